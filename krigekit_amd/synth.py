@@ -1,0 +1,107 @@
+"""Seeded synthetic workloads for the kriging hot path.
+
+Each generator returns a dict describing one kriging problem: observations,
+target nodes, variogram structures and search settings, in the argument
+conventions of the ``Kriging`` class (coordinates (n, ndim), 1-based ivar).
+All randomness comes from ``numpy.random.default_rng(seed)`` (PCG64).
+
+Workloads (sizes are the full ones; ``scale`` shrinks the target grid for
+parity tests while keeping observations, model and search identical):
+
+- ``ok2d``   2-D ordinary kriging: 2,000 obs uniform in [0, 10000]^2 m, value =
+  8-term random-phase sinusoid sum (wavelengths 500-4000 m) + N(0, 0.1^2);
+  200 x 200 nodes at 50 m; spherical, sill 1.0, nugget 0.05, range 1500 m;
+  nmax = 16, no sectors.
+- ``ok3d``   3-D ordinary kriging: 50,000 obs from 20 Gaussian clusters
+  (sigma 300/300/10 m, reflected into a 12.8 km x 12.8 km x 320 m box);
+  256 x 256 x 64 = 4,194,304 nodes at (50, 50, 5) m; exponential, sill 1.0,
+  nugget 0.02, ranges 2000/800/50 m, azimuth 30 deg; octant search with
+  at most 4 per octant (<= 32 neighbours from 128 candidates).
+- ``loo2d``  leave-one-out cross-validation: 90,000 uniform obs in
+  [0, 20000]^2 + 10,000 near-duplicates displaced 0.01 m; anisotropic
+  spherical (nugget 0.1, sill 1.0, ranges 1500/600, azimuth 45 deg); nmax 16.
+"""
+import numpy as np
+
+
+def _sinusoid_field_2d(rng, xy, nterm=8, lmin=500.0, lmax=4000.0):
+    lam = rng.uniform(lmin, lmax, nterm)
+    ang = rng.uniform(0.0, 2 * np.pi, nterm)
+    phi = rng.uniform(0.0, 2 * np.pi, nterm)
+    u = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    proj = xy @ u.T  # (n, nterm)
+    return np.sin(2 * np.pi * proj / lam + phi).sum(axis=1)
+
+
+def _sinusoid_field_3d(rng, xyz, nterm=8):
+    lam_h = rng.uniform(500.0, 4000.0, nterm)
+    lam_v = rng.uniform(40.0, 320.0, nterm)
+    ang = rng.uniform(0.0, 2 * np.pi, nterm)
+    phi = rng.uniform(0.0, 2 * np.pi, nterm)
+    u = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    proj = xyz[:, :2] @ u.T
+    return np.sin(2 * np.pi * (proj / lam_h + xyz[:, 2:3] / lam_v) + phi).sum(axis=1)
+
+
+def _grid(shape, spacing, origin):
+    axes = [origin[d] + spacing[d] * np.arange(shape[d]) for d in range(len(shape))]
+    mesh = np.meshgrid(*axes, indexing="ij")
+    return np.stack([m.ravel() for m in mesh], axis=1)
+
+
+def ok2d(scale=1.0, seed=1001):
+    rng = np.random.default_rng(seed)
+    n = 2000
+    xy = rng.uniform(0.0, 10000.0, (n, 2))
+    val = _sinusoid_field_2d(rng, xy) + rng.normal(0.0, 0.1, n)
+    ng = max(2, int(round(200 * np.sqrt(scale))))
+    grid = _grid((ng, ng), (50.0 * 200 / ng,) * 2, (25.0, 25.0))
+    return dict(name="ok2d", ndim=2, obs=[dict(coord=xy, value=val, nmax=16)], grid=grid,
+                vgm=[dict(ivar=1, jvar=1, vtype="sph", nugget=0.05, sill=1.0, a_major=1500.0)],
+                search=[dict(ivar=1, sector_search=False)])
+
+
+def ok3d_obs(seed=1002):
+    rng = np.random.default_rng(seed)
+    box = np.array([12800.0, 12800.0, 320.0])
+    sig = np.array([300.0, 300.0, 10.0])
+    ncl, n = 20, 50000
+    centres = rng.uniform(0.0, 1.0, (ncl, 3)) * box
+    lab = rng.integers(0, ncl, n)
+    pts = centres[lab] + rng.normal(0.0, 1.0, (n, 3)) * sig
+    # reflect into the box
+    pts = np.abs(pts)
+    pts = np.where(pts > box, 2 * box - pts, pts)
+    pts = np.clip(pts, 0.0, box)
+    val = _sinusoid_field_3d(rng, pts) + rng.normal(0.0, 0.1, n)
+    return rng, pts, val
+
+
+def ok3d(scale=1.0, seed=1002, grid_shape=(256, 256, 64)):
+    """Full size is 256 x 256 x 64 nodes; ``scale`` < 1 keeps every
+    ``1/scale``-th node of the same lattice (a strided subset), so parity
+    runs see the same geometry at a fraction of the nodes."""
+    _, pts, val = ok3d_obs(seed)
+    grid = _grid(grid_shape, (50.0, 50.0, 5.0), (25.0, 25.0, 2.5))
+    if scale < 1.0:
+        step = max(1, int(round(1.0 / scale)))
+        grid = grid[::step]
+    return dict(name="ok3d", ndim=3, obs=[dict(coord=pts, value=val, nmax=4)], grid=grid,
+                vgm=[dict(ivar=1, jvar=1, vtype="exp", nugget=0.02, sill=1.0, a_major=2000.0,
+                          a_minor1=800.0, a_minor2=50.0, azimuth=30.0)],
+                search=[dict(ivar=1, sector_search=True)])
+
+
+def loo2d(n_uniform=90000, n_dup=10000, seed=1004, nmax=16):
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(0.0, 20000.0, (n_uniform, 2))
+    src = xy[rng.choice(n_uniform, n_dup, replace=False)] if n_dup else xy[:0]
+    ang = rng.uniform(0.0, 2 * np.pi, n_dup)
+    dup = src + 0.01 * np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    pts = np.concatenate([xy, dup])
+    val = _sinusoid_field_2d(rng, pts) + rng.normal(0.0, 0.1, len(pts))
+    return dict(name="loo2d", ndim=2, cross_validation=True,
+                obs=[dict(coord=pts, value=val, nmax=nmax)], grid=None,
+                vgm=[dict(ivar=1, jvar=1, vtype="sph", nugget=0.1, sill=1.0, a_major=1500.0,
+                          a_minor1=600.0, azimuth=45.0)],
+                search=[dict(ivar=1, sector_search=False)])

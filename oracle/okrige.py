@@ -1,0 +1,193 @@
+"""ctypes front end of the C oracle ``oracle/liboracle.so`` (test infrastructure).
+
+``solve(spec)`` runs the CPU restatement of the reference hot path on a problem
+described in the same dict format the workload generators produce
+(``krigekit_amd.synth``).  Only tests, the smoke entry point and bench.py's
+cpu_baseline leg use this module.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+VTYPES = {"nug": 0, "sph": 1, "exp": 2, "gau": 3, "hol": 4, "pow": 5, "bsq": 6, "cir": 7,
+          "lin": 8, "cyc": 9, "dco": 10}
+
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
+
+
+class Struct(ctypes.Structure):
+    _fields_ = [("vtype", ctypes.c_int), ("is_product", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in
+        ("nugget", "sill", "a_major", "a_minor1", "a_minor2", "azimuth", "dip", "plunge")]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int), ("nobs", ctypes.c_int),
+        ("obs_coord", _pd), ("obs_value", _pd), ("obs_var", _pd),
+        ("nmax", ctypes.c_int), ("maxdist", ctypes.c_double), ("sk_mean", ctypes.c_double),
+        ("unbias", ctypes.c_int), ("sector", ctypes.c_int), ("cross_validation", ctypes.c_int),
+        ("aniso_search", ctypes.c_int),
+        ("s_anis1", ctypes.c_double), ("s_anis2", ctypes.c_double), ("s_azimuth", ctypes.c_double),
+        ("s_dip", ctypes.c_double), ("s_plunge", ctypes.c_double),
+        ("nblock", ctypes.c_int), ("block_coord", _pd), ("nsub", ctypes.c_int),
+        ("sub_coord", _pd), ("sub_weight", _pd), ("rangescale", _pd), ("localnugget", _pd),
+        ("nstruct", ctypes.c_int), ("structs", ctypes.POINTER(Struct)),
+        ("table_mode", ctypes.c_int), ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double),
+        ("neglect_error", ctypes.c_int), ("nthread", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("estimate", _pd), ("variance", _pd), ("mmax", ctypes.c_int), ("nnear", _pi),
+                ("inear", _pi), ("weights", _pd), ("stats", ctypes.c_int * 5),
+                ("errmsg", ctypes.c_char * 512)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"oracle not built: {LIB_PATH} (make -C oracle oracle)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.okr_solve.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Result)]
+        _lib.okr_solve.restype = ctypes.c_int
+        _lib.okr_mmax.argtypes = [ctypes.POINTER(Problem)]
+        _lib.okr_mmax.restype = ctypes.c_int
+        _lib.okr_cov.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, _pd, _pd]
+        _lib.okr_cov.restype = ctypes.c_int
+        _lib.okr_knn.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pd, ctypes.c_int, ctypes.c_int,
+                                 _pi, _pd]
+        _lib.okr_knn.restype = ctypes.c_int
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_pd)
+
+
+def _structs(vgms):
+    arr = (Struct * len(vgms))()
+    for i, v in enumerate(vgms):
+        am = v.get("a_major", 1.0)
+        a1 = v.get("a_minor1", am)
+        a1 = am if a1 is None else a1
+        a2 = v.get("a_minor2", a1)
+        a2 = a1 if a2 is None else a2
+        arr[i] = Struct(VTYPES[v["vtype"].lower()[:3]], int(v.get("product", False)),
+                        v.get("nugget", 0.0), v.get("sill", 1.0), am, a1, a2,
+                        v.get("azimuth", 0.0), v.get("dip", 0.0), v.get("plunge", 0.0))
+    return arr
+
+
+class _Holder:
+    """keeps numpy buffers alive while a Problem refers to them"""
+
+
+def make_problem(spec, table_mode=True, nthread=8, bounds=None, neglect_error=True):
+    h = _Holder()
+    o = spec["obs"][0]
+    nd = spec["ndim"]
+    h.oc = np.ascontiguousarray(np.asarray(o["coord"], dtype=np.float64))  # (n, nd) == (nd, n) F
+    h.ov = np.ascontiguousarray(o["value"], dtype=np.float64)
+    h.ovar = np.ascontiguousarray(o.get("variance", np.zeros(len(h.ov))), dtype=np.float64)
+    cv = bool(spec.get("cross_validation", False))
+    P = Problem()
+    P.ndim, P.nobs = nd, len(h.ov)
+    P.obs_coord, P.obs_value, P.obs_var = _d(h.oc), _d(h.ov), _d(h.ovar)
+    nmax = o.get("nmax")
+    P.nmax = nmax if nmax is not None else np.iinfo(np.int32).max
+    md = o.get("maxdist")
+    P.maxdist = md if md is not None else sys.float_info.max
+    P.sk_mean = o.get("sk_mean", 0.0)
+    P.unbias = spec.get("unbias", 1)
+    s = spec["search"][0]
+    P.sector = int(s.get("sector_search", False))
+    P.cross_validation = int(cv)
+    P.aniso_search = int(spec.get("anisotropic_search", False))
+    P.s_anis1, P.s_anis2 = s.get("anis1", 1.0), s.get("anis2", 1.0)
+    P.s_azimuth, P.s_dip, P.s_plunge = s.get("azimuth", 0.0), s.get("dip", 0.0), s.get("plunge", 0.0)
+    if cv:
+        P.nblock = P.nobs
+        h.bc = h.oc
+    else:
+        h.bc = np.ascontiguousarray(np.asarray(spec["grid"], dtype=np.float64))
+        P.nblock = len(h.bc)
+    P.block_coord = _d(h.bc)
+    P.nsub = 1
+    if spec.get("sub_coord") is not None:
+        h.sc = np.ascontiguousarray(spec["sub_coord"], dtype=np.float64)
+        h.sw = np.ascontiguousarray(spec["sub_weight"], dtype=np.float64)
+        P.nsub = int(spec["nsub"])
+        P.sub_coord, P.sub_weight = _d(h.sc), _d(h.sw)
+    if spec.get("rangescale") is not None:
+        h.rs = np.ascontiguousarray(spec["rangescale"], dtype=np.float64)
+        P.rangescale = _d(h.rs)
+    if spec.get("localnugget") is not None:
+        h.ln = np.ascontiguousarray(spec["localnugget"], dtype=np.float64)
+        P.localnugget = _d(h.ln)
+    h.st = _structs(spec["vgm"])
+    P.nstruct = len(spec["vgm"])
+    P.structs = ctypes.cast(h.st, ctypes.POINTER(Struct))
+    P.table_mode = int(table_mode)
+    huge = sys.float_info.max * 1e3
+    b = bounds if bounds is not None else spec.get("bounds", (-huge, huge))
+    P.bound_lo, P.bound_hi = b
+    P.neglect_error = int(neglect_error)
+    P.nthread = nthread
+    return P, h
+
+
+def solve(spec, table_mode=True, nthread=8, want_weights=False, **kw):
+    P, h = make_problem(spec, table_mode, nthread, **kw)
+    L = lib()
+    mmax = L.okr_mmax(ctypes.byref(P))
+    nb = P.nblock
+    est = np.empty(nb)
+    var = np.empty(nb)
+    nnear = np.zeros(nb, dtype=np.int32)
+    inear = np.zeros((nb, mmax), dtype=np.int32)
+    w = np.zeros((nb, mmax)) if want_weights else None
+    R = Result()
+    R.estimate, R.variance, R.mmax = _d(est), _d(var), mmax
+    R.nnear = nnear.ctypes.data_as(_pi)
+    R.inear = inear.ctypes.data_as(_pi)
+    if w is not None:
+        R.weights = _d(w)
+    rc = L.okr_solve(ctypes.byref(P), ctypes.byref(R))
+    if rc:
+        raise RuntimeError(R.errmsg.decode())
+    out = dict(estimate=est, variance=var, nnear=nnear, inear=inear,
+               stats=dict(zip(["fail", "chol_fact", "chol_reuse", "ssytrf_fact", "ssytrf_reuse"],
+                              list(R.stats))))
+    if w is not None:
+        out["weights"] = w
+    return out
+
+
+def cov(spec, lags, table_mode=True):
+    P, h = make_problem(spec, table_mode, 1)
+    lags = np.ascontiguousarray(lags, dtype=np.float64)
+    out = np.empty(len(lags))
+    if lib().okr_cov(ctypes.byref(P), len(lags), _d(lags), _d(out)):
+        raise RuntimeError("okr_cov failed")
+    return out
+
+
+def knn(coord, q, k, sort=False):
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    idx = np.zeros(k, dtype=np.int32)
+    dis = np.zeros(k)
+    if lib().okr_knn(coord.shape[1], coord.shape[0], _d(coord), _d(q), k, int(sort),
+                     idx.ctypes.data_as(_pi), _d(dis)):
+        raise RuntimeError("okr_knn failed")
+    return idx, dis
