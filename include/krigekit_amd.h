@@ -1,0 +1,156 @@
+/*
+ * C ABI of libkrigekit_amd.so, the MI355X (gfx950) kriging backend.
+ *
+ * The krige_* entry points have the names, argument meaning, array layouts
+ * and error behaviour of the reference engine's C ABI, so krigekit's ctypes
+ * wrapper can bind them unchanged.  Each declaration cites the reference
+ * interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions (src/libkriging/kriging_capi_common.F90, kriging_err.f90):
+ *   - handles are 1-based int64 slots of a process-global registry;
+ *   - every entry clears the error slot first and returns 0 on success,
+ *     1 on error; krige_get_last_error copies the first error message;
+ *   - coordinates are (ndim, n) column-major; results are block-first;
+ *   - input arrays are copied; outputs go to caller-allocated buffers.
+ * Entry points for features outside this backend (indicator kriging, SGSIM,
+ * drift, gradient constraints, varying variograms, factor cache, weight
+ * replay) exist and return 1 with a message.
+ */
+#ifndef KRIGEKIT_AMD_H
+#define KRIGEKIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/libkriging/kriging_capi.F90:43 */
+int krige_create(int64_t *handle);
+/* src/libkriging/kriging_capi.F90 (krige_ind_create): returns 1, not provided */
+int krige_ind_create(int64_t *handle);
+/* src/libkriging/kriging_capi.F90 (krige_ind_set_ncat): returns 1, not provided */
+int krige_ind_set_ncat(int64_t handle, int ncat);
+/* src/libkriging/kriging_capi.F90 (krige_destroy): zeroes *handle */
+int krige_destroy(int64_t *handle);
+
+/* src/libkriging/kriging_capi.F90:143-192 */
+int krige_initialize(int64_t handle, int ndim, int nvar, int ndrift, int unbias, int nsim,
+                     int anisotropic_search, int weight_correction, int use_old_weight,
+                     int store_weight, int cross_validation, int write_mat, int neglect_error,
+                     int varying_vgm, int std_ck, int verbose, int pf_cache,
+                     const char *weight_file, const double bounds[2], int seed);
+
+/* src/libkriging/kriging_capi.F90:197-224; coord (ndim, nobs) column-major */
+int krige_set_obs(int64_t handle, int ivar, int nobs, int ndim, const double *coord,
+                  const double *value, const double *variance, int nmax, double maxdist,
+                  double sk_mean);
+/* src/libkriging/kriging_capi_common.F90:178: returns 1, not provided */
+int krige_set_obs_drift(int64_t handle, int ivar, int ndrift, int nobs, const double *drift);
+/* src/libkriging/kriging_capi_common.F90:136 */
+int krige_update_obs_value(int64_t handle, int ivar, int nobs, const double *value);
+/* src/libkriging/kriging_capi_common.F90:156: returns 1, not provided */
+int krige_set_nscore(int64_t handle, int ivar, double zmin, double zmax, int ltail, int utail,
+                     double ltpar, double utpar, int nwt, const double *wt);
+
+/* src/libkriging/kriging_capi.F90 (krige_reset_vgm) */
+int krige_reset_vgm(int64_t handle, int ivar, int jvar);
+/* src/libkriging/kriging_capi.F90:245-270 */
+int krige_set_vgm(int64_t handle, int ivar, int jvar, const char *vtype, double nugget,
+                  double sill, double a_major, double a_minor1, double a_minor2, double azimuth,
+                  double dip, double plunge, int is_product);
+/* src/libkriging/kriging_capi.F90 (krige_set_vgm_block): returns 1, not provided */
+int krige_set_vgm_block(int64_t handle, int ivar, int jvar, int ib, const char *vtype,
+                        double nugget, double sill, double a_major, double a_minor1,
+                        double a_minor2, double azimuth, double dip, double plunge,
+                        int is_product);
+
+/* src/libkriging/kriging_capi.F90:311-332 */
+int krige_set_grid(int64_t handle, int ngrid, int ndim, const double *coord,
+                   const double *rangescale, const double *localnugget);
+/* src/libkriging/kriging_capi.F90:334-374 (block_type -4: 4^ndim Gauss-Legendre
+ * sub-points; block_type > 0: user sub-points with equal counts per block) */
+int krige_set_grid_block(int64_t handle, int block_type, int ngrid, int ndim, const double *coord,
+                         int nblock, const int *nblockpnt, const double *pointweight,
+                         const double *blocksize, const double *rangescale,
+                         const double *localnugget);
+/* src/libkriging/kriging_capi_common.F90:195 */
+int krige_set_grid_cv(int64_t handle);
+/* src/libkriging/kriging_capi.F90 (krige_set_grid_drift): returns 1, not provided */
+int krige_set_grid_drift(int64_t handle, int ivar, int ndrift, int nblocks, const double *drift);
+/* src/libkriging/kriging_capi.F90 (krige_set_sim): returns 1, not provided */
+int krige_set_sim(int64_t handle, int nblocks, const void *randpath, int nsim, int nvar,
+                  const void *sample);
+/* src/libkriging/kriging_capi.F90:448-467 (builds the kd-tree) */
+int krige_set_search(int64_t handle, int ivar, double anis1, double anis2, double azimuth,
+                     double dip, double plunge, int sector_search);
+/* src/libkriging/kriging_capi.F90 (krige_set_grad): returns 1, not provided */
+int krige_set_grad(int64_t handle, int ivar, int ngrad, int ndim, const double *coord1,
+                   const double *coord2, const double *grad_val, const double *variance,
+                   int ndrift, const double *drift_ext);
+
+/* src/libkriging/kriging_capi_common.F90:212 (validation only) */
+int krige_prepare(int64_t handle);
+/* src/libkriging/kriging_capi_common.F90:229-239: the hot path, run on the GPU;
+ * nthread and ncache are accepted and ignored */
+int krige_solve(int64_t handle, int nthread, int ncache);
+
+/* src/libkriging/kriging_capi_common.F90:246-392 */
+int krige_get_nblocks(int64_t handle, int *n);
+int krige_get_nsim(int64_t handle, int *n);
+int krige_get_block_coord(int64_t handle, int ndim, int nblocks, double *out);
+int krige_get_estimate(int64_t handle, int nsim, int nblocks, double *out);
+int krige_get_estimate_all(int64_t handle, int nblocks, int nvar, int nsim, double *out);
+int krige_get_variance(int64_t handle, int nblocks, double *out);
+int krige_get_variance_all(int64_t handle, int nblocks, int nvar, double *out);
+
+/* src/libkriging/kriging_capi_common.F90:417-484: return 1, not provided */
+int krige_get_factor_info(int64_t handle, int *valid, int *npp, int *p);
+int krige_get_factor_matrices(int64_t handle, int npp, int p, double *L, double *kinv_drift,
+                              double *schur);
+int krige_get_factor_system(int64_t handle, int matsize, int nvar, int which, double *matA,
+                            double *rhsB);
+
+/* src/libkriging/kriging_capi_common.F90:90-96 */
+int krige_get_last_error(char *buffer, int nbuf);
+/* src/libkriging/kriging_capi_common.F90:113-128: fail, chol_fact, chol_reuse,
+ * ssytrf_fact, ssytrf_reuse */
+void krige_solver_stats(int64_t handle, int out[5]);
+/* src/libkriging/kriging_capi_common.F90:397: library-owned string */
+intptr_t krige_to_str(int64_t handle);
+/* src/krigekit/kriging.py:282-283 (OpenMP probes; this backend reports 1) */
+void krige_get_max_threads(int *n);
+void krige_get_num_threads(int *n);
+
+/* weight-store views, src/libkriging/kriging_capi_common.F90:491-577:
+ * neighbour lists of the last solve when store_weight was set */
+int krige_free_weight_store(int64_t handle);
+int krige_get_weight_dims(int64_t handle, int *nmax, int *ngroups, int *nblock);
+int krige_get_weight_nnear(int64_t handle, int ngroups, int nblock, int *out);
+int krige_get_weight_inear(int64_t handle, int nmax, int ngroups, int nblock, int *out);
+/* returns 1, not provided */
+int krige_get_weight_data(int64_t handle, int nmax, int ngroups, int nvar, int nblock, double *out);
+int krige_get_weight_var(int64_t handle, int nvar, int nblock, double *out);
+int krige_set_weights(int64_t handle, int nmax, int ngroups, int nvar, int nblock, const int *nnear,
+                      const int *inear, const double *weight, const int *order, const double *var);
+
+/* ---- extensions: no counterpart in the reference -------------------------- */
+/* precision 0 = FP64 solve, 1 = FP32 factorisation + FP64 refinement;
+ * pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64;
+ * table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic */
+int kk_set_options(int64_t handle, int precision, double pivot_tol, int table_mode);
+/* solve with inputs and results left resident in HBM (no result copy-back) */
+int kk_solve_resident(int64_t handle);
+/* copy the results of the last kk_solve_resident to the host */
+int kk_fetch(int64_t handle);
+/* out[8]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
+ * nodes, LU-fallback nodes, reserved */
+int kk_get_timing(int64_t handle, double *out);
+/* neighbour lists of the last solve (0-based ids ascending, row stride mmax) */
+int kk_get_mmax(int64_t handle, int *mmax);
+int kk_get_neighbours(int64_t handle, int nblock, int mmax, int *nnear, int *inear);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KRIGEKIT_AMD_H */

@@ -1,0 +1,1731 @@
+// C-ABI of the MI355X kriging backend: the krige_* entry points that
+// krigekit's ctypes wrapper binds (src/krigekit/kriging.py:142-286), with the
+// reference's handle registry and first-error conventions
+// (src/libkriging/kriging_capi_common.F90:61-65,627-695;
+//  src/libkriging/kriging_err.f90:15-118).
+//
+// Setup calls copy caller arrays (kriging_base.F90:600).  solve() builds the
+// kd-tree (at set_search, as the reference does) and covariance tables on the
+// host, keeps inputs resident in HBM across solves, runs search -> bucket ->
+// per-node solve on the GPU and copies estimates and variances back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/krigekit_amd.h"
+#include "kk_device.hpp"
+#include "kk_types.hpp"
+
+using kk::Model;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// first-error slot (kriging_err.f90): every entry clears it; the first error
+// raised in a call chain is kept.
+// ---------------------------------------------------------------------------
+int g_ierr = 0;
+std::string g_msg;
+
+void clear_error() {
+  g_ierr = 0;
+  g_msg.clear();
+}
+void set_error(const char* where, const std::string& msg, long iblock = -1) {
+  if (g_ierr) return;
+  g_ierr = 1;
+  g_msg = "\nKRIGING ERROR: Location: ";
+  g_msg += where;
+  if (iblock >= 0) g_msg += "; Block: " + std::to_string(iblock);
+  g_msg += "\nDescription: " + msg;
+}
+bool failed() { return g_ierr != 0; }
+
+#define HIPCHK(expr, where)                                                        \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(where, std::string("HIP error: ") + hipGetErrorString(e_));        \
+      return false;                                                                \
+    }                                                                              \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// model: variogram.f90 structures, anisotropy (rotation.f90:51-95), cov0
+// (variogram.f90:853-869), piecewise tables (variogram.f90:223-325,497-680)
+// ---------------------------------------------------------------------------
+struct VStruct {
+  int vtype, is_product;
+  double nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge;
+};
+
+const double kPI = 4.0 * std::atan(1.0);
+
+void matmul3(const double* a, const double* b, double* c) {
+  double t[9];
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s = s + a[i + 3 * k] * b[k + 3 * j];
+      t[i + 3 * j] = s;
+    }
+  std::memcpy(c, t, sizeof t);
+}
+
+// calc_rotmat: diag(1/anis1, 1, 1/anis2) * Ry(plunge) * Rx(dip) * Rz(azimuth)
+void calc_rotmat(double ang1, double ang2, double ang3, double anis1, double anis2, double* R) {
+  const double d2r = kPI / 180.0, eps = 1.e-10;
+  std::fill(R, R + 9, 0.0);
+  R[0] = 1.0 / anis1;
+  R[4] = 1.0;
+  R[8] = 1.0 / anis2;
+  if (std::fabs(ang3) > eps) {
+    double a = -ang3 * d2r, s = std::sin(a), c = std::cos(a);
+    double T[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
+    matmul3(R, T, R);
+  }
+  if (std::fabs(ang2) > eps) {
+    double a = -ang2 * d2r, s = std::sin(a), c = std::cos(a);
+    double T[9] = {1.0, 0.0, 0.0, 0.0, c, s, 0.0, -s, c};
+    matmul3(R, T, R);
+  }
+  if (std::fabs(ang1) > eps) {
+    double a = ang1 * d2r, s = std::sin(a), c = std::cos(a);
+    double T[9] = {c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0};
+    matmul3(R, T, R);
+  }
+}
+
+double corefunc_host(int t, double r) {
+  switch (t) {
+    case kk::SPH: return (r < 1.0) ? (1.0 - 1.5 * r) + 0.5 * (r * r * r) : 0.0;
+    case kk::EXP: return std::exp(-3.0 * r);
+    case kk::GAU: return std::exp(-3.0625 * (r * r));
+    case kk::HOL: return std::cos(kPI * r);
+    case kk::POW: return (r < 1.0) ? 1.0 - std::pow(r, 1.5) : 0.0;
+    case kk::BSQ: { double a = 1.0 - r * r; return (r < 1.0) ? a * a : 0.0; }
+    case kk::CIR: return (r < 1.0) ? 1.0 - (2.0 * r * std::sqrt(1.0 - r * r) + 2.0 * std::asin(r)) / kPI : 0.0;
+    case kk::LIN: return (r < 1.0) ? 1.0 - r : 0.0;
+    case kk::CYC: { double s = std::sin(kPI * r); return std::exp(-2.0 * (s * s)); }
+    case kk::DCO: return std::exp(-3.0 * r) * std::cos(kPI * r);
+    default: return 0.0;
+  }
+}
+bool has_tail_host(int t) { return t == kk::EXP || t == kk::GAU || t == kk::HOL || t == kk::CYC || t == kk::DCO; }
+
+int vtype_from_str(const char* s) {
+  static const char* names[] = {"nug", "sph", "exp", "gau", "hol", "pow", "bsq", "cir", "lin", "cyc", "dco"};
+  for (int i = 0; i < 11; ++i)
+    if (strncasecmp(s, names[i], 3) == 0) return i;
+  return -1;
+}
+
+// Builds the kernel Model plus its table storage.  table_mode follows the
+// reference's default build (USE_COV_TABLE); analytic is the alternative.
+bool build_model(const std::vector<VStruct>& vs, int ndim, bool table_mode, Model& M,
+                 std::vector<double>& tab) {
+  std::memset(&M, 0, sizeof M);
+  M.ndim = ndim;
+  M.nstruct = (int)vs.size();
+  if (M.nstruct < 1 || M.nstruct > kk::MAXS) return false;
+  for (int i = 0; i < M.nstruct; ++i) {
+    M.vtype[i] = vs[i].vtype;
+    M.is_product[i] = vs[i].is_product;
+    M.sill[i] = vs[i].sill;
+    M.nugget[i] = vs[i].nugget;
+    double R[9];
+    calc_rotmat(vs[i].azimuth, vs[i].dip, vs[i].plunge, vs[i].a_minor1 / vs[i].a_major,
+                vs[i].a_minor2 / vs[i].a_major, R);
+    for (int k = 0; k < 9; ++k) M.mat[i][k] = R[k] / vs[i].a_major;
+  }
+  double tot = 0.0;
+  int iv = 0;
+  while (iv < M.nstruct) {
+    double g = vs[iv].sill + vs[iv].nugget;
+    while (iv + 1 < M.nstruct && vs[iv + 1].is_product) {
+      ++iv;
+      g = g * (vs[iv].sill + vs[iv].nugget);
+    }
+    tot = tot + g;
+    ++iv;
+  }
+  M.cov0 = tot;
+  M.mode = kk::COV_ANALYTIC;
+  tab.clear();
+  if (!table_mode) return true;
+  const double hb[4] = {0.0, 0.1, 0.5, 3.5};
+  const double dh[3] = {1e-5, 1e-4, 1e-3};
+  int off = 0;
+  for (int k = 0; k < 3; ++k) {
+    int n = (int)std::lround((hb[k + 1] - hb[k]) / dh[k]);
+    n = std::max(1, n);
+    M.zn[k] = n;
+    M.zoff[k] = off;
+    M.zinv[k] = (double)n / (hb[k + 1] - hb[k]);
+    off += n;
+  }
+  for (int k = 0; k < 4; ++k) M.zb[k] = hb[k];
+  M.ntab = off;
+  // Level B if all non-nugget structures share one matrix (MAT_TOL 1e-6;
+  // product members only need equal angles, ANG_TOL 1e-4)
+  int ref = -1;
+  bool levelB = true;
+  for (int i = 0; i < M.nstruct; ++i) {
+    if (vs[i].vtype == kk::NUG) continue;
+    if (ref < 0) {
+      ref = i;
+      continue;
+    }
+    if (vs[i].is_product) {
+      if (std::fabs(vs[i].azimuth - vs[ref].azimuth) > 1e-4 || std::fabs(vs[i].dip - vs[ref].dip) > 1e-4 ||
+          std::fabs(vs[i].plunge - vs[ref].plunge) > 1e-4) {
+        levelB = false;
+        break;
+      }
+    } else {
+      double mx = 0.0;
+      for (int k = 0; k < 9; ++k) mx = std::max(mx, std::fabs(M.mat[i][k] - M.mat[ref][k]));
+      if (mx > 1.0e-6) {
+        levelB = false;
+        break;
+      }
+    }
+  }
+  if (levelB && ref < 0) return true;  // all nuggets: no table is built, analytic path
+  if (levelB) {
+    M.mode = kk::COV_TABLE_B;
+    std::memcpy(M.bmat, M.mat[ref], sizeof M.bmat);
+    const double aref = vs[ref].a_major;
+    tab.assign(M.ntab + 1, 0.0);
+    tab[0] = M.cov0;
+    for (int k = 0; k < 3; ++k) {
+      const double hlo = hb[k], step = 1.0 / M.zinv[k];
+      for (int i = 1; i <= M.zn[k]; ++i) {
+        const double rdist = hlo + (double)i * step;
+        double val = 0.0;
+        int j = 0;
+        while (j < M.nstruct) {
+          if (vs[j].vtype == kk::NUG) {
+            ++j;
+            continue;
+          }
+          double rk = rdist * aref / vs[j].a_major;
+          double g = vs[j].sill * corefunc_host(vs[j].vtype, rk);
+          while (j + 1 < M.nstruct && vs[j + 1].is_product) {
+            ++j;
+            rk = rdist * aref / vs[j].a_major;
+            g = g * vs[j].sill * corefunc_host(vs[j].vtype, rk);
+          }
+          val = val + g;
+          ++j;
+        }
+        tab[M.zoff[k] + i] = val;
+      }
+    }
+    int tail = 0, j = 0;
+    while (j < M.nstruct) {
+      if (has_tail_host(vs[j].vtype) && vs[j].vtype != kk::NUG) {
+        int gt = has_tail_host(vs[j].vtype);
+        while (j + 1 < M.nstruct && vs[j + 1].is_product) {
+          ++j;
+          gt = gt && has_tail_host(vs[j].vtype);
+        }
+        tail = tail || gt;
+      }
+      ++j;
+    }
+    M.b_tail = tail;
+  } else {
+    M.mode = kk::COV_TABLE_A;
+    tab.assign((size_t)M.nstruct * (M.ntab + 1), 0.0);
+    for (int c = 0; c < M.nstruct; ++c) {
+      double* t = tab.data() + (size_t)c * (M.ntab + 1);
+      t[0] = vs[c].sill + vs[c].nugget;
+      for (int k = 0; k < 3; ++k) {
+        const double hlo = hb[k], step = 1.0 / M.zinv[k];
+        for (int i = 1; i <= M.zn[k]; ++i)
+          t[M.zoff[k] + i] = vs[c].sill * corefunc_host(vs[c].vtype, hlo + (double)i * step);
+      }
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// kd-tree build on the host (kdtree2_maxidx.f90:629-998): mean split on the
+// widest dimension, buckets of at most 13 points, boxes of the non-cut
+// dimensions inherited from the parent, rearranged coordinate copy.
+// Flattened into node records with parent links for the GPU traversal.
+// ---------------------------------------------------------------------------
+struct HostTree {
+  int dim = 0, n = 0, root = -1;
+  std::vector<int> ind;
+  std::vector<double> rdata;
+  std::vector<int> ni;     // NODE_I per node
+  std::vector<double> nd;  // NODE_D per node
+  int nnodes() const { return (int)(ni.size() / kk::NODE_I); }
+};
+
+struct TreeBuilder {
+  HostTree& t;
+  const double* data;
+  int dim;
+  double lo(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + 3 + k]; }
+  double hi(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + 6 + k]; }
+  void set_box(int node, int k, double l, double h) {
+    t.nd[(size_t)node * kk::NODE_D + 3 + k] = l;
+    t.nd[(size_t)node * kk::NODE_D + 6 + k] = h;
+  }
+  void spread(int c, int l, int u, double& mn, double& mx) const {
+    mn = mx = data[c + dim * (size_t)t.ind[l]];
+    for (int i = l + 1; i <= u; ++i) {
+      double v = data[c + dim * (size_t)t.ind[i]];
+      mn = std::min(mn, v);
+      mx = std::max(mx, v);
+    }
+  }
+  int add_node() {
+    int id = t.nnodes();
+    t.ni.resize(t.ni.size() + kk::NODE_I, -1);
+    t.nd.resize(t.nd.size() + kk::NODE_D, 0.0);
+    return id;
+  }
+  int build(int l, int u, int parent, int depth) {
+    if (u < l) return -1;
+    if (depth > 256) return -2;
+    const int id = add_node();
+    int* ni = &t.ni[(size_t)id * kk::NODE_I];
+    ni[1] = l;
+    ni[2] = u;
+    ni[5] = parent;
+    if (u - l <= 12) {
+      for (int k = 0; k < dim; ++k) {
+        double a, b;
+        spread(k, l, u, a, b);
+        set_box(id, k, a, b);
+      }
+      ni[0] = -1;
+      return id;
+    }
+    const int pcut = parent >= 0 ? t.ni[(size_t)parent * kk::NODE_I] : -1;
+    for (int k = 0; k < dim; ++k) {
+      if (parent < 0 || k == pcut) {
+        double a, b;
+        spread(k, l, u, a, b);
+        set_box(id, k, a, b);
+      } else {
+        set_box(id, k, lo(parent, k), hi(parent, k));
+      }
+    }
+    int c = 0;
+    double best = hi(id, 0) - lo(id, 0);
+    for (int k = 1; k < dim; ++k)
+      if (hi(id, k) - lo(id, k) > best) {
+        best = hi(id, k) - lo(id, k);
+        c = k;
+      }
+    double sum = 0.0;
+    for (int j = l; j <= u; ++j) sum = sum + data[c + dim * (size_t)t.ind[j]];
+    const double average = sum / (double)(u - l + 1);
+    // partition: points with coordinate <= average to the left
+    int lb = l, rb = u;
+    while (lb < rb) {
+      if (data[c + dim * (size_t)t.ind[lb]] <= average) {
+        ++lb;
+      } else {
+        std::swap(t.ind[lb], t.ind[rb]);
+        --rb;
+      }
+    }
+    const int m = (data[c + dim * (size_t)t.ind[lb]] <= average) ? lb : lb - 1;
+    t.ni[(size_t)id * kk::NODE_I + 0] = c;
+    t.nd[(size_t)id * kk::NODE_D + 0] = average;
+    const int left = build(l, m, id, depth + 1);
+    if (left == -2) return -2;
+    const int right = build(m + 1, u, id, depth + 1);
+    if (right == -2) return -2;
+    ni = &t.ni[(size_t)id * kk::NODE_I];
+    double* nd = &t.nd[(size_t)id * kk::NODE_D];
+    ni[3] = left;
+    ni[4] = right;
+    if (right < 0) {
+      for (int k = 0; k < dim; ++k) set_box(id, k, lo(left, k), hi(left, k));
+      nd[1] = hi(left, c);
+      nd[0] = nd[1];
+    } else if (left < 0) {
+      for (int k = 0; k < dim; ++k) set_box(id, k, lo(right, k), hi(right, k));
+      nd[2] = lo(right, c);
+      nd[0] = nd[2];
+    } else {
+      nd[2] = lo(right, c);
+      nd[1] = hi(left, c);
+      nd[0] = (nd[1] + nd[2]) / 2;
+      for (int k = 0; k < dim; ++k)
+        set_box(id, k, std::min(lo(left, k), lo(right, k)), std::max(hi(left, k), hi(right, k)));
+    }
+    return id;
+  }
+};
+
+bool build_tree(HostTree& t, int dim, int n, const double* data) {
+  t = HostTree();
+  t.dim = dim;
+  t.n = n;
+  t.ind.resize(n);
+  for (int j = 0; j < n; ++j) t.ind[j] = j;
+  t.ni.reserve((size_t)kk::NODE_I * (2 * n / 6 + 8));
+  t.nd.reserve((size_t)kk::NODE_D * (2 * n / 6 + 8));
+  TreeBuilder b{t, data, dim};
+  t.root = b.build(0, n - 1, -1, 0);
+  if (t.root < 0) return false;
+  t.rdata.resize((size_t)dim * n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < dim; ++k) t.rdata[k + dim * (size_t)i] = data[k + dim * (size_t)t.ind[i]];
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  bool ensure(size_t nb) {
+    if (nb <= bytes && p) return true;
+    release();
+    if (hipMalloc(&p, std::max<size_t>(nb, 16)) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    bytes = std::max<size_t>(nb, 16);
+    return true;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// kriging object
+// ---------------------------------------------------------------------------
+struct Obs {
+  int n = 0;
+  std::vector<double> coord, value, var;
+  int nmax = INT_MAX;
+  double maxdist2 = HUGE_VAL;
+  double sk_mean = 0.0;
+  bool sector = false, search_set = false, need_search = false, aniso = false;
+  double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::vector<double> rcoord;  // rotated coordinates (aniso search)
+  HostTree tree;
+};
+
+struct Krige {
+  int ndim = 2, nvar = 1, ndrift = 0, unbias = 1, nsim = 0;
+  bool aniso_search = false, weight_correction = false, use_old_weight = false, store_weight = false;
+  bool cross_validation = false, write_mat = false, neglect_error = true, varying_vgm = false;
+  bool std_ck = false, verbose = false, pf_cache = false;
+  double bounds[2] = {-DBL_MAX, DBL_MAX};
+  int seed = 0;
+  bool initialized = false;
+  std::vector<Obs> obs;
+  std::vector<std::vector<VStruct>> vgm;  // nvar*nvar, (ivar-1)+nvar*(jvar-1)
+  // blocks
+  int nblock = 0;
+  std::vector<double> bcoord;             // (ndim, nblock) centres
+  std::vector<int> soff;                  // nblock+1 offsets into sub-points (empty: point kriging)
+  std::vector<double> scoord, sweight;
+  std::vector<double> rangescale, localnugget;
+  // options
+  int precision = kk::PREC_FP64;
+  double pivot_tol = 1.0e-3;
+  bool table_mode = true;
+  // results
+  bool solved = false;
+  std::vector<double> est, var;
+  std::vector<int> nnear, inear;          // weight-store view (0-based ids)
+  int mmax = 0;
+  int stats[5] = {0, 0, 0, 0, 0};
+  double timing[8] = {0};
+  std::string info;
+  // device residency
+  bool dev_inputs_ok = false;
+  DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
+  DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
+  DBuf d_model;
+  DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
+  Model model{};
+  std::vector<double> tab;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {nullptr};
+  ~Krige() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+std::vector<Krige*> g_reg;
+
+Krige* get_obj(int64_t h) {
+  if (h == 0) {
+    set_error("kriging_capi", "Null kriging object handle");
+    return nullptr;
+  }
+  if (h < 1 || h > (int64_t)g_reg.size() || !g_reg[h - 1]) {
+    set_error("kriging_capi", "Invalid kriging object handle");
+    return nullptr;
+  }
+  return g_reg[h - 1];
+}
+
+int ierr() { return g_ierr; }
+
+// normalise nmax like kriging_normalize_nmax (kriging_base.F90:1966-1971)
+int normalize_nmax(int nmax, int ntot) {
+  int v = std::min(nmax, ntot);
+  return v <= 0 ? ntot : v;
+}
+
+// ---------------------------------------------------------------------------
+// device pipeline
+// ---------------------------------------------------------------------------
+template <class T>
+bool upload(DBuf& b, const std::vector<T>& v, const char* where) {
+  if (!b.ensure(v.size() * sizeof(T))) {
+    set_error(where, "device allocation failed");
+    return false;
+  }
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), where);
+  return true;
+}
+
+bool ensure_stream(Krige& K) {
+  if (!K.stream) {
+    HIPCHK(hipStreamCreateWithFlags(&K.stream, hipStreamNonBlocking), "solve");
+    for (auto& e : K.ev) HIPCHK(hipEventCreate(&e), "solve");
+  }
+  return true;
+}
+
+// Uploads everything solve() reads.  Inputs stay resident until a setter
+// changes them.
+bool stage_inputs(Krige& K) {
+  const char* W = "t_kriging_base%solve_base";
+  Obs& O = K.obs[0];
+  const int nd = K.ndim;
+  if (!build_model(K.vgm[0], nd, K.table_mode, K.model, K.tab)) {
+    set_error(W, "invalid variogram model");
+    return false;
+  }
+  if (!upload(K.d_tab, K.tab, W)) return false;
+  K.model.tab = K.tab.empty() ? nullptr : K.d_tab.as<double>();
+  if (!K.d_model.ensure(sizeof(Model))) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  HIPCHK(hipMemcpy(K.d_model.p, &K.model, sizeof(Model), hipMemcpyHostToDevice), W);
+  if (K.dev_inputs_ok) return true;
+  if (!upload(K.d_ocoord, O.coord, W) || !upload(K.d_ovalue, O.value, W) || !upload(K.d_ovar, O.var, W))
+    return false;
+  if (!upload(K.d_socoord, O.aniso ? O.rcoord : O.coord, W)) return false;
+  if (O.need_search) {
+    if (!upload(K.d_rdata, O.tree.rdata, W) || !upload(K.d_ind, O.tree.ind, W) ||
+        !upload(K.d_ni, O.tree.ni, W) || !upload(K.d_nd, O.tree.nd, W))
+      return false;
+  }
+  // block centres, and their rotated copy for the search
+  const std::vector<double>& bc = K.cross_validation ? O.coord : K.bcoord;
+  if (!upload(K.d_bcoord, bc, W)) return false;
+  if (O.aniso) {
+    std::vector<double> q(bc.size());
+    kk::Rot3 R;
+    std::memcpy(R.m, O.rot, sizeof R.m);
+    for (int b = 0; b < K.nblock; ++b) {
+      if (nd == 1) R.apply<1>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
+      else if (nd == 2) R.apply<2>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
+      else R.apply<3>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
+    }
+    if (!upload(K.d_qcoord, q, W)) return false;
+  }
+  if (!upload(K.d_rs, K.rangescale, W) || !upload(K.d_ln, K.localnugget, W)) return false;
+  if (!K.soff.empty()) {
+    if (!upload(K.d_soff, K.soff, W) || !upload(K.d_scoord, K.scoord, W) || !upload(K.d_sweight, K.sweight, W))
+      return false;
+  }
+  K.dev_inputs_ok = true;
+  return true;
+}
+
+bool run_pipeline(Krige& K) {
+  const char* W = "t_kriging_base%solve_base";
+  if (!ensure_stream(K)) return false;
+  hipStream_t st = K.stream;
+  Obs& O = K.obs[0];
+  const int nd = K.ndim, nb = K.nblock;
+  int nmax = O.nmax;  // already normalised at set_search
+  const bool sector = O.sector && O.need_search;
+  const int mmax = sector ? (1 << nd) * nmax : nmax;
+  K.mmax = mmax;
+  if (mmax > 64) {
+    set_error(W, "neighbourhoods larger than 64 data rows are not supported by the GPU solver (mmax=" +
+                     std::to_string(mmax) + ")");
+    return false;
+  }
+  if (!K.d_nnear.ensure(sizeof(int) * nb) || !K.d_inear.ensure(sizeof(int) * (size_t)nb * mmax) ||
+      !K.d_est.ensure(sizeof(double) * nb) || !K.d_var.ensure(sizeof(double) * nb) ||
+      !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 16) ||
+      !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb + 64)) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  HIPCHK(hipEventRecord(K.ev[0], st), W);
+  // ---- search -----------------------------------------------------------
+  kk::SearchArgs sa{};
+  sa.ndim = nd;
+  sa.qcoord = O.aniso ? K.d_qcoord.as<double>() : K.d_bcoord.as<double>();
+  sa.ocoord = K.d_socoord.as<double>();
+  sa.nmax = nmax;
+  sa.cv = K.cross_validation;
+  sa.maxdist2 = O.maxdist2;
+  sa.mmax = mmax;
+  sa.nnear = K.d_nnear.as<int>();
+  sa.inear = K.d_inear.as<int>();
+  if (!O.need_search) {
+    kk::Rot3 R;
+    std::memcpy(R.m, O.rot, sizeof R.m);
+    sa.nq = nb;
+    sa.q0 = 0;
+    HIPCHK(kk::launch_search_all(sa, O.n, K.d_ocoord.as<double>(), R, st), W);
+  } else {
+    sa.tree.dim = nd;
+    sa.tree.n = O.n;
+    sa.tree.nnodes = O.tree.nnodes();
+    sa.tree.root = O.tree.root;
+    sa.tree.rdata = K.d_rdata.as<double>();
+    sa.tree.ind = K.d_ind.as<int>();
+    sa.tree.ni = K.d_ni.as<int>();
+    sa.tree.nd = K.d_nd.as<double>();
+    sa.sort = sector;
+    sa.k = sector ? std::min(O.n, (1 << nd) * nmax * 4) : nmax;
+    const int chunk = std::min(nb, 1 << 20);
+    if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
+        !K.d_hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
+      set_error(W, "device allocation failed (search scratch)");
+      return false;
+    }
+    sa.hdis = K.d_hdis.as<double>();
+    sa.hidx = K.d_hidx.as<int>();
+    sa.hstride = chunk;
+    for (int q0 = 0; q0 < nb; q0 += chunk) {
+      sa.q0 = q0;
+      sa.nq = std::min(chunk, nb - q0);
+      HIPCHK(kk::launch_search(sa, st), W);
+    }
+  }
+  HIPCHK(hipEventRecord(K.ev[1], st), W);
+  // ---- bucket by neighbour count -------------------------------------------
+  HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+  HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), nb, K.d_counts.as<int>(), K.d_lists.as<int>(), nb, st), W);
+  int counts[16];
+  HIPCHK(hipMemcpyAsync(counts, K.d_counts.p, sizeof(int) * 16, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  if (counts[kk::NBUCKET] > 0) {
+    set_error(W, "neighbourhood larger than 64 rows");
+    return false;
+  }
+  HIPCHK(hipEventRecord(K.ev[2], st), W);
+  // ---- per-node solve ---------------------------------------------------------
+  kk::SolveArgs va{};
+  va.model = K.d_model.as<Model>();
+  va.ndim = nd;
+  va.unbias = K.unbias;
+  va.sk_mean = O.sk_mean;
+  va.bound_lo = K.bounds[0];
+  va.bound_hi = K.bounds[1];
+  va.nsub = 1;
+  va.bcoord = K.d_bcoord.as<double>();
+  va.rangescale = K.rangescale.empty() ? nullptr : K.d_rs.as<double>();
+  va.localnugget = K.localnugget.empty() ? nullptr : K.d_ln.as<double>();
+  va.ocoord = K.d_ocoord.as<double>();
+  va.ovalue = K.d_ovalue.as<double>();
+  va.ovar = K.d_ovar.as<double>();
+  va.nnear = K.d_nnear.as<int>();
+  va.inear = K.d_inear.as<int>();
+  va.mmax = mmax;
+  va.pivot_tol = K.pivot_tol;
+  va.est = K.d_est.as<double>();
+  va.var = K.d_var.as<double>();
+  va.flag = K.d_flag.as<int>();
+  if (!K.soff.empty()) {
+    // uniform sub-point count per block (Gauss-Legendre blocks)
+    va.nsub = K.soff[1] - K.soff[0];
+    va.scoord = K.d_scoord.as<double>();
+    va.sweight = K.d_sweight.as<double>();
+    va.base_cov = K.d_basecov.as<double>();
+  }
+  const bool tb = K.model.mode == kk::COV_TABLE_B;
+  for (int bk = 0; bk < kk::NBUCKET; ++bk) {
+    va.list = K.d_lists.as<int>() + (size_t)bk * nb;
+    va.nlist = counts[bk];
+    if (bk == 0) HIPCHK(kk::launch_empty(va, st), W);
+    else HIPCHK(kk::launch_solve(va, kk::bucket_width(bk), K.precision, tb, st), W);
+  }
+  HIPCHK(hipEventRecord(K.ev[3], st), W);
+  // ---- FP64 re-solve of flagged FP32 systems, LU fallback ----------------
+  int nref = 0, nfb = 0;
+  if (K.precision == kk::PREC_FP32R) {
+    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 1, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
+    HIPCHK(hipMemcpyAsync(&nref, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    if (nref > 0) {
+      va.list = K.d_lists.as<int>();
+      va.nlist = nref;
+      HIPCHK(kk::launch_solve(va, mmax <= 8 ? 8 : mmax <= 16 ? 16 : mmax <= 32 ? 32 : 64, kk::PREC_FP64, tb, st), W);
+    }
+  }
+  HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+  HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 2, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
+  HIPCHK(hipMemcpyAsync(&nfb, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  if (nfb > 0) {
+    const int fchunk = 4096;
+    const size_t ld = (size_t)mmax + 1;
+    if (!K.d_fb.ensure(sizeof(double) * fchunk * (2 * ld * ld + 3 * ld))) {
+      set_error(W, "device allocation failed (fallback scratch)");
+      return false;
+    }
+    for (int f0 = 0; f0 < nfb; f0 += fchunk) {
+      va.list = K.d_lists.as<int>() + f0;
+      va.nlist = std::min(fchunk, nfb - f0);
+      HIPCHK(kk::launch_fallback(va, K.d_fb.as<double>(), st), W);
+    }
+  }
+  HIPCHK(hipEventRecord(K.ev[4], st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  float ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[1]), W);
+  K.timing[0] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[1], K.ev[2]), W);
+  K.timing[1] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[2], K.ev[3]), W);
+  K.timing[2] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[3], K.ev[4]), W);
+  K.timing[3] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
+  K.timing[4] = ms;
+  K.stats[0] = 0;
+  K.stats[1] = nb - counts[0] - nfb;
+  K.stats[2] = 0;
+  K.stats[3] = nfb;
+  K.stats[4] = 0;
+  K.timing[5] = nref;
+  K.timing[6] = nfb;
+  return true;
+}
+
+bool fetch_results(Krige& K) {
+  const char* W = "t_kriging_base%solve_base";
+  const int nb = K.nblock;
+  K.est.resize(nb);
+  K.var.resize(nb);
+  HIPCHK(hipMemcpy(K.est.data(), K.d_est.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
+  HIPCHK(hipMemcpy(K.var.data(), K.d_var.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
+  // LU fallback failures: NaN results, counted in n_fail (kriging_base.F90:3369-3383)
+  if (K.stats[3] > 0) {
+    std::vector<int> flag(nb);
+    HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+    int nf = 0;
+    long first = -1;
+    for (int b = 0; b < nb; ++b)
+      if (flag[b] == 3) {
+        ++nf;
+        if (first < 0) first = b + 1;
+      }
+    K.stats[0] = nf;
+    K.stats[3] -= nf;
+    if (nf > 0 && !K.neglect_error) {
+      set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
+      return false;
+    }
+  }
+  if (K.store_weight) {
+    K.nnear.resize(nb);
+    K.inear.resize((size_t)nb * K.mmax);
+    HIPCHK(hipMemcpy(K.nnear.data(), K.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+    HIPCHK(hipMemcpy(K.inear.data(), K.d_inear.p, sizeof(int) * (size_t)nb * K.mmax, hipMemcpyDeviceToHost), W);
+  }
+  // blocks without neighbours are an error unless neglect_error
+  // (kriging_base.F90:3198-3207)
+  if (!K.neglect_error) {
+    std::vector<int> nn(nb);
+    HIPCHK(hipMemcpy(nn.data(), K.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+    for (int b = 0; b < nb; ++b)
+      if (nn[b] == 0) {
+        set_error("t_kriging_base%assemble_linear_system", "not enough neighbors for kriging at block", b + 1);
+        return false;
+      }
+  }
+  return true;
+}
+
+bool validate_for_solve(Krige& K) {
+  const char* W = "t_kriging%prepare";
+  if (!K.initialized) {
+    set_error(W, "Call initialize() before solve.");
+    return false;
+  }
+  if (K.nvar != 1 || K.nsim != 0 || K.ndrift != 0) {
+    set_error(W, "this backend solves single-variable kriging (nvar=1, nsim=0, ndrift=0)");
+    return false;
+  }
+  if (K.use_old_weight || K.varying_vgm || K.weight_correction) {
+    set_error(W, "use_old_weight, varying_vgm and weight_correction are not supported by this backend");
+    return false;
+  }
+  if (K.obs[0].n == 0) {
+    set_error(W, "set_obs() needs to be called before solve().");
+    return false;
+  }
+  if (K.vgm[0].empty()) {
+    set_error("t_kriging_sva%validate_vgm",
+              "t_kriging_sva: variogram not set for block 1, ivar=1, jvar=1. Call set_vgm_block() or "
+              "set_vgm_block_all().");
+    return false;
+  }
+  for (const auto& s : K.vgm[0]) {
+    if (s.nugget < 0.0 || s.sill < 0.0 || s.a_major <= 0.0 || s.a_minor1 <= 0.0 || s.a_minor2 <= 0.0 ||
+        (s.vtype == kk::HOL && K.ndim == 3)) {
+      set_error("t_kriging_sva%validate_vgm",
+                "t_kriging_sva: variogram is not valid for block 1, ivar=1, jvar=1. Check your variogram "
+                "parameters.");
+      return false;
+    }
+  }
+  if (K.nblock == 0 && !K.cross_validation) {
+    set_error(W, "set_grid() needs to be called before solve().");
+    return false;
+  }
+  if (!K.obs[0].search_set) {
+    set_error(W, "set_search() needs to be called before solve().");
+    return false;
+  }
+  return true;
+}
+
+bool compute_base_cov(Krige& K) {
+  // prior covariance of each block over its integration points
+  // (kriging.F90:992-1008), evaluated once per block on the host
+  const char* W = "t_kriging%calc_variance";
+  if (K.soff.empty()) return true;
+  std::vector<double> base(K.nblock);
+  const int nd = K.ndim;
+  // identical relative layouts share one value: hoisted when the block
+  // geometry repeats (uniform Gauss-Legendre blocks)
+  std::vector<double> tabcopy = K.tab;
+  Model M = K.model;
+  M.tab = tabcopy.empty() ? nullptr : tabcopy.data();
+  auto cov_host = [&](const double* lag) -> double {
+    // host evaluation through the same model description
+    auto aniso = [&](const double* m) {
+      double rx, ry, rz = 0.0;
+      if (nd == 1) return std::fabs(lag[0] * m[0]);
+      if (nd == 2) {
+        rx = m[0] * lag[0] + m[3] * lag[1];
+        ry = m[1] * lag[0] + m[4] * lag[1];
+        return std::sqrt(rx * rx + ry * ry);
+      }
+      rx = m[0] * lag[0] + m[3] * lag[1] + m[6] * lag[2];
+      ry = m[1] * lag[0] + m[4] * lag[1] + m[7] * lag[2];
+      rz = m[2] * lag[0] + m[5] * lag[1] + m[8] * lag[2];
+      return std::sqrt(rx * rx + ry * ry + rz * rz);
+    };
+    auto comp_h = [&](int c, double h) {
+      return (h > DBL_MIN * 1.0e3) ? M.sill[c] * corefunc_host(M.vtype[c], h) : M.sill[c] + M.nugget[c];
+    };
+    auto interp = [&](const double* tab, double h) {
+      int k = (h >= M.zb[1]) + (h >= M.zb[2]);
+      double fi = (h - M.zb[k]) * M.zinv[k];
+      int ifi = (int)fi;
+      int i = M.zoff[k] + std::min(ifi, M.zn[k] - 1);
+      fi = fi - (double)ifi;
+      return tab[i] + fi * (tab[i + 1] - tab[i]);
+    };
+    if (M.mode == kk::COV_TABLE_B) {
+      double h = aniso(M.bmat);
+      if (h >= M.zb[3]) {
+        if (!M.b_tail) return 0.0;
+        double res = 0.0;
+        int iv = 0;
+        while (iv < M.nstruct) {
+          double g = comp_h(iv, h);
+          while (iv + 1 < M.nstruct && M.is_product[iv + 1]) {
+            ++iv;
+            g = g * comp_h(iv, h);
+          }
+          res = res + g;
+          ++iv;
+        }
+        return res;
+      }
+      if (h <= 0.0) return M.tab[0];
+      return interp(M.tab, h);
+    }
+    double res = 0.0;
+    int iv = 0;
+    while (iv < M.nstruct) {
+      double g = 0.0;
+      bool first = true;
+      do {
+        if (!first) ++iv;
+        double h = aniso(M.mat[iv]), c;
+        if (M.mode == kk::COV_TABLE_A) {
+          const double* tab = M.tab + (size_t)iv * (M.ntab + 1);
+          if (h >= M.zb[3]) c = has_tail_host(M.vtype[iv]) ? comp_h(iv, h) : 0.0;
+          else if (h <= 0.0) c = tab[0];
+          else c = interp(tab, h);
+        } else {
+          c = comp_h(iv, h);
+        }
+        g = first ? c : g * c;
+        first = false;
+      } while (iv + 1 < M.nstruct && M.is_product[iv + 1]);
+      res = res + g;
+      ++iv;
+    }
+    return res;
+  };
+  std::vector<double> prev_rel;
+  double prev_val = 0.0;
+  for (int b = 0; b < K.nblock; ++b) {
+    const int s0 = K.soff[b], s1 = K.soff[b + 1], ns = s1 - s0;
+    std::vector<double> rel((size_t)ns * (nd + 1));
+    for (int i = 0; i < ns; ++i) {
+      for (int k = 0; k < nd; ++k)
+        rel[(size_t)i * (nd + 1) + k] = K.scoord[(size_t)nd * (s0 + i) + k] - K.scoord[(size_t)nd * s0 + k];
+      rel[(size_t)i * (nd + 1) + nd] = K.sweight[s0 + i];
+    }
+    if (b > 0 && rel == prev_rel) {
+      base[b] = prev_val;
+      continue;
+    }
+    double v = 0.0;
+    for (int i = 0; i < ns; ++i) {
+      const double wi = K.sweight[s0 + i];
+      v = v + K.model.cov0 * wi * wi;
+      for (int j = i + 1; j < ns; ++j) {
+        double lag[3] = {0, 0, 0};
+        for (int k = 0; k < nd; ++k) lag[k] = K.scoord[(size_t)nd * (s0 + i) + k] - K.scoord[(size_t)nd * (s0 + j) + k];
+        v = v + cov_host(lag) * wi * K.sweight[s0 + j] * 2.0;
+      }
+    }
+    base[b] = v;
+    prev_rel.swap(rel);
+    prev_val = v;
+  }
+  return upload(K.d_basecov, base, W);
+}
+
+bool do_solve(Krige& K, bool fetch) {
+  K.solved = false;
+  if (!validate_for_solve(K)) return false;
+  if (!stage_inputs(K)) return false;
+  if (!compute_base_cov(K)) return false;
+  if (!run_pipeline(K)) return false;
+  if (fetch && !fetch_results(K)) return false;
+  K.solved = true;
+  return true;
+}
+
+Krige* checked(int64_t h) { return get_obj(h); }
+
+bool check_solved(Krige* K, const char* caller) {
+  if (!K->solved) {
+    set_error(caller, "solve() has not been called; call solve() before retrieving results");
+    return false;
+  }
+  return true;
+}
+
+int unsupported(const char* name, const char* what) {
+  set_error(name, what);
+  return ierr();
+}
+
+}  // namespace
+
+// ===========================================================================
+// exported C ABI (everything else in the library has hidden visibility)
+// ===========================================================================
+#pragma GCC visibility push(default)
+extern "C" {
+
+int krige_create(int64_t* handle) {
+  clear_error();
+  Krige* K = new Krige();
+  for (size_t i = 0; i < g_reg.size(); ++i)
+    if (!g_reg[i]) {
+      g_reg[i] = K;
+      *handle = (int64_t)i + 1;
+      return 0;
+    }
+  g_reg.push_back(K);
+  *handle = (int64_t)g_reg.size();
+  return 0;
+}
+
+int krige_ind_create(int64_t* handle) {
+  clear_error();
+  *handle = 0;
+  return unsupported("krige_ind_create", "indicator kriging is not provided by this backend");
+}
+
+int krige_ind_set_ncat(int64_t, int) {
+  clear_error();
+  return unsupported("krige_ind_set_ncat", "indicator kriging is not provided by this backend");
+}
+
+int krige_destroy(int64_t* handle) {
+  clear_error();
+  if (!handle) return 0;
+  const int64_t h = *handle;
+  if (h >= 1 && h <= (int64_t)g_reg.size() && g_reg[h - 1]) {
+    delete g_reg[h - 1];
+    g_reg[h - 1] = nullptr;
+  }
+  *handle = 0;
+  return 0;
+}
+
+int krige_initialize(int64_t h, int ndim, int nvar, int ndrift, int unbias, int nsim, int anisotropic_search,
+                     int weight_correction, int use_old_weight, int store_weight, int cross_validation,
+                     int write_mat, int neglect_error, int varying_vgm, int std_ck, int verbose, int pf_cache,
+                     const char* weight_file, const double* bounds, int seed) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  (void)weight_file;
+  const char* W = "t_kriging_base%initialize";
+  if (store_weight && use_old_weight) {
+    set_error(W, "store_weight and use_old_weight are mutually exclusive");
+    return ierr();
+  }
+  if (pf_cache && use_old_weight) {
+    set_error(W, "pf_cache and use_old_weight are mutually exclusive");
+    return ierr();
+  }
+  if (cross_validation && nsim > 0) {
+    set_error(W, "cross_validation and nsim>0 are mutually exclusive");
+    return ierr();
+  }
+  if (ndim < 1 || ndim > 3 || nvar < 1) {
+    set_error(W, "ndim must be 1, 2 or 3 and nvar >= 1");
+    return ierr();
+  }
+  const int prec = K->precision;
+  const double ptol = K->pivot_tol;
+  const bool tmode = K->table_mode;
+  hipStream_t st = K->stream;
+  hipEvent_t ev[8];
+  std::memcpy(ev, K->ev, sizeof ev);
+  K->stream = nullptr;
+  std::memset(K->ev, 0, sizeof K->ev);
+  Krige* fresh = new Krige();
+  delete K;
+  K = fresh;
+  g_reg[h - 1] = K;
+  K->stream = st;
+  std::memcpy(K->ev, ev, sizeof ev);
+  K->precision = prec;
+  K->pivot_tol = ptol;
+  K->table_mode = tmode;
+  K->ndim = ndim;
+  K->nvar = nvar;
+  K->ndrift = ndrift;
+  K->unbias = unbias;
+  K->nsim = nsim;
+  K->aniso_search = anisotropic_search;
+  K->weight_correction = weight_correction;
+  K->use_old_weight = use_old_weight;
+  K->store_weight = store_weight;
+  K->cross_validation = cross_validation;
+  K->write_mat = write_mat;
+  K->neglect_error = neglect_error;
+  K->varying_vgm = varying_vgm;
+  K->std_ck = std_ck;
+  K->verbose = verbose;
+  K->pf_cache = pf_cache;
+  if (bounds) {
+    K->bounds[0] = bounds[0];
+    K->bounds[1] = bounds[1];
+  }
+  K->seed = seed;
+  K->obs.assign(nvar, Obs());
+  K->vgm.assign((size_t)nvar * nvar, {});
+  K->initialized = true;
+  return 0;
+}
+
+int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord, const double* value,
+                  const double* variance, int nmax, double maxdist, double sk_mean) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_obs";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_obs.");
+    return ierr();
+  }
+  if (ivar < 1 || ivar > K->nvar) {
+    set_error(W, "ivar out of range");
+    return ierr();
+  }
+  if (ndim_c != K->ndim) {
+    set_error(W, "nlag /= size(coord, 1) for obs");
+    return ierr();
+  }
+  if (nobs < 1) {
+    set_error(W, "coord column count != nobs");
+    return ierr();
+  }
+  // duplicate coordinate tuples are rejected (utils.F90:43-133)
+  std::vector<int> perm(nobs);
+  for (int i = 0; i < nobs; ++i) perm[i] = i;
+  std::sort(perm.begin(), perm.end(), [&](int a, int b) {
+    for (int k = 0; k < ndim_c; ++k) {
+      double x = coord[k + (size_t)ndim_c * a], y = coord[k + (size_t)ndim_c * b];
+      if (x != y) return x < y;
+    }
+    return a < b;
+  });
+  for (int i = 0; i + 1 < nobs; ++i) {
+    bool same = true;
+    for (int k = 0; k < ndim_c && same; ++k)
+      same = coord[k + (size_t)ndim_c * perm[i]] == coord[k + (size_t)ndim_c * perm[i + 1]];
+    if (same) {
+      set_error(W, "ERROR: Duplicate coordinate found! Station " + std::to_string(perm[i] + 1) + " and Station " +
+                       std::to_string(perm[i + 1] + 1) + " share identical coordinates.");
+      return ierr();
+    }
+  }
+  Obs& O = K->obs[ivar - 1];
+  O = Obs();
+  O.n = nobs;
+  O.coord.assign(coord, coord + (size_t)ndim_c * nobs);
+  O.value.assign(value, value + nobs);
+  O.var.assign(nobs, 0.0);
+  if (variance) O.var.assign(variance, variance + nobs);
+  O.nmax = nmax;
+  O.maxdist2 = maxdist * maxdist;
+  O.sk_mean = sk_mean;
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  return 0;
+}
+
+int krige_set_obs_drift(int64_t h, int, int, int, const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_obs_drift", "external drift is not provided by this backend");
+}
+
+int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (ivar < 1 || ivar > K->nvar || K->obs[ivar - 1].n != nobs) {
+    set_error("krige_update_obs_value", "nobs does not match the previous set_obs call");
+    return ierr();
+  }
+  K->obs[ivar - 1].value.assign(value, value + nobs);
+  K->dev_inputs_ok = false;
+  return 0;
+}
+
+int krige_set_nscore(int64_t h, int, double, double, int, int, double, double, int, const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_nscore", "normal-score transform (SGSIM) is not provided by this backend");
+}
+
+int krige_reset_vgm(int64_t h, int ivar, int jvar) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (ivar < 1 || jvar < 1 || ivar > K->nvar || jvar > K->nvar) return 0;
+  K->vgm[(ivar - 1) + (size_t)K->nvar * (jvar - 1)].clear();
+  K->vgm[(jvar - 1) + (size_t)K->nvar * (ivar - 1)].clear();
+  return 0;
+}
+
+int krige_set_vgm(int64_t h, int ivar, int jvar, const char* vtype, double nugget, double sill, double a_major,
+                  double a_minor1, double a_minor2, double azimuth, double dip, double plunge, int is_product) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging%set_vgm";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_vgm.");
+    return ierr();
+  }
+  if (ivar < 1 || jvar < 1 || ivar > K->nvar || jvar > K->nvar) {
+    set_error(W, "variable index out of range");
+    return ierr();
+  }
+  if (jvar < ivar) {
+    set_error(W, "jvar must be >= ivar to set the upper triangle of the variogram matrix");
+    return ierr();
+  }
+  const int id = vtype_from_str(vtype ? vtype : "sph");
+  if (id < 0) {
+    set_error("vtype_from_str", std::string("unknown vtype: ") + (vtype ? vtype : ""));
+    return ierr();
+  }
+  auto& v = K->vgm[(ivar - 1) + (size_t)K->nvar * (jvar - 1)];
+  if (is_product && v.empty()) {
+    set_error("vgm_struct%add",
+              "first structure cannot be a product member (product=.true. requires a preceding structure)");
+    return ierr();
+  }
+  if ((int)v.size() >= kk::MAXS) {
+    set_error("vgm_struct%add", "exceeded maxvgm nested structures");
+    return ierr();
+  }
+  VStruct s{id, is_product ? 1 : 0, nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge};
+  v.push_back(s);
+  if (ivar != jvar) K->vgm[(jvar - 1) + (size_t)K->nvar * (ivar - 1)].push_back(s);
+  K->solved = false;
+  return 0;
+}
+
+int krige_set_vgm_block(int64_t h, int, int, int, const char*, double, double, double, double, double, double,
+                        double, double, int) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_vgm_block", "spatially varying variograms are not provided by this backend");
+}
+
+static int set_grid_common(Krige* K, int nblock, const double* rangescale, const double* localnugget) {
+  K->nblock = nblock;
+  K->rangescale.clear();
+  K->localnugget.clear();
+  bool rs_unit = true, ln_zero = true;
+  for (int b = 0; rangescale && b < nblock; ++b) rs_unit &= rangescale[b] == 1.0;
+  for (int b = 0; localnugget && b < nblock; ++b) ln_zero &= localnugget[b] == 0.0;
+  if (rangescale && !rs_unit) K->rangescale.assign(rangescale, rangescale + nblock);
+  if (localnugget && !ln_zero) K->localnugget.assign(localnugget, localnugget + nblock);
+  K->cross_validation = false;
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  return 0;
+}
+
+int krige_set_grid(int64_t h, int ngrid, int ndim_c, const double* coord, const double* rangescale,
+                   const double* localnugget) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_grid_point";
+  if (!K->initialized || K->obs.empty() || K->obs[0].n == 0) {
+    set_error(W, "set_obs() needs to be called before set_grid().");
+    return ierr();
+  }
+  if (ndim_c != K->ndim) {
+    set_error(W, "nlag /= size(coord, 1) for self%grid");
+    return ierr();
+  }
+  K->bcoord.assign(coord, coord + (size_t)ndim_c * ngrid);
+  K->soff.clear();
+  K->scoord.clear();
+  K->sweight.clear();
+  return set_grid_common(K, ngrid, rangescale, localnugget);
+}
+
+int krige_set_grid_block(int64_t h, int block_type, int ngrid, int ndim_c, const double* coord, int nblock,
+                         const int* nblockpnt, const double* pointweight, const double* blocksize,
+                         const double* rangescale, const double* localnugget) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_grid_block";
+  if (!K->initialized || K->obs.empty() || K->obs[0].n == 0) {
+    set_error(W, "set_obs() needs to be called before set_grid().");
+    return ierr();
+  }
+  if (ndim_c != K->ndim) {
+    set_error(W, "nlag /= size(coord, 1) for self%grid");
+    return ierr();
+  }
+  const int nd = ndim_c;
+  if (block_type == -4) {
+    // Gauss-Legendre 4^ndim sub-points (gaussian_quadrature.f90:3-41; the
+    // decimal-truncated constants are the reference's)
+    const double frac[4] = {0.4305681558, 0.1699905218, -0.1699905218, -0.4305681558};
+    const double wgq4[4] = {0.1739274226, 0.3260725774, 0.3260725774, 0.1739274226};
+    const int ngq = nd == 1 ? 4 : nd == 2 ? 16 : 64;
+    K->bcoord.assign(coord, coord + (size_t)nd * ngrid);
+    K->soff.resize(ngrid + 1);
+    K->scoord.resize((size_t)nd * ngrid * ngq);
+    K->sweight.resize((size_t)ngrid * ngq);
+    for (int b = 0; b < ngrid; ++b) {
+      K->soff[b] = b * ngq;
+      for (int g = 0; g < ngq; ++g) {
+        const int ii[3] = {g % 4, (g / 4) % 4, g / 16};
+        double w = 1.0;
+        for (int k = 0; k < nd; ++k) {
+          K->scoord[(size_t)nd * ((size_t)b * ngq + g) + k] =
+              coord[k + (size_t)nd * b] + frac[ii[k]] * blocksize[k + (size_t)nd * b];
+          w = w * wgq4[ii[k]];
+        }
+        K->sweight[(size_t)b * ngq + g] = w;
+      }
+    }
+    K->soff[ngrid] = ngrid * ngq;
+    return set_grid_common(K, ngrid, rangescale, localnugget);
+  }
+  // user-supplied sub-points: centroid = weighted mean (kriging_base.F90:1500-1563)
+  int tot = 0;
+  for (int b = 0; b < nblock; ++b) {
+    if (nblockpnt[b] <= 0) {
+      set_error("set_grid_centroid_common", "nblockpnt entries must be positive");
+      return ierr();
+    }
+    tot += nblockpnt[b];
+  }
+  if (tot != ngrid) {
+    set_error("set_grid_centroid_common", "gridcoord columns != sum(nblockpnt)");
+    return ierr();
+  }
+  for (int b = 1; b < nblock; ++b)
+    if (nblockpnt[b] != nblockpnt[0]) {
+      set_error(W, "blocks with different sub-point counts are not supported by this backend");
+      return ierr();
+    }
+  K->soff.resize(nblock + 1);
+  K->scoord.assign(coord, coord + (size_t)nd * ngrid);
+  K->sweight.resize(ngrid);
+  K->bcoord.resize((size_t)nd * nblock);
+  int off = 0;
+  for (int b = 0; b < nblock; ++b) {
+    const int nn = nblockpnt[b];
+    K->soff[b] = off;
+    double denom = 0.0;
+    for (int i = 0; i < nn; ++i) {
+      K->sweight[off + i] = pointweight ? pointweight[off + i] : 1.0 / nn;
+      denom = denom + K->sweight[off + i];
+    }
+    if (!(denom > 0.0)) {
+      set_error("set_grid_centroid_common", "sum(pointweight) must be positive for every block");
+      return ierr();
+    }
+    for (int k = 0; k < nd; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < nn; ++i) s = s + coord[k + (size_t)nd * (off + i)] * K->sweight[off + i];
+      K->bcoord[k + (size_t)nd * b] = s / denom;
+    }
+    off += nn;
+  }
+  K->soff[nblock] = off;
+  if (nblockpnt[0] == 1) {
+    K->soff.clear();
+    K->scoord.clear();
+    K->sweight.clear();
+  }
+  return set_grid_common(K, nblock, rangescale, localnugget);
+}
+
+int krige_set_grid_cv(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_grid_cv";
+  if (!K->initialized || K->obs.empty() || K->obs[0].n == 0) {
+    set_error(W, "set_obs() needs to be called before set_grid_cv().");
+    return ierr();
+  }
+  set_grid_common(K, K->obs[0].n, nullptr, nullptr);
+  K->cross_validation = true;
+  K->bcoord.clear();
+  K->soff.clear();
+  // kriging_base.F90:1167: one extra neighbour so self can be excluded
+  Obs& O = K->obs[0];
+  if (O.nmax > 0) O.nmax = (int)((unsigned)O.nmax + 1u);
+  return 0;
+}
+
+int krige_set_grid_drift(int64_t h, int, int, int, const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_grid_drift", "external drift is not provided by this backend");
+}
+
+int krige_set_sim(int64_t h, int, const void*, int, int, const void*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_sim", "sequential simulation is not provided by this backend");
+}
+
+int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azimuth, double dip, double plunge,
+                     int sector_search) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging%set_search";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_search.");
+    return ierr();
+  }
+  if (ivar < 1 || ivar > K->nvar) {
+    set_error(W, "ivar out of range");
+    return ierr();
+  }
+  Obs& O = K->obs[ivar - 1];
+  if (O.n == 0) {
+    set_error(W, "set_obs() needs to be called before set_search().");
+    return ierr();
+  }
+  if (sector_search && K->ndim < 2) {
+    set_error(W, "sector search needs ndim >= 2");
+    return ierr();
+  }
+  O.sector = sector_search != 0;
+  O.aniso = (std::fabs(anis1 - 1.0) > 1e-10 || std::fabs(anis2 - 1.0) > 1e-10) && K->aniso_search;
+  O.nmax = normalize_nmax(O.nmax, O.n);
+  O.need_search = O.n > O.nmax;
+  O.tree = HostTree();
+  O.rcoord.clear();
+  std::fill(O.rot, O.rot + 9, 0.0);
+  O.rot[0] = O.rot[4] = O.rot[8] = 1.0;
+  if (O.aniso) {
+    calc_rotmat(azimuth, dip, plunge, anis1, anis2, O.rot);
+    O.rcoord.resize(O.coord.size());
+    kk::Rot3 R;
+    std::memcpy(R.m, O.rot, sizeof R.m);
+    for (int i = 0; i < O.n; ++i) {
+      if (K->ndim == 1) R.apply<1>(&O.coord[i], &O.rcoord[i]);
+      else if (K->ndim == 2) R.apply<2>(&O.coord[(size_t)2 * i], &O.rcoord[(size_t)2 * i]);
+      else R.apply<3>(&O.coord[(size_t)3 * i], &O.rcoord[(size_t)3 * i]);
+    }
+  }
+  if (O.need_search) {
+    if (!build_tree(O.tree, K->ndim, O.n, O.aniso ? O.rcoord.data() : O.coord.data())) {
+      set_error("kdtree2_create", "kd-tree construction did not terminate (degenerate coordinates)");
+      return ierr();
+    }
+  }
+  O.search_set = true;
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  return 0;
+}
+
+int krige_set_grad(int64_t h, int, int, int, const double*, const double*, const double*, const double*, int,
+                   const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_grad", "gradient constraints are not provided by this backend");
+}
+
+int krige_prepare(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  validate_for_solve(*K);
+  return ierr();
+}
+
+int krige_solve(int64_t h, int nthread, int ncache) {
+  clear_error();
+  (void)nthread;
+  (void)ncache;
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  do_solve(*K, true);
+  return ierr();
+}
+
+int krige_get_nblocks(int64_t h, int* n) {
+  clear_error();
+  *n = 0;
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  *n = K->nblock;
+  return 0;
+}
+
+int krige_get_nsim(int64_t h, int* n) {
+  clear_error();
+  *n = 0;
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  *n = K->nsim;
+  return 0;
+}
+
+int krige_get_block_coord(int64_t h, int ndim_c, int nblocks, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const std::vector<double>& bc = K->cross_validation ? K->obs[0].coord : K->bcoord;
+  for (int b = 0; b < nblocks && b < K->nblock; ++b)
+    for (int k = 0; k < ndim_c; ++k) out[k + (size_t)ndim_c * b] = bc[k + (size_t)K->ndim * b];
+  return 0;
+}
+
+int krige_get_estimate(int64_t h, int nsim_c, int nblocks, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!check_solved(K, "krige_get_estimate")) return ierr();
+  for (int s = 0; s < nsim_c; ++s)
+    for (int b = 0; b < nblocks; ++b) out[b + (size_t)nblocks * s] = K->est[b];
+  return 0;
+}
+
+int krige_get_estimate_all(int64_t h, int nblocks, int nvar_c, int nsim_c, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!check_solved(K, "krige_get_estimate_all")) return ierr();
+  for (int s = 0; s < nsim_c; ++s)
+    for (int v = 0; v < nvar_c; ++v)
+      for (int b = 0; b < nblocks; ++b)
+        out[b + (size_t)nblocks * (v + (size_t)nvar_c * s)] = K->est[b];
+  return 0;
+}
+
+int krige_get_variance(int64_t h, int nblocks, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!check_solved(K, "krige_get_variance")) return ierr();
+  for (int b = 0; b < nblocks; ++b) out[b] = K->var[b];
+  return 0;
+}
+
+int krige_get_variance_all(int64_t h, int nblocks, int nvar_c, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!check_solved(K, "krige_get_variance_all")) return ierr();
+  for (int i = 0; i < nvar_c; ++i)
+    for (int j = 0; j < nvar_c; ++j)
+      for (int b = 0; b < nblocks; ++b)
+        out[b + (size_t)nblocks * (i + (size_t)nvar_c * j)] = K->var[b];
+  return 0;
+}
+
+int krige_get_factor_info(int64_t h, int*, int*, int*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_get_factor_info", "persistent factor cache is not provided by this backend");
+}
+
+int krige_get_factor_matrices(int64_t h, int, int, double*, double*, double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_get_factor_matrices", "persistent factor cache is not provided by this backend");
+}
+
+int krige_get_factor_system(int64_t h, int, int, int, double*, double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_get_factor_system", "persistent factor cache is not provided by this backend");
+}
+
+int krige_get_last_error(char* buffer, int nbuf) {
+  if (nbuf <= 0) return g_ierr;
+  const int n = std::min((int)g_msg.size(), nbuf - 1);
+  std::memcpy(buffer, g_msg.data(), n);
+  std::memset(buffer + n, 0, nbuf - n);
+  return g_ierr;
+}
+
+void krige_solver_stats(int64_t h, int* out) {
+  if (h < 1 || h > (int64_t)g_reg.size() || !g_reg[h - 1]) {
+    for (int i = 0; i < 5; ++i) out[i] = 0;
+    return;
+  }
+  for (int i = 0; i < 5; ++i) out[i] = g_reg[h - 1]->stats[i];
+}
+
+intptr_t krige_to_str(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return 0;
+  char buf[512];
+  std::snprintf(buf, sizeof buf,
+                " MI355X kriging backend\n ndim=%d nvar=%d unbias=%d nblock=%d nobs=%d precision=%s table=%d\n",
+                K->ndim, K->nvar, K->unbias, K->nblock, K->obs.empty() ? 0 : K->obs[0].n,
+                K->precision == kk::PREC_FP64 ? "fp64" : "fp32+refine", (int)K->table_mode);
+  K->info = buf;
+  return (intptr_t)K->info.c_str();
+}
+
+void krige_get_max_threads(int* n) { *n = 1; }
+void krige_get_num_threads(int* n) { *n = 1; }
+
+// ---- weight-store views (neighbour lists of the last solve) ------------
+int krige_free_weight_store(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  K->nnear.clear();
+  K->inear.clear();
+  return 0;
+}
+
+int krige_get_weight_dims(int64_t h, int* nm, int* ng, int* nb) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (K->nnear.empty()) {
+    set_error("krige_get_weight_dims", "Weight store not allocated");
+    return ierr();
+  }
+  *nm = K->mmax;
+  *ng = K->nvar;
+  *nb = K->nblock;
+  return 0;
+}
+
+int krige_get_weight_nnear(int64_t h, int ngroups_c, int nblock_c, int* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (K->nnear.empty()) {
+    set_error("krige_get_weight_nnear", "Weight store not allocated");
+    return ierr();
+  }
+  for (int b = 0; b < nblock_c; ++b)
+    for (int g = 0; g < ngroups_c; ++g) out[g + (size_t)ngroups_c * b] = g == 0 ? K->nnear[b] : 0;
+  return 0;
+}
+
+int krige_get_weight_inear(int64_t h, int nmax_c, int ngroups_c, int nblock_c, int* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (K->nnear.empty()) {
+    set_error("krige_get_weight_inear", "Weight store not allocated");
+    return ierr();
+  }
+  for (int b = 0; b < nblock_c; ++b)
+    for (int g = 0; g < ngroups_c; ++g)
+      for (int i = 0; i < nmax_c; ++i) {
+        int v = 0;
+        if (g == 0 && i < K->nnear[b]) v = K->inear[(size_t)K->mmax * b + i] + 1;
+        out[i + (size_t)nmax_c * (g + (size_t)ngroups_c * b)] = v;
+      }
+  return 0;
+}
+
+int krige_get_weight_data(int64_t h, int, int, int, int, double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_get_weight_data", "kriging weights are not stored by this backend");
+}
+
+int krige_get_weight_var(int64_t h, int, int, double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_get_weight_var", "kriging weights are not stored by this backend");
+}
+
+int krige_set_weights(int64_t h, int, int, int, int, const int*, const int*, const double*, const int*,
+                      const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_set_weights", "weight replay is not provided by this backend");
+}
+
+// ---- extensions of this backend (no counterpart in the reference) -------
+
+// precision: 0 = FP64 solve, 1 = FP32 factorisation + FP64 refinement.
+// pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64.
+// table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic.
+int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (precision != kk::PREC_FP64 && precision != kk::PREC_FP32R) {
+    set_error("kk_set_options", "precision must be 0 (fp64) or 1 (fp32 + refinement)");
+    return ierr();
+  }
+  K->precision = precision;
+  K->pivot_tol = pivot_tol;
+  K->table_mode = table_mode != 0;
+  K->solved = false;
+  return 0;
+}
+
+// Runs the device pipeline only: inputs resident in HBM, results left there.
+int kk_solve_resident(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  do_solve(*K, false);
+  return ierr();
+}
+
+// Copies the results of the last kk_solve_resident back to the host.
+int kk_fetch(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!fetch_results(*K)) K->solved = false;
+  return ierr();
+}
+
+// out[0..7]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
+// FP64 re-solved nodes, LU-fallback nodes, reserved
+int kk_get_timing(int64_t h, double* out) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  for (int i = 0; i < 8; ++i) out[i] = K->timing[i];
+  return 0;
+}
+
+// neighbour lists of the last solve straight from HBM (0-based ids, ascending)
+int kk_get_neighbours(int64_t h, int nblock, int mmax, int* nnear, int* inear) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->d_nnear.p || nblock != K->nblock || mmax != K->mmax) {
+    set_error("kk_get_neighbours", "no neighbour lists of matching shape");
+    return ierr();
+  }
+  if (hipMemcpy(nnear, K->d_nnear.p, sizeof(int) * nblock, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(inear, K->d_inear.p, sizeof(int) * (size_t)nblock * mmax, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("kk_get_neighbours", "device copy failed");
+    return ierr();
+  }
+  return 0;
+}
+
+int kk_get_mmax(int64_t h, int* mmax) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  *mmax = K->mmax;
+  return 0;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

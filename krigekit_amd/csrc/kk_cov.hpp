@@ -1,0 +1,148 @@
+// Device-side covariance evaluation.
+//
+// C(lag) for one variable pair follows the reference's COV macro
+// (src/libkriging/cov_dispatch.fh:14-19):
+//   table mode  -> struct_cov_tab   (src/libkriging/variogram.f90:750-771)
+//   analytic    -> struct_cov_lag   (src/libkriging/variogram.f90:731-747)
+// The lookup reproduces cov_struct_tab_h (variogram.f90:682-710) exactly:
+// zone search over h_bounds, index clamped to n_zone-1, fraction taken from
+// the unclamped integer part, entry 0 = C(0) including the nugget.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kk_types.hpp"
+
+namespace kk {
+
+// correlation shapes, src/libkriging/vgmfunc.f90:48-87
+__device__ inline double corefunc(int t, double r) {
+  const double pi = 3.141592653589793238462643383279502884;
+  switch (t) {
+    case SPH: return (r < 1.0) ? (1.0 - 1.5 * r) + 0.5 * (r * r * r) : 0.0;
+    case EXP: return exp(-3.0 * r);
+    case GAU: return exp(-3.0625 * (r * r));
+    case HOL: return cos(pi * r);
+    case POW: return (r < 1.0) ? 1.0 - pow(r, 1.5) : 0.0;
+    case BSQ: { double a = 1.0 - r * r; return (r < 1.0) ? a * a : 0.0; }
+    case CIR: return (r < 1.0) ? 1.0 - (2.0 * r * sqrt(1.0 - r * r) + 2.0 * asin(r)) / pi : 0.0;
+    case LIN: return (r < 1.0) ? 1.0 - r : 0.0;
+    case CYC: { double s = sin(pi * r); return exp(-2.0 * (s * s)); }
+    case DCO: return exp(-3.0 * r) * cos(pi * r);
+    default: return 0.0;
+  }
+}
+
+__device__ inline bool has_tail(int t) {
+  return t == EXP || t == GAU || t == HOL || t == CYC || t == DCO;
+}
+
+// h = |mat * lag|, src/libkriging/variogram.f90:194-210
+template <int DIM>
+__device__ inline double aniso_h(const double* m, double dx, double dy, double dz) {
+  if (DIM == 1) {
+    return fabs(dx * m[0]);
+  } else if (DIM == 2) {
+    double rx = m[0] * dx + m[3] * dy;
+    double ry = m[1] * dx + m[4] * dy;
+    return sqrt(rx * rx + ry * ry);
+  } else {
+    double rx = m[0] * dx + m[3] * dy + m[6] * dz;
+    double ry = m[1] * dx + m[4] * dy + m[7] * dz;
+    double rz = m[2] * dx + m[5] * dy + m[8] * dz;
+    return sqrt(rx * rx + ry * ry + rz * rz);
+  }
+}
+
+// comp_cov_h, variogram.f90:327-337
+__device__ inline double comp_cov_h(const Model& M, int c, double h) {
+  const double eps = 2.2250738585072014e-308 * 1.0e3;
+  return (h > eps) ? M.sill[c] * corefunc(M.vtype[c], h) : M.sill[c] + M.nugget[c];
+}
+
+// struct_cov_h, variogram.f90:712-728 (one h for every structure)
+__device__ inline double struct_cov_h(const Model& M, double h) {
+  double res = 0.0;
+  int iv = 0;
+  while (iv < M.nstruct) {
+    double g = comp_cov_h(M, iv, h);
+    while (iv + 1 < M.nstruct && M.is_product[iv + 1]) {
+      ++iv;
+      g = g * comp_cov_h(M, iv, h);
+    }
+    res = res + g;
+    ++iv;
+  }
+  return res;
+}
+
+// in-table part of cov_struct_tab_h / comp_cov_tab_h; h in (0, zb[3])
+__device__ inline double table_interp(const Model& M, const double* tab, double h) {
+  int k = (h >= M.zb[1]) + (h >= M.zb[2]);
+  double fi = (h - M.zb[k]) * M.zinv[k];
+  int ifi = (int)fi;
+  int i = M.zoff[k] + min(ifi, M.zn[k] - 1);
+  fi = fi - (double)ifi;
+  double t0 = tab[i], t1 = tab[i + 1];
+  return t0 + fi * (t1 - t0);
+}
+
+// analytic tail beyond the table (h >= 3.5), kept out of line: it is rare
+// and its shape switch would otherwise inflate every caller's registers
+__device__ __noinline__ double struct_cov_h_tail(const Model* M, double h) { return struct_cov_h(*M, h); }
+
+// Level-B table evaluation: one anisotropy transform and one lookup
+template <int DIM>
+__device__ inline double cov_tableB(const Model& M, double dx, double dy, double dz) {
+  const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
+  if (h >= M.zb[3]) return M.b_tail ? struct_cov_h_tail(&M, h) : 0.0;
+  if (h <= 0.0) return M.tab[0];
+  return table_interp(M, M.tab, h);
+}
+
+template <int DIM>
+__device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy, double dz);
+
+template <int DIM>
+__device__ inline double cov(const Model& M, double dx, double dy, double dz) {
+  if (M.mode == COV_TABLE_B) return cov_tableB<DIM>(M, dx, dy, dz);
+  double res = 0.0;
+  int iv = 0;
+  while (iv < M.nstruct) {
+    double g = 1.0;
+    bool first = true;
+    do {
+      if (!first) ++iv;
+      double h = aniso_h<DIM>(M.mat[iv], dx, dy, dz);
+      double c;
+      if (M.mode == COV_TABLE_A) {
+        const double* tab = M.tab + (size_t)iv * (M.ntab + 1);
+        if (h >= M.zb[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h(M, iv, h) : 0.0;
+        else if (h <= 0.0) c = tab[0];
+        else c = table_interp(M, tab, h);
+      } else {
+        c = comp_cov_h(M, iv, h);
+      }
+      g = first ? c : g * c;
+      first = false;
+    } while (iv + 1 < M.nstruct && M.is_product[iv + 1]);
+    res = res + g;
+    ++iv;
+  }
+  return res;
+}
+
+// every mode (Level A tables, analytic), out of line
+template <int DIM>
+__device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy, double dz) {
+  return cov<DIM>(*Mp, dx, dy, dz);
+}
+
+// kernels specialise on the model mode: TB = Level-B table inline,
+// otherwise the general path out of line
+template <int DIM, bool TB>
+__device__ inline double cov_eval(const Model& M, double dx, double dy, double dz) {
+  if (TB) return cov_tableB<DIM>(M, dx, dy, dz);
+  return cov_general<DIM>(&M, dx, dy, dz);
+}
+
+}  // namespace kk

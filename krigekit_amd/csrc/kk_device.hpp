@@ -1,0 +1,46 @@
+// Kernel launchers shared between the host orchestration and the HIP files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kk_types.hpp"
+
+namespace kk {
+
+// 3x3 rotation applied like the reference's mat_mul (src/libkriging/rotation.f90:224-246)
+struct Rot3 {
+  double m[9];  // column-major
+  template <int DIM>
+  __host__ __device__ void apply(const double* x, double* y) const {
+    if (DIM == 1) {
+      y[0] = m[0] * x[0];
+    } else if (DIM == 2) {
+      y[0] = m[0] * x[0] + m[3] * x[1];
+      y[1] = m[1] * x[0] + m[4] * x[1];
+    } else {
+      y[0] = m[0] * x[0] + m[3] * x[1] + m[6] * x[2];
+      y[1] = m[1] * x[0] + m[4] * x[1] + m[7] * x[2];
+      y[2] = m[2] * x[0] + m[5] * x[1] + m[8] * x[2];
+    }
+  }
+};
+
+hipError_t launch_search(const SearchArgs& a, hipStream_t st);
+hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord_orig, const Rot3& r,
+                             hipStream_t st);
+
+// precision of the per-node solve
+enum Precision : int { PREC_FP64 = 0, PREC_FP32R = 1 };
+
+// bucket blocks by neighbour count: counts[NB] and lists (device), returns via counts
+constexpr int NBUCKET = 5;  // n==0, <=8, <=16, <=32, <=64 ; larger -> overflow bucket
+hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,
+                         hipStream_t st);
+int bucket_width(int b);  // lanes per system of bucket b (0 for the empty bucket)
+
+hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
+hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
+hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
+hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
+                          hipStream_t st);
+
+}  // namespace kk

@@ -1,0 +1,106 @@
+// Shared host/device data layouts of the MI355X kriging backend.
+//
+// Everything the GPU reads is flattened into plain arrays: the kd-tree
+// (node records + rearranged coordinates), the covariance model (structure
+// parameters + lookup tables) and the per-solve block/observation arrays.
+#pragma once
+#include <cstdint>
+
+namespace kk {
+
+constexpr int MAXS = 8;        // nested structures per variogram
+constexpr int MAXDIM = 3;
+constexpr int NODE_I = 6;      // ints per tree node: cut_dim, l, u, left, right, parent
+constexpr int NODE_D = 9;      // doubles per tree node: cut_val, cut_left, cut_right, lo[3], hi[3]
+
+// variogram shape ids (src/libkriging/vgmfunc.f90:6-16)
+enum VType : int { NUG = 0, SPH = 1, EXP = 2, GAU = 3, HOL = 4, POW = 5, BSQ = 6,
+                   CIR = 7, LIN = 8, CYC = 9, DCO = 10 };
+
+// covariance evaluation mode
+enum CovMode : int { COV_ANALYTIC = 0, COV_TABLE_B = 1, COV_TABLE_A = 2 };
+
+// Covariance model of one variable pair, passed by value to kernels.
+struct Model {
+  int ndim;
+  int mode;
+  int nstruct;
+  int b_tail;                 // Level-B table has an analytic tail
+  int vtype[MAXS];
+  int is_product[MAXS];
+  double sill[MAXS];
+  double nugget[MAXS];
+  double mat[MAXS][9];        // per-structure anisotropy matrix, column-major
+  double bmat[9];             // Level-B table matrix (first non-nugget structure)
+  double cov0;
+  // piecewise-uniform table zones (h_bounds 0, .1, .5, 3.5)
+  double zb[4];
+  double zinv[3];
+  int zoff[3];
+  int zn[3];
+  int ntab;                   // entries 0..ntab per table
+  const double* tab;          // Level B: ntab+1; Level A: nstruct*(ntab+1)
+};
+
+// kd-tree as flattened arrays in device memory
+struct Tree {
+  int dim;
+  int n;
+  int nnodes;
+  int root;
+  const double* rdata;        // (dim, n) coordinates in tree order
+  const int* ind;             // tree position -> 0-based observation index
+  const int* ni;              // nnodes * NODE_I
+  const double* nd;           // nnodes * NODE_D
+};
+
+struct SearchArgs {
+  Tree tree;
+  int ndim;
+  int nq;                     // queries in this launch
+  int q0;                     // first query id of this launch
+  const double* qcoord;       // (ndim, nblock) query coordinates (already rotated)
+  const double* ocoord;       // (ndim, nobs) original-order obs coordinates (rotated if aniso)
+  int k;                      // candidates requested from the tree (heap size)
+  int sort;                   // sector mode: heapsort then octant filter
+  int nmax;                   // per-sector cap (sector) or list size (plain)
+  int cv;                     // leave-one-out: skip query's own index (query id == obs id)
+  int all_obs;                // nmax >= nobs: every observation, no tree
+  double maxdist2;
+  int mmax;                   // row stride of inear
+  double* hdis;               // heap scratch: k * hstride
+  int* hidx;
+  int hstride;
+  int* nnear;                 // [nblock]
+  int* inear;                 // [nblock * mmax], 0-based obs ids, ascending
+};
+
+struct SolveArgs {
+  const Model* model;         // device copy of the covariance model
+  int ndim;
+  int unbias;                 // 1 ordinary, 0 simple
+  double sk_mean;
+  double bound_lo, bound_hi;
+  int nsub;                   // integration points per block (1 = point)
+  const double* bcoord;       // (ndim, nblock) block centres
+  const double* scoord;       // (ndim, nblock*nsub) sub-points or nullptr
+  const double* sweight;      // nblock*nsub or nullptr
+  const double* base_cov;     // per-block prior covariance (block kriging) or nullptr
+  const double* rangescale;   // nblock or nullptr
+  const double* localnugget;  // nblock or nullptr
+  const double* ocoord;       // (ndim, nobs)
+  const double* ovalue;       // nobs
+  const double* ovar;         // nobs
+  const int* nnear;
+  const int* inear;
+  int mmax;
+  const int* list;            // block ids of this bucket
+  int nlist;
+  double pivot_tol;           // FP32 pivot ratio below which a system is re-solved in FP64
+  double* est;
+  double* var;
+  double* weights;            // nblock*mmax or nullptr
+  int* flag;                  // per block: 0 ok, 1 needs FP64 re-solve, 2 Cholesky failed
+};
+
+}  // namespace kk

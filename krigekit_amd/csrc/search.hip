@@ -1,0 +1,317 @@
+// Neighbour search on the GPU: exact kdtree2 semantics, one lane per query.
+//
+// Reproduces, per query, what the reference's search_neighbors computes
+// (src/libkriging/kriging.F90:490-666):
+//   * kdtree2_n_nearest traversal order and pruning
+//     (src/libkriging/kdtree2_maxidx.f90:1383-1457), bounded max-heap with
+//     replace-on-equal (:307-446, :1510-1613)
+//   * sector mode: unstable heapsort of the candidates (:1860-1914) and the
+//     quadrant/octant caps (kriging.F90:1093-1154)
+//   * leave-one-out self exclusion (kriging.F90:647-657), maxdist filter
+//     (kriging_base.F90:2491-2505) and the ascending index sort
+//     (kriging_base.F90:3510-3524)
+// so neighbour lists, including which of equal-distance points is kept, are
+// bit-identical to the reference.
+//
+// Traversal is stackless: the tree carries parent links, and whether a lane
+// returns from the closer or the farther child is recomputed from the query.
+// Squared distances accumulate dimension by dimension with separately
+// rounded multiply and add (this file is built with -ffp-contract=off), which
+// is how the reference's x86-64 build rounds them.
+//
+// The heap of each lane lives in HBM scratch, lane-interleaved
+// (entry p of lane t at p*hstride + t), so the top of the heap, touched on
+// every insert, is read by a wave as one contiguous segment.
+#include <hip/hip_runtime.h>
+
+#include "kk_device.hpp"
+#include "kk_types.hpp"
+
+namespace kk {
+
+#pragma clang fp contract(off)
+
+struct Heap {
+  double* d;
+  int* i;
+  size_t s;
+  __device__ double& D(int p) const { return d[(size_t)(p - 1) * s]; }
+  __device__ int& I(int p) const { return i[(size_t)(p - 1) * s]; }
+};
+
+// pq_insert, kdtree2_maxidx.f90:307-346
+__device__ inline double pq_insert(const Heap& h, int& size, double dis, int idx) {
+  size += 1;
+  int i = size;
+  while (i > 1) {
+    int ip = i >> 1;
+    double pd = h.D(ip);
+    if (dis > pd) {
+      h.D(i) = pd;
+      h.I(i) = h.I(ip);
+      i = ip;
+    } else {
+      break;
+    }
+  }
+  h.D(i) = dis;
+  h.I(i) = idx;
+  return h.D(1);
+}
+
+// pq_replace_max, kdtree2_maxidx.f90:392-446
+__device__ inline double pq_replace_max(const Heap& h, int N, double dis, int idx) {
+  int parent = 1, child = 2;
+  while (child <= N) {
+    double pc = h.D(child);
+    if (child < N) {
+      double pc1 = h.D(child + 1);
+      if (pc < pc1) {
+        child = child + 1;
+        pc = pc1;
+      }
+    }
+    if (dis >= pc) break;
+    h.D(parent) = pc;
+    h.I(parent) = h.I(child);
+    parent = child;
+    child = 2 * parent;
+  }
+  h.D(parent) = dis;
+  h.I(parent) = idx;
+  return h.D(1);
+}
+
+// heapsort_struct, kdtree2_maxidx.f90:1860-1914 (unstable, ascending)
+__device__ inline void heapsort(const Heap& a, int n) {
+  if (n <= 1) return;
+  int ileft = n / 2 + 1, iright = n;
+  double vd;
+  int vi;
+  for (;;) {
+    if (ileft > 1) {
+      ileft = ileft - 1;
+      vd = a.D(ileft);
+      vi = a.I(ileft);
+    } else {
+      vd = a.D(iright);
+      vi = a.I(iright);
+      a.D(iright) = a.D(1);
+      a.I(iright) = a.I(1);
+      iright = iright - 1;
+      if (iright == 1) {
+        a.D(1) = vd;
+        a.I(1) = vi;
+        return;
+      }
+    }
+    int i = ileft, j = 2 * ileft;
+    while (j <= iright) {
+      if (j < iright && a.D(j) < a.D(j + 1)) j = j + 1;
+      double aj = a.D(j);
+      if (vd < aj) {
+        a.D(i) = aj;
+        a.I(i) = a.I(j);
+        i = j;
+        j = j + j;
+      } else {
+        j = iright + 1;
+      }
+    }
+    a.D(i) = vd;
+    a.I(i) = vi;
+  }
+}
+
+__device__ inline double sq(double x) { return __dmul_rn(x, x); }
+
+__device__ inline double dis2_bnd(double x, double lo, double hi) {
+  if (x > hi) return sq(__dsub_rn(x, hi));
+  if (x < lo) return sq(__dsub_rn(lo, x));
+  return 0.0;
+}
+
+// ascending insertion sort of out[0..n) in global memory
+__device__ inline void isort(int* a, int n) {
+  for (int i = 1; i < n; ++i) {
+    int t = a[i], j = i - 1;
+    while (j >= 0 && a[j] > t) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = t;
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int qid = a.q0 + t;
+  double q[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
+  int* out = a.inear + (size_t)qid * a.mmax;
+  const Tree& T = a.tree;
+  Heap h{a.hdis + t, a.hidx + t, (size_t)a.hstride};
+
+  // ---- kdtree2_n_nearest: stackless traversal -------------------------
+  double ballsize = 1.7976931348623157e308;  // huge(1.0)
+  int nfound = 0;
+  const int nn = a.k;
+  int node = T.root, prev = -1;
+  while (node >= 0) {
+    const int* ni = T.ni + (size_t)node * NODE_I;
+    const int cut = ni[0], left = ni[3], right = ni[4], parent = ni[5];
+    if (prev == parent) {
+      // arriving from above
+      if (left < 0 || right < 0) {
+        // process_terminal_node, kdtree2_maxidx.f90:1510-1613
+        const int l = ni[1], u = ni[2];
+        for (int i = l; i <= u; ++i) {
+          double sd = 0.0;
+          bool skip = false;
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) {
+            sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
+            if (sd > ballsize) {
+              skip = true;
+              break;
+            }
+          }
+          if (skip) continue;
+          const int idx = T.ind[i];
+          if (nfound < nn) {
+            double newpri = pq_insert(h, nfound, sd, idx);
+            if (nfound == nn) ballsize = newpri;
+          } else {
+            ballsize = pq_replace_max(h, nn, sd, idx);
+          }
+        }
+        prev = node;
+        node = parent;
+        continue;
+      }
+      const double cv = T.nd[(size_t)node * NODE_D + 0];
+      prev = node;
+      node = (q[cut] < cv) ? left : right;  // closer child
+      continue;
+    }
+    // returning from a child
+    const double* nd = T.nd + (size_t)node * NODE_D;
+    const bool goes_left = q[cut] < nd[0];
+    const int closer = goes_left ? left : right;
+    if (prev == closer) {
+      const int farther = goes_left ? right : left;
+      double dis = goes_left ? sq(__dsub_rn(nd[2], q[cut])) : sq(__dsub_rn(nd[1], q[cut]));
+      bool visit = dis <= ballsize;
+      if (visit) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+          if (i != cut && visit) {
+            dis = __dadd_rn(dis, dis2_bnd(q[i], nd[3 + i], nd[6 + i]));
+            if (dis > ballsize) visit = false;
+          }
+        }
+      }
+      if (visit) {
+        prev = node;
+        node = farther;
+        continue;
+      }
+    }
+    prev = node;
+    node = parent;
+  }
+
+  // ---- candidate list -> neighbour list --------------------------------
+  int cnt = 0;
+  if (a.sort) {
+    heapsort(h, nn);
+    // apply_sector_filter, kriging.F90:1093-1154
+    int counts[1 << DIM];
+#pragma unroll
+    for (int s = 0; s < (1 << DIM); ++s) counts[s] = 0;
+    const int skip_idx = a.cv ? qid : -1;
+    for (int c = 1; c <= nn; ++c) {
+      const int idx = h.I(c);
+      if (idx == skip_idx) continue;
+      if (h.D(c) > a.maxdist2) continue;
+      const double* o = a.ocoord + DIM * (size_t)idx;
+      int sec = (__dsub_rn(o[0], q[0]) < 0.0);
+      if (DIM >= 2) sec += 2 * (__dsub_rn(o[1], q[1]) < 0.0);
+      if (DIM == 3) sec += 4 * (__dsub_rn(o[2], q[2]) < 0.0);
+      int cs = 0;
+#pragma unroll
+      for (int s = 0; s < (1 << DIM); ++s) cs = (s == sec) ? counts[s] : cs;
+      if (cs < a.nmax) {
+#pragma unroll
+        for (int s = 0; s < (1 << DIM); ++s) counts[s] += (s == sec);
+        out[cnt++] = idx;
+      }
+    }
+  } else {
+    // plain k-nearest in heap order, then self exclusion and maxdist
+    bool removed = !a.cv;
+    for (int c = 1; c <= nn; ++c) {
+      const int idx = h.I(c);
+      if (!removed && idx == qid) {
+        removed = true;
+        continue;
+      }
+      if (h.D(c) <= a.maxdist2) out[cnt++] = idx;
+    }
+  }
+  isort(out, cnt);
+  a.nnear[qid] = cnt;
+}
+
+// every observation is a neighbour (nmax >= nobs): kriging.F90:640-657
+template <int DIM>
+__global__ __launch_bounds__(256) void k_search_all(SearchArgs a, int nobs, const double* ocoord_orig,
+                                                    Rot3 srot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int qid = a.q0 + t;
+  double q[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
+  int* out = a.inear + (size_t)qid * a.mmax;
+  int cnt = 0;
+  bool removed = !a.cv;
+  for (int i = 0; i < nobs; ++i) {
+    if (!removed && i == qid) {
+      removed = true;
+      continue;
+    }
+    // rotated_dists (src/libkriging/rotation.f90:193-218): rotate (obs - query)
+    double df[3] = {0.0, 0.0, 0.0}, r[3];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) df[k] = __dsub_rn(ocoord_orig[k + DIM * (size_t)i], q[k]);
+    srot.apply<DIM>(df, r);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) s = __dadd_rn(s, sq(r[k]));
+    if (s <= a.maxdist2) out[cnt++] = i;
+  }
+  a.nnear[qid] = cnt;
+}
+
+hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
+  dim3 blk(256), grd((a.nq + 255) / 256);
+  if (a.ndim == 1) hipLaunchKernelGGL(k_search<1>, grd, blk, 0, st, a);
+  else if (a.ndim == 2) hipLaunchKernelGGL(k_search<2>, grd, blk, 0, st, a);
+  else hipLaunchKernelGGL(k_search<3>, grd, blk, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord_orig, const Rot3& r,
+                             hipStream_t st) {
+  dim3 blk(256), grd((a.nq + 255) / 256);
+  if (a.ndim == 1) hipLaunchKernelGGL(k_search_all<1>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
+  else if (a.ndim == 2) hipLaunchKernelGGL(k_search_all<2>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
+  else hipLaunchKernelGGL(k_search_all<3>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
+  return hipGetLastError();
+}
+
+}  // namespace kk

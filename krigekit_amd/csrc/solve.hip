@@ -1,0 +1,489 @@
+// Per-node kriging solve on the GPU.
+//
+// One system per group of G lanes (G = 8/16/32/64 by neighbour-count
+// bucket), one matrix row per lane, held in registers:
+//   1. assemble K (kriging.F90:714-849) and b (kriging.F90:864-939): each
+//      lane evaluates G/2 pairs of a balanced "fold" (r, r+d mod G) and the
+//      pair values are exchanged through LDS, so every covariance is
+//      evaluated once;
+//   2. factor K = L D L^T right-looking; column k is broadcast through LDS;
+//   3. forward-substitute b, F (unbias rows) and the data values z;
+//   4. the ordinary-kriging Schur step (solver.f90:224-300), the variance
+//      (kriging.F90:968-1028) and the estimate (kriging_base.F90:3407-3503)
+//      come out of five quadratic forms b'K^-1b, F'K^-1F, F'K^-1b, z'K^-1b,
+//      z'K^-1F, so no backward substitution is needed for the outputs.
+//
+// PREC_FP32R factors a float copy of K and refines the two solves K^-1 b and
+// K^-1 F once against the FP64 matrix (residual in FP64).  A system whose
+// FP32 pivot ratio D_k / K_kk falls below pivot_tol is flagged and solved
+// again by the FP64 kernel.  A system whose FP64 pivot is not positive is
+// flagged for the LU fallback (the reference's SSYTRF fallback,
+// kriging_base.F90:3299-3366).
+#include <hip/hip_runtime.h>
+
+#include "kk_cov.hpp"
+#include "kk_device.hpp"
+#include "kk_types.hpp"
+
+namespace kk {
+
+template <int G>
+__device__ inline double gsum(double v) {
+#pragma unroll
+  for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+  return v;
+}
+template <int G>
+__device__ inline int gmax(int v) {
+#pragma unroll
+  for (int m = G / 2; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, G));
+  return v;
+}
+__device__ inline int wmax(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+template <int G, int DIM, int PREC, bool TB>
+__global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
+  constexpr int S = 64 / G;
+  __shared__ double sx[S][DIM][G];
+  __shared__ double sm[S][G][G + 1];
+  __shared__ double col[S][G];
+
+  const int lane = threadIdx.x, s = lane / G, r = lane % G;
+  const int sys = blockIdx.x * S + s;
+  const bool act = sys < a.nlist;
+  const int b = act ? a.list[sys] : 0;
+  const int n = act ? a.nnear[b] : 0;
+  const bool live = r < n;
+  const int idx = live ? a.inear[(size_t)b * a.mmax + r] : 0;
+  const int nw = wmax(n);  // largest system in this wave (uniform)
+
+  double x[3] = {0.0, 0.0, 0.0};
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) x[k] = a.ocoord[k + DIM * (size_t)idx];
+  }
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) sx[s][k][r] = x[k];
+  const double z = live ? a.ovalue[idx] : 0.0;
+  const double ov = live ? a.ovar[idx] : 0.0;
+  const double rs = (act && a.rangescale) ? a.rangescale[b] : 1.0;
+  const double ln = (act && a.localnugget) ? a.localnugget[b] : 0.0;
+  const Model& M = *a.model;
+  __syncthreads();
+
+  // ---- assemble K by the fold (each pair once) --------------------------
+#pragma unroll 1
+  for (int d = 1; d <= G / 2; ++d) {
+    const int c = (r + d) & (G - 1);
+    double v = 0.0;
+    const bool mine = (d < G / 2) || (r < G / 2);
+    if (d < nw && live && c < n && mine) {
+      const double dx = (x[0] - sx[s][0][c]) / rs;
+      const double dy = DIM >= 2 ? (x[1] - sx[s][1][c]) / rs : 0.0;
+      const double dz = DIM == 3 ? (x[2] - sx[s][2][c]) / rs : 0.0;
+      v = cov_eval<DIM, TB>(M, dx, dy, dz);
+    }
+    if (mine) {
+      sm[s][r][c] = v;
+      sm[s][c][r] = v;
+    }
+  }
+  sm[s][r][r] = live ? (M.cov0 + ov) + ln : 1.0;
+  // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
+  double rhs = 0.0;
+  if (live) {
+    for (int q = 0; q < a.nsub; ++q) {
+      const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
+                                                 : a.scoord + DIM * ((size_t)b * a.nsub + q);
+      const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
+      const double dx = (x[0] - g[0]) / rs;
+      const double dy = DIM >= 2 ? (x[1] - g[1]) / rs : 0.0;
+      const double dz = DIM == 3 ? (x[2] - g[2]) / rs : 0.0;
+      rhs = rhs + cov_eval<DIM, TB>(M, dx, dy, dz) * w;
+    }
+  }
+  const double f = (live && a.unbias) ? 1.0 : 0.0;
+  __syncthreads();
+  double A[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) A[j] = sm[s][r][j];
+  const double kdiag = sm[s][r][r];
+
+  int fail = 0;
+  double uu, uy, yy, vy, vu;
+
+  if (PREC == PREC_FP64) {
+    // ---- L D L^T, row r in A[] ------------------------------------------
+    double D = 1.0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < nw) {
+        col[s][r] = A[k];
+        __syncthreads();
+        const double akk = col[s][k];
+        if (r == k) D = akk;
+        fail |= (k < n) && !(akk > 0.0);
+        const double t = A[k] / akk;
+#pragma unroll
+        for (int j = k + 1; j < G; ++j)
+          if (j < nw) A[j] = fma(-t, col[s][j], A[j]);
+        A[k] = (r > k) ? t : A[k];
+        __syncthreads();
+      }
+    }
+    // ---- forward substitution for b, F, z ---------------------------------
+    double y = rhs, u = f, v = z;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < nw) {
+        const double yk = __shfl(y, k, G), uk = __shfl(u, k, G), vk = __shfl(v, k, G);
+        const double l = (r > k) ? A[k] : 0.0;
+        y = fma(-l, yk, y);
+        u = fma(-l, uk, u);
+        v = fma(-l, vk, v);
+      }
+    }
+    const double id = live ? 1.0 / D : 0.0;
+    uu = gsum<G>(u * u * id);
+    uy = gsum<G>(u * y * id);
+    yy = gsum<G>(y * y * id);
+    vy = gsum<G>(v * y * id);
+    vu = gsum<G>(v * u * id);
+  } else {
+    // ---- FP32 L D L^T with FP64 residual refinement ----------------------
+    float* col32 = reinterpret_cast<float*>(&col[s][0]);
+    float L[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) L[j] = (float)A[j];
+    float D = 1.0f;
+    float pmin = 1.0f;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < nw) {
+        col32[r] = L[k];
+        __syncthreads();
+        const float akk = col32[k];
+        if (r == k) D = akk;
+        fail |= (k < n) && !(akk > 0.0f);
+        const float t = L[k] * __builtin_amdgcn_rcpf(akk);
+#pragma unroll
+        for (int j = k + 1; j < G; ++j)
+          if (j < nw) L[j] = fmaf(-t, col32[j], L[j]);
+        L[k] = (r > k) ? t : L[k];
+        __syncthreads();
+      }
+    }
+    if (live) pmin = (float)((double)D / kdiag);
+    // transpose L through LDS for the backward sweeps
+    float* sm32 = reinterpret_cast<float*>(&sm[s][0][0]);
+#pragma unroll
+    for (int j = 0; j < G; ++j) sm32[r * (G + 1) + j] = L[j];
+    __syncthreads();
+    float Lt[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) Lt[j] = sm32[j * (G + 1) + r];
+    const float iD = 1.0f / D;
+    // solve K32 [xb x1] = [b f]
+    auto solve2 = [&](float p, float q, float& op, float& oq) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        if (k < nw) {
+          const float pk = __shfl(p, k, G), qk = __shfl(q, k, G);
+          const float l = (r > k) ? L[k] : 0.0f;
+          p = fmaf(-l, pk, p);
+          q = fmaf(-l, qk, q);
+        }
+      }
+      p *= iD;
+      q *= iD;
+#pragma unroll
+      for (int k = G - 1; k >= 0; --k) {
+        if (k < nw) {
+          const float pk = __shfl(p, k, G), qk = __shfl(q, k, G);
+          const float l = (r < k) ? Lt[k] : 0.0f;
+          p = fmaf(-l, pk, p);
+          q = fmaf(-l, qk, q);
+        }
+      }
+      op = p;
+      oq = q;
+    };
+    float xb32, x132;
+    solve2((float)rhs, (float)f, xb32, x132);
+    double xb = xb32, x1 = x132;
+    // FP64 residual r = t - K x (K row r is A[], still FP64)
+    double rb = rhs, r1 = f;
+    double* xsh = &col[s][0];
+    __syncthreads();
+    xsh[r] = xb;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < nw) rb = fma(-A[j], xsh[j], rb);
+    __syncthreads();
+    xsh[r] = x1;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < nw) r1 = fma(-A[j], xsh[j], r1);
+    float db, d1;
+    solve2((float)rb, (float)r1, db, d1);
+    xb += (double)db;
+    x1 += (double)d1;
+    uu = gsum<G>(f * x1);
+    uy = gsum<G>(f * xb);
+    yy = gsum<G>(rhs * xb);
+    vy = gsum<G>(z * xb);
+    vu = gsum<G>(z * x1);
+    // pivot ratio check: re-solve in FP64 below the threshold
+    float pm = live ? pmin : 1.0f;
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) pm = fminf(pm, __shfl_xor(pm, m, G));
+    if (act && r == 0 && pm < (float)a.pivot_tol) fail |= 4;
+  }
+
+  fail = gmax<G>(fail);
+  if (!act || r != 0) return;
+  if (fail & 1) {
+    a.flag[b] = 2;  // not positive definite: LU fallback
+    return;
+  }
+  if (fail & 4) {
+    a.flag[b] = 1;  // FP32 pivot too small: FP64 re-solve
+    return;
+  }
+  double var, est;
+  const double base = a.base_cov ? a.base_cov[b] : M.cov0;
+  if (a.unbias) {
+    const double lam = (uy - 1.0) / uu;
+    var = (base - yy) + lam * (uy - 1.0);
+    est = vy - lam * vu;
+  } else {
+    var = base - yy;
+    est = vy;
+    if (a.sk_mean != 0.0) est = est + (1.0 - uy) * a.sk_mean;
+  }
+  var = var > 0.0 ? var : 0.0;
+  est = est < a.bound_lo ? a.bound_lo : est;
+  est = est > a.bound_hi ? a.bound_hi : est;
+  a.est[b] = est;
+  a.var[b] = var;
+  a.flag[b] = 0;
+}
+
+// blocks without neighbours: NaN outputs (kriging_base.F90:3421-3425)
+__global__ void k_empty(SolveArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nlist) return;
+  const int b = a.list[t];
+  a.est[b] = __builtin_nan("");
+  a.var[b] = __builtin_nan("");
+  a.flag[b] = 0;
+}
+
+__global__ void k_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblock) return;
+  const int n = nnear[t];
+  const int bk = n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
+  const int pos = atomicAdd(&counts[bk], 1);
+  lists[(size_t)bk * cap + pos] = t;
+}
+
+__global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblock) return;
+  if (flag[t] == code) list[atomicAdd(count, 1)] = t;
+}
+
+// Dense LU with partial pivoting on the full augmented system, one thread per
+// system, matrix in HBM scratch: the reference's fallback when Cholesky fails
+// (SSYTRF, then one retry with the data diagonal raised by 1e-6*mean(diag),
+// then NaN; kriging_base.F90:3299-3383).
+template <int DIM>
+__global__ void k_fallback(SolveArgs a, double* scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nlist) return;
+  const int b = a.list[t];
+  const int n = a.nnear[b];
+  const int p = a.unbias ? 1 : 0, m = n + p;
+  const int ld = a.mmax + 1;
+  double* K = scratch + (size_t)t * (2 * ld * ld + 3 * ld);
+  double* A = K + ld * ld;
+  double* rhs = A + ld * ld;
+  double* xs = rhs + ld;
+  const int* in = a.inear + (size_t)b * a.mmax;
+  const double rs = a.rangescale ? a.rangescale[b] : 1.0;
+  const double ln = a.localnugget ? a.localnugget[b] : 0.0;
+  const Model& M = *a.model;
+  for (int i = 0; i < n; ++i) {
+    const double* xi = a.ocoord + DIM * (size_t)in[i];
+    K[i + ld * i] = (M.cov0 + a.ovar[in[i]]) + ln;
+    for (int j = i + 1; j < n; ++j) {
+      const double* xj = a.ocoord + DIM * (size_t)in[j];
+      const double c = cov<DIM>(M, (xi[0] - xj[0]) / rs, DIM >= 2 ? (xi[1] - xj[1]) / rs : 0.0,
+                                DIM == 3 ? (xi[2] - xj[2]) / rs : 0.0);
+      K[j + ld * i] = c;
+      K[i + ld * j] = c;
+    }
+    if (p) {
+      K[n + ld * i] = 1.0;
+      K[i + ld * n] = 1.0;
+    }
+    double tmp = 0.0;
+    for (int q = 0; q < a.nsub; ++q) {
+      const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
+                                                 : a.scoord + DIM * ((size_t)b * a.nsub + q);
+      const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
+      tmp = tmp + cov<DIM>(M, (xi[0] - g[0]) / rs, DIM >= 2 ? (xi[1] - g[1]) / rs : 0.0,
+                           DIM == 3 ? (xi[2] - g[2]) / rs : 0.0) * w;
+    }
+    rhs[i] = tmp;
+  }
+  if (p) {
+    K[n + ld * n] = 0.0;
+    rhs[n] = 1.0;
+  }
+  int info = 1;
+  for (int attempt = 0; attempt < 2 && info; ++attempt) {
+    if (attempt == 1) {
+      double ds = 0.0;
+      for (int i = 0; i < n; ++i) ds = ds + K[i + ld * i];
+      if (!(ds > 0.0)) break;
+      ds = 1.0e-6 * (ds / (double)n);
+      for (int i = 0; i < n; ++i) K[i + ld * i] = K[i + ld * i] + ds;
+    }
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) A[i + ld * j] = K[i + ld * j];
+    for (int i = 0; i < m; ++i) xs[i] = rhs[i];
+    info = 0;
+    for (int k = 0; k < m && !info; ++k) {
+      int pv = k;
+      double mx = fabs(A[k + ld * k]);
+      for (int i = k + 1; i < m; ++i)
+        if (fabs(A[i + ld * k]) > mx) {
+          mx = fabs(A[i + ld * k]);
+          pv = i;
+        }
+      if (mx == 0.0) {
+        info = k + 1;
+        break;
+      }
+      if (pv != k) {
+        for (int j = 0; j < m; ++j) {
+          const double tt = A[k + ld * j];
+          A[k + ld * j] = A[pv + ld * j];
+          A[pv + ld * j] = tt;
+        }
+        const double tt = xs[k];
+        xs[k] = xs[pv];
+        xs[pv] = tt;
+      }
+      for (int i = k + 1; i < m; ++i) {
+        const double fct = A[i + ld * k] / A[k + ld * k];
+        for (int j = k + 1; j < m; ++j) A[i + ld * j] -= fct * A[k + ld * j];
+        xs[i] -= fct * xs[k];
+      }
+    }
+    if (!info)
+      for (int i = m - 1; i >= 0; --i) {
+        double v = xs[i];
+        for (int j = i + 1; j < m; ++j) v -= A[i + ld * j] * xs[j];
+        xs[i] = v / A[i + ld * i];
+      }
+  }
+  a.flag[b] = info ? 3 : 0;
+  if (info) {
+    a.est[b] = __builtin_nan("");
+    a.var[b] = __builtin_nan("");
+    return;
+  }
+  double dot = 0.0, est = 0.0, tw = 0.0;
+  for (int i = 0; i < m; ++i) dot = dot + xs[i] * rhs[i];
+  for (int i = 0; i < n; ++i) {
+    est = est + a.ovalue[in[i]] * xs[i];
+    tw = tw + xs[i];
+  }
+  const double base = a.base_cov ? a.base_cov[b] : M.cov0;
+  double var = base - dot;
+  if (!a.unbias && a.sk_mean != 0.0) est = est + (1.0 - tw) * a.sk_mean;
+  var = var > 0.0 ? var : 0.0;
+  est = est < a.bound_lo ? a.bound_lo : est;
+  est = est > a.bound_hi ? a.bound_hi : est;
+  a.est[b] = est;
+  a.var[b] = var;
+}
+
+int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 64 : 0; }
+
+hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_bucket, dim3((nblock + 255) / 256), dim3(256), 0, st, nnear, nblock, counts,
+                     lists, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_collect, dim3((nblock + 255) / 256), dim3(256), 0, st, flag, nblock, code,
+                     count, list);
+  return hipGetLastError();
+}
+
+template <int G, int PREC>
+static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
+  constexpr int S = 64 / G;
+  dim3 grd((a.nlist + S - 1) / S), blk(64);
+  if (a.ndim == 1) {
+    if (tb) hipLaunchKernelGGL((k_solve<G, 1, PREC, true>), grd, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve<G, 1, PREC, false>), grd, blk, 0, st, a);
+  } else if (a.ndim == 2) {
+    if (tb) hipLaunchKernelGGL((k_solve<G, 2, PREC, true>), grd, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve<G, 2, PREC, false>), grd, blk, 0, st, a);
+  } else {
+    if (tb) hipLaunchKernelGGL((k_solve<G, 3, PREC, true>), grd, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve<G, 3, PREC, false>), grd, blk, 0, st, a);
+  }
+}
+
+hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st) {
+  if (a.nlist <= 0) return hipSuccess;
+  if (prec == PREC_FP64) {
+    switch (width) {
+      case 8: launch_g<8, PREC_FP64>(a, tableB, st); break;
+      case 16: launch_g<16, PREC_FP64>(a, tableB, st); break;
+      case 32: launch_g<32, PREC_FP64>(a, tableB, st); break;
+      case 64: launch_g<64, PREC_FP64>(a, tableB, st); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (width) {
+      case 8: launch_g<8, PREC_FP32R>(a, tableB, st); break;
+      case 16: launch_g<16, PREC_FP32R>(a, tableB, st); break;
+      case 32: launch_g<32, PREC_FP32R>(a, tableB, st); break;
+      case 64: launch_g<64, PREC_FP32R>(a, tableB, st); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_empty(const SolveArgs& a, hipStream_t st) {
+  if (a.nlist <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_empty, dim3((a.nlist + 255) / 256), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st) {
+  if (a.nlist <= 0) return hipSuccess;
+  if (a.ndim == 1) hipLaunchKernelGGL(k_fallback<1>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
+  else if (a.ndim == 2) hipLaunchKernelGGL(k_fallback<2>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
+  else hipLaunchKernelGGL(k_fallback<3>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
+  return hipGetLastError();
+}
+
+}  // namespace kk

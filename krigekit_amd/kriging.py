@@ -1,0 +1,381 @@
+"""Python front end of the MI355X kriging backend.
+
+Mirrors the public surface of krigekit's ``Kriging`` class
+(``src/krigekit/kriging.py:403-1985``): same constructor flags and defaults
+(``neglect_error=True``, ``std_ck=False``, ``pf_cache=False``,
+``anisotropic_search=False``), same setters, the same (n, ndim) ->
+(ndim, n) column-major transposes, the same ``RuntimeError(last_error)``
+convention and the same result shapes and DataFrame columns.  ``solve()``
+runs the hot path (neighbour search, assembly, factorisation, estimate and
+variance) on the GPU through the C ABI of ``libkrigekit_amd.so``.
+
+Additional keyword arguments (no counterpart in the reference):
+``precision`` ("fp64" or "fp32r": FP32 factorisation with one FP64
+refinement step), ``table_mode`` (True: the reference's default lookup-table
+covariance; False: analytic covariance) and ``pivot_tol``.
+"""
+import ctypes
+import os
+import random
+import sys
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkrigekit_amd.so")
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_i64 = ctypes.c_int64
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
+
+_lib = None
+
+
+def _load_lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"HIP kriging library not found at {LIB_PATH!r}; build it with "
+            "`python -m krigekit_amd.build`.")
+    lib = ctypes.CDLL(LIB_PATH)
+
+    def sig(name, args, res=_c_int):
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+
+    sig("krige_create", [ctypes.POINTER(_i64)])
+    sig("krige_destroy", [ctypes.POINTER(_i64)])
+    sig("krige_initialize", [_i64] + [_c_int] * 16 + [ctypes.c_char_p, _pd, _c_int])
+    sig("krige_set_obs", [_i64, _c_int, _c_int, _c_int, _pd, _pd, _pd, _c_int, _c_double, _c_double])
+    sig("krige_update_obs_value", [_i64, _c_int, _c_int, _pd])
+    sig("krige_reset_vgm", [_i64, _c_int, _c_int])
+    sig("krige_set_vgm", [_i64, _c_int, _c_int, ctypes.c_char_p] + [_c_double] * 8 + [_c_int])
+    sig("krige_set_grid", [_i64, _c_int, _c_int, _pd, _pd, _pd])
+    sig("krige_set_grid_block", [_i64, _c_int, _c_int, _c_int, _pd, _c_int, _pi, _pd, _pd, _pd, _pd])
+    sig("krige_set_grid_cv", [_i64])
+    sig("krige_set_search", [_i64, _c_int] + [_c_double] * 5 + [_c_int])
+    sig("krige_solve", [_i64, _c_int, _c_int])
+    sig("krige_get_nblocks", [_i64, _pi])
+    sig("krige_get_nsim", [_i64, _pi])
+    sig("krige_get_block_coord", [_i64, _c_int, _c_int, _pd])
+    sig("krige_get_estimate_all", [_i64, _c_int, _c_int, _c_int, _pd])
+    sig("krige_get_variance_all", [_i64, _c_int, _c_int, _pd])
+    sig("krige_get_last_error", [ctypes.POINTER(ctypes.c_char), _c_int])
+    sig("krige_solver_stats", [_i64, _pi], None)
+    sig("krige_to_str", [_i64], ctypes.c_void_p)
+    sig("krige_get_weight_dims", [_i64, _pi, _pi, _pi])
+    sig("krige_get_weight_nnear", [_i64, _c_int, _c_int, _pi])
+    sig("krige_get_weight_inear", [_i64, _c_int, _c_int, _c_int, _pi])
+    sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
+    sig("kk_solve_resident", [_i64])
+    sig("kk_fetch", [_i64])
+    sig("kk_get_timing", [_i64, _pd])
+    sig("kk_get_neighbours", [_i64, _c_int, _c_int, _pi, _pi])
+    sig("kk_get_mmax", [_i64, _pi])
+    _lib = lib
+    return lib
+
+
+def _last_error() -> str:
+    buf = ctypes.create_string_buffer(4096)
+    _load_lib().krige_get_last_error(buf, 4096)
+    return buf.value.decode("utf-8", errors="replace").strip()
+
+
+def _check(ierr, name):
+    if int(ierr) != 0:
+        raise RuntimeError(_last_error() or f"{name} failed with ierr={int(ierr)}")
+
+
+def _fcoord(coord):
+    """(n, ndim) -> (ndim, n) column-major, as kriging.py:342-360."""
+    a = np.asarray(coord, dtype=np.float64)
+    if a.ndim == 1:
+        return np.asfortranarray(a.reshape(-1, 1))
+    return np.asfortranarray(a.T)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_pd)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_pi)
+
+
+_PRECISIONS = {"fp64": 0, "fp32r": 1}
+
+
+class Kriging:
+    """GPU drop-in for krigekit's ``Kriging`` (same methods and argument meaning)."""
+
+    def __init__(self, ndim: int = 2, nvar: int = 1, ndrift: int = 0, unbias: int = 1, nsim: int = 0,
+                 anisotropic_search: bool = False, weight_correction: bool = False,
+                 use_old_weight: bool = False, store_weight: bool = False,
+                 cross_validation: bool = False, write_mat: bool = False, neglect_error: bool = True,
+                 varying_vgm: bool = False, std_ck: bool = False, verbose: bool = False,
+                 pf_cache: bool = False, weight_file: str = "", bounds: Optional[tuple] = None,
+                 seed: Optional[int] = None, precision: str = "fp64", table_mode: bool = True,
+                 pivot_tol: float = 1.0e-3):
+        lib = _load_lib()
+        h = _i64(0)
+        _check(lib.krige_create(ctypes.byref(h)), "krige_create")
+        self._handle = h.value
+        huge = sys.float_info.max * 1e3
+        self.bounds = np.asfortranarray(bounds if bounds is not None else [-huge, huge], dtype=np.float64)
+        seed = seed or random.randint(0, 2 ** 31 - 1)
+        self.store_weight = store_weight
+        self.weight_file = weight_file
+        if store_weight and weight_file:
+            raise ValueError("writing a weight file is not provided by this backend")
+        if store_weight and use_old_weight:
+            raise ValueError("store_weight and use_old_weight are mutually exclusive")
+        if cross_validation and nsim > 0:
+            raise ValueError("nsim>0 and cross_validation are mutually exclusive")
+        _check(lib.krige_initialize(
+            self._handle, ndim, nvar, ndrift, unbias, nsim, int(anisotropic_search),
+            int(weight_correction), int(use_old_weight), int(store_weight), int(cross_validation),
+            int(write_mat), int(neglect_error), int(varying_vgm), int(std_ck), int(verbose),
+            int(pf_cache), weight_file.encode("utf-8") if weight_file else b"", _dp(self.bounds),
+            int(seed) & 0x7FFFFFFF), "krige_initialize")
+        self.ndim, self.nvar, self.ndrift, self.nsim = ndim, nvar, ndrift, nsim
+        self.unbias = unbias
+        self.cross_validation = cross_validation
+        self.verbose = verbose
+        self.seed = seed
+        self._nblock = 0
+        self._nobs = np.zeros(nvar, dtype=np.int64)
+        self.set_options(precision=precision, table_mode=table_mode, pivot_tol=pivot_tol)
+
+    # ------------------------------------------------------------------
+    def set_options(self, precision: str = "fp64", table_mode: bool = True, pivot_tol: float = 1.0e-3):
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+        _check(_load_lib().kk_set_options(self._handle, _PRECISIONS[precision], pivot_tol,
+                                          int(table_mode)), "kk_set_options")
+        self.precision, self.table_mode, self.pivot_tol = precision, table_mode, pivot_tol
+
+    def set_obs(self, ivar: int, coord, value, variance=None, nmax: Optional[int] = None,
+                maxdist: Optional[float] = None, sk_mean: float = 0.0):
+        cf = _fcoord(coord)
+        v = np.asfortranarray(np.asarray(value, dtype=np.float64).ravel())
+        nobs = cf.shape[1]
+        if v.size != nobs:
+            raise ValueError(f"value length ({v.size}) must match nobs ({nobs})")
+        if variance is not None:
+            var = np.asfortranarray(np.asarray(variance, dtype=np.float64).ravel())
+            if var.size != nobs:
+                raise ValueError(f"variance length ({var.size}) must match nobs ({nobs})")
+        else:
+            var = np.zeros(nobs)
+        nm = nmax if nmax is not None else np.iinfo(np.int32).max
+        md = maxdist if maxdist is not None else sys.float_info.max
+        _check(_load_lib().krige_set_obs(self._handle, ivar, nobs, cf.shape[0], _dp(cf), _dp(v),
+                                         _dp(var), nm, md, sk_mean), "krige_set_obs")
+        self._nobs[ivar - 1] = nobs
+
+    def update_obs_value(self, ivar: int, value):
+        v = np.asfortranarray(np.asarray(value, dtype=np.float64).ravel())
+        if v.size != self._nobs[ivar - 1]:
+            raise ValueError(f"value length ({v.size}) must match nobs ({self._nobs[ivar - 1]}) "
+                             f"set by the previous set_obs call for ivar={ivar}")
+        _check(_load_lib().krige_update_obs_value(self._handle, ivar, v.size, _dp(v)),
+               "krige_update_obs_value")
+
+    def set_vgm(self, ivar: int, jvar: int, vtype: str, nugget: float = 0.0, sill: float = 1.0,
+                a_major: float = 1.0, a_minor1: Optional[float] = None,
+                a_minor2: Optional[float] = None, azimuth: float = 0.0, dip: float = 0.0,
+                plunge: float = 0.0, append: bool = True, product: bool = False):
+        if a_minor1 is None:
+            a_minor1 = a_major
+        if a_minor2 is None:
+            a_minor2 = a_minor1
+        lib = _load_lib()
+        if not append:
+            _check(lib.krige_reset_vgm(self._handle, ivar, jvar), "krige_reset_vgm")
+        if jvar < ivar:  # same pair collapse as kriging.py:878-881
+            jvar = ivar
+        _check(lib.krige_set_vgm(self._handle, ivar, jvar, vtype.lower()[:3].encode("utf-8"),
+                                 nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge,
+                                 int(product)), "krige_set_vgm")
+
+    def set_grid(self, coord=None, rangescale=None, localnugget=None):
+        if coord is None:
+            self.set_grid_cv()
+            return
+        cf = _fcoord(coord)
+        n = cf.shape[1]
+        rs = np.asfortranarray(rangescale if rangescale is not None else np.ones(n), dtype=np.float64)
+        ln = np.asfortranarray(localnugget if localnugget is not None else np.zeros(n), dtype=np.float64)
+        _check(_load_lib().krige_set_grid(self._handle, n, cf.shape[0], _dp(cf), _dp(rs), _dp(ln)),
+               "krige_set_grid")
+        self._nblock = n
+
+    def set_grid_block(self, coord, block_type: int, nblockpnt=None, pointweight=None,
+                       blocksize=None, rangescale=None, localnugget=None):
+        cf = _fcoord(coord)
+        ngrid = cf.shape[1]
+        if block_type == -4:
+            nblock = ngrid
+            if blocksize is None:
+                raise ValueError("blocksize must be specified for Gaussian quadrature blocks.")
+            bs = np.asarray(blocksize, dtype=np.float64)
+            if bs.ndim == 1:
+                bs = np.tile(bs, (nblock, 1))
+            bsf = _fcoord(bs)
+            nbp = np.ones(nblock, dtype=np.int32)
+            pw = np.ones(nblock)
+        else:
+            nbp = np.ascontiguousarray(nblockpnt, dtype=np.int32)
+            nblock = len(nbp)
+            npoint = int(nbp.sum())
+            if np.any(nbp <= 0):
+                raise ValueError("nblockpnt must contain positive counts")
+            if npoint != ngrid:
+                raise ValueError(f"sum(nblockpnt) ({npoint}) must match coord rows ({ngrid})")
+            bsf = _fcoord(np.zeros((nblock, self.ndim)))
+            pw = (np.asfortranarray(pointweight, dtype=np.float64) if pointweight is not None
+                  else np.repeat(1.0 / nbp, nbp))
+        rs = np.asfortranarray(rangescale if rangescale is not None else np.ones(nblock), dtype=np.float64)
+        ln = np.asfortranarray(localnugget if localnugget is not None else np.zeros(nblock),
+                               dtype=np.float64)
+        _check(_load_lib().krige_set_grid_block(self._handle, block_type, ngrid, cf.shape[0], _dp(cf),
+                                                nblock, _ip(nbp), _dp(pw), _dp(bsf), _dp(rs), _dp(ln)),
+               "krige_set_grid_block")
+        self._nblock = nblock
+
+    def set_grid_cv(self):
+        _check(_load_lib().krige_set_grid_cv(self._handle), "krige_set_grid_cv")
+        self._nblock = int(self._nobs[0])
+
+    def set_search(self, ivar: int = 1, anis1: float = 1.0, anis2: float = 1.0, azimuth: float = 0.0,
+                   dip: float = 0.0, plunge: float = 0.0, sector_search: bool = False):
+        _check(_load_lib().krige_set_search(self._handle, ivar, anis1, anis2, azimuth, dip, plunge,
+                                            int(sector_search)), "krige_set_search")
+
+    def solve(self, nthread: int = 0, ncache: Optional[int] = None):
+        _check(_load_lib().krige_solve(self._handle, nthread, -1 if ncache is None else int(ncache)),
+               "krige_solve")
+
+    # ---- resident-data entry points (inputs and results stay in HBM) ----
+    def solve_resident(self):
+        _check(_load_lib().kk_solve_resident(self._handle), "kk_solve_resident")
+
+    def fetch(self):
+        _check(_load_lib().kk_fetch(self._handle), "kk_fetch")
+
+    @property
+    def timing(self) -> dict:
+        out = np.zeros(8)
+        _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
+        return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
+                    total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]))
+
+    def neighbours(self):
+        """(nnear[nblock], inear[nblock, mmax]) of the last solve; 1-based ids, ascending."""
+        lib = _load_lib()
+        mm = ctypes.c_int(0)
+        _check(lib.kk_get_mmax(self._handle, ctypes.byref(mm)), "kk_get_mmax")
+        nb = self._nblock
+        nnear = np.zeros(nb, dtype=np.int32)
+        inear = np.zeros((nb, mm.value), dtype=np.int32)
+        _check(lib.kk_get_neighbours(self._handle, nb, mm.value, _ip(nnear), _ip(inear)),
+               "kk_get_neighbours")
+        mask = np.arange(mm.value)[None, :] < nnear[:, None]
+        return nnear, np.where(mask, inear + 1, 0)
+
+    @property
+    def solver_stats(self) -> dict:
+        buf = (ctypes.c_int * 5)()
+        _load_lib().krige_solver_stats(self._handle, buf)
+        return dict(zip(["fail", "chol_fact", "chol_reuse", "ssytrf_fact", "ssytrf_reuse"], list(buf)))
+
+    def get_results(self, copy: bool = False, squeeze: bool = True):
+        lib = _load_lib()
+        nb, ns = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.krige_get_nblocks(self._handle, ctypes.byref(nb)), "krige_get_nblocks")
+        _check(lib.krige_get_nsim(self._handle, ctypes.byref(ns)), "krige_get_nsim")
+        nb, nv, ns = nb.value, self.nvar, max(1, ns.value)
+        est = np.empty((nb, nv, ns), order="F")
+        var = np.empty((nb, nv, nv), order="F")
+        _check(lib.krige_get_estimate_all(self._handle, nb, nv, ns, _dp(est)), "krige_get_estimate_all")
+        _check(lib.krige_get_variance_all(self._handle, nb, nv, _dp(var)), "krige_get_variance_all")
+        if squeeze:
+            if nv == 1:
+                est = est[:, 0]
+                var = var[:, 0, 0]
+            if ns <= 1:
+                est = est[..., 0]
+        if copy:
+            est = np.array(est, order="C", copy=True)
+            var = np.array(var, order="C", copy=True)
+        return est, var
+
+    def get_result_array(self) -> np.ndarray:
+        lib = _load_lib()
+        nb = ctypes.c_int(0)
+        _check(lib.krige_get_nblocks(self._handle, ctypes.byref(nb)), "krige_get_nblocks")
+        nb = nb.value
+        nd, nv = self.ndim, self.nvar
+        cf = np.empty((nd, nb), order="F")
+        _check(lib.krige_get_block_coord(self._handle, nd, nb, _dp(cf)), "krige_get_block_coord")
+        est, var = self.get_results(squeeze=False)
+        names = ["x", "y", "z"][:nd]
+        fields = [(n, np.float64) for n in names]
+        fields += [("estimate", np.float64)] if nv == 1 else [(f"est_v{i + 1}", np.float64) for i in range(nv)]
+        fields += [("variance", np.float64)] if nv == 1 else [(f"var_v{i + 1}", np.float64) for i in range(nv)]
+        out = np.empty(nb, dtype=fields)
+        for i, n in enumerate(names):
+            out[n] = cf[i]
+        if nv == 1:
+            out["estimate"] = est[:, 0, 0]
+            out["variance"] = var[:, 0, 0]
+        else:
+            for i in range(nv):
+                out[f"est_v{i + 1}"] = est[:, i, 0]
+                out[f"var_v{i + 1}"] = var[:, i, i]
+        return out
+
+    def get_result_df(self):
+        import pandas as pd
+        return pd.DataFrame(self.get_result_array())
+
+    def to_str(self) -> str:
+        p = _load_lib().krige_to_str(self._handle)
+        return ctypes.string_at(p).decode() if p else ""
+
+    def close(self):
+        if getattr(self, "_handle", 0):
+            h = _i64(self._handle)
+            _load_lib().krige_destroy(ctypes.byref(h))
+            self._handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def from_spec(spec, **kw):
+    """Builds a ``Kriging`` object from a workload description
+    (``krigekit_amd.synth``) and returns it ready to ``solve()``."""
+    k = Kriging(ndim=spec["ndim"], cross_validation=bool(spec.get("cross_validation", False)), **kw)
+    for iv, o in enumerate(spec["obs"], start=1):
+        k.set_obs(iv, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
+                  maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
+    if spec.get("grid") is None:
+        k.set_grid_cv()
+    else:
+        k.set_grid(spec["grid"], rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
+    for v in spec["vgm"]:
+        k.set_vgm(**v)
+    for s in spec["search"]:
+        k.set_search(**s)
+    return k

@@ -770,7 +770,9 @@ typedef struct {
 } ctx_t;
 
 static void rot_pt(const double *R, int ndim, const double *x, double *y) {
-  if (ndim == 2) {
+  if (ndim == 1) {
+    y[0] = R[0] * x[0];
+  } else if (ndim == 2) {
     y[0] = R[0] * x[0] + R[3] * x[1];
     y[1] = R[1] * x[0] + R[4] * x[1];
   } else {
@@ -1007,8 +1009,8 @@ int okr_cov(const okr_problem *P, int n, const double *lags, double *out) {
 int okr_solve(const okr_problem *P, okr_result *R) {
   R->errmsg[0] = 0;
   memset(R->stats, 0, sizeof R->stats);
-  if (P->ndim < 2 || P->ndim > 3 || P->nobs < 1) {
-    snprintf(R->errmsg, sizeof R->errmsg, "bad ndim/nobs");
+  if (P->ndim < 1 || P->ndim > 3 || P->nobs < 1 || (P->ndim == 1 && P->sector)) {
+    snprintf(R->errmsg, sizeof R->errmsg, "bad ndim/nobs (sector search needs ndim >= 2)");
     return 1;
   }
   model_t M;
