@@ -1,0 +1,186 @@
+"""Benchmark: kriged nodes/s for 3-D ordinary kriging of 4,194,304 nodes on one
+MI355X, FP32 factorisation + FP64 refinement against the plain FP64 GPU path.
+
+Workload (krigekit_amd.synth.ok3d): 50,000 observations from 20 Gaussian
+clusters in a 12.8 km x 12.8 km x 320 m box; 256 x 256 x 64 nodes at
+(50, 50, 5) m; anisotropic exponential variogram (sill 1.0, nugget 0.02,
+ranges 2000/800/50 m, azimuth 30 deg); octant search, at most 4 per octant
+(<= 32 neighbours out of the 128 nearest).  One step = one full pass of the
+hot path (neighbour search, bucketing, per-node solve, estimate, variance)
+over all nodes, with observations, tree, tables and outputs resident in HBM.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "kriged nodes/s, 3D OK 4.2M nodes: FP32+refine vs FP64 GPU at 1e-6 rel err"
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_search_bytes(nnear):
+    """per launch of the search kernel: query coordinates (3 x FP64) read,
+    neighbour count, flag and ascending id list (int32) written"""
+    nb = len(nnear)
+    return 24.0 * nb + 8.0 * nb + 4.0 * float(np.sum(nnear))
+
+
+def solve_flops(nnear, c=30, p=1, q=1):
+    """algorithmic flops of assembly + factorisation + solves + outputs per node
+    (covariance c flops per evaluation, p constraint rows, q right-hand sides)"""
+    n = nnear.astype(np.float64)
+    f = c * (n * (n - 1) / 2 + n * q) + n ** 3 / 3 + n ** 2 / 2 + n / 6 + p ** 3 / 3
+    f += 2 * n ** 2 * (p + q) + 2 * n * p ** 2 + 4 * n * p * q + 2 * n * q + 2 * (n + p) * q ** 2
+    return float(np.sum(f))
+
+
+def time_path(spec, precision, steps, warmup):
+    from krigekit_amd.kriging import device_synchronize, from_spec
+    k = from_spec(spec, precision=precision)
+    k.solve()  # uploads inputs, first full pass
+    for _ in range(warmup):
+        k.solve_resident()
+    device_synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        k.solve_resident()
+        phases.append(k.timing)
+    device_synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    k.fetch()
+    est, var = k.get_results(copy=True)
+    nnear, _ = k.neighbours()
+    k.close()
+    agg = {key: float(np.mean([p[key] for p in phases])) for key in phases[0]}
+    return wall, agg, est, var, nnear
+
+
+def rel_stats(x, ref, nnear, floor=1e-4):
+    ok = nnear != 1
+    e = np.abs(x[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), floor)
+    return dict(max=float(np.max(e)), p99=float(np.percentile(e, 99)))
+
+
+def cpu_baseline(spec, stride=4):
+    """the compiled reference engine on every `stride`-th node, all host cores"""
+    nthread = min(os.cpu_count() or 1, 16)
+    sub = dict(spec)
+    sub["grid"] = spec["grid"][::stride]
+    n = len(sub["grid"])
+    try:
+        from oracle import refdrv
+        if not refdrv.ref_available("table"):
+            raise FileNotFoundError
+        from oracle.refdrv import RefKriging
+        k = RefKriging(ndim=3)
+        o = sub["obs"][0]
+        k.set_obs(1, o["coord"], o["value"], nmax=o["nmax"])
+        k.set_grid(sub["grid"])
+        for v in sub["vgm"]:
+            k.set_vgm(**v)
+        for s in sub["search"]:
+            k.set_search(**s)
+        t = time.perf_counter()
+        k.solve(nthread=nthread)
+        t = time.perf_counter() - t
+        k.close()
+        kind = "reference"
+    except (FileNotFoundError, OSError):
+        from oracle import okrige
+        t = time.perf_counter()
+        okrige.solve(sub, nthread=nthread)
+        t = time.perf_counter() - t
+        kind = "port"
+    return dict(value=n / t, unit="nodes/s", cores=nthread, kind=kind,
+                sample=f"every {stride}th node of the 256x256x64 grid ({n} nodes), solve() only, "
+                       f"kd-tree built beforehand; {t:.2f} s wall")
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "search_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
+                 "(nodes shard independently; see DESIGN.md).")
+    from krigekit_amd import synth
+    spec = synth.ok3d()
+    nb = len(spec["grid"])
+    w64, t64, e64, v64, nn = time_path(spec, "fp64", args.steps, args.warmup)
+    w32, t32, e32, v32, _ = time_path(spec, "fp32r", args.steps, args.warmup)
+    value = nb / w32
+    search_ms = t32["fast_search_kernel_ms"]
+    achieved = algorithmic_search_bytes(nn) / (search_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    flops = solve_flops(nn)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": w32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factorisation + f64 refinement (f64 assembly, distances, outputs)",
+        "data": "synthetic (seeded clustered observations, regular target grid)",
+        "config": {"workload": "3-D ordinary kriging, octant search (<=4 per octant, 128 candidates)",
+                   "nodes": nb, "grid": [256, 256, 64], "obs": 50000,
+                   "variogram": "exponential sill 1.0 nugget 0.02 ranges 2000/800/50 m azimuth 30",
+                   "covariance": "lookup table (reference default)", "parallelism": "1 GPU"},
+        "fp64_nodes_per_s": nb / w64,
+        "fp64_ms_per_step": w64 * 1e3,
+        "speedup_fp32r_vs_fp64": w64 / w32,
+        "solve_kernel_speedup_fp32r_vs_fp64": t64["solve_ms"] / t32["solve_ms"],
+        "err_fp32r_vs_fp64": {"estimate": rel_stats(e32, e64, nn), "variance": rel_stats(v32, v64, nn),
+                              "floor": 1e-4},
+        "phase_ms_fp32r": {k: round(v, 4) for k, v in t32.items()},
+        "phase_ms_fp64": {k: round(v, 4) for k, v in t64.items()},
+        "roofline": {
+            "kernel": "k_search_fast (octant neighbour search)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        },
+        "solve_roofline": {
+            "kernel": "k_solve (assembly + LDL^T + forward solves, FP64)",
+            "bound": "valu_fp64",
+            "achieved": flops / (t64["solve_ms"] * 1e-3) / 1e12,
+            "peak": FP64_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": flops / (t64["solve_ms"] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

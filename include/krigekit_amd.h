@@ -139,13 +139,20 @@ int krige_set_weights(int64_t handle, int nmax, int ngroups, int nvar, int nbloc
  * pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64;
  * table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic */
 int kk_set_options(int64_t handle, int precision, double pivot_tol, int table_mode);
+/* fast = 1 (default): two-pass LDS search, exact kernel only for undecided
+ * queries; fast = 0: exact-emulation search kernel for every query */
+int kk_set_search_mode(int64_t handle, int fast);
 /* solve with inputs and results left resident in HBM (no result copy-back) */
 int kk_solve_resident(int64_t handle);
 /* copy the results of the last kk_solve_resident to the host */
 int kk_fetch(int64_t handle);
-/* out[8]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
- * nodes, LU-fallback nodes, reserved */
+/* out[12]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
+ * nodes, LU-fallback nodes, queries decided by the exact search kernel, fast
+ * search used, fast search kernel ms, exact search kernel ms, reserved.
+ * Times come from HIP events on the stream the kernels run on. */
 int kk_get_timing(int64_t handle, double *out);
+/* hipDeviceSynchronize */
+int kk_device_synchronize(void);
 /* neighbour lists of the last solve (0-based ids ascending, row stride mmax) */
 int kk_get_mmax(int64_t handle, int *mmax);
 int kk_get_neighbours(int64_t handle, int nblock, int mmax, int *nnear, int *inear);

@@ -460,18 +460,20 @@ struct Krige {
   std::vector<int> nnear, inear;          // weight-store view (0-based ids)
   int mmax = 0;
   int stats[5] = {0, 0, 0, 0, 0};
-  double timing[8] = {0};
+  double timing[16] = {0};
   std::string info;
   // device residency
   bool dev_inputs_ok = false;
+  bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model;
+  DBuf d_model, d_sflag;
+  bool fast_search = true;
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
   Model model{};
   std::vector<double> tab;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {nullptr};
+  hipEvent_t ev[12] = {nullptr};
   ~Krige() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -528,17 +530,20 @@ bool stage_inputs(Krige& K) {
   const char* W = "t_kriging_base%solve_base";
   Obs& O = K.obs[0];
   const int nd = K.ndim;
-  if (!build_model(K.vgm[0], nd, K.table_mode, K.model, K.tab)) {
-    set_error(W, "invalid variogram model");
-    return false;
+  if (!K.model_ok) {
+    if (!build_model(K.vgm[0], nd, K.table_mode, K.model, K.tab)) {
+      set_error(W, "invalid variogram model");
+      return false;
+    }
+    if (!upload(K.d_tab, K.tab, W)) return false;
+    K.model.tab = K.tab.empty() ? nullptr : K.d_tab.as<double>();
+    if (!K.d_model.ensure(sizeof(Model))) {
+      set_error(W, "device allocation failed");
+      return false;
+    }
+    HIPCHK(hipMemcpy(K.d_model.p, &K.model, sizeof(Model), hipMemcpyHostToDevice), W);
+    K.model_ok = true;
   }
-  if (!upload(K.d_tab, K.tab, W)) return false;
-  K.model.tab = K.tab.empty() ? nullptr : K.d_tab.as<double>();
-  if (!K.d_model.ensure(sizeof(Model))) {
-    set_error(W, "device allocation failed");
-    return false;
-  }
-  HIPCHK(hipMemcpy(K.d_model.p, &K.model, sizeof(Model), hipMemcpyHostToDevice), W);
   if (K.dev_inputs_ok) return true;
   if (!upload(K.d_ocoord, O.coord, W) || !upload(K.d_ovalue, O.value, W) || !upload(K.d_ovar, O.var, W))
     return false;
@@ -622,20 +627,47 @@ bool run_pipeline(Krige& K) {
     sa.tree.nd = K.d_nd.as<double>();
     sa.sort = sector;
     sa.k = sector ? std::min(O.n, (1 << nd) * nmax * 4) : nmax;
-    const int chunk = std::min(nb, 1 << 20);
-    if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
-        !K.d_hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
-      set_error(W, "device allocation failed (search scratch)");
-      return false;
+    const size_t lds = kk::fast_search_lds(nd, sector, sa.k, nmax);
+    int nexact = nb;
+    const int* exact_list = nullptr;
+    if (K.fast_search && lds <= 64 * 1024) {
+      if (!K.d_sflag.ensure(sizeof(int) * nb)) {
+        set_error(W, "device allocation failed (search flags)");
+        return false;
+      }
+      sa.qflag = K.d_sflag.as<int>();
+      sa.q0 = 0;
+      sa.nq = nb;
+      HIPCHK(hipEventRecord(K.ev[8], st), W);
+      HIPCHK(kk::launch_search_fast(sa, st), W);
+      HIPCHK(hipEventRecord(K.ev[9], st), W);
+      HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_lists.as<int>(), st), W);
+      HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
+      HIPCHK(hipStreamSynchronize(st), W);
+      exact_list = K.d_lists.as<int>();
     }
-    sa.hdis = K.d_hdis.as<double>();
-    sa.hidx = K.d_hidx.as<int>();
-    sa.hstride = chunk;
-    for (int q0 = 0; q0 < nb; q0 += chunk) {
-      sa.q0 = q0;
-      sa.nq = std::min(chunk, nb - q0);
-      HIPCHK(kk::launch_search(sa, st), W);
+    K.timing[7] = exact_list ? nexact : -1;
+    K.timing[8] = exact_list ? 1.0 : 0.0;  // marks that ev[8]/ev[9] hold the fast kernel
+    HIPCHK(hipEventRecord(K.ev[10], st), W);
+    if (nexact > 0) {
+      const int chunk = std::min(nexact, 1 << 20);
+      if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
+          !K.d_hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
+        set_error(W, "device allocation failed (search scratch)");
+        return false;
+      }
+      sa.hdis = K.d_hdis.as<double>();
+      sa.hidx = K.d_hidx.as<int>();
+      sa.hstride = chunk;
+      for (int q0 = 0; q0 < nexact; q0 += chunk) {
+        sa.q0 = exact_list ? 0 : q0;
+        sa.qlist = exact_list ? exact_list + q0 : nullptr;
+        sa.nq = std::min(chunk, nexact - q0);
+        HIPCHK(kk::launch_search(sa, st), W);
+      }
     }
+    HIPCHK(hipEventRecord(K.ev[11], st), W);
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
   // ---- bucket by neighbour count -------------------------------------------
@@ -729,6 +761,15 @@ bool run_pipeline(Krige& K) {
   K.timing[3] = ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
   K.timing[4] = ms;
+  K.timing[9] = K.timing[10] = 0.0;
+  if (K.timing[8] > 0.0) {
+    HIPCHK(hipEventElapsedTime(&ms, K.ev[8], K.ev[9]), W);
+    K.timing[9] = ms;
+  }
+  if (O.need_search) {
+    HIPCHK(hipEventElapsedTime(&ms, K.ev[10], K.ev[11]), W);
+    K.timing[10] = ms;
+  }
   K.stats[0] = 0;
   K.stats[1] = nb - counts[0] - nfb;
   K.stats[2] = 0;
@@ -1041,7 +1082,7 @@ int krige_initialize(int64_t h, int ndim, int nvar, int ndrift, int unbias, int 
   const double ptol = K->pivot_tol;
   const bool tmode = K->table_mode;
   hipStream_t st = K->stream;
-  hipEvent_t ev[8];
+  hipEvent_t ev[12];
   std::memcpy(ev, K->ev, sizeof ev);
   K->stream = nullptr;
   std::memset(K->ev, 0, sizeof K->ev);
@@ -1170,6 +1211,7 @@ int krige_reset_vgm(int64_t h, int ivar, int jvar) {
   if (ivar < 1 || jvar < 1 || ivar > K->nvar || jvar > K->nvar) return 0;
   K->vgm[(ivar - 1) + (size_t)K->nvar * (jvar - 1)].clear();
   K->vgm[(jvar - 1) + (size_t)K->nvar * (ivar - 1)].clear();
+  K->model_ok = false;
   return 0;
 }
 
@@ -1210,6 +1252,7 @@ int krige_set_vgm(int64_t h, int ivar, int jvar, const char* vtype, double nugge
   v.push_back(s);
   if (ivar != jvar) K->vgm[(jvar - 1) + (size_t)K->nvar * (ivar - 1)].push_back(s);
   K->solved = false;
+  K->model_ok = false;
   return 0;
 }
 
@@ -1671,6 +1714,17 @@ int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
   K->pivot_tol = pivot_tol;
   K->table_mode = table_mode != 0;
   K->solved = false;
+  K->model_ok = false;
+  return 0;
+}
+
+// fast = 0 forces the exact-emulation search kernel for every query
+int kk_set_search_mode(int64_t h, int fast) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  K->fast_search = fast != 0;
+  K->solved = false;
   return 0;
 }
 
@@ -1692,13 +1746,24 @@ int kk_fetch(int64_t h) {
   return ierr();
 }
 
-// out[0..7]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
-// FP64 re-solved nodes, LU-fallback nodes, reserved
+// out[0..11]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
+// FP64 re-solved nodes, LU-fallback nodes, queries decided by the exact
+// search kernel (-1 when the fast search was not used), fast search used
+// (1/0), fast search kernel ms, exact search kernel ms, reserved
 int kk_get_timing(int64_t h, double* out) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  for (int i = 0; i < 8; ++i) out[i] = K->timing[i];
+  for (int i = 0; i < 12; ++i) out[i] = K->timing[i];
+  return 0;
+}
+
+int kk_device_synchronize(void) {
+  clear_error();
+  if (hipDeviceSynchronize() != hipSuccess) {
+    set_error("kk_device_synchronize", "hipDeviceSynchronize failed");
+    return ierr();
+  }
   return 0;
 }
 

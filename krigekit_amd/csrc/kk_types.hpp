@@ -73,6 +73,8 @@ struct SearchArgs {
   int hstride;
   int* nnear;                 // [nblock]
   int* inear;                 // [nblock * mmax], 0-based obs ids, ascending
+  const int* qlist;           // exact kernel: query ids to process (nullptr: q0..q0+nq)
+  int* qflag;                 // fast kernel: 1 where the exact kernel must decide
 };
 
 struct SolveArgs {

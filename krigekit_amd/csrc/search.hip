@@ -24,6 +24,8 @@
 // every insert, is read by a wave as one contiguous segment.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kk_device.hpp"
 #include "kk_types.hpp"
 
@@ -143,74 +145,40 @@ __device__ inline void isort(int* a, int n) {
   }
 }
 
-template <int DIM>
-__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nq) return;
-  const int qid = a.q0 + t;
-  double q[DIM];
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
-  int* out = a.inear + (size_t)qid * a.mmax;
-  const Tree& T = a.tree;
-  Heap h{a.hdis + t, a.hidx + t, (size_t)a.hstride};
-
-  // ---- kdtree2_n_nearest: stackless traversal -------------------------
-  double ballsize = 1.7976931348623157e308;  // huge(1.0)
-  int nfound = 0;
-  const int nn = a.k;
+// Stackless kdtree2 traversal (kdtree2_maxidx.f90:1383-1457): closer child
+// first; the farther child is visited when its box lower bound is <= the
+// current ball.  leaf(l, u) processes a bucket and may shrink `ball`.
+template <int DIM, class Leaf>
+__device__ inline void traverse(const Tree& T, const double* q, double& ball, Leaf&& leaf) {
   int node = T.root, prev = -1;
   while (node >= 0) {
     const int* ni = T.ni + (size_t)node * NODE_I;
     const int cut = ni[0], left = ni[3], right = ni[4], parent = ni[5];
     if (prev == parent) {
-      // arriving from above
       if (left < 0 || right < 0) {
-        // process_terminal_node, kdtree2_maxidx.f90:1510-1613
-        const int l = ni[1], u = ni[2];
-        for (int i = l; i <= u; ++i) {
-          double sd = 0.0;
-          bool skip = false;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) {
-            sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
-            if (sd > ballsize) {
-              skip = true;
-              break;
-            }
-          }
-          if (skip) continue;
-          const int idx = T.ind[i];
-          if (nfound < nn) {
-            double newpri = pq_insert(h, nfound, sd, idx);
-            if (nfound == nn) ballsize = newpri;
-          } else {
-            ballsize = pq_replace_max(h, nn, sd, idx);
-          }
-        }
+        leaf(ni[1], ni[2]);
         prev = node;
         node = parent;
         continue;
       }
       const double cv = T.nd[(size_t)node * NODE_D + 0];
       prev = node;
-      node = (q[cut] < cv) ? left : right;  // closer child
+      node = (q[cut] < cv) ? left : right;
       continue;
     }
-    // returning from a child
     const double* nd = T.nd + (size_t)node * NODE_D;
     const bool goes_left = q[cut] < nd[0];
     const int closer = goes_left ? left : right;
     if (prev == closer) {
       const int farther = goes_left ? right : left;
       double dis = goes_left ? sq(__dsub_rn(nd[2], q[cut])) : sq(__dsub_rn(nd[1], q[cut]));
-      bool visit = dis <= ballsize;
+      bool visit = dis <= ball;
       if (visit) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
           if (i != cut && visit) {
             dis = __dadd_rn(dis, dis2_bnd(q[i], nd[3 + i], nd[6 + i]));
-            if (dis > ballsize) visit = false;
+            if (dis > ball) visit = false;
           }
         }
       }
@@ -223,8 +191,61 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     prev = node;
     node = parent;
   }
+}
 
-  // ---- candidate list -> neighbour list --------------------------------
+// squared distance accumulated like process_terminal_node, with early exit
+template <int DIM>
+__device__ inline bool point_sd(const Tree& T, int i, const double* q, double ball, double& sd) {
+  sd = 0.0;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
+    if (sd > ball) return false;
+  }
+  return true;
+}
+
+template <int DIM>
+__device__ inline int octant(const double* o, const double* q) {
+  int sec = (__dsub_rn(o[0], q[0]) < 0.0);
+  if (DIM >= 2) sec += 2 * (__dsub_rn(o[1], q[1]) < 0.0);
+  if (DIM == 3) sec += 4 * (__dsub_rn(o[2], q[2]) < 0.0);
+  return sec;
+}
+
+// Exact emulation of the reference search for one query per lane (heap in
+// HBM scratch).  Runs over every query, or over the queries the fast kernel
+// could not decide (a.qlist).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int qid = a.qlist ? a.qlist[t] : a.q0 + t;
+  double q[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
+  int* out = a.inear + (size_t)qid * a.mmax;
+  const Tree& T = a.tree;
+  Heap h{a.hdis + t, a.hidx + t, (size_t)a.hstride};
+
+  double ballsize = 1.7976931348623157e308;  // huge(1.0)
+  int nfound = 0;
+  const int nn = a.k;
+  traverse<DIM>(T, q, ballsize, [&](int l, int u) {
+    // process_terminal_node, kdtree2_maxidx.f90:1510-1613
+    for (int i = l; i <= u; ++i) {
+      double sd;
+      if (!point_sd<DIM>(T, i, q, ballsize, sd)) continue;
+      const int idx = T.ind[i];
+      if (nfound < nn) {
+        double newpri = pq_insert(h, nfound, sd, idx);
+        if (nfound == nn) ballsize = newpri;
+      } else {
+        ballsize = pq_replace_max(h, nn, sd, idx);
+      }
+    }
+  });
+
   int cnt = 0;
   if (a.sort) {
     heapsort(h, nn);
@@ -237,10 +258,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       const int idx = h.I(c);
       if (idx == skip_idx) continue;
       if (h.D(c) > a.maxdist2) continue;
-      const double* o = a.ocoord + DIM * (size_t)idx;
-      int sec = (__dsub_rn(o[0], q[0]) < 0.0);
-      if (DIM >= 2) sec += 2 * (__dsub_rn(o[1], q[1]) < 0.0);
-      if (DIM == 3) sec += 4 * (__dsub_rn(o[2], q[2]) < 0.0);
+      const int sec = octant<DIM>(a.ocoord + DIM * (size_t)idx, q);
       int cs = 0;
 #pragma unroll
       for (int s = 0; s < (1 << DIM); ++s) cs = (s == sec) ? counts[s] : cs;
@@ -251,7 +269,6 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
     }
   } else {
-    // plain k-nearest in heap order, then self exclusion and maxdist
     bool removed = !a.cv;
     for (int c = 1; c <= nn; ++c) {
       const int idx = h.I(c);
@@ -264,6 +281,150 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   }
   isort(out, cnt);
   a.nnear[qid] = cnt;
+}
+
+__device__ inline float f_next(float f) { return __int_as_float(__float_as_int(f) + 1); }
+__device__ inline float f_prev(float f) { return f > 0.0f ? __int_as_float(__float_as_int(f) - 1) : 0.0f; }
+
+// Fast search, one query per lane, all per-lane state in LDS (lane-
+// interleaved words, no cross-lane traffic).
+//
+// Pass 1: f = the k-th smallest of RN_fp32(d^2) over all observations, with a
+//   float max-heap of k keys.  Rounding is monotonic, so f = RN_fp32(r_k)
+//   exactly, where r_k is the k-th smallest FP64 squared distance.
+// Pass 2: fixed-radius traversal.  Points with key < prev(f) are certainly
+//   among the k nearest; points with key in [prev(f), next(f)] form the
+//   boundary set.  If (#certain + #boundary) == k the k-nearest set is known
+//   exactly; otherwise equal or adjacent distances at rank k make the answer
+//   depend on the reference's visit order, and the query is flagged for the
+//   exact kernel.  In sector mode each octant keeps its nmax+1 nearest (exact
+//   FP64 distances); an exact tie between rank nmax and nmax+1 of an octant
+//   (where the reference's unstable heapsort decides) also flags the query.
+template <int DIM, bool SECTOR>
+__global__ __launch_bounds__(64) void k_search_fast(SearchArgs a) {
+  extern __shared__ double lds_d[];
+  const int lane = threadIdx.x;
+  const int t = blockIdx.x * 64 + lane;
+  if (t >= a.nq) return;
+  const int qid = a.q0 + t;
+  double q[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
+  const Tree& T = a.tree;
+  const int K = a.k;
+  float* hk = reinterpret_cast<float*>(lds_d);
+  auto HK = [&](int p) -> float& { return hk[(p - 1) * 64 + lane]; };
+
+  // ---- pass 1: k-th smallest FP32 key ---------------------------------
+  double ball = 1.7976931348623157e308;
+  int nf = 0;
+  traverse<DIM>(T, q, ball, [&](int l, int u) {
+    for (int i = l; i <= u; ++i) {
+      double sd;
+      if (!point_sd<DIM>(T, i, q, ball, sd)) continue;
+      const float key = (float)sd;
+      if (nf < K) {
+        int c = ++nf;
+        while (c > 1) {
+          const int pp = c >> 1;
+          const float pk = HK(pp);
+          if (key > pk) {
+            HK(c) = pk;
+            c = pp;
+          } else {
+            break;
+          }
+        }
+        HK(c) = key;
+        if (nf == K) ball = (double)f_next(HK(1));
+      } else if (key < HK(1)) {
+        int par = 1, ch = 2;
+        while (ch <= K) {
+          float ck = HK(ch);
+          if (ch < K) {
+            const float c1 = HK(ch + 1);
+            if (ck < c1) {
+              ++ch;
+              ck = c1;
+            }
+          }
+          if (key >= ck) break;
+          HK(par) = ck;
+          par = ch;
+          ch = 2 * par;
+        }
+        HK(par) = key;
+        ball = (double)f_next(HK(1));
+      }
+    }
+  });
+  const float f = HK(1);
+  const float flo = f_prev(f), fhi = f_next(f);
+  double bound = (double)f_next(fhi);
+
+  // ---- pass 2: exact collection ------------------------------------------
+  constexpr int NOCT = 1 << DIM;
+  const int C = a.nmax + 1;  // per-octant capacity (sector mode)
+  int* ocnt = reinterpret_cast<int*>(lds_d);
+  double* od = lds_d + (size_t)(NOCT / 2 + (NOCT & 1)) * 64;
+  int* oi = reinterpret_cast<int*>(od + (size_t)NOCT * C * 64);
+  if (SECTOR) {
+#pragma unroll
+    for (int s = 0; s < NOCT; ++s) ocnt[s * 64 + lane] = 0;
+  }
+  int* out = a.inear + (size_t)qid * a.mmax;
+  int nin = 0, nb = 0, cnt = 0;
+  bool flag = false;
+  traverse<DIM>(T, q, bound, [&](int l, int u) {
+    for (int i = l; i <= u; ++i) {
+      double sd;
+      if (!point_sd<DIM>(T, i, q, bound, sd)) continue;
+      const float key = (float)sd;
+      if (key > fhi) continue;
+      if (key < flo) ++nin;
+      else ++nb;
+      const int idx = T.ind[i];
+      if (a.cv && idx == qid) continue;
+      if (sd > a.maxdist2) continue;
+      if (SECTOR) {
+        const int s = octant<DIM>(a.ocoord + DIM * (size_t)idx, q);
+        const int c = ocnt[s * 64 + lane];
+        double* dd = od + (size_t)s * C * 64 + lane;
+        int* ii = oi + (size_t)s * C * 64 + lane;
+        if (c == C && sd >= dd[(size_t)(C - 1) * 64]) continue;
+        int j = c < C ? c : C - 1;
+        while (j > 0 && dd[(size_t)(j - 1) * 64] > sd) {
+          dd[(size_t)j * 64] = dd[(size_t)(j - 1) * 64];
+          ii[(size_t)j * 64] = ii[(size_t)(j - 1) * 64];
+          --j;
+        }
+        dd[(size_t)j * 64] = sd;
+        ii[(size_t)j * 64] = idx;
+        ocnt[s * 64 + lane] = c < C ? c + 1 : C;
+      } else {
+        if (cnt < a.mmax) out[cnt++] = idx;
+        else flag = true;
+      }
+    }
+  });
+  if (nin + nb != K) flag = true;
+  if (SECTOR) {
+    for (int s = 0; s < NOCT; ++s) {
+      const int c = ocnt[s * 64 + lane];
+      const double* dd = od + (size_t)s * C * 64 + lane;
+      const int* ii = oi + (size_t)s * C * 64 + lane;
+      if (c == C && dd[(size_t)(C - 2) * 64] == dd[(size_t)(C - 1) * 64]) flag = true;
+      const int take = c < a.nmax ? c : a.nmax;
+      for (int j = 0; j < take; ++j) out[cnt++] = ii[(size_t)j * 64];
+    }
+  }
+  if (flag) {
+    a.qflag[qid] = 1;
+    return;
+  }
+  isort(out, cnt);
+  a.nnear[qid] = cnt;
+  a.qflag[qid] = 0;
 }
 
 // every observation is a neighbour (nmax >= nobs): kriging.F90:640-657
@@ -295,6 +456,26 @@ __global__ __launch_bounds__(256) void k_search_all(SearchArgs a, int nobs, cons
     if (s <= a.maxdist2) out[cnt++] = i;
   }
   a.nnear[qid] = cnt;
+}
+
+size_t fast_search_lds(int ndim, bool sector, int k, int nmax) {
+  const int noct = 1 << ndim;
+  const int words = sector ? std::max(k, 2 * ((noct + 1) / 2) + 3 * noct * (nmax + 1)) : k;
+  return (size_t)64 * 4 * (size_t)words;
+}
+
+hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st) {
+  const size_t lds = fast_search_lds(a.ndim, a.sort != 0, a.k, a.nmax);
+  dim3 blk(64), grd((a.nq + 63) / 64);
+  if (a.sort) {
+    if (a.ndim == 2) hipLaunchKernelGGL((k_search_fast<2, true>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_fast<3, true>), grd, blk, lds, st, a);
+  } else {
+    if (a.ndim == 1) hipLaunchKernelGGL((k_search_fast<1, false>), grd, blk, lds, st, a);
+    else if (a.ndim == 2) hipLaunchKernelGGL((k_search_fast<2, false>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_fast<3, false>), grd, blk, lds, st, a);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_search(const SearchArgs& a, hipStream_t st) {

@@ -106,7 +106,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       rhs = rhs + cov_eval<DIM, TB>(M, dx, dy, dz) * w;
     }
   }
-  const double f = (live && a.unbias) ? 1.0 : 0.0;
+  // ones on the live rows: the unbias row F for ordinary kriging, and the
+  // weight sum 1^T K^-1 b that simple kriging needs for its mean term
+  const double f = live ? 1.0 : 0.0;
   __syncthreads();
   double A[G];
 #pragma unroll
@@ -248,6 +250,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 
   fail = gmax<G>(fail);
   if (!act || r != 0) return;
+  if (n == 1) {
+    // the reference skips the solve for a single neighbour (solve_base,
+    // kriging_base.F90:2127): variance keeps its NaN fill; the estimate would
+    // reuse another block's weights and is reported as NaN
+    a.est[b] = __builtin_nan("");
+    a.var[b] = __builtin_nan("");
+    a.flag[b] = 0;
+    return;
+  }
   if (fail & 1) {
     a.flag[b] = 2;  // not positive definite: LU fallback
     return;
@@ -285,19 +296,39 @@ __global__ void k_empty(SolveArgs a) {
   a.flag[b] = 0;
 }
 
+// One atomic per wave and bucket: lanes of a wave agree on their offsets
+// through a ballot, so 4M blocks cost ~65k atomics instead of 4M.
 __global__ void k_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblock) return;
-  const int n = nnear[t];
-  const int bk = n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
-  const int pos = atomicAdd(&counts[bk], 1);
-  lists[(size_t)bk * cap + pos] = t;
+  const int lane = threadIdx.x & 63;
+  const bool valid = t < nblock;
+  const int n = valid ? nnear[t] : 0;
+  const int bk = !valid ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int b = 0; b <= NBUCKET; ++b) {
+    const unsigned long long m = __ballot(bk == b);
+    if (m == 0) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counts[b], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (bk == b) lists[(size_t)b * cap + base + __popcll(m & below)] = t;
+  }
 }
 
 __global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblock) return;
-  if (flag[t] == code) list[atomicAdd(count, 1)] = t;
+  const int lane = threadIdx.x & 63;
+  const bool hit = t < nblock && flag[t] == code;
+  const unsigned long long m = __ballot(hit);
+  if (m == 0) return;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (hit) list[base + __popcll(m & below)] = t;
 }
 
 // Dense LU with partial pivoting on the full augmented system, one thread per

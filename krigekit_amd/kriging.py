@@ -73,11 +73,13 @@ def _load_lib():
     sig("krige_get_weight_nnear", [_i64, _c_int, _c_int, _pi])
     sig("krige_get_weight_inear", [_i64, _c_int, _c_int, _c_int, _pi])
     sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
+    sig("kk_set_search_mode", [_i64, _c_int])
     sig("kk_solve_resident", [_i64])
     sig("kk_fetch", [_i64])
     sig("kk_get_timing", [_i64, _pd])
     sig("kk_get_neighbours", [_i64, _c_int, _c_int, _pi, _pi])
     sig("kk_get_mmax", [_i64, _pi])
+    sig("kk_device_synchronize", [])
     _lib = lib
     return lib
 
@@ -272,10 +274,16 @@ class Kriging:
 
     @property
     def timing(self) -> dict:
-        out = np.zeros(8)
+        out = np.zeros(12)
         _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
-                    total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]))
+                    total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]),
+                    exact_search_queries=int(out[7]), fast_search_kernel_ms=out[9],
+                    exact_search_kernel_ms=out[10])
+
+    def set_search_mode(self, fast: bool = True):
+        """fast=False runs the exact-emulation search kernel for every query."""
+        _check(_load_lib().kk_set_search_mode(self._handle, int(fast)), "kk_set_search_mode")
 
     def neighbours(self):
         """(nnear[nblock], inear[nblock, mmax]) of the last solve; 1-based ids, ascending."""
@@ -363,10 +371,15 @@ class Kriging:
             pass
 
 
+def device_synchronize():
+    _check(_load_lib().kk_device_synchronize(), "kk_device_synchronize")
+
+
 def from_spec(spec, **kw):
     """Builds a ``Kriging`` object from a workload description
     (``krigekit_amd.synth``) and returns it ready to ``solve()``."""
-    k = Kriging(ndim=spec["ndim"], cross_validation=bool(spec.get("cross_validation", False)), **kw)
+    k = Kriging(ndim=spec["ndim"], unbias=spec.get("unbias", 1),
+                cross_validation=bool(spec.get("cross_validation", False)), **kw)
     for iv, o in enumerate(spec["obs"], start=1):
         k.set_obs(iv, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))

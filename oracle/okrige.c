@@ -869,6 +869,16 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
     return 0;
   }
   isort(inear, n);
+  if (n == 1) {
+    /* solve_base (kriging_base.F90:2127) skips the solve when npp == 1: the
+     * variance keeps its NaN fill, the estimate would reuse the weights the
+     * same OpenMP thread computed for its previous block.  Both are reported
+     * as NaN here (the estimate deliberately: the reference value depends on
+     * thread scheduling). */
+    R->estimate[ib] = NAN;
+    R->variance[ib] = NAN;
+    return 0;
+  }
   double rs = P->rangescale ? P->rangescale[ib] : 1.0;
   double ln = P->localnugget ? P->localnugget[ib] : 0.0;
   int p = P->unbias ? 1 : 0, m = n + p;
