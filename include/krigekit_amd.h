@@ -139,9 +139,11 @@ int krige_set_weights(int64_t handle, int nmax, int ngroups, int nvar, int nbloc
  * pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64;
  * table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic */
 int kk_set_options(int64_t handle, int precision, double pivot_tol, int table_mode);
-/* fast = 1 (default): two-pass LDS search, exact kernel only for undecided
- * queries; fast = 0: exact-emulation search kernel for every query */
-int kk_set_search_mode(int64_t handle, int fast);
+/* mode 2 (default): wave-cooperative two-pass LDS search, mode 1: per-lane
+ * two-pass LDS search (both run the exact kernel only for the queries they
+ * flag), mode 0: exact-emulation search kernel for every query.
+ * reorder = 1: the wave-cooperative search visits queries in Morton order. */
+int kk_set_search_mode(int64_t handle, int mode, int reorder);
 /* solve with inputs and results left resident in HBM (no result copy-back) */
 int kk_solve_resident(int64_t handle);
 /* copy the results of the last kk_solve_resident to the host */

@@ -467,8 +467,10 @@ struct Krige {
   bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag;
-  bool fast_search = true;
+  DBuf d_model, d_sflag, d_perm;
+  int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
+  bool reorder = true;          // Morton order of queries for the wave-cooperative search
+  bool perm_ok = false;
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
   Model model{};
   std::vector<double> tab;
@@ -576,6 +578,43 @@ bool stage_inputs(Krige& K) {
   return true;
 }
 
+// Morton (Z-order) permutation of the query points, so that the 64 queries
+// of a wave are spatially compact; built once per grid on the host.
+bool morton_perm(Krige& K) {
+  if (K.perm_ok) return true;
+  const char* W = "t_kriging_base%solve_base";
+  const std::vector<double>& bc = K.cross_validation ? K.obs[0].coord : K.bcoord;
+  const int nd = K.ndim, nb = K.nblock;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int k = 0; k < nd; ++k) {
+    lo[k] = hi[k] = bc[k];
+    for (int b = 1; b < nb; ++b) {
+      lo[k] = std::min(lo[k], bc[k + (size_t)nd * b]);
+      hi[k] = std::max(hi[k], bc[k + (size_t)nd * b]);
+    }
+  }
+  double ext = 0.0;
+  for (int k = 0; k < nd; ++k) ext = std::max(ext, hi[k] - lo[k]);
+  if (!(ext > 0.0)) ext = 1.0;
+  const int bits = nd == 1 ? 30 : nd == 2 ? 21 : 20;
+  const double scale = (double)((1u << bits) - 1) / ext;
+  std::vector<std::pair<uint64_t, int>> key(nb);
+  for (int b = 0; b < nb; ++b) {
+    uint64_t code = 0;
+    uint32_t c[3] = {0, 0, 0};
+    for (int k = 0; k < nd; ++k) c[k] = (uint32_t)((bc[k + (size_t)nd * b] - lo[k]) * scale);
+    for (int bit = bits - 1; bit >= 0; --bit)
+      for (int k = 0; k < nd; ++k) code = (code << 1) | ((c[k] >> bit) & 1u);
+    key[b] = {code, b};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> perm(nb);
+  for (int b = 0; b < nb; ++b) perm[b] = key[b].second;
+  if (!upload(K.d_perm, perm, W)) return false;
+  K.perm_ok = true;
+  return true;
+}
+
 bool run_pipeline(Krige& K) {
   const char* W = "t_kriging_base%solve_base";
   if (!ensure_stream(K)) return false;
@@ -630,7 +669,7 @@ bool run_pipeline(Krige& K) {
     const size_t lds = kk::fast_search_lds(nd, sector, sa.k, nmax);
     int nexact = nb;
     const int* exact_list = nullptr;
-    if (K.fast_search && lds <= 64 * 1024) {
+    if (K.search_mode > 0 && lds <= 64 * 1024) {
       if (!K.d_sflag.ensure(sizeof(int) * nb)) {
         set_error(W, "device allocation failed (search flags)");
         return false;
@@ -639,7 +678,14 @@ bool run_pipeline(Krige& K) {
       sa.q0 = 0;
       sa.nq = nb;
       HIPCHK(hipEventRecord(K.ev[8], st), W);
-      HIPCHK(kk::launch_search_fast(sa, st), W);
+      if (K.search_mode == 2) {
+        if (K.reorder && !morton_perm(K)) return false;
+        sa.qlist = K.reorder ? K.d_perm.as<int>() : nullptr;
+        HIPCHK(kk::launch_search_wave(sa, st), W);
+        sa.qlist = nullptr;
+      } else {
+        HIPCHK(kk::launch_search_fast(sa, st), W);
+      }
       HIPCHK(hipEventRecord(K.ev[9], st), W);
       HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
       HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_lists.as<int>(), st), W);
@@ -1167,6 +1213,7 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
   Obs& O = K->obs[ivar - 1];
   O = Obs();
   O.n = nobs;
+  K->perm_ok = false;
   O.coord.assign(coord, coord + (size_t)ndim_c * nobs);
   O.value.assign(value, value + nobs);
   O.var.assign(nobs, 0.0);
@@ -1274,6 +1321,7 @@ static int set_grid_common(Krige* K, int nblock, const double* rangescale, const
   if (localnugget && !ln_zero) K->localnugget.assign(localnugget, localnugget + nblock);
   K->cross_validation = false;
   K->dev_inputs_ok = false;
+  K->perm_ok = false;
   K->solved = false;
   return 0;
 }
@@ -1403,6 +1451,7 @@ int krige_set_grid_cv(int64_t h) {
   }
   set_grid_common(K, K->obs[0].n, nullptr, nullptr);
   K->cross_validation = true;
+  K->perm_ok = false;
   K->bcoord.clear();
   K->soff.clear();
   // kriging_base.F90:1167: one extra neighbour so self can be excluded
@@ -1718,12 +1767,19 @@ int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
   return 0;
 }
 
-// fast = 0 forces the exact-emulation search kernel for every query
-int kk_set_search_mode(int64_t h, int fast) {
+// mode 0: exact-emulation search kernel for every query; 1: per-lane two-pass
+// search; 2: wave-cooperative two-pass search (default).  reorder: process
+// queries of the wave-cooperative search in Morton order.
+int kk_set_search_mode(int64_t h, int mode, int reorder) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  K->fast_search = fast != 0;
+  if (mode < 0 || mode > 2) {
+    set_error("kk_set_search_mode", "mode must be 0, 1 or 2");
+    return ierr();
+  }
+  K->search_mode = mode;
+  K->reorder = reorder != 0;
   K->solved = false;
   return 0;
 }

@@ -27,6 +27,8 @@ struct Rot3 {
 hipError_t launch_search(const SearchArgs& a, hipStream_t st);
 // two-pass LDS search; a.qflag marks queries the exact kernel must decide
 hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st);
+// wave-cooperative traversal, same results and flags as launch_search_fast
+hipError_t launch_search_wave(const SearchArgs& a, hipStream_t st);
 size_t fast_search_lds(int ndim, bool sector, int k, int nmax);
 hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord_orig, const Rot3& r,
                              hipStream_t st);

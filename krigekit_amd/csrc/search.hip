@@ -216,8 +216,11 @@ __device__ inline int octant(const double* o, const double* q) {
 // Exact emulation of the reference search for one query per lane (heap in
 // HBM scratch).  Runs over every query, or over the queries the fast kernel
 // could not decide (a.qlist).
-template <int DIM>
+// LDSHEAP: the heap lives in LDS (lane-interleaved, 32-lane workgroups),
+// used when only the queries the fast kernel flagged are left.
+template <int DIM, bool LDSHEAP>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+  extern __shared__ double lds_heap[];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nq) return;
   const int qid = a.qlist ? a.qlist[t] : a.q0 + t;
@@ -226,7 +229,10 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   for (int k = 0; k < DIM; ++k) q[k] = a.qcoord[k + DIM * (size_t)qid];
   int* out = a.inear + (size_t)qid * a.mmax;
   const Tree& T = a.tree;
-  Heap h{a.hdis + t, a.hidx + t, (size_t)a.hstride};
+  Heap h = LDSHEAP ? Heap{lds_heap + threadIdx.x,
+                          reinterpret_cast<int*>(lds_heap + (size_t)a.k * blockDim.x) + threadIdx.x,
+                          (size_t)blockDim.x}
+                   : Heap{a.hdis + t, a.hidx + t, (size_t)a.hstride};
 
   double ballsize = 1.7976931348623157e308;  // huge(1.0)
   int nfound = 0;
@@ -427,6 +433,276 @@ __global__ __launch_bounds__(64) void k_search_fast(SearchArgs a) {
   a.qflag[qid] = 0;
 }
 
+// Wave-cooperative variant of the fast search: the 64 lanes of a wave hold 64
+// spatially coherent queries and walk the tree together.  Control flow is
+// uniform: a node is entered when ANY lane's exact box lower bound is within
+// that lane's current ball (a conservative test for every lane), children
+// are ordered by the wave's first query, and leaf points are read with
+// uniform (scalar) loads and offered to every lane.  Each lane keeps its own
+// LDS heap / octant lists, so its result is the same as the per-lane
+// kernel's; the exactness flags are the same.
+template <int DIM>
+__device__ inline double box_lb(const double* nd, const double* q) {
+  double d = 0.0;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) d = __dadd_rn(d, dis2_bnd(q[k], nd[3 + k], nd[6 + k]));
+  return d;
+}
+
+template <int DIM, class Leaf, class Ball>
+__device__ inline void traverse_wave(const Tree& T, const double* q, const double* q0, Ball&& ball,
+                                     Leaf&& leaf) {
+  int node = T.root, prev = -1;
+  while (node >= 0) {
+    node = __builtin_amdgcn_readfirstlane(node);
+    const int* ni = T.ni + (size_t)node * NODE_I;
+    const int cut = ni[0], left = ni[3], right = ni[4], parent = ni[5];
+    const double* nd = T.nd + (size_t)node * NODE_D;
+    if (prev == parent) {
+      // inflate by 1e-12 relative: rounding in the bound must never prune a
+      // point whose squared distance sits at the ball
+      const bool need = box_lb<DIM>(nd, q) <= ball() * (1.0 + 1e-12);
+      if (__ballot(need) == 0) {
+        prev = node;
+        node = parent;
+        continue;
+      }
+      if (left < 0 || right < 0) {
+        leaf(ni[1], ni[2], node);
+        prev = node;
+        node = parent;
+        continue;
+      }
+      prev = node;
+      node = (q0[cut] < nd[0]) ? left : right;
+      continue;
+    }
+    const int closer = (q0[cut] < nd[0]) ? left : right;
+    const int farther = closer == left ? right : left;
+    if (prev == closer) {
+      prev = node;
+      node = farther;
+      continue;
+    }
+    prev = node;
+    node = parent;
+  }
+}
+
+// Boundary candidates of pass 2 (FP32 key within one ulp of the k-th key),
+// kept sorted by exact FP64 distance in registers.
+constexpr int BCAP = 4;
+struct Boundary {
+  double d[BCAP];
+  int i[BCAP];
+  int n = 0;
+  __device__ void add(double sd, int idx, bool& overflow) {
+    if (n == BCAP) {
+      overflow = true;
+      return;
+    }
+    // insertion into the sorted prefix by compare-select (static indices)
+    double cd = sd;
+    int ci = idx;
+#pragma unroll
+    for (int j = 0; j < BCAP; ++j) {
+      if (j < n) {
+        if (d[j] > cd) {
+          const double td = d[j];
+          const int ti = i[j];
+          d[j] = cd;
+          i[j] = ci;
+          cd = td;
+          ci = ti;
+        }
+      } else if (j == n) {
+        d[j] = cd;
+        i[j] = ci;
+      }
+    }
+    ++n;
+  }
+};
+
+constexpr int LEAFCAP = 1024;  // leaves recorded per wave in pass 1
+
+template <int DIM, bool SECTOR>
+__global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
+  extern __shared__ double lds_d[];
+  __shared__ int leaves[LEAFCAP];
+  const int lane = threadIdx.x;
+  const int t = blockIdx.x * 64 + lane;
+  const bool valid = t < a.nq;
+  const int qid = a.qlist ? a.qlist[valid ? t : a.nq - 1] : a.q0 + (valid ? t : a.nq - 1);
+  double q[DIM], q0[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    q[k] = a.qcoord[k + DIM * (size_t)qid];
+    q0[k] = __builtin_amdgcn_readfirstlane(q[k]);
+  }
+  const Tree& T = a.tree;
+  const int K = a.k;
+  float* hk = reinterpret_cast<float*>(lds_d);
+  auto HK = [&](int p) -> float& { return hk[(p - 1) * 64 + lane]; };
+
+  // ---- pass 1: k-th smallest FP32 key; records the leaves it visits --------
+  // The ball is next(next(max key)) so the recorded leaves cover every point
+  // pass 2 needs (its bound is next(next(f))).
+  double ball = 1.7976931348623157e308;
+  int nf = 0, nleaf = 0;
+  traverse_wave<DIM>(T, q, q0, [&]() { return ball; }, [&](int l, int u, int node) {
+    if (nleaf < LEAFCAP && lane == 0) leaves[nleaf] = node;
+    ++nleaf;
+    for (int i = l; i <= u; ++i) {
+      double sd = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
+      const float key = (float)sd;
+      if (nf < K) {
+        int c = ++nf;
+        while (c > 1) {
+          const int pp = c >> 1;
+          const float pk = HK(pp);
+          if (key > pk) {
+            HK(c) = pk;
+            c = pp;
+          } else {
+            break;
+          }
+        }
+        HK(c) = key;
+        if (nf == K) ball = (double)f_next(f_next(HK(1)));
+      } else if (key < HK(1)) {
+        int par = 1, ch = 2;
+        while (ch <= K) {
+          float ck = HK(ch);
+          if (ch < K) {
+            const float c1 = HK(ch + 1);
+            if (ck < c1) {
+              ++ch;
+              ck = c1;
+            }
+          }
+          if (key >= ck) break;
+          HK(par) = ck;
+          par = ch;
+          ch = 2 * par;
+        }
+        HK(par) = key;
+        ball = (double)f_next(f_next(HK(1)));
+      }
+    }
+  });
+  const float f = HK(1);
+  const float flo = f_prev(f), fhi = f_next(f);
+  const double bound = (double)f_next(fhi);
+
+  // ---- pass 2: exact collection over the recorded leaves -------------------
+  constexpr int NOCT = 1 << DIM;
+  const int C = a.nmax + 1;
+  int* ocnt = reinterpret_cast<int*>(lds_d);
+  double* od = lds_d + (size_t)(NOCT / 2 + (NOCT & 1)) * 64;
+  int* oi = reinterpret_cast<int*>(od + (size_t)NOCT * C * 64);
+  if (SECTOR) {
+#pragma unroll
+    for (int s = 0; s < NOCT; ++s) ocnt[s * 64 + lane] = 0;
+  }
+  int* out = a.inear + (size_t)qid * a.mmax;
+  int nin = 0, cnt = 0;
+  bool flag = false;
+  Boundary B;
+  auto keep = [&](double sd, int idx) {
+    // a point of the k-nearest set: sector caps or the plain list
+    if (a.cv && idx == qid) return;
+    if (sd > a.maxdist2) return;
+    if (SECTOR) {
+      const int s = octant<DIM>(a.ocoord + DIM * (size_t)idx, q);
+      const int c = ocnt[s * 64 + lane];
+      double* dd = od + (size_t)s * C * 64 + lane;
+      int* ii = oi + (size_t)s * C * 64 + lane;
+      if (c == C && sd >= dd[(size_t)(C - 1) * 64]) return;
+      int j = c < C ? c : C - 1;
+      while (j > 0 && dd[(size_t)(j - 1) * 64] > sd) {
+        dd[(size_t)j * 64] = dd[(size_t)(j - 1) * 64];
+        ii[(size_t)j * 64] = ii[(size_t)(j - 1) * 64];
+        --j;
+      }
+      dd[(size_t)j * 64] = sd;
+      ii[(size_t)j * 64] = idx;
+      ocnt[s * 64 + lane] = c < C ? c + 1 : C;
+    } else {
+      if (cnt < a.mmax) out[cnt++] = idx;
+      else flag = true;
+    }
+  };
+  auto visit = [&](int l, int u, int) {
+    for (int i = l; i <= u; ++i) {
+      double sd = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
+      if (!(sd <= bound)) continue;
+      const float key = (float)sd;
+      if (key > fhi) continue;
+      const int idx = T.ind[i];
+      if (key < flo) {
+        ++nin;
+        keep(sd, idx);
+      } else {
+        B.add(sd, idx, flag);
+      }
+    }
+  };
+  __syncthreads();
+  if (nleaf <= LEAFCAP) {
+    for (int j = 0; j < nleaf; ++j) {
+      const int node = __builtin_amdgcn_readfirstlane(leaves[j]);
+      const int* ni = T.ni + (size_t)node * NODE_I;
+      const bool need_leaf = box_lb<DIM>(T.nd + (size_t)node * NODE_D, q) <= bound * (1.0 + 1e-12);
+      if (__ballot(need_leaf) == 0) continue;
+      visit(ni[1], ni[2], node);
+    }
+  } else {
+    traverse_wave<DIM>(T, q, q0, [&]() { return bound; }, visit);
+  }
+  // the k-nearest set: all certain points plus the need closest boundary points
+  const int need = K - nin;
+  if (need < 0 || need > B.n) {
+    flag = true;
+  } else if (need < B.n) {
+    // cut inside the boundary set: exact FP64 order decides; a (near-)tie at
+    // the cut is left to the exact kernel
+    double dcut = 0.0, dnext = 0.0;
+#pragma unroll
+    for (int j = 0; j < BCAP; ++j) {
+      if (j == need - 1) dcut = B.d[j];
+      if (j == need) dnext = B.d[j];
+    }
+    if (need > 0 && !(dnext > dcut * (1.0 + 1e-12))) flag = true;
+    if (need == 0 && !(B.d[0] > 0.0)) flag = true;
+  }
+#pragma unroll
+  for (int j = 0; j < BCAP; ++j)
+    if (j < B.n && j < need) keep(B.d[j], B.i[j]);
+  if (!valid) return;
+  if (SECTOR) {
+    for (int s = 0; s < NOCT; ++s) {
+      const int c = ocnt[s * 64 + lane];
+      const double* dd = od + (size_t)s * C * 64 + lane;
+      const int* ii = oi + (size_t)s * C * 64 + lane;
+      if (c == C && dd[(size_t)(C - 2) * 64] == dd[(size_t)(C - 1) * 64]) flag = true;
+      const int take = c < a.nmax ? c : a.nmax;
+      for (int j = 0; j < take; ++j) out[cnt++] = ii[(size_t)j * 64];
+    }
+  }
+  if (flag) {
+    a.qflag[qid] = 1;
+    return;
+  }
+  isort(out, cnt);
+  a.nnear[qid] = cnt;
+  a.qflag[qid] = 0;
+}
+
 // every observation is a neighbour (nmax >= nobs): kriging.F90:640-657
 template <int DIM>
 __global__ __launch_bounds__(256) void k_search_all(SearchArgs a, int nobs, const double* ocoord_orig,
@@ -464,6 +740,20 @@ size_t fast_search_lds(int ndim, bool sector, int k, int nmax) {
   return (size_t)64 * 4 * (size_t)words;
 }
 
+hipError_t launch_search_wave(const SearchArgs& a, hipStream_t st) {
+  const size_t lds = fast_search_lds(a.ndim, a.sort != 0, a.k, a.nmax);
+  dim3 blk(64), grd((a.nq + 63) / 64);
+  if (a.sort) {
+    if (a.ndim == 2) hipLaunchKernelGGL((k_search_wave<2, true>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_wave<3, true>), grd, blk, lds, st, a);
+  } else {
+    if (a.ndim == 1) hipLaunchKernelGGL((k_search_wave<1, false>), grd, blk, lds, st, a);
+    else if (a.ndim == 2) hipLaunchKernelGGL((k_search_wave<2, false>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_wave<3, false>), grd, blk, lds, st, a);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st) {
   const size_t lds = fast_search_lds(a.ndim, a.sort != 0, a.k, a.nmax);
   dim3 blk(64), grd((a.nq + 63) / 64);
@@ -478,11 +768,21 @@ hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+bool exact_search_fits_lds(int k) { return (size_t)k * 12 * 32 <= 64 * 1024; }
+
 hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
+  if (a.qlist && exact_search_fits_lds(a.k)) {
+    const size_t lds = (size_t)a.k * 12 * 32;
+    dim3 blk(32), grd((a.nq + 31) / 32);
+    if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, true>), grd, blk, lds, st, a);
+    else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, true>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);
+    return hipGetLastError();
+  }
   dim3 blk(256), grd((a.nq + 255) / 256);
-  if (a.ndim == 1) hipLaunchKernelGGL(k_search<1>, grd, blk, 0, st, a);
-  else if (a.ndim == 2) hipLaunchKernelGGL(k_search<2>, grd, blk, 0, st, a);
-  else hipLaunchKernelGGL(k_search<3>, grd, blk, 0, st, a);
+  if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, false>), grd, blk, 0, st, a);
+  else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, false>), grd, blk, 0, st, a);
+  else hipLaunchKernelGGL((k_search<3, false>), grd, blk, 0, st, a);
   return hipGetLastError();
 }
 

@@ -296,25 +296,24 @@ __global__ void k_empty(SolveArgs a) {
   a.flag[b] = 0;
 }
 
-// One atomic per wave and bucket: lanes of a wave agree on their offsets
-// through a ballot, so 4M blocks cost ~65k atomics instead of 4M.
-__global__ void k_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap) {
+// Bucket counts are aggregated in LDS per 1024-thread workgroup, so each
+// workgroup issues one global atomic per bucket.
+__global__ __launch_bounds__(1024) void k_bucket(const int* nnear, int nblock, int* counts, int* lists,
+                                                 int cap) {
+  __shared__ int lcount[NBUCKET + 1];
+  __shared__ int lbase[NBUCKET + 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+  if (threadIdx.x <= NBUCKET) lcount[threadIdx.x] = 0;
+  __syncthreads();
   const bool valid = t < nblock;
   const int n = valid ? nnear[t] : 0;
   const int bk = !valid ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
-  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int b = 0; b <= NBUCKET; ++b) {
-    const unsigned long long m = __ballot(bk == b);
-    if (m == 0) continue;
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counts[b], __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (bk == b) lists[(size_t)b * cap + base + __popcll(m & below)] = t;
-  }
+  const int local = valid ? atomicAdd(&lcount[bk], 1) : 0;
+  __syncthreads();
+  if (threadIdx.x <= NBUCKET)
+    lbase[threadIdx.x] = lcount[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], lcount[threadIdx.x]) : 0;
+  __syncthreads();
+  if (valid) lists[(size_t)bk * cap + lbase[bk] + local] = t;
 }
 
 __global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
@@ -453,7 +452,7 @@ int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk
 
 hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((nblock + 255) / 256), dim3(256), 0, st, nnear, nblock, counts,
+  hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, nblock, counts,
                      lists, cap);
   return hipGetLastError();
 }

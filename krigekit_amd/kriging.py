@@ -73,7 +73,7 @@ def _load_lib():
     sig("krige_get_weight_nnear", [_i64, _c_int, _c_int, _pi])
     sig("krige_get_weight_inear", [_i64, _c_int, _c_int, _c_int, _pi])
     sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
-    sig("kk_set_search_mode", [_i64, _c_int])
+    sig("kk_set_search_mode", [_i64, _c_int, _c_int])
     sig("kk_solve_resident", [_i64])
     sig("kk_fetch", [_i64])
     sig("kk_get_timing", [_i64, _pd])
@@ -281,9 +281,11 @@ class Kriging:
                     exact_search_queries=int(out[7]), fast_search_kernel_ms=out[9],
                     exact_search_kernel_ms=out[10])
 
-    def set_search_mode(self, fast: bool = True):
-        """fast=False runs the exact-emulation search kernel for every query."""
-        _check(_load_lib().kk_set_search_mode(self._handle, int(fast)), "kk_set_search_mode")
+    def set_search_mode(self, fast=True, reorder: bool = True):
+        """fast: True/2 wave-cooperative two-pass search (default), 1 per-lane
+        two-pass search, False/0 exact-emulation kernel for every query."""
+        mode = 2 if fast is True else 0 if fast is False else int(fast)
+        _check(_load_lib().kk_set_search_mode(self._handle, mode, int(reorder)), "kk_set_search_mode")
 
     def neighbours(self):
         """(nnear[nblock], inear[nblock, mmax]) of the last solve; 1-based ids, ascending."""

@@ -32,7 +32,7 @@ def _ref(vec, key):
 
 
 @pytest.mark.parametrize("spec", CASES, ids=IDS)
-@pytest.mark.parametrize("fast", [True, False], ids=["fast_search", "exact_search"])
+@pytest.mark.parametrize("fast", [2, 1, 0], ids=["wave_search", "lane_search", "exact_search"])
 def test_fp64_matches_reference_and_oracle(spec, fast, ref_vectors):
     print("case", spec["name"], "fast", fast, flush=True)
     g = run_gpu(spec, fast=fast)
