@@ -120,7 +120,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        from krigekit_amd.kriging import copy_probe
+        ms = copy_probe(args.copy_probe, reps=5)
+        print(json.dumps({"copy_bytes": args.copy_probe, "copy_ms": ms,
+                          "copy_GBps": 2 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
                  "(nodes shard independently; see DESIGN.md).")
@@ -160,7 +168,7 @@ def main():
         "phase_ms_fp32r": {k: round(v, 4) for k, v in t32.items()},
         "phase_ms_fp64": {k: round(v, 4) for k, v in t64.items()},
         "roofline": {
-            "kernel": "k_search_fast (octant neighbour search)",
+            "kernel": "k_search_wave (octant neighbour search, two FP32-key passes + exact boundary)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -155,6 +155,9 @@ int kk_fetch(int64_t handle);
 int kk_get_timing(int64_t handle, double *out);
 /* hipDeviceSynchronize */
 int kk_device_synchronize(void);
+/* plain device copy of nbytes, reps times; *ms = mean kernel time.  A kernel
+ * of known size for calibrating HBM traffic counters. */
+int kk_copy_probe(double nbytes, int reps, double *ms);
 /* neighbour lists of the last solve (0-based ids ascending, row stride mmax) */
 int kk_get_mmax(int64_t handle, int *mmax);
 int kk_get_neighbours(int64_t handle, int nblock, int mmax, int *nnear, int *inear);

@@ -1814,6 +1814,37 @@ int kk_get_timing(int64_t h, double* out) {
   return 0;
 }
 
+// Copies nbytes between two device buffers with a plain copy kernel, reps
+// times; *ms receives the mean kernel time (HIP events).  Used to calibrate
+// HBM traffic counters against a kernel of known size.
+int kk_copy_probe(double nbytes, int reps, double* ms) {
+  clear_error();
+  const size_t n = (size_t)(nbytes / 8.0);
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  if (hipMalloc(&a, n * 8) != hipSuccess || hipMalloc(&b, n * 8) != hipSuccess) {
+    set_error("kk_copy_probe", "device allocation failed");
+    if (a) (void)hipFree(a);
+    return ierr();
+  }
+  (void)hipMemset(a, 0, n * 8);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)kk::launch_copy(a, b, n, nullptr);
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; ++r) (void)kk::launch_copy(a, b, n, nullptr);
+  (void)hipEventRecord(e1, nullptr);
+  (void)hipEventSynchronize(e1);
+  float t = 0.0f;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / std::max(reps, 1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 int kk_device_synchronize(void) {
   clear_error();
   if (hipDeviceSynchronize() != hipSuccess) {

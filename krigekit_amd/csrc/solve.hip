@@ -448,6 +448,18 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   a.var[b] = var;
 }
 
+// plain copy of n doubles (reads 8n bytes, writes 8n bytes): calibrates HBM
+// traffic counters against a kernel of known size
+__global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
 int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 64 : 0; }
 
 hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,

@@ -80,6 +80,7 @@ def _load_lib():
     sig("kk_get_neighbours", [_i64, _c_int, _c_int, _pi, _pi])
     sig("kk_get_mmax", [_i64, _pi])
     sig("kk_device_synchronize", [])
+    sig("kk_copy_probe", [_c_double, _c_int, _pd])
     _lib = lib
     return lib
 
@@ -371,6 +372,13 @@ class Kriging:
             self.close()
         except Exception:
             pass
+
+
+def copy_probe(nbytes, reps=10):
+    """mean time (ms) of a plain device copy of nbytes"""
+    ms = ctypes.c_double(0.0)
+    _check(_load_lib().kk_copy_probe(float(nbytes), int(reps), ctypes.byref(ms)), "kk_copy_probe")
+    return ms.value
 
 
 def device_synchronize():
