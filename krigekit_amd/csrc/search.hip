@@ -238,16 +238,41 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   int nfound = 0;
   const int nn = a.k;
   traverse<DIM>(T, q, ballsize, [&](int l, int u) {
-    // process_terminal_node, kdtree2_maxidx.f90:1510-1613
-    for (int i = l; i <= u; ++i) {
-      double sd;
-      if (!point_sd<DIM>(T, i, q, ballsize, sd)) continue;
-      const int idx = T.ind[i];
-      if (nfound < nn) {
-        double newpri = pq_insert(h, nfound, sd, idx);
-        if (nfound == nn) ballsize = newpri;
-      } else {
-        ballsize = pq_replace_max(h, nn, sd, idx);
+    // process_terminal_node, kdtree2_maxidx.f90:1510-1613.  The bucket's
+    // points (at most 13, contiguous in tree order) are loaded up front so
+    // their latencies overlap; a partial sum above the ball rejects a point
+    // exactly when the full sum does, so the full squared distance is formed.
+    // (a node with one empty child -- 14 or more coincident points -- is
+    // processed as a bucket too, in chunks of 13)
+    constexpr int BK = 13;
+    for (int c0 = l; c0 <= u; c0 += BK) {
+      const int cu = c0 + BK - 1 < u ? c0 + BK - 1 : u;
+      double sds[BK];
+      int ids[BK];
+#pragma unroll
+      for (int p = 0; p < BK; ++p) {
+        const int i = c0 + p <= cu ? c0 + p : cu;
+        double sd = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)i], q[k])));
+        sds[p] = sd;
+        ids[p] = T.ind[i];
+      }
+      for (int p = 0; p <= cu - c0; ++p) {
+        double sd = sds[0];
+        int idx = ids[0];
+#pragma unroll
+        for (int r = 1; r < BK; ++r) {
+          sd = p == r ? sds[r] : sd;
+          idx = p == r ? ids[r] : idx;
+        }
+        if (sd > ballsize) continue;
+        if (nfound < nn) {
+          double newpri = pq_insert(h, nfound, sd, idx);
+          if (nfound == nn) ballsize = newpri;
+        } else {
+          ballsize = pq_replace_max(h, nn, sd, idx);
+        }
       }
     }
   });
@@ -703,6 +728,386 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
   a.qflag[qid] = 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Sector (octant/quadrant) search in one traversal, without a k-heap.
+//
+// The reference keeps the k = min(n, 2^ndim * nmax * 4) nearest points, sorts
+// them and walks them in ascending order keeping at most nmax per sector
+// (kriging.F90:1093-1154).  For a sector s that equals: the nmax nearest
+// points of s overall, restricted to those among the k nearest.  So each lane
+// keeps, in registers, the C = nmax+1 nearest points of every sector (float
+// key RN_fp32(d^2) and tree position), after the maxdist and leave-one-out
+// filters, and decides membership in the k-nearest set from a per-lane
+// histogram of the keys of every point it has seen:
+//   * 64 quarter-octave bins of the key's float bits relative to a per-lane
+//     base (bin 0 and bin 63 catch everything below and above), 16-bit
+//     counters packed two to a word (a wave that visits more than 65000
+//     points leaves its queries to the exact kernel);
+//   * jK = the lowest bin whose cumulative count reaches k, tracked as points
+//     arrive; its upper edge E bounds the k-th key, so boxes beyond E are
+//     pruned, and so are boxes beyond every sector's (nmax+1)-th key;
+//   * all counts are of distinct visited points, hence lower bounds of the
+//     true counts below any key, which is all the decisions below use:
+//       entry bin < jK, or bin == jK with cum(jK) <= k : #(d2 <= d2_e) <= k, in;
+//       entry bin > jK                                   : #(d2 <  d2_e) >= k, out;
+//       otherwise the rank of the entry straddles k: a walk over the
+//       recorded leaves splits bin jK into 32 sub-bins and counts
+//       #(d2 < d2_e) and #(d2 == d2_e) in FP64 for up to NAMB entries, and a
+//       last walk counts the entries still undecided.
+// Anything the reference's visit order could decide -- equal FP64 distances
+// straddling rank k, equal keys at a sector's nmax / nmax+1 cut -- flags the
+// query for the exact kernel.
+// Points of a leaf are read with one vector load (a leaf is contiguous in
+// tree order) and broadcast with readlane.
+// ---------------------------------------------------------------------------
+constexpr int NBIN = 64;      // quarter-octave bins, 16-bit counters
+constexpr int BSH = 21;       // key bits >> BSH = exponent and 2 mantissa bits
+constexpr int SLEAFCAP = 512;
+constexpr int MAXVISIT = 65000;
+constexpr int NAMB = 4;  // straddling entries resolved per lane
+
+__device__ inline double readlane_d(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int C>
+__device__ inline void sector_insert(float (&k)[C], int (&p)[C], float key, int pos) {
+#pragma unroll
+  for (int j = C - 1; j > 0; --j) {
+    if (key < k[j - 1]) {
+      k[j] = k[j - 1];
+      p[j] = p[j - 1];
+    } else if (key < k[j]) {
+      k[j] = key;
+      p[j] = pos;
+    }
+  }
+  if (key < k[0]) {
+    k[0] = key;
+    p[0] = pos;
+  }
+}
+
+__device__ inline int key_bin(float key, unsigned base) {
+  const int d = (int)(__float_as_uint(key) >> BSH) - (int)(base >> BSH);
+  return d < 0 ? 0 : (d >= NBIN ? NBIN - 1 : d);
+}
+
+template <int DIM, int C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_search_sector(SearchArgs a) {
+  __shared__ unsigned hist[NBIN / 2 * 64];
+  __shared__ int leaves[SLEAFCAP];
+  constexpr int NOCT = 1 << DIM;
+  constexpr float FINF = __builtin_huge_valf();
+  constexpr double BIG = 1.7976931348623157e308;
+  const int lane = threadIdx.x;
+  const int t = blockIdx.x * 64 + lane;
+  const bool valid = t < a.nq;
+  const int qid = a.qlist ? a.qlist[valid ? t : a.nq - 1] : a.q0 + (valid ? t : a.nq - 1);
+  double q[DIM], q0[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    q[k] = a.qcoord[k + DIM * (size_t)qid];
+    q0[k] = __builtin_amdgcn_readfirstlane(q[k]);
+  }
+  const Tree& T = a.tree;
+  const int K = a.k;
+  float ok[NOCT][C];
+  int op[NOCT][C];
+#pragma unroll
+  for (int s = 0; s < NOCT; ++s)
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      ok[s][j] = FINF;
+      op[s][j] = -1;
+    }
+#pragma unroll
+  for (int b = 0; b < NBIN / 2; ++b) hist[b * 64 + lane] = 0u;
+  auto hcount = [&](int b) { return (int)((hist[(b >> 1) * 64 + lane] >> (16 * (b & 1))) & 0xffffu); };
+
+  unsigned base = 0;
+  bool have_base = false;
+  int jK = NBIN - 1, tot = 0, nleaf = 0, nvisit = 0;
+  double ball = BIG;
+
+  auto point = [&](const double* x, int idx, int pos) {
+    double sd = 0.0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(x[k], q[k])));
+    const float key = (float)sd;
+    const int bin = key_bin(key, base);
+    __hip_atomic_fetch_add(&hist[(bin >> 1) * 64 + lane], 1u << (16 * (bin & 1)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    tot += (bin <= jK);
+    if ((a.cv && idx == qid) || !(sd <= a.maxdist2)) return;
+    int oct = (__dsub_rn(x[0], q[0]) < 0.0);
+    if (DIM >= 2) oct += 2 * (__dsub_rn(x[1], q[1]) < 0.0);
+    if (DIM == 3) oct += 4 * (__dsub_rn(x[2], q[2]) < 0.0);
+    float thr = ok[0][C - 1];
+#pragma unroll
+    for (int s = 1; s < NOCT; ++s) thr = oct == s ? ok[s][C - 1] : thr;
+    if (!(key < thr)) return;
+#pragma unroll
+    for (int s = 0; s < NOCT; ++s)
+      if (oct == s) sector_insert<C>(ok[s], op[s], key, pos);
+  };
+
+  // a bucket holds at most 13 points; a node with one empty child (14 or
+  // more coincident points) is processed as a bucket, in chunks of 64/DIM
+  constexpr int CHK = 64 / DIM;
+  traverse_wave<DIM>(T, q, q0, [&]() { return ball; }, [&](int l0, int u, int node) {
+    if (lane == 0 && nleaf < SLEAFCAP) leaves[nleaf] = node;
+    ++nleaf;
+    nvisit += u - l0 + 1;
+    for (int l = l0; l <= u; l += CHK) {
+      const int m = u - l + 1 < CHK ? u - l + 1 : CHK;
+      const double v = lane < DIM * m ? T.rdata[DIM * (size_t)l + lane] : 0.0;
+      const int iv = lane < m ? T.ind[l + lane] : 0;
+      if (!have_base) {
+        // bins start 8 octaves below the largest key of the first leaf (64
+        // quarter-octave bins span 16 octaves)
+        float mk = 0.0f;
+        for (int p = 0; p < m; ++p) {
+          double sd = 0.0;
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(readlane_d(v, DIM * p + k), q[k])));
+          mk = fmaxf(mk, (float)sd);
+        }
+        const unsigned mb = __float_as_uint(mk) > (1u << 23) ? __float_as_uint(mk) : (1u << 23);
+        base = mb > (32u << BSH) ? ((mb - (32u << BSH)) >> BSH) << BSH : 0u;
+        have_base = true;
+      }
+      for (int p = 0; p < m; ++p) {
+        double x[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) x[k] = readlane_d(v, DIM * p + k);
+        point(x, __builtin_amdgcn_readlane(iv, p), l + p);
+      }
+    }
+    // lowest bin reaching k, and the ball: min(upper edge of that bin,
+    // next(largest (nmax+1)-th sector key))
+    while (jK > 0) {
+      const int h = hcount(jK);
+      if (tot - h < K) break;
+      tot -= h;
+      --jK;
+    }
+    double bE = BIG;
+    if (tot >= K && jK < NBIN - 1) {
+      const unsigned eb = base + ((unsigned)(jK + 1) << BSH);
+      if (eb < 0x7f800000u) bE = (double)__uint_as_float(eb);
+    }
+    float mo = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NOCT; ++s) mo = fmaxf(mo, ok[s][C - 1]);
+    const double bO = mo < FINF ? (double)f_next(mo) : BIG;
+    ball = bE < bO ? bE : bO;
+  });
+
+  // ---- k-nearest membership of every kept entry ------------------------------
+  // state per entry (bit s*C+j): in, or undecided; neither = out
+  static_assert(NOCT * C <= 64, "sector lists exceed the state masks");
+  unsigned long long in_m = 0, und_m = 0;
+  auto bit = [](int s, int j) { return 1ull << (s * C + j); };
+  bool flag = nvisit > MAXVISIT;  // 16-bit bin counters could have wrapped
+  auto entry_sd = [&](int pos) {
+    double sd = 0.0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[k + DIM * (size_t)pos], q[k])));
+    return sd;
+  };
+#pragma unroll
+  for (int s = 0; s < NOCT; ++s)
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      if (op[s][j] < 0) continue;
+      const int b = key_bin(ok[s][j], base);
+      if (b < jK || (b == jK && tot <= K)) in_m |= bit(s, j);
+      else if (b == jK) und_m |= bit(s, j);
+    }
+  if (__ballot(und_m != 0)) {
+    // walks over the points of the recorded leaves (every point with a key
+    // below the final ball) whose box is within some lane's bound
+    auto walk = [&](double bound, auto&& fn) {
+      auto leaf = [&](int l0, int mm) {
+        for (int l = l0; l < l0 + mm; l += CHK) {
+          const int m = l0 + mm - l < CHK ? l0 + mm - l : CHK;
+          const double v = lane < DIM * m ? T.rdata[DIM * (size_t)l + lane] : 0.0;
+          for (int p = 0; p < m; ++p) {
+            double sd = 0.0;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(readlane_d(v, DIM * p + k), q[k])));
+            fn(sd);
+          }
+        }
+      };
+      if (nleaf <= SLEAFCAP) {
+        for (int j = 0; j < nleaf; ++j) {
+          const int node = __builtin_amdgcn_readfirstlane(leaves[j]);
+          if (__ballot(box_lb<DIM>(T.nd + (size_t)node * NODE_D, q) <= bound * (1.0 + 1e-12)) == 0) continue;
+          const int* ni = T.ni + (size_t)node * NODE_I;
+          leaf(ni[1], ni[2] - ni[1] + 1);
+        }
+      } else {
+        traverse_wave<DIM>(T, q, q0, [&]() { return bound; }, [&](int l, int u, int) { leaf(l, u - l + 1); });
+      }
+    };
+    // exact FP64 ranks for up to NAMB distinct undecided distances per walk
+    double amb[NAMB];
+    int clt[NAMB], ceq[NAMB], namb = 0;
+    auto pick = [&]() {
+      namb = 0;
+#pragma unroll
+      for (int z = 0; z < NAMB; ++z) {
+        amb[z] = -1.0;
+        clt[z] = ceq[z] = 0;
+      }
+#pragma unroll
+      for (int s = 0; s < NOCT; ++s)
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          if (!(und_m & bit(s, j))) continue;
+          const double sd = entry_sd(op[s][j]);
+          bool known = false;
+#pragma unroll
+          for (int z = 0; z < NAMB; ++z) known |= (z < namb && amb[z] == sd);
+          if (!known && namb < NAMB) {
+#pragma unroll
+            for (int z = 0; z < NAMB; ++z)
+              if (z == namb) amb[z] = sd;
+            ++namb;
+          }
+        }
+    };
+    // in if every point at or below d2_e fits in k, out if k points are
+    // strictly closer, otherwise equal distances straddle rank k
+    auto settle = [&]() {
+#pragma unroll
+      for (int s = 0; s < NOCT; ++s)
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          if (!(und_m & bit(s, j))) continue;
+          const double sd = entry_sd(op[s][j]);
+#pragma unroll
+          for (int z = 0; z < NAMB; ++z)
+            if (z < namb && amb[z] == sd) {
+              if (clt[z] + ceq[z] <= K) {
+                in_m |= bit(s, j);
+                und_m &= ~bit(s, j);
+              } else if (clt[z] >= K) {
+                und_m &= ~bit(s, j);
+              } else {
+                flag = true;
+              }
+            }
+        }
+    };
+    // walk 1: sub-histogram of bin jK (bits 16..20 of the key: 1/128
+    // octave, 32-bit counters) plus exact ranks for the first NAMB undecided
+    // distances
+    const int hK = hcount(jK);
+    const int cb = tot - hK;  // counted points in bins below jK
+    const bool subok = jK > 0 && jK < NBIN - 1;
+    const unsigned ekb = base + ((unsigned)(jK + 1) << BSH);
+    const double eK = !subok ? 0.0 : (ekb < 0x7f800000u ? (double)__uint_as_float(ekb) : BIG);
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 32; ++b) hist[b * 64 + lane] = 0u;
+    pick();
+    double b1 = eK;
+#pragma unroll
+    for (int z = 0; z < NAMB; ++z) b1 = fmax(b1, amb[z]);
+    walk(b1, [&](double sd) {
+      const float key = (float)sd;
+      if (subok && key_bin(key, base) == jK)
+        __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> 16) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+      for (int z = 0; z < NAMB; ++z) {
+        clt[z] += (sd < amb[z]);
+        ceq[z] += (sd == amb[z]);
+      }
+    });
+    settle();
+    if (subok) {
+      int c = cb, jK2 = 31, tot2 = cb;
+      for (int b = 0; b < 32; ++b) {
+        c += (int)hist[b * 64 + lane];
+        if (c >= K) {
+          jK2 = b;
+          tot2 = c;
+          break;
+        }
+        tot2 = c;
+      }
+#pragma unroll
+      for (int s = 0; s < NOCT; ++s)
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          if (!(und_m & bit(s, j))) continue;
+          const int b = (int)((__float_as_uint(ok[s][j]) >> 16) & 31);
+          if (b < jK2 || (b == jK2 && tot2 <= K)) {
+            in_m |= bit(s, j);
+            und_m &= ~bit(s, j);
+          } else if (b > jK2) {
+            und_m &= ~bit(s, j);
+          }
+        }
+    }
+    // walk 2: exact ranks for what is still undecided
+    if (__ballot(und_m != 0)) {
+      pick();
+      double b2 = 0.0;
+#pragma unroll
+      for (int z = 0; z < NAMB; ++z) b2 = fmax(b2, amb[z]);
+      walk(b2, [&](double sd) {
+#pragma unroll
+        for (int z = 0; z < NAMB; ++z) {
+          clt[z] += (sd < amb[z]);
+          ceq[z] += (sd == amb[z]);
+        }
+      });
+      settle();
+      flag |= und_m != 0;
+    }
+  }
+  if (!valid) return;
+  // per sector: the in-entries must be a prefix of the key order (only equal
+  // keys could break that), take the first nmax, and a key tie at the cut
+  // between two in-entries is left to the exact kernel
+  int* out = a.inear + (size_t)qid * a.mmax;
+  int cnt = 0;
+#pragma unroll
+  for (int s = 0; s < NOCT; ++s) {
+    int nin = 0;
+    bool gap = false;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      if (in_m & bit(s, j)) {
+        if (gap) flag = true;
+        ++nin;
+      } else {
+        gap = true;
+      }
+    }
+    if (nin == C && ok[s][C - 2] == ok[s][C - 1]) flag = true;
+#pragma unroll
+    for (int j = 0; j < C - 1; ++j)
+      if (j < nin) out[cnt++] = T.ind[op[s][j]];
+  }
+  if (flag) {
+    a.qflag[qid] = 1;
+    return;
+  }
+  isort(out, cnt);
+  a.nnear[qid] = cnt;
+  a.qflag[qid] = 0;
+}
+
 // every observation is a neighbour (nmax >= nobs): kriging.F90:640-657
 template <int DIM>
 __global__ __launch_bounds__(256) void k_search_all(SearchArgs a, int nobs, const double* ocoord_orig,
@@ -740,9 +1145,27 @@ size_t fast_search_lds(int ndim, bool sector, int k, int nmax) {
   return (size_t)64 * 4 * (size_t)words;
 }
 
+template <int DIM>
+static void launch_sector(const SearchArgs& a, dim3 grd, dim3 blk, hipStream_t st) {
+  switch (a.nmax) {
+    case 1: hipLaunchKernelGGL((k_search_sector<DIM, 2>), grd, blk, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_search_sector<DIM, 3>), grd, blk, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_search_sector<DIM, 4>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_search_sector<DIM, 5>), grd, blk, 0, st, a); break;
+  }
+}
+
+// register-resident sector lists: up to 4 per sector
+bool sector_kernel_fits(const SearchArgs& a) { return a.sort && a.ndim >= 2 && a.nmax >= 1 && a.nmax <= 4; }
+
 hipError_t launch_search_wave(const SearchArgs& a, hipStream_t st) {
   const size_t lds = fast_search_lds(a.ndim, a.sort != 0, a.k, a.nmax);
   dim3 blk(64), grd((a.nq + 63) / 64);
+  if (sector_kernel_fits(a)) {
+    if (a.ndim == 2) launch_sector<2>(a, grd, blk, st);
+    else launch_sector<3>(a, grd, blk, st);
+    return hipGetLastError();
+  }
   if (a.sort) {
     if (a.ndim == 2) hipLaunchKernelGGL((k_search_wave<2, true>), grd, blk, lds, st, a);
     else hipLaunchKernelGGL((k_search_wave<3, true>), grd, blk, lds, st, a);
