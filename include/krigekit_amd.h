@@ -151,7 +151,10 @@ int kk_fetch(int64_t handle);
 /* out[12]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
  * nodes, LU-fallback nodes, queries decided by the exact search kernel, fast
  * search used, fast search kernel ms, exact search kernel ms, reserved.
- * Times come from HIP events on the stream the kernels run on. */
+ * Times come from HIP events on the stream the kernels run on.  When the fast
+ * search leaves queries to the exact kernel, that kernel runs on a second
+ * stream while the other blocks are solved: it is outside "search" and its
+ * wait, if any, is inside "solve". */
 int kk_get_timing(int64_t handle, double *out);
 /* hipDeviceSynchronize */
 int kk_device_synchronize(void);

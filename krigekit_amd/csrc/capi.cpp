@@ -467,7 +467,7 @@ struct Krige {
   bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm;
+  DBuf d_model, d_sflag, d_perm, d_xlist;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool perm_ok = false;
@@ -475,11 +475,13 @@ struct Krige {
   Model model{};
   std::vector<double> tab;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // exact search of flagged queries, overlapped with the solve
   hipEvent_t ev[12] = {nullptr};
   ~Krige() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -521,6 +523,7 @@ bool upload(DBuf& b, const std::vector<T>& v, const char* where) {
 bool ensure_stream(Krige& K) {
   if (!K.stream) {
     HIPCHK(hipStreamCreateWithFlags(&K.stream, hipStreamNonBlocking), "solve");
+    HIPCHK(hipStreamCreateWithFlags(&K.stream2, hipStreamNonBlocking), "solve");
     for (auto& e : K.ev) HIPCHK(hipEventCreate(&e), "solve");
   }
   return true;
@@ -638,6 +641,8 @@ bool run_pipeline(Krige& K) {
     return false;
   }
   HIPCHK(hipEventRecord(K.ev[0], st), W);
+  bool overlap = false;  // exact search of flagged queries runs on stream2
+  int nx = 0;            // blocks solved in the second round
   // ---- search -----------------------------------------------------------
   kk::SearchArgs sa{};
   sa.ndim = nd;
@@ -687,15 +692,24 @@ bool run_pipeline(Krige& K) {
         HIPCHK(kk::launch_search_fast(sa, st), W);
       }
       HIPCHK(hipEventRecord(K.ev[9], st), W);
+      if (!K.d_xlist.ensure(sizeof(int) * nb)) {
+        set_error(W, "device allocation failed (search flags)");
+        return false;
+      }
       HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_lists.as<int>(), st), W);
+      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_xlist.as<int>(), st), W);
       HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
       HIPCHK(hipStreamSynchronize(st), W);
-      exact_list = K.d_lists.as<int>();
+      exact_list = K.d_xlist.as<int>();
     }
     K.timing[7] = exact_list ? nexact : -1;
     K.timing[8] = exact_list ? 1.0 : 0.0;  // marks that ev[8]/ev[9] hold the fast kernel
-    HIPCHK(hipEventRecord(K.ev[10], st), W);
+    // the queries the fast kernel left undecided run on the second stream,
+    // overlapped with the solve of every other block
+    overlap = exact_list != nullptr && nexact > 0;
+    hipStream_t xs = overlap ? K.stream2 : st;
+    if (overlap) HIPCHK(hipStreamWaitEvent(xs, K.ev[9], 0), W);
+    HIPCHK(hipEventRecord(K.ev[10], xs), W);
     if (nexact > 0) {
       const int chunk = std::min(nexact, 1 << 20);
       if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
@@ -710,22 +724,13 @@ bool run_pipeline(Krige& K) {
         sa.q0 = exact_list ? 0 : q0;
         sa.qlist = exact_list ? exact_list + q0 : nullptr;
         sa.nq = std::min(chunk, nexact - q0);
-        HIPCHK(kk::launch_search(sa, st), W);
+        HIPCHK(kk::launch_search(sa, xs), W);
       }
     }
-    HIPCHK(hipEventRecord(K.ev[11], st), W);
+    HIPCHK(hipEventRecord(K.ev[11], xs), W);
+    nx = overlap ? nexact : 0;
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
-  // ---- bucket by neighbour count -------------------------------------------
-  HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-  HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), nb, K.d_counts.as<int>(), K.d_lists.as<int>(), nb, st), W);
-  int counts[16];
-  HIPCHK(hipMemcpyAsync(counts, K.d_counts.p, sizeof(int) * 16, hipMemcpyDeviceToHost, st), W);
-  HIPCHK(hipStreamSynchronize(st), W);
-  if (counts[kk::NBUCKET] > 0) {
-    set_error(W, "neighbourhood larger than 64 rows");
-    return false;
-  }
   HIPCHK(hipEventRecord(K.ev[2], st), W);
   // ---- per-node solve ---------------------------------------------------------
   kk::SolveArgs va{};
@@ -757,11 +762,35 @@ bool run_pipeline(Krige& K) {
     va.base_cov = K.d_basecov.as<double>();
   }
   const bool tb = K.model.mode == kk::COV_TABLE_B;
-  for (int bk = 0; bk < kk::NBUCKET; ++bk) {
-    va.list = K.d_lists.as<int>() + (size_t)bk * nb;
-    va.nlist = counts[bk];
-    if (bk == 0) HIPCHK(kk::launch_empty(va, st), W);
-    else HIPCHK(kk::launch_solve(va, kk::bucket_width(bk), K.precision, tb, st), W);
+  // ---- bucket by neighbour count, solve per bucket ---------------------------
+  // round 0: every block the search has decided; round 1: the blocks the
+  // exact search decided on the second stream
+  int nempty = 0;
+  float bucket_ms = 0.0f;
+  for (int round = 0; round < (nx > 0 ? 2 : 1); ++round) {
+    if (round == 1) HIPCHK(hipStreamWaitEvent(st, K.ev[11], 0), W);
+    HIPCHK(hipEventRecord(K.ev[5], st), W);
+    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), round ? K.d_xlist.as<int>() : nullptr, round ? nx : nb,
+                             K.d_counts.as<int>(), K.d_lists.as<int>(), nb, st), W);
+    int counts[16];
+    HIPCHK(hipMemcpyAsync(counts, K.d_counts.p, sizeof(int) * 16, hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipEventRecord(K.ev[6], st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    float bms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
+    bucket_ms += bms;
+    if (counts[kk::NBUCKET] > 0) {
+      set_error(W, "neighbourhood larger than 64 rows");
+      return false;
+    }
+    nempty += counts[0];
+    for (int bk = 0; bk < kk::NBUCKET; ++bk) {
+      va.list = K.d_lists.as<int>() + (size_t)bk * nb;
+      va.nlist = counts[bk];
+      if (bk == 0) HIPCHK(kk::launch_empty(va, st), W);
+      else HIPCHK(kk::launch_solve(va, kk::bucket_width(bk), K.precision, tb, st), W);
+    }
   }
   HIPCHK(hipEventRecord(K.ev[3], st), W);
   // ---- FP64 re-solve of flagged FP32 systems, LU fallback ----------------
@@ -799,10 +828,9 @@ bool run_pipeline(Krige& K) {
   float ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[1]), W);
   K.timing[0] = ms;
-  HIPCHK(hipEventElapsedTime(&ms, K.ev[1], K.ev[2]), W);
-  K.timing[1] = ms;
+  K.timing[1] = bucket_ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[2], K.ev[3]), W);
-  K.timing[2] = ms;
+  K.timing[2] = ms - bucket_ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[3], K.ev[4]), W);
   K.timing[3] = ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
@@ -817,7 +845,7 @@ bool run_pipeline(Krige& K) {
     K.timing[10] = ms;
   }
   K.stats[0] = 0;
-  K.stats[1] = nb - counts[0] - nfb;
+  K.stats[1] = nb - nempty - nfb;
   K.stats[2] = 0;
   K.stats[3] = nfb;
   K.stats[4] = 0;

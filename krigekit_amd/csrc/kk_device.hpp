@@ -38,7 +38,9 @@ enum Precision : int { PREC_FP64 = 0, PREC_FP32R = 1 };
 
 // bucket blocks by neighbour count: counts[NB] and lists (device), returns via counts
 constexpr int NBUCKET = 5;  // n==0, <=8, <=16, <=32, <=64 ; larger -> overflow bucket
-hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,
+// blocks 0..nblock-1, or the nblock blocks listed in sel; blocks with a
+// negative neighbour count are left out
+hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st);
 int bucket_width(int b);  // lanes per system of bucket b (0 for the empty bucket)
 

@@ -451,6 +451,7 @@ __global__ __launch_bounds__(64) void k_search_fast(SearchArgs a) {
   }
   if (flag) {
     a.qflag[qid] = 1;
+    a.nnear[qid] = -1;  // decided by the exact kernel
     return;
   }
   isort(out, cnt);
@@ -721,6 +722,7 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
   }
   if (flag) {
     a.qflag[qid] = 1;
+    a.nnear[qid] = -1;  // decided by the exact kernel
     return;
   }
   isort(out, cnt);
@@ -1101,6 +1103,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
   if (flag) {
     a.qflag[qid] = 1;
+    a.nnear[qid] = -1;  // decided by the exact kernel
     return;
   }
   isort(out, cnt);

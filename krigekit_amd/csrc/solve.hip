@@ -298,22 +298,24 @@ __global__ void k_empty(SolveArgs a) {
 
 // Bucket counts are aggregated in LDS per 1024-thread workgroup, so each
 // workgroup issues one global atomic per bucket.
-__global__ __launch_bounds__(1024) void k_bucket(const int* nnear, int nblock, int* counts, int* lists,
-                                                 int cap) {
+// Blocks 0..nblock-1 (or the blocks listed in `sel`) into per-size lists.
+// A negative count marks a block whose neighbours are still being searched.
+__global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* sel, int nblock, int* counts,
+                                                 int* lists, int cap) {
   __shared__ int lcount[NBUCKET + 1];
   __shared__ int lbase[NBUCKET + 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (threadIdx.x <= NBUCKET) lcount[threadIdx.x] = 0;
   __syncthreads();
-  const bool valid = t < nblock;
-  const int n = valid ? nnear[t] : 0;
-  const int bk = !valid ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
-  const int local = valid ? atomicAdd(&lcount[bk], 1) : 0;
+  const int b = t < nblock ? (sel ? sel[t] : t) : -1;
+  const int n = b >= 0 ? nnear[b] : -1;
+  const int bk = n < 0 ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
+  const int local = bk >= 0 ? atomicAdd(&lcount[bk], 1) : 0;
   __syncthreads();
   if (threadIdx.x <= NBUCKET)
     lbase[threadIdx.x] = lcount[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], lcount[threadIdx.x]) : 0;
   __syncthreads();
-  if (valid) lists[(size_t)bk * cap + lbase[bk] + local] = t;
+  if (bk >= 0) lists[(size_t)bk * cap + lbase[bk] + local] = b;
 }
 
 __global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
@@ -462,9 +464,10 @@ hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st)
 
 int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 64 : 0; }
 
-hipError_t launch_bucket(const int* nnear, int nblock, int* counts, int* lists, int cap,
+hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, nblock, counts,
+  if (nblock <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, sel, nblock, counts,
                      lists, cap);
   return hipGetLastError();
 }
