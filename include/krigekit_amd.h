@@ -156,6 +156,14 @@ int kk_fetch(int64_t handle);
  * stream while the other blocks are solved: it is outside "search" and its
  * wait, if any, is inside "solve". */
 int kk_get_timing(int64_t handle, double *out);
+/* Leave-one-out cross-validation MSE and its gradient with respect to the
+ * variogram parameters (nugget, sill, a_major, a_minor1, a_minor2, azimuth,
+ * dip, plunge) of a single-structure ordinary-kriging model, by an adjoint
+ * kernel over the per-block systems (no counterpart in the reference; the
+ * target of its LOO workflow, tests/test_cross_validation.py).  Needs
+ * cross_validation=1 and set_grid_cv.  grad[8]; zhat[nblock] and
+ * dz[nblock*8] (block-major) may be NULL. */
+int kk_loo_grad(int64_t handle, double *mse, double *grad, double *zhat, double *dz);
 /* hipDeviceSynchronize */
 int kk_device_synchronize(void);
 /* plain device copy of nbytes, reps times; *ms = mean kernel time.  A kernel

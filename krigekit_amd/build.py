@@ -27,6 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 UNITS = [
     ("search.hip", ["-ffp-contract=off"]),
     ("solve.hip", []),
+    ("loograd.hip", []),
     ("capi.cpp", ["-x", "hip"]),
 ]
 

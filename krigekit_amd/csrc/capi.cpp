@@ -59,6 +59,16 @@ bool failed() { return g_ierr != 0; }
     }                                                                              \
   } while (0)
 
+// as HIPCHK, in functions that return the error status
+#define HIPCHK_I(expr, where)                                                      \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(where, std::string("HIP error: ") + hipGetErrorString(e_));        \
+      return ierr();                                                               \
+    }                                                                              \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // model: variogram.f90 structures, anisotropy (rotation.f90:51-95), cov0
 // (variogram.f90:853-869), piecewise tables (variogram.f90:223-325,497-680)
@@ -103,6 +113,62 @@ void calc_rotmat(double ang1, double ang2, double ang3, double anis1, double ani
     double T[9] = {c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0};
     matmul3(R, T, R);
   }
+}
+
+// Derivatives of the anisotropy matrix M = calc_rotmat(...) / a_major with
+// respect to (a_major, a_minor1, a_minor2, azimuth, dip, plunge).  Written as
+// M = diag(1/a_minor1, 1/a_major, 1/a_minor2) * Ty(plunge) Tx(dip) Tz(azimuth)
+// (the reference's product, rotation.f90:51-95, with the scaling folded in).
+void aniso_dmat(const VStruct& v, double dm[6][9]) {
+  const double d2r = kPI / 180.0;
+  auto rot = [&](int axis, double ang, bool deriv, double* T) {
+    // column-major rotation matrices of calc_rotmat and their derivatives
+    // with respect to the angle in degrees
+    const double a = (axis == 2 ? 1.0 : -1.0) * ang * d2r, da = (axis == 2 ? 1.0 : -1.0) * d2r;
+    const double s = std::sin(a), c = std::cos(a);
+    if (axis == 0) {  // Y (plunge)
+      const double t[9] = {c, 0.0, s, 0.0, 1.0, 0.0, -s, 0.0, c};
+      const double d[9] = {-s * da, 0.0, c * da, 0.0, 0.0, 0.0, -c * da, 0.0, -s * da};
+      std::memcpy(T, deriv ? d : t, sizeof t);
+    } else if (axis == 1) {  // X (dip)
+      const double t[9] = {1.0, 0.0, 0.0, 0.0, c, s, 0.0, -s, c};
+      const double d[9] = {0.0, 0.0, 0.0, 0.0, -s * da, c * da, 0.0, -c * da, -s * da};
+      std::memcpy(T, deriv ? d : t, sizeof t);
+    } else {  // Z (azimuth)
+      const double t[9] = {c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0};
+      const double d[9] = {-s * da, c * da, 0.0, -c * da, -s * da, 0.0, 0.0, 0.0, 0.0};
+      std::memcpy(T, deriv ? d : t, sizeof t);
+    }
+  };
+  const double ang[3] = {v.plunge, v.dip, v.azimuth};
+  auto chain = [&](int dax, double* out) {  // Ty Tx Tz with factor dax differentiated
+    double A[9], B[9], C[9], t[9];
+    rot(0, ang[0], dax == 0, A);
+    rot(1, ang[1], dax == 1, B);
+    rot(2, ang[2], dax == 2, C);
+    matmul3(A, B, t);
+    matmul3(t, C, out);
+  };
+  const double sc[3] = {1.0 / v.a_minor1, 1.0 / v.a_major, 1.0 / v.a_minor2};
+  auto scale_rows = [](const double* f, const double* T, double* out) {
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) out[r + 3 * c] = f[r] * T[r + 3 * c];
+  };
+  double T[9];
+  chain(-1, T);
+  const double fa[3] = {0.0, -sc[1] * sc[1], 0.0};
+  const double f1[3] = {-sc[0] * sc[0], 0.0, 0.0};
+  const double f2[3] = {0.0, 0.0, -sc[2] * sc[2]};
+  scale_rows(fa, T, dm[0]);
+  scale_rows(f1, T, dm[1]);
+  scale_rows(f2, T, dm[2]);
+  double dT[9];
+  chain(2, dT);
+  scale_rows(sc, dT, dm[3]);
+  chain(1, dT);
+  scale_rows(sc, dT, dm[4]);
+  chain(0, dT);
+  scale_rows(sc, dT, dm[5]);
 }
 
 double corefunc_host(int t, double r) {
@@ -467,7 +533,7 @@ struct Krige {
   bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm, d_xlist;
+  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool perm_ok = false;
@@ -1830,10 +1896,80 @@ int kk_fetch(int64_t h) {
   return ierr();
 }
 
+// Leave-one-out MSE and its gradient with respect to the variogram parameters
+// (nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge) of the
+// single structure of an ordinary-kriging cross-validation setup.  Runs the
+// search and forward solve, then the adjoint kernel.  zhat (nblock) and dz
+// (nblock x 8, block-major) are optional per-block outputs.
+int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) {
+  clear_error();
+  const char* W = "kk_loo_grad";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->cross_validation || K->nvar != 1 || K->unbias != 1) {
+    set_error(W, "the gradient needs single-variable ordinary kriging with cross_validation=True");
+    return ierr();
+  }
+  if (K->vgm.empty() || K->vgm[0].size() != 1 || K->vgm[0][0].vtype == kk::NUG) {
+    set_error(W, "the gradient supports one variogram structure (with its nugget)");
+    return ierr();
+  }
+  if (!K->rangescale.empty() || !K->localnugget.empty()) {
+    set_error(W, "the gradient does not support per-block range scale or local nugget");
+    return ierr();
+  }
+  if (!do_solve(*K, false)) return ierr();
+  const int nb = K->nblock, nd = K->ndim;
+  if (!K->d_gz.ensure(sizeof(double) * nb) || !K->d_gdz.ensure(sizeof(double) * (size_t)nb * kk::NGRAD)) {
+    set_error(W, "device allocation failed");
+    return ierr();
+  }
+  kk::GradArgs ga{};
+  ga.model = K->d_model.as<Model>();
+  ga.ndim = nd;
+  ga.nb = nb;
+  ga.mmax = K->mmax;
+  ga.bcoord = K->d_bcoord.as<double>();
+  ga.ocoord = K->d_ocoord.as<double>();
+  ga.ovalue = K->d_ovalue.as<double>();
+  ga.ovar = K->d_ovar.as<double>();
+  ga.nnear = K->d_nnear.as<int>();
+  ga.inear = K->d_inear.as<int>();
+  aniso_dmat(K->vgm[0][0], ga.dmat);
+  ga.zhat = K->d_gz.as<double>();
+  ga.dz = K->d_gdz.as<double>();
+  hipStream_t st = K->stream;
+  HIPCHK_I(hipEventRecord(K->ev[5], st), W);
+  HIPCHK_I(kk::launch_loo_grad(ga, st), W);
+  HIPCHK_I(hipEventRecord(K->ev[6], st), W);
+  std::vector<double> z(nb), d((size_t)nb * kk::NGRAD);
+  HIPCHK_I(hipMemcpyAsync(z.data(), K->d_gz.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st), W);
+  HIPCHK_I(hipMemcpyAsync(d.data(), K->d_gdz.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, st), W);
+  HIPCHK_I(hipStreamSynchronize(st), W);
+  float ms = 0.0f;
+  HIPCHK_I(hipEventElapsedTime(&ms, K->ev[5], K->ev[6]), W);
+  K->timing[11] = ms;
+  const std::vector<double>& val = K->obs[0].value;
+  double s = 0.0, g[kk::NGRAD] = {0.0};
+  int cnt = 0;
+  for (int b = 0; b < nb; ++b) {
+    if (!std::isfinite(z[b])) continue;
+    const double r = z[b] - val[b];
+    s += r * r;
+    for (int p = 0; p < kk::NGRAD; ++p) g[p] += r * d[(size_t)b * kk::NGRAD + p];
+    ++cnt;
+  }
+  *mse = cnt ? s / cnt : std::nan("");
+  for (int p = 0; p < kk::NGRAD; ++p) grad[p] = cnt ? 2.0 * g[p] / cnt : std::nan("");
+  if (zhat) std::memcpy(zhat, z.data(), sizeof(double) * nb);
+  if (dz) std::memcpy(dz, d.data(), sizeof(double) * d.size());
+  return 0;
+}
+
 // out[0..11]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
 // FP64 re-solved nodes, LU-fallback nodes, queries decided by the exact
 // search kernel (-1 when the fast search was not used), fast search used
-// (1/0), fast search kernel ms, exact search kernel ms, reserved
+// (1/0), fast search kernel ms, exact search kernel ms, LOO gradient kernel ms
 int kk_get_timing(int64_t h, double* out) {
   clear_error();
   Krige* K = checked(h);

@@ -47,6 +47,8 @@ int bucket_width(int b);  // lanes per system of bucket b (0 for the empty bucke
 hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
 hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
+// leave-one-out estimates and their parameter derivatives, one block per workgroup
+hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
                           hipStream_t st);

@@ -77,6 +77,25 @@ struct SearchArgs {
   int* qflag;                 // fast kernel: 1 where the exact kernel must decide
 };
 
+// Leave-one-out gradient: per block, the estimate and its derivatives with
+// respect to the variogram parameters of the single structure.
+constexpr int NGRAD = 8;  // nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge
+struct GradArgs {
+  const Model* model;
+  int ndim;
+  int nb;
+  int mmax;
+  const double* bcoord;       // (ndim, nb) targets (the observations themselves)
+  const double* ocoord;       // (ndim, nobs)
+  const double* ovalue;
+  const double* ovar;
+  const int* nnear;
+  const int* inear;
+  double dmat[6][9];          // d(anisotropy matrix)/d(a_major, a_minor1, a_minor2, azimuth, dip, plunge)
+  double* zhat;               // [nb]
+  double* dz;                 // [nb * NGRAD]
+};
+
 struct SolveArgs {
   const Model* model;         // device copy of the covariance model
   int ndim;

@@ -1,0 +1,269 @@
+// Gradient of the leave-one-out estimates with respect to the variogram
+// parameters, for the ordinary-kriging system the reference solves per block
+// (kriging.F90:714-939, solver.f90:224-300):
+//
+//   A = [K 1; 1' 0],  x = A^-1 [k; 1] (weights w, multiplier),  zhat = w'z.
+//
+// With the adjoint y = A^-1 [z; 0] (A is symmetric), for any parameter t
+//
+//   d zhat / dt = y_w' dk/dt - y_w' (dK/dt) w,
+//
+// because only K and k depend on the variogram; the neighbour set does not
+// (the search metric is fixed).  K is recomputed here rather than stored.
+//
+// Covariance derivatives of one structure with a nugget:
+//   analytic: C = sill * rho(h), h = |M lag| > 0;  C(0) = sill + nugget
+//   table:    C = t_i + f (t_{i+1} - t_i) on the reference's lookup table
+//             (t_0 = sill + nugget, t_i = sill * rho(h_i)), so dC/dh is the
+//             slope of the table segment, dC/dsill interpolates rho(h_i) and
+//             dC/dnugget = 1 - f in the first segment;
+//   dh/dt = (M lag) . (dM/dt lag) / h for the ranges and angles.
+// One 64-lane workgroup per block, K in LDS, FP64 throughout.
+#include <hip/hip_runtime.h>
+
+#include "kk_cov.hpp"
+#include "kk_device.hpp"
+#include "kk_types.hpp"
+
+namespace kk {
+
+// d rho / dr for the correlation shapes of vgmfunc.f90:48-87
+__device__ inline double corefunc_d(int t, double r) {
+  const double pi = 3.141592653589793238462643383279502884;
+  switch (t) {
+    case SPH: return (r < 1.0) ? -1.5 + 1.5 * (r * r) : 0.0;
+    case EXP: return -3.0 * exp(-3.0 * r);
+    case GAU: return -6.125 * r * exp(-3.0625 * (r * r));
+    case HOL: return -pi * sin(pi * r);
+    case POW: return (r < 1.0) ? -1.5 * sqrt(r) : 0.0;
+    case BSQ: return (r < 1.0) ? -4.0 * r * (1.0 - r * r) : 0.0;
+    case CIR: return (r < 1.0) ? -4.0 * sqrt(1.0 - r * r) / pi : 0.0;
+    case LIN: return (r < 1.0) ? -1.0 : 0.0;
+    case CYC: { double s = sin(pi * r); return -2.0 * pi * sin(2.0 * pi * r) * exp(-2.0 * (s * s)); }
+    case DCO: return -exp(-3.0 * r) * (3.0 * cos(pi * r) + pi * sin(pi * r));
+    default: return 0.0;
+  }
+}
+
+template <int DIM>
+__device__ inline void mat_apply(const double* m, const double* l, double* u) {
+#pragma unroll
+  for (int r = 0; r < DIM; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) s += m[r + 3 * c] * l[c];
+    u[r] = s;
+  }
+}
+
+// covariance of one structure at lag l and its derivatives dc[NGRAD]
+template <int DIM>
+__device__ inline double cov_grad(const Model& M, const double (*dmat)[9], const double* l, double* dc) {
+  const int vt = M.vtype[0];
+  const double sill = M.sill[0];
+  double u[DIM];
+  mat_apply<DIM>(M.mat[0], l, u);
+  double hh = 0.0;
+#pragma unroll
+  for (int r = 0; r < DIM; ++r) hh += u[r] * u[r];
+  const double h = sqrt(hh);
+#pragma unroll
+  for (int p = 0; p < NGRAD; ++p) dc[p] = 0.0;
+  const double eps = 2.2250738585072014e-308 * 1.0e3;
+  double c, dcdh = 0.0;
+  if (M.mode == COV_TABLE_B && h > 0.0 && h < M.zb[3]) {
+    const int k = (h >= M.zb[1]) + (h >= M.zb[2]);
+    double fi = (h - M.zb[k]) * M.zinv[k];
+    const int ifi = (int)fi;
+    const int il = min(ifi, M.zn[k] - 1);
+    const int i = M.zoff[k] + il;
+    fi = fi - (double)ifi;
+    const double t0 = M.tab[i], t1 = M.tab[i + 1];
+    c = t0 + fi * (t1 - t0);
+    dcdh = (t1 - t0) * M.zinv[k];
+    const double step = 1.0 / M.zinv[k];
+    const double r0 = i == 0 ? 1.0 : corefunc(vt, M.zb[k] + (double)il * step);
+    const double r1 = corefunc(vt, M.zb[k] + (double)(il + 1) * step);
+    dc[1] = r0 + fi * (r1 - r0);
+    if (i == 0) dc[0] = 1.0 - fi;
+  } else if (M.mode == COV_TABLE_B && h <= 0.0) {
+    c = M.tab[0];
+    dc[0] = 1.0;
+    dc[1] = 1.0;
+  } else if (h > eps && (M.mode != COV_TABLE_B || M.b_tail)) {
+    const double rho = corefunc(vt, h);
+    c = sill * rho;
+    dc[1] = rho;
+    dcdh = sill * corefunc_d(vt, h);
+  } else if (h > eps) {
+    c = 0.0;  // beyond the table of a shape without a tail
+  } else {
+    c = sill + M.nugget[0];
+    dc[0] = 1.0;
+    dc[1] = 1.0;
+  }
+  if (dcdh != 0.0 && h > 0.0) {
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+      double du[DIM];
+      mat_apply<DIM>(dmat[g], l, du);
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) s += u[r] * du[r];
+      dc[2 + g] = dcdh * s / h;
+    }
+  }
+  return c;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+constexpr int GMAX = 64;
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_loo_grad(GradArgs a) {
+  extern __shared__ double Ks[];  // mmax x (mmax + 1)
+  __shared__ double sx[DIM][GMAX];
+  __shared__ double rh[3][GMAX];
+  const int b = blockIdx.x;
+  const int j = threadIdx.x;
+  const Model& M = *a.model;
+  const int n = a.nnear[b];
+  const double qnan = __builtin_nan("");
+  if (n < 2) {
+    // no neighbour: NaN; one neighbour: the reference skips the solve
+    // (kriging_base.F90:2127), NaN here as in the forward path
+    if (j == 0) a.zhat[b] = qnan;
+    if (j < NGRAD) a.dz[(size_t)b * NGRAD + j] = 0.0;
+    return;
+  }
+  const int ld = a.mmax + 1;
+  const int* in = a.inear + (size_t)b * a.mmax;
+  const bool row = j < n;
+  const int idx = row ? in[j] : 0;
+  double x0[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) x0[k] = a.bcoord[k + DIM * (size_t)b];
+  // neighbour coordinates relative to the target (node-local origin)
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) sx[k][j] = row ? a.ocoord[k + DIM * (size_t)idx] - x0[k] : 0.0;
+  __syncthreads();
+  double dc[NGRAD];
+  if (row) {
+    for (int m = 0; m <= j; ++m) {
+      double l[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) l[k] = sx[k][j] - sx[k][m];
+      Ks[j * ld + m] = m == j ? M.cov0 + a.ovar[idx] : cov_grad<DIM>(M, a.dmat, l, dc);
+    }
+    double l[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) l[k] = sx[k][j];
+    rh[0][j] = cov_grad<DIM>(M, a.dmat, l, dc);
+    rh[1][j] = 1.0;
+    rh[2][j] = a.ovalue[idx];
+  }
+  __syncthreads();
+  // Cholesky K = L L' (lower triangle, in place), lanes own rows
+  bool ok = true;
+  for (int c = 0; c < n; ++c) {
+    const double piv = Ks[c * ld + c];
+    if (!(piv > 0.0)) {
+      ok = false;
+      break;
+    }
+    const double d = sqrt(piv);
+    __syncthreads();
+    if (j == c) Ks[c * ld + c] = d;
+    if (j > c && row) Ks[j * ld + c] = Ks[j * ld + c] / d;
+    __syncthreads();
+    if (j > c && row) {
+      const double ljc = Ks[j * ld + c];
+      for (int m = c + 1; m <= j; ++m) Ks[j * ld + m] -= ljc * Ks[m * ld + c];
+    }
+    __syncthreads();
+  }
+  if (!ok) {
+    if (j == 0) a.zhat[b] = qnan;
+    if (j < NGRAD) a.dz[(size_t)b * NGRAD + j] = qnan;
+    return;
+  }
+  // K^-1 k, K^-1 1, K^-1 z: forward then backward substitution
+  for (int c = 0; c < n; ++c) {
+    if (j < 3) rh[j][c] = rh[j][c] / Ks[c * ld + c];
+    __syncthreads();
+    if (j > c && row) {
+      const double l = Ks[j * ld + c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) rh[r][j] -= l * rh[r][c];
+    }
+    __syncthreads();
+  }
+  for (int c = n - 1; c >= 0; --c) {
+    if (j < 3) rh[j][c] = rh[j][c] / Ks[c * ld + c];
+    __syncthreads();
+    if (j < c) {
+      const double l = Ks[c * ld + j];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) rh[r][j] -= l * rh[r][c];
+    }
+    __syncthreads();
+  }
+  const double kk = row ? rh[0][j] : 0.0, k1 = row ? rh[1][j] : 0.0, kz = row ? rh[2][j] : 0.0;
+  const double uu = wave_sum(k1), uk = wave_sum(kk), uz = wave_sum(kz);
+  const double mu = (uk - 1.0) / uu;
+  const double w = kk - mu * k1;      // kriging weight of row j
+  const double y = kz - (uz / uu) * k1;  // adjoint weight of row j
+  __shared__ double sw[GMAX];
+  sw[j] = row ? w : 0.0;
+  const double zj = row ? a.ovalue[idx] : 0.0;
+  const double zh = wave_sum(row ? w * zj : 0.0);
+  __syncthreads();
+  // row j of  y' dk - y' dK w
+  double g[NGRAD];
+#pragma unroll
+  for (int p = 0; p < NGRAD; ++p) g[p] = 0.0;
+  if (row) {
+    double l[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) l[k] = sx[k][j];
+    (void)cov_grad<DIM>(M, a.dmat, l, dc);
+#pragma unroll
+    for (int p = 0; p < NGRAD; ++p) g[p] = y * dc[p];
+    for (int m = 0; m < n; ++m) {
+      const double wm = sw[m];
+      if (m == j) {
+        g[0] -= y * wm;  // diagonal: d cov0 / d nugget = d cov0 / d sill = 1
+        g[1] -= y * wm;
+        continue;
+      }
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) l[k] = sx[k][j] - sx[k][m];
+      (void)cov_grad<DIM>(M, a.dmat, l, dc);
+#pragma unroll
+      for (int p = 0; p < NGRAD; ++p) g[p] -= y * wm * dc[p];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NGRAD; ++p) g[p] = wave_sum(g[p]);
+  if (j == 0) a.zhat[b] = zh;
+#pragma unroll
+  for (int p = 0; p < NGRAD; ++p)
+    if (j == p) a.dz[(size_t)b * NGRAD + p] = g[p];
+}
+
+hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st) {
+  if (a.nb <= 0) return hipSuccess;
+  if (a.mmax > GMAX) return hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * (size_t)a.mmax * (a.mmax + 1);
+  if (a.ndim == 1) hipLaunchKernelGGL(k_loo_grad<1>, dim3(a.nb), dim3(64), lds, st, a);
+  else if (a.ndim == 2) hipLaunchKernelGGL(k_loo_grad<2>, dim3(a.nb), dim3(64), lds, st, a);
+  else hipLaunchKernelGGL(k_loo_grad<3>, dim3(a.nb), dim3(64), lds, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace kk

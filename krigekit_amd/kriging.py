@@ -81,6 +81,7 @@ def _load_lib():
     sig("kk_get_mmax", [_i64, _pi])
     sig("kk_device_synchronize", [])
     sig("kk_copy_probe", [_c_double, _c_int, _pd])
+    sig("kk_loo_grad", [_i64, _pd, _pd, _pd, _pd])
     _lib = lib
     return lib
 
@@ -280,7 +281,32 @@ class Kriging:
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
                     total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]),
                     exact_search_queries=int(out[7]), fast_search_kernel_ms=out[9],
-                    exact_search_kernel_ms=out[10])
+                    exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11])
+
+    GRAD_PARAMS = ("nugget", "sill", "a_major", "a_minor1", "a_minor2", "azimuth", "dip", "plunge")
+
+    def loo_grad(self, per_block: bool = False):
+        """Leave-one-out MSE and its gradient with respect to the parameters of
+        the single variogram structure, from the adjoint kernel (FP64).
+
+        Needs ``cross_validation=True`` and ``set_grid_cv()``.  Returns
+        ``(mse, grad)`` with ``grad`` a dict over ``GRAD_PARAMS`` (angles in
+        degrees, as ``set_vgm`` takes them); with ``per_block`` also the
+        leave-one-out estimates and their (nblock, 8) derivatives."""
+        nbc = ctypes.c_int(0)
+        _check(_load_lib().krige_get_nblocks(self._handle, ctypes.byref(nbc)), "krige_get_nblocks")
+        nb = nbc.value
+        mse = ctypes.c_double(0.0)
+        grad = np.zeros(8)
+        zhat = np.zeros(nb) if per_block else None
+        dz = np.zeros((nb, 8)) if per_block else None
+        _check(_load_lib().kk_loo_grad(self._handle, ctypes.byref(mse), _dp(grad),
+                                        _dp(zhat) if per_block else None,
+                                        _dp(dz) if per_block else None), "kk_loo_grad")
+        g = dict(zip(self.GRAD_PARAMS, grad.tolist()))
+        if per_block:
+            return mse.value, g, zhat, dz
+        return mse.value, g
 
     def set_search_mode(self, fast=True, reorder: bool = True):
         """fast: True/2 wave-cooperative two-pass search (default), 1 per-lane

@@ -189,6 +189,15 @@ def onedim(coords, values, target, vgm, nmax=50):
                  [dict(ivar=1, jvar=1, **vgm)], nmax=nmax)
 
 
+def loo_grad_problem(n_uniform=900, n_dup=100, nmax=16, shrink=0.15):
+    """C4-shaped leave-one-out problem (anisotropic spherical, jittered
+    duplicates), shrunk so that neighbour lags span the variogram ranges"""
+    p = synth.loo2d(n_uniform=n_uniform, n_dup=n_dup, nmax=nmax)
+    p["obs"][0]["coord"] = p["obs"][0]["coord"] * shrink
+    p["name"] = "loo_grad"
+    return p
+
+
 def _named(p, name):
     p["name"] = name
     return p
