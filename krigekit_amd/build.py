@@ -28,6 +28,7 @@ UNITS = [
     ("search.hip", ["-ffp-contract=off"]),
     ("solve.hip", []),
     ("loograd.hip", []),
+    ("cokrige.hip", []),
     ("capi.cpp", ["-x", "hip"]),
 ]
 

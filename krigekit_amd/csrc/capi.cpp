@@ -500,6 +500,13 @@ struct Obs {
   HostTree tree;
 };
 
+// device copies of one variable's observations, tree and neighbour lists
+// (multi-variable solves)
+struct DevVar {
+  DBuf ocoord, socoord, ovalue, ovar, rdata, ind, ni, nd, qcoord, nnear, inear, sflag, xlist, hdis, hidx;
+  int mmax = 0;
+};
+
 struct Krige {
   int ndim = 2, nvar = 1, ndrift = 0, unbias = 1, nsim = 0;
   bool aniso_search = false, weight_correction = false, use_old_weight = false, store_weight = false;
@@ -540,6 +547,10 @@ struct Krige {
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
   Model model{};
   std::vector<double> tab;
+  // multi-variable path
+  DevVar dv[kk::MAXVAR];
+  DBuf d_models, d_mvtab, d_mvest, d_mvvar;
+  std::vector<double> est_mv, var_mv;  // [nb * nvar], [nb * nvar * nvar]
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // exact search of flagged queries, overlapped with the solve
   hipEvent_t ev[12] = {nullptr};
@@ -971,10 +982,39 @@ bool validate_for_solve(Krige& K) {
     set_error(W, "Call initialize() before solve.");
     return false;
   }
-  if (K.nvar != 1 || K.nsim != 0 || K.ndrift != 0) {
-    set_error(W, "this backend solves single-variable kriging (nvar=1, nsim=0, ndrift=0)");
+  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift != 0) {
+    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift=0");
     return false;
   }
+  if (K.nvar > 1 && K.cross_validation) {
+    set_error(W, "cross validation with nvar > 1 is not supported by this backend");
+    return false;
+  }
+  for (int v = 1; v < K.nvar; ++v) {
+    if (K.obs[v].n == 0) {
+      set_error(W, "set_obs() needs to be called for every variable before solve().");
+      return false;
+    }
+    if (!K.obs[v].search_set) {
+      set_error(W, "set_search() needs to be called for every variable before solve().");
+      return false;
+    }
+  }
+  for (int i = 0; i < K.nvar; ++i)
+    for (int j = i; j < K.nvar; ++j) {
+      const auto& vs = K.vgm[(size_t)i + (size_t)K.nvar * j];
+      if (vs.empty()) {
+        set_error("t_kriging%validate_vgm", "variogram not set for ivar=" + std::to_string(i + 1) +
+                                                ", jvar=" + std::to_string(j + 1));
+        return false;
+      }
+      for (const auto& st : vs)
+        if (st.nugget < 0.0 || st.sill < 0.0 || st.a_major <= 0.0 || st.a_minor1 <= 0.0 || st.a_minor2 <= 0.0) {
+          set_error("t_kriging%validate_vgm", "variogram is not valid for ivar=" + std::to_string(i + 1) +
+                                                  ", jvar=" + std::to_string(j + 1));
+          return false;
+        }
+    }
   if (K.use_old_weight || K.varying_vgm || K.weight_correction) {
     set_error(W, "use_old_weight, varying_vgm and weight_correction are not supported by this backend");
     return false;
@@ -1122,9 +1162,252 @@ bool compute_base_cov(Krige& K) {
   return upload(K.d_basecov, base, W);
 }
 
+// ---------------------------------------------------------------------------
+// multi-variable (co)kriging: one neighbour search per variable, then one
+// workgroup per block solving the bordered system (cokrige.hip)
+// ---------------------------------------------------------------------------
+bool search_var(Krige& K, int v, hipStream_t st) {
+  const char* W = "t_kriging_base%solve_base";
+  Obs& O = K.obs[v];
+  DevVar& D = K.dv[v];
+  const int nd = K.ndim, nb = K.nblock;
+  const bool sector = O.sector && O.need_search;
+  const int nmax = O.nmax;
+  D.mmax = sector ? (1 << nd) * nmax : nmax;
+  if (!D.nnear.ensure(sizeof(int) * nb) || !D.inear.ensure(sizeof(int) * (size_t)nb * D.mmax)) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  kk::SearchArgs sa{};
+  sa.ndim = nd;
+  sa.qcoord = O.aniso ? D.qcoord.as<double>() : K.d_bcoord.as<double>();
+  sa.ocoord = D.socoord.as<double>();
+  sa.nmax = nmax;
+  sa.cv = 0;
+  sa.maxdist2 = O.maxdist2;
+  sa.mmax = D.mmax;
+  sa.nnear = D.nnear.as<int>();
+  sa.inear = D.inear.as<int>();
+  if (!O.need_search) {
+    kk::Rot3 R;
+    std::memcpy(R.m, O.rot, sizeof R.m);
+    sa.nq = nb;
+    HIPCHK(kk::launch_search_all(sa, O.n, D.ocoord.as<double>(), R, st), W);
+    return true;
+  }
+  sa.tree.dim = nd;
+  sa.tree.n = O.n;
+  sa.tree.nnodes = O.tree.nnodes();
+  sa.tree.root = O.tree.root;
+  sa.tree.rdata = D.rdata.as<double>();
+  sa.tree.ind = D.ind.as<int>();
+  sa.tree.ni = D.ni.as<int>();
+  sa.tree.nd = D.nd.as<double>();
+  sa.sort = sector;
+  sa.k = sector ? std::min(O.n, (1 << nd) * nmax * 4) : nmax;
+  int nexact = nb;
+  const int* list = nullptr;
+  if (K.search_mode > 0 && kk::fast_search_lds(nd, sector, sa.k, nmax) <= 64 * 1024) {
+    if (!D.sflag.ensure(sizeof(int) * nb) || !D.xlist.ensure(sizeof(int) * nb)) {
+      set_error(W, "device allocation failed (search flags)");
+      return false;
+    }
+    sa.qflag = D.sflag.as<int>();
+    sa.nq = nb;
+    if (K.search_mode == 2) HIPCHK(kk::launch_search_wave(sa, st), W);
+    else HIPCHK(kk::launch_search_fast(sa, st), W);
+    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_collect(D.sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, D.xlist.as<int>(), st), W);
+    HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    list = D.xlist.as<int>();
+  }
+  if (nexact > 0) {
+    const int chunk = std::min(nexact, 1 << 20);
+    if (!D.hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) || !D.hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
+      set_error(W, "device allocation failed (search scratch)");
+      return false;
+    }
+    sa.hdis = D.hdis.as<double>();
+    sa.hidx = D.hidx.as<int>();
+    sa.hstride = chunk;
+    sa.qflag = nullptr;
+    for (int q0 = 0; q0 < nexact; q0 += chunk) {
+      sa.q0 = list ? 0 : q0;
+      sa.qlist = list ? list + q0 : nullptr;
+      sa.nq = std::min(chunk, nexact - q0);
+      HIPCHK(kk::launch_search(sa, st), W);
+    }
+  }
+  return true;
+}
+
+bool do_solve_mv(Krige& K, bool fetch) {
+  const char* W = "t_kriging_base%solve_base";
+  if (!ensure_stream(K)) return false;
+  hipStream_t st = K.stream;
+  const int nd = K.ndim, nb = K.nblock, nvar = K.nvar;
+  // models of every ordered pair; in table mode only vgm(a, b) with a <= b
+  // carries tables (the pairs the reference builds them for,
+  // kriging.F90:410-420), the others are evaluated analytically
+  if (!K.model_ok) {
+    std::vector<Model> ms((size_t)nvar * nvar);
+    std::vector<std::vector<double>> tabs(ms.size());
+    size_t total = 0;
+    for (int a = 0; a < nvar; ++a)
+      for (int b = 0; b < nvar; ++b) {
+        const size_t i = (size_t)a + (size_t)nvar * b;
+        if (!build_model(K.vgm[i], nd, K.table_mode && a <= b, ms[i], tabs[i])) {
+          set_error(W, "invalid variogram model");
+          return false;
+        }
+        total += tabs[i].size();
+      }
+    std::vector<double> all;
+    all.reserve(total);
+    std::vector<size_t> off(ms.size());
+    for (size_t i = 0; i < ms.size(); ++i) {
+      off[i] = all.size();
+      all.insert(all.end(), tabs[i].begin(), tabs[i].end());
+    }
+    if (!upload(K.d_mvtab, all, W)) return false;
+    for (size_t i = 0; i < ms.size(); ++i) ms[i].tab = tabs[i].empty() ? nullptr : K.d_mvtab.as<double>() + off[i];
+    if (!upload(K.d_models, ms, W)) return false;
+    K.model_ok = true;
+  }
+  if (!K.dev_inputs_ok) {
+    if (!upload(K.d_bcoord, K.bcoord, W) || !upload(K.d_rs, K.rangescale, W) || !upload(K.d_ln, K.localnugget, W))
+      return false;
+    if (!K.soff.empty() && (!upload(K.d_scoord, K.scoord, W) || !upload(K.d_sweight, K.sweight, W))) return false;
+    for (int v = 0; v < nvar; ++v) {
+      Obs& O = K.obs[v];
+      DevVar& D = K.dv[v];
+      if (!upload(D.ocoord, O.coord, W) || !upload(D.ovalue, O.value, W) || !upload(D.ovar, O.var, W) ||
+          !upload(D.socoord, O.aniso ? O.rcoord : O.coord, W))
+        return false;
+      if (O.need_search && (!upload(D.rdata, O.tree.rdata, W) || !upload(D.ind, O.tree.ind, W) ||
+                            !upload(D.ni, O.tree.ni, W) || !upload(D.nd, O.tree.nd, W)))
+        return false;
+      if (O.aniso) {
+        std::vector<double> q(K.bcoord.size());
+        kk::Rot3 R;
+        std::memcpy(R.m, O.rot, sizeof R.m);
+        for (int b = 0; b < nb; ++b) {
+          if (nd == 1) R.apply<1>(&K.bcoord[(size_t)nd * b], &q[(size_t)nd * b]);
+          else if (nd == 2) R.apply<2>(&K.bcoord[(size_t)nd * b], &q[(size_t)nd * b]);
+          else R.apply<3>(&K.bcoord[(size_t)nd * b], &q[(size_t)nd * b]);
+        }
+        if (!upload(D.qcoord, q, W)) return false;
+      }
+    }
+    K.dev_inputs_ok = true;
+  }
+  if (!K.d_counts.ensure(sizeof(int) * 16) || !K.d_mvest.ensure(sizeof(double) * (size_t)nb * nvar) ||
+      !K.d_mvvar.ensure(sizeof(double) * (size_t)nb * nvar * nvar) || !K.d_flag.ensure(sizeof(int) * nb)) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  HIPCHK(hipEventRecord(K.ev[0], st), W);
+  int rows = 0;
+  for (int v = 0; v < nvar; ++v) {
+    if (!search_var(K, v, st)) return false;
+    rows += K.dv[v].mmax;
+  }
+  HIPCHK(hipEventRecord(K.ev[1], st), W);
+  if (rows > 128) {
+    set_error(W, "cokriging neighbourhoods larger than 128 data rows are not supported by the GPU solver (" +
+                     std::to_string(rows) + ")");
+    return false;
+  }
+  const int nsub = K.soff.empty() ? 1 : K.soff[1] - K.soff[0];
+  if (nsub > 64) {
+    set_error(W, "blocks with more than 64 integration nodes are not supported with nvar > 1");
+    return false;
+  }
+  kk::MVArgs a{};
+  a.models = K.d_models.as<Model>();
+  a.ndim = nd;
+  a.nvar = nvar;
+  a.nb = nb;
+  a.std_ck = K.std_ck;
+  a.unbias = K.unbias;
+  a.nsub = nsub;
+  a.ld = rows + 1;
+  for (int v = 0; v < nvar; ++v) a.sk_mean[v] = K.obs[v].sk_mean;
+  a.bound_lo = K.bounds[0];
+  a.bound_hi = K.bounds[1];
+  a.bcoord = K.d_bcoord.as<double>();
+  a.scoord = nsub > 1 ? K.d_scoord.as<double>() : nullptr;
+  a.sweight = nsub > 1 ? K.d_sweight.as<double>() : nullptr;
+  a.rangescale = K.rangescale.empty() ? nullptr : K.d_rs.as<double>();
+  a.localnugget = K.localnugget.empty() ? nullptr : K.d_ln.as<double>();
+  for (int v = 0; v < nvar; ++v) {
+    a.ocoord[v] = K.dv[v].ocoord.as<double>();
+    a.ovalue[v] = K.dv[v].ovalue.as<double>();
+    a.ovar[v] = K.dv[v].ovar.as<double>();
+    a.nnear[v] = K.dv[v].nnear.as<int>();
+    a.inear[v] = K.dv[v].inear.as<int>();
+    a.mmax[v] = K.dv[v].mmax;
+  }
+  a.est = K.d_mvest.as<double>();
+  a.var = K.d_mvvar.as<double>();
+  a.flag = K.d_flag.as<int>();
+  HIPCHK(kk::launch_solve_mv(a, st), W);
+  HIPCHK(hipEventRecord(K.ev[4], st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[1]), W);
+  for (double& x : K.timing) x = 0.0;
+  K.timing[0] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[1], K.ev[4]), W);
+  K.timing[2] = ms;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
+  K.timing[4] = ms;
+  K.timing[7] = -1;
+  if (!fetch) return true;
+  K.est_mv.resize((size_t)nb * nvar);
+  K.var_mv.resize((size_t)nb * nvar * nvar);
+  HIPCHK(hipMemcpy(K.est_mv.data(), K.d_mvest.p, sizeof(double) * K.est_mv.size(), hipMemcpyDeviceToHost), W);
+  HIPCHK(hipMemcpy(K.var_mv.data(), K.d_mvvar.p, sizeof(double) * K.var_mv.size(), hipMemcpyDeviceToHost), W);
+  std::vector<int> flag(nb), tot(nb, 0), nn(nb);
+  HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+  for (int v = 0; v < nvar; ++v) {
+    HIPCHK(hipMemcpy(nn.data(), K.dv[v].nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+    for (int b = 0; b < nb; ++b) tot[b] += nn[b];
+  }
+  int nfail = 0, nsolved = 0;
+  long first = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (flag[b] == 3) {
+      ++nfail;
+      if (first < 0) first = b + 1;
+    } else if (tot[b] > 0) {
+      ++nsolved;
+    }
+    if (tot[b] == 0 && !K.neglect_error) {
+      set_error("t_kriging_base%assemble_linear_system", "not enough neighbors for kriging at block", b + 1);
+      return false;
+    }
+  }
+  K.stats[0] = nfail;
+  K.stats[1] = nsolved;
+  K.stats[2] = K.stats[3] = K.stats[4] = 0;
+  if (nfail > 0 && !K.neglect_error) {
+    set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
+    return false;
+  }
+  return true;
+}
+
+
 bool do_solve(Krige& K, bool fetch) {
   K.solved = false;
   if (!validate_for_solve(K)) return false;
+  if (K.nvar > 1) {
+    if (!do_solve_mv(K, fetch)) return false;
+    K.solved = true;
+    return true;
+  }
   if (!stage_inputs(K)) return false;
   if (!compute_base_cov(K)) return false;
   if (!run_pipeline(K)) return false;
@@ -1679,7 +1962,8 @@ int krige_get_estimate(int64_t h, int nsim_c, int nblocks, double* out) {
   if (!K) return ierr();
   if (!check_solved(K, "krige_get_estimate")) return ierr();
   for (int s = 0; s < nsim_c; ++s)
-    for (int b = 0; b < nblocks; ++b) out[b + (size_t)nblocks * s] = K->est[b];
+    for (int b = 0; b < nblocks; ++b)
+      out[b + (size_t)nblocks * s] = K->nvar > 1 ? K->est_mv[b] : K->est[b];
   return 0;
 }
 
@@ -1691,7 +1975,8 @@ int krige_get_estimate_all(int64_t h, int nblocks, int nvar_c, int nsim_c, doubl
   for (int s = 0; s < nsim_c; ++s)
     for (int v = 0; v < nvar_c; ++v)
       for (int b = 0; b < nblocks; ++b)
-        out[b + (size_t)nblocks * (v + (size_t)nvar_c * s)] = K->est[b];
+        out[b + (size_t)nblocks * (v + (size_t)nvar_c * s)] =
+            K->nvar > 1 ? K->est_mv[b + (size_t)nblocks * v] : K->est[b];
   return 0;
 }
 
@@ -1700,7 +1985,7 @@ int krige_get_variance(int64_t h, int nblocks, double* out) {
   Krige* K = checked(h);
   if (!K) return ierr();
   if (!check_solved(K, "krige_get_variance")) return ierr();
-  for (int b = 0; b < nblocks; ++b) out[b] = K->var[b];
+  for (int b = 0; b < nblocks; ++b) out[b] = K->nvar > 1 ? K->var_mv[b] : K->var[b];
   return 0;
 }
 
@@ -1712,7 +1997,8 @@ int krige_get_variance_all(int64_t h, int nblocks, int nvar_c, double* out) {
   for (int i = 0; i < nvar_c; ++i)
     for (int j = 0; j < nvar_c; ++j)
       for (int b = 0; b < nblocks; ++b)
-        out[b + (size_t)nblocks * (i + (size_t)nvar_c * j)] = K->var[b];
+        out[b + (size_t)nblocks * (i + (size_t)nvar_c * j)] =
+            K->nvar > 1 ? K->var_mv[b + (size_t)nblocks * (i + (size_t)K->nvar * j)] : K->var[b];
   return 0;
 }
 

@@ -49,6 +49,9 @@ hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
 hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
+// multi-variable block solve; lds bytes from mv_lds_bytes
+size_t mv_lds_bytes(int ld, int ncol);
+hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
                           hipStream_t st);

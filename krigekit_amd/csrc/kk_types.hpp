@@ -96,6 +96,35 @@ struct GradArgs {
   double* dz;                 // [nb * NGRAD]
 };
 
+// Multi-variable (co)kriging, one workgroup per block.
+constexpr int MAXVAR = 4;
+struct MVArgs {
+  const Model* models;        // vgm(a+1, b+1) at a + nvar * b
+  int ndim;
+  int nvar;
+  int nb;
+  int std_ck;                 // one unbiasedness row per variable (else one in total)
+  int unbias;                 // 0: simple kriging, no constraint rows
+  int nsub;                   // integration nodes per block (1 = point)
+  int ld;                     // leading dimension of the LDS system (> rows)
+  double sk_mean[MAXVAR];
+  double bound_lo, bound_hi;
+  const double* bcoord;       // (ndim, nb) block centres
+  const double* scoord;       // (ndim, nb * nsub) integration nodes
+  const double* sweight;      // (nb * nsub)
+  const double* rangescale;   // [nb] or null
+  const double* localnugget;  // [nb] or null
+  const double* ocoord[MAXVAR];
+  const double* ovalue[MAXVAR];
+  const double* ovar[MAXVAR];
+  const int* nnear[MAXVAR];
+  const int* inear[MAXVAR];
+  int mmax[MAXVAR];
+  double* est;                // est[b + nb * q]
+  double* var;                // var[b + nb * (i + nvar * j)]
+  int* flag;                  // 3: factorisation failed (NaN results)
+};
+
 struct SolveArgs {
   const Model* model;         // device copy of the covariance model
   int ndim;

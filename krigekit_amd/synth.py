@@ -105,3 +105,60 @@ def loo2d(n_uniform=90000, n_dup=10000, seed=1004, nmax=16):
                 vgm=[dict(ivar=1, jvar=1, vtype="sph", nugget=0.1, sill=1.0, a_major=1500.0,
                           a_minor1=600.0, azimuth=45.0)],
                 search=[dict(ivar=1, sector_search=False)])
+
+
+def _sinusoid_field_2d_fixed(seed, xy, nterm=8, lmin=500.0, lmax=4000.0):
+    """a sinusoid-sum field whose wavelengths/phases come from their own seed,
+    so that observation sets of different sizes sample the same field"""
+    return _sinusoid_field_2d(np.random.default_rng(seed), xy, nterm, lmin, lmax)
+
+
+def gq_nodes(centres, blocksize):
+    """Gauss-Legendre 4-point nodes and weights per block (block_type=-4):
+    index i fastest, then j, then k; fractions +-0.4305681558, +-0.1699905218
+    of the block size, weights 0.1739274226 and 0.3260725774 per axis"""
+    frac = np.array([0.4305681558, 0.1699905218, -0.1699905218, -0.4305681558])
+    wgt = np.array([0.1739274226, 0.3260725774, 0.3260725774, 0.1739274226])
+    centres = np.asarray(centres, dtype=np.float64)
+    nd = centres.shape[1]
+    ngq = 4 ** nd
+    ii = np.array([[g % 4, (g // 4) % 4, g // 16] for g in range(ngq)])[:, :nd]
+    off = frac[ii] * np.asarray(blocksize, dtype=np.float64)[None, :]
+    w = np.prod(wgt[ii], axis=1)
+    sub = (centres[:, None, :] + off[None, :, :]).reshape(-1, nd)
+    return sub, np.tile(w, len(centres))
+
+
+def ck2d(scale=1.0, seed=1003, grid_side=None, std_ck=True):
+    """Two-variable cokriging under a linear model of coregionalisation.
+
+    20,000 primary and 200,000 secondary observations (times `scale`) uniform
+    in a square of side 20 km * sqrt(scale); z1 = f + N(0, 0.1^2),
+    z2 = 0.7 f + 0.3 g + N(0, 0.1^2) with f, g seeded sinusoid-sum fields.
+    LMC: spherical range 1200 m, sills [[1.0, 0.6], [0.6, 0.8]], plus
+    Gaussian range 400 m, sills [[0.3, 0.1], [0.1, 0.2]].  Neighbours: 24
+    primary + 48 secondary.  Targets: blocks at 20 m spacing (centres
+    10 + 20 i), 20 m x 20 m with 4 x 4 Gauss-Legendre nodes."""
+    rng = np.random.default_rng(seed)
+    side = 20000.0 * np.sqrt(scale)
+    n1, n2 = max(int(20000 * scale), 30), max(int(200000 * scale), 60)
+    x1 = rng.uniform(0.0, side, (n1, 2))
+    x2 = rng.uniform(0.0, side, (n2, 2))
+    f1, f2 = _sinusoid_field_2d_fixed(seed + 1, x1), _sinusoid_field_2d_fixed(seed + 1, x2)
+    g2 = _sinusoid_field_2d_fixed(seed + 2, x2)
+    z1 = f1 + rng.normal(0.0, 0.1, n1)
+    z2 = 0.7 * f2 + 0.3 * g2 + rng.normal(0.0, 0.1, n2)
+    ns = int(round(side / 20.0)) if grid_side is None else grid_side
+    ax = 10.0 + 20.0 * np.arange(ns)
+    gx, gy = np.meshgrid(ax, ax, indexing="ij")
+    centres = np.stack([gx.ravel(), gy.ravel()], axis=1)
+    bs = np.array([20.0, 20.0])
+    sub, w = gq_nodes(centres, bs)
+    vgm = []
+    for (i, j), s1, s2 in (((1, 1), 1.0, 0.3), ((1, 2), 0.6, 0.1), ((2, 2), 0.8, 0.2)):
+        vgm.append(dict(ivar=i, jvar=j, vtype="sph", sill=s1, a_major=1200.0))
+        vgm.append(dict(ivar=i, jvar=j, vtype="gau", sill=s2, a_major=400.0))
+    return dict(name="ck2d", ndim=2, nvar=2, std_ck=std_ck, unbias=1,
+                obs=[dict(coord=x1, value=z1, nmax=24), dict(coord=x2, value=z2, nmax=48)],
+                grid=centres, blocksize=bs, nsub=16, sub_coord=sub, sub_weight=w, vgm=vgm,
+                search=[dict(ivar=1, sector_search=False), dict(ivar=2, sector_search=False)])

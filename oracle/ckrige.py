@@ -1,0 +1,173 @@
+"""Multi-variable (co)kriging restated with numpy -- TEST INFRASTRUCTURE
+(parity oracle for the library's cokriging path; only tests/ use it).
+
+Follows the reference's per-block computation for nvar > 1:
+  * neighbours: one search per variable, each on its own observation set
+    (kriging_base.F90:3172-3175), here taken from the C oracle's search;
+  * rows: the variables' neighbour groups in order (istart, :3177-3190),
+    each group sorted by index;
+  * K(r, c) = C_{v(r) v(c)}((x_r - x_c) / rs), diagonal cov0 + obs variance +
+    local nugget (kriging.F90:714-849), covariances from the C oracle
+    (okrige.cov on the variable pair's structures);
+  * table mode: the reference builds lookup tables only for vgm(a, b) with
+    a <= b (kriging.F90:410-420) and evaluates the others analytically
+    (variogram.f90:365); K's cross blocks use vgm(col var, row var) with the
+    column group after the row group, so they are analytic, and the
+    right-hand side uses vgm(group var, target var);
+  * constraint rows (kriging_base.F90:614-624, 940-950): std_ck -> one per
+    variable (row v(r) of F is 1), otherwise one row of ones; none for
+    simple kriging;
+  * right-hand side per target variable q: sum over the block's integration
+    nodes of w_k C_{v(r) q}((x_r - g_k) / rs), constraint part e_q (std_ck) or
+    1 (kriging.F90:864-939);
+  * estimate (kriging_base.F90:3407-3503): sum of weights x values, the
+    Isaaks-Srivastava mean shift for non-standard cokriging, the simple
+    kriging mean term, bounds;
+  * variance (kriging.F90:968-1028): base_ij - x_i' b_j over weights and
+    multipliers, base = cov0 (points) or the double sum over integration
+    nodes (lag not range-scaled); symmetrised; diagonal and off-diagonal
+    clamped at 0.
+The bordered system is solved with numpy.linalg.solve (any exact solver gives
+the reference's solution to rounding).
+"""
+import numpy as np
+
+from oracle import okrige
+
+
+def _pair_vgm(spec, i, j):
+    a, b = min(i, j), max(i, j)
+    return [v for v in spec["vgm"] if (min(v["ivar"], v["jvar"]), max(v["ivar"], v["jvar"])) == (a, b)]
+
+
+def _cov0(structs):
+    tot, i = 0.0, 0
+    while i < len(structs):
+        g = structs[i].get("sill", 1.0) + structs[i].get("nugget", 0.0)
+        while i + 1 < len(structs) and structs[i + 1].get("product", False):
+            i += 1
+            g *= structs[i].get("sill", 1.0) + structs[i].get("nugget", 0.0)
+        tot += g
+        i += 1
+    return tot
+
+
+def neighbour_lists(spec, nthread=8):
+    """per variable: (nnear, inear 0-based) from the C oracle's search"""
+    out = []
+    for v in range(spec["nvar"]):
+        sv = dict(name="ckv", ndim=spec["ndim"], obs=[spec["obs"][v]], grid=spec["grid"],
+                  vgm=_pair_vgm(spec, v + 1, v + 1), search=[spec["search"][v]],
+                  unbias=spec.get("unbias", 1))
+        o = okrige.solve(sv, table_mode=True, nthread=nthread)
+        out.append((o["nnear"], o["inear"] - 1))
+    return out
+
+
+def solve(spec, table_mode=True, nthread=8):
+    nvar, nd = spec["nvar"], spec["ndim"]
+    std = spec.get("std_ck", True)
+    unbias = spec.get("unbias", 1)
+    centres = np.asarray(spec["grid"], dtype=np.float64)
+    nb = len(centres)
+    nsub = int(spec.get("nsub", 1))
+    if spec.get("sub_coord") is not None:
+        sub = np.asarray(spec["sub_coord"], dtype=np.float64).reshape(nb, nsub, nd)
+        sw = np.asarray(spec["sub_weight"], dtype=np.float64).reshape(nb, nsub)
+    else:
+        sub = centres[:, None, :]
+        sw = np.ones((nb, 1))
+    rs = np.ones(nb) if spec.get("rangescale") is None else np.asarray(spec["rangescale"])
+    ln = np.zeros(nb) if spec.get("localnugget") is None else np.asarray(spec["localnugget"])
+    huge = 1.7976931348623157e308
+    blo, bhi = spec.get("bounds", (-huge, huge))
+    lists = neighbour_lists(spec, nthread)
+    X = [np.asarray(o["coord"], dtype=np.float64) for o in spec["obs"]]
+    Z = [np.asarray(o["value"], dtype=np.float64) for o in spec["obs"]]
+    V = [np.zeros(len(z)) if o.get("variance") is None else np.asarray(o["variance"], dtype=np.float64)
+         for o, z in zip(spec["obs"], Z)]
+    skm = [o.get("sk_mean", 0.0) for o in spec["obs"]]
+    pairs = {(i, j): _pair_vgm(spec, i + 1, j + 1) for i in range(nvar) for j in range(nvar)}
+    cov0 = {k: _cov0(v) for k, v in pairs.items()}
+    base_spec = dict(ndim=nd, obs=[spec["obs"][0]], grid=centres[:1], search=[spec["search"][0]])
+
+    def cov(a, b, lags):
+        """C from vgm(a+1, b+1): table only where the reference built one"""
+        s = dict(base_spec)
+        s["vgm"] = pairs[(a, b)]
+        return okrige.cov(s, np.asarray(lags, dtype=np.float64).reshape(-1, nd),
+                          table_mode=table_mode and a <= b)
+
+    p = (nvar if std else 1) if unbias else 0
+    est = np.full((nb, nvar), np.nan)
+    var = np.full((nb, nvar, nvar), np.nan)
+    nnear = np.stack([l[0] for l in lists], axis=1)
+    for b in range(nb):
+        groups = [l[1][b, :l[0][b]] for l in lists]
+        rv = np.concatenate([np.full(len(g), v) for v, g in enumerate(groups)]).astype(int)
+        n = len(rv)
+        if n == 0:
+            continue
+        xr = np.concatenate([X[v][g] for v, g in enumerate(groups)])
+        zr = np.concatenate([Z[v][g] for v, g in enumerate(groups)])
+        vr = np.concatenate([V[v][g] for v, g in enumerate(groups)])
+        K = np.zeros((n + p, n + p))
+        for i in range(nvar):
+            for j in range(nvar):
+                ri, rj = np.nonzero(rv == i)[0], np.nonzero(rv == j)[0]
+                if len(ri) == 0 or len(rj) == 0:
+                    continue
+                lag = (xr[ri][:, None, :] - xr[rj][None, :, :]) / rs[b]
+                K[np.ix_(ri, rj)] = cov(max(i, j), min(i, j), lag).reshape(len(ri), len(rj))
+        K[np.arange(n), np.arange(n)] = np.array([cov0[(v, v)] for v in rv]) + vr + ln[b]
+        if p:
+            F = np.zeros((n, p))
+            if std:
+                F[np.arange(n), rv] = 1.0
+            else:
+                F[:, 0] = 1.0
+            K[:n, n:] = F
+            K[n:, :n] = F.T
+        R = np.zeros((n + p, nvar))
+        for q in range(nvar):
+            for i in range(nvar):
+                ri = np.nonzero(rv == i)[0]
+                if len(ri) == 0:
+                    continue
+                lag = (xr[ri][:, None, :] - sub[b][None, :, :]) / rs[b]
+                R[ri, q] = cov(i, q, lag).reshape(len(ri), -1) @ sw[b]
+            if p:
+                R[n + (q if std else 0), q] = 1.0
+        if n == 1:
+            # the reference skips the solve of a one-neighbour block
+            # (kriging_base.F90:2127): NaN, as the library reports
+            continue
+        x = np.linalg.solve(K, R)
+        w = x[:n]
+        for q in range(nvar):
+            e = w[:, q] @ zr
+            if not std and unbias and nvar > 1:
+                means = [zr[rv == i].mean() if np.any(rv == i) else 0.0 for i in range(nvar)]
+                for i in range(nvar):
+                    if i != q and np.any(rv == i):
+                        e += (means[q] - means[i]) * w[rv == i, q].sum()
+            if unbias == 0 and skm[q] != 0.0:
+                e += (1.0 - w[:, q].sum()) * skm[q]
+            est[b, q] = min(max(e, blo), bhi)
+        for i in range(nvar):
+            for j in range(i, nvar):
+                if nsub == 1:
+                    base = cov0[(i, j)]
+                else:
+                    g = sub[b]
+                    lag = g[:, None, :] - g[None, :, :]
+                    C = cov(i, j, lag).reshape(nsub, nsub)
+                    iu = np.triu_indices(nsub, 1)
+                    base = cov0[(i, j)] * np.sum(sw[b] ** 2) + 2.0 * np.sum(C[iu] * sw[b][iu[0]] * sw[b][iu[1]])
+                s = base - x[:, i] @ R[:, j]
+                var[b, i, j] = var[b, j, i] = s
+        for i in range(nvar):
+            var[b, i, i] = max(var[b, i, i], 0.0)
+            for j in range(i + 1, nvar):
+                var[b, j, i] = var[b, i, j] = max(var[b, j, i], 0.0)
+    return dict(estimate=est, variance=var, nnear=nnear, lists=lists)

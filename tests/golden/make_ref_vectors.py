@@ -1,7 +1,9 @@
 """Writes ref_vectors.npz: results of the compiled reference engine
 (oracle/_ref/libkriging_ref.so, table-mode covariance as shipped) on the small
-problems of tests/problems.py.  Run in a container where the reference tree
-and oracle/_ref exist:  python tests/golden/make_ref_vectors.py
+problems of tests/problems.py, and ck_vectors.npz: its multi-variable
+(cokriging) results on problems.ck_cases() in table and analytic mode
+(oracle/_ref/libkriging_ref_analytic.so).  Run in a container where the
+reference tree and oracle/_ref exist:  python tests/golden/make_ref_vectors.py
 """
 import os
 import sys
@@ -13,7 +15,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import problems  # noqa: E402
-from runners import run_ref  # noqa: E402
+from runners import run_ref, run_ref_ck  # noqa: E402
 
 
 def main():
@@ -29,6 +31,14 @@ def main():
         out[f"{key}/inear"] = inear.astype(np.uint16) if inear.max() < 65535 else inear
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print("wrote", len(out) // 4, "problems")
+    ck = {}
+    for spec in problems.ck_cases():
+        for mode in ("table", "analytic"):
+            r = run_ref_ck(spec, mode)
+            ck[f"{spec['name']}/{mode}/estimate"] = r["estimate"]
+            ck[f"{spec['name']}/{mode}/variance"] = r["variance"]
+    np.savez_compressed(os.path.join(HERE, "ck_vectors.npz"), **ck)
+    print("wrote", len(ck) // 2, "cokriging results")
 
 
 if __name__ == "__main__":

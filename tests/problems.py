@@ -198,6 +198,48 @@ def loo_grad_problem(n_uniform=900, n_dup=100, nmax=16, shrink=0.15):
     return p
 
 
+def ck_small(std=True, gq=True, nb=200, seed=7, unbias=1, sk=(0.0, 0.0)):
+    """two-variable LMC cokriging (krigekit_amd.synth.ck2d at 0.64% of the
+    observations, same densities), a random subset of its 20 m blocks"""
+    s = synth.ck2d(scale=0.0064, std_ck=std)
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(len(s["grid"]), nb, replace=False)
+    s["grid"] = s["grid"][sel]
+    if gq:
+        s["sub_coord"] = s["sub_coord"].reshape(-1, 16, 2)[sel].reshape(-1, 2)
+        s["sub_weight"] = s["sub_weight"].reshape(-1, 16)[sel].ravel()
+    else:
+        for k in ("sub_coord", "sub_weight", "blocksize"):
+            s.pop(k)
+        s["nsub"] = 1
+    s["unbias"] = unbias
+    for o, m in zip(s["obs"], sk):
+        o["sk_mean"] = m
+    s["name"] = f"ck_std{int(std)}_gq{int(gq)}_ub{unbias}"
+    return s
+
+
+def ck3_small(nb=150, seed=9):
+    """three variables (third = noisy mix of the first two), point support"""
+    s = ck_small(std=True, gq=False, nb=nb, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    x3 = rng.uniform(0.0, 1600.0, (600, 2))
+    z3 = np.sin(x3[:, 0] / 300.0) + rng.normal(0.0, 0.2, len(x3))
+    s["nvar"] = 3
+    s["obs"].append(dict(coord=x3, value=z3, nmax=12))
+    s["search"].append(dict(ivar=3, sector_search=False))
+    sills = {(1, 1): 1.0, (1, 2): 0.6, (1, 3): 0.3, (2, 2): 0.8, (2, 3): 0.2, (3, 3): 0.7}
+    s["vgm"] = [dict(ivar=i, jvar=j, vtype="exp", sill=v, a_major=700.0, nugget=0.05 if i == j else 0.0)
+                for (i, j), v in sills.items()]
+    s["name"] = "ck3_point"
+    return s
+
+
+def ck_cases():
+    return [ck_small(True, True), ck_small(True, False), ck_small(False, True), ck_small(False, False),
+            ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small()]
+
+
 def _named(p, name):
     p["name"] = name
     return p

@@ -113,3 +113,63 @@ def max_errors(x, ref):
     if not nan_ok:
         return np.inf, np.inf
     return float(np.max(e, initial=0.0)), float(np.max(v, initial=0.0))
+
+
+def run_ref_ck(spec, mode="table", nthread=4):
+    """the compiled reference engine on a multi-variable problem"""
+    from oracle.refdrv import RefKriging
+    k = RefKriging(mode=mode, ndim=spec["ndim"], nvar=spec["nvar"], std_ck=spec.get("std_ck", True),
+                   unbias=spec.get("unbias", 1))
+    for v, o in enumerate(spec["obs"], 1):
+        k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
+                  sk_mean=o.get("sk_mean", 0.0))
+    if spec.get("sub_coord") is not None:
+        k.set_grid_block_gq(spec["grid"], spec["blocksize"])
+    else:
+        k.set_grid(spec["grid"])
+    for v in spec["vgm"]:
+        k.set_vgm(**v)
+    for s in spec["search"]:
+        k.set_search(**s)
+    k.solve(nthread=nthread)
+    e, v = k.results()
+    k.close()
+    return dict(estimate=e, variance=v)
+
+
+def make_gpu_ck(spec, **kw):
+    from krigekit_amd.kriging import Kriging
+    k = Kriging(ndim=spec["ndim"], nvar=spec["nvar"], std_ck=spec.get("std_ck", True),
+                unbias=spec.get("unbias", 1), **kw)
+    for v, o in enumerate(spec["obs"], 1):
+        k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
+                  sk_mean=o.get("sk_mean", 0.0))
+    if spec.get("sub_coord") is not None:
+        k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]))
+    else:
+        k.set_grid(spec["grid"])
+    for v in spec["vgm"]:
+        k.set_vgm(**v)
+    for s in spec["search"]:
+        k.set_search(**s)
+    return k
+
+
+def run_gpu_ck(spec, **kw):
+    k = make_gpu_ck(spec, **kw)
+    k.solve()
+    e, v = k.get_results(copy=True)
+    out = dict(estimate=e, variance=v, timing=k.timing, stats=k.solver_stats)
+    k.close()
+    return out
+
+
+def ck_errors(x, ref):
+    """max relative errors (floor 1e-4) of estimates and variances; NaN
+    positions must agree"""
+    e = rel_err(x["estimate"], ref["estimate"])
+    v = rel_err(x["variance"], ref["variance"])
+    if not (np.array_equal(np.isnan(x["estimate"]), np.isnan(ref["estimate"])) and
+            np.array_equal(np.isnan(x["variance"]), np.isnan(ref["variance"]))):
+        return np.inf, np.inf
+    return float(np.max(e, initial=0.0)), float(np.max(v, initial=0.0))

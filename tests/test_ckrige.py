@@ -1,0 +1,76 @@
+"""The numpy cokriging restatement (oracle/ckrige.py) pinned on the CPU
+against the compiled reference's results: the committed vectors
+(tests/golden/ck_vectors.npz) always, the live compiled reference when
+oracle/_ref is built."""
+import numpy as np
+import pytest
+
+import problems
+from oracle import ckrige
+from runners import ck_errors
+
+CASES = problems.ck_cases()
+IDS = [c["name"] for c in CASES]
+
+
+@pytest.fixture(scope="module")
+def vectors():
+    return np.load(problems.GOLDEN + "/ck_vectors.npz")
+
+
+@pytest.mark.parametrize("mode", ["table", "analytic"])
+@pytest.mark.parametrize("spec", CASES, ids=IDS)
+def test_oracle_matches_reference_vectors(spec, mode, vectors):
+    o = ckrige.solve(spec, table_mode=(mode == "table"))
+    ref = dict(estimate=vectors[f"{spec['name']}/{mode}/estimate"],
+               variance=vectors[f"{spec['name']}/{mode}/variance"])
+    ee, ev = ck_errors(o, ref)
+    assert ee < 1e-10 and ev < 1e-10, (ee, ev)
+
+
+def test_oracle_matches_live_reference():
+    from oracle import refdrv
+    if not refdrv.ref_available("table"):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    from runners import run_ref_ck
+    spec = problems.ck_small(True, True, nb=60, seed=3)
+    ee, ev = ck_errors(ckrige.solve(spec), run_ref_ck(spec))
+    assert ee < 1e-10 and ev < 1e-10
+
+
+def test_table_rule_matters():
+    """cross blocks of K are analytic in table mode; evaluating them from the
+    tables instead (a planted mistake) must break the agreement"""
+    from oracle import okrige
+    spec = problems.ck_small(True, False, nb=40, seed=5)
+    v = np.load(problems.GOLDEN + "/ck_vectors.npz")
+    orig = okrige.cov
+
+    def all_tables(s, lags, table_mode=True):
+        return orig(s, lags, table_mode=True)
+
+    okrige.cov = all_tables
+    try:
+        o = ckrige.solve(spec, table_mode=True)
+    finally:
+        okrige.cov = orig
+    ref = ckrige.solve(spec, table_mode=True)
+    assert np.max(np.abs(o["estimate"] - ref["estimate"])) > 1e-9
+
+
+def test_point_cokriging_closed_form():
+    """one neighbour per variable, std_ck: the constraints fix both weights,
+    so the primary estimate is the primary value and the secondary weight 0"""
+    rng = np.random.default_rng(0)
+    spec = dict(name="ck_closed", ndim=2, nvar=2, std_ck=True, unbias=1,
+                obs=[dict(coord=np.array([[0.0, 0.0], [500.0, 0.0]]), value=np.array([2.0, 5.0]), nmax=1),
+                     dict(coord=np.array([[10.0, 0.0], [480.0, 0.0]]), value=np.array([-1.0, 3.0]), nmax=1)],
+                grid=np.array([[5.0, 1.0]]),
+                vgm=[dict(ivar=1, jvar=1, vtype="sph", sill=1.0, a_major=300.0, nugget=0.1),
+                     dict(ivar=1, jvar=2, vtype="sph", sill=0.5, a_major=300.0),
+                     dict(ivar=2, jvar=2, vtype="sph", sill=1.0, a_major=300.0, nugget=0.1)],
+                search=[dict(ivar=1, sector_search=False), dict(ivar=2, sector_search=False)])
+    o = ckrige.solve(spec, table_mode=False)
+    assert o["estimate"][0, 0] == pytest.approx(2.0, rel=1e-12)
+    assert o["estimate"][0, 1] == pytest.approx(-1.0, rel=1e-12)
+    del rng
