@@ -150,6 +150,50 @@ def loo_gradient(steps=3):
                 forward_device_ms=tm["total_ms"], fd_rel_err=rel, fd_max_rel_err=max(rel.values()))
 
 
+def resident_rate(k, nodes, steps=3):
+    """nodes/s of repeated device passes over resident inputs (one warm pass
+    first), with the last pass's per-phase device timings"""
+    from krigekit_amd.kriging import device_synchronize
+    k.solve()
+    k.solve_resident()
+    device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        k.solve_resident()
+    device_synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    tm = k.timing
+    return dict(nodes=nodes, nodes_per_s=nodes / wall, ms_per_pass=wall * 1e3,
+                phase_ms={key: round(float(v), 4) for key, v in tm.items()})
+
+
+def space_time():
+    """space-time product-sum kriging: 500 stations x 365 days (182,500
+    observations), 500 x 500 km grid at 1 km x 30 times (7.5 million nodes),
+    Cs exponential (nugget 0.05, sill 1, 50 km), Ct exponential (sill 0.5,
+    5 days), k 0.5, time_at 10 km/day, nmax 48, maxdist 40, FP64"""
+    from krigekit_amd import synth
+    from krigekit_amd.kriging_st import from_st_spec
+    spec = synth.st()
+    k = from_st_spec(spec)
+    out = resident_rate(k, len(spec["grid"]))
+    k.close()
+    return out
+
+
+def cokriging():
+    """2-variable standard cokriging (linear model of coregionalisation):
+    20,000 + 200,000 observations, 10^6 blocks of 4 x 4 Gauss-Legendre nodes,
+    24 + 48 neighbours, FP64"""
+    from krigekit_amd import synth
+    from krigekit_amd.kriging import from_spec
+    spec = synth.ck2d()
+    k = from_spec(spec)
+    out = resident_rate(k, len(spec["grid"]))
+    k.close()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(path):
@@ -165,6 +209,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-loo-gradient", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the space-time and cokriging workload lines")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -213,7 +259,7 @@ def main():
         "phase_ms_fp32r": {k: round(v, 4) for k, v in t32.items()},
         "phase_ms_fp64": {k: round(v, 4) for k, v in t64.items()},
         "roofline": {
-            "kernel": "k_search_wave (octant neighbour search, two FP32-key passes + exact boundary)",
+            "kernel": "k_search_sector (octant neighbour search: one kd-tree traversal, per-octant register lists, key histogram bound, exact FP64 ranks)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -232,6 +278,9 @@ def main():
     }
     if not args.no_loo_gradient:
         line["loo_gradient"] = loo_gradient()
+    if not args.no_secondary:
+        line["space_time_product_sum"] = space_time()
+        line["cokriging_2var_blocks"] = cokriging()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec)
     print(json.dumps(line), flush=True)

@@ -134,6 +134,61 @@ int krige_get_weight_var(int64_t handle, int nvar, int nblock, double *out);
 int krige_set_weights(int64_t handle, int nmax, int ngroups, int nvar, int nblock, const int *nnear,
                       const int *inear, const double *weight, const int *order, const double *var);
 
+/* ---- space-time kriging, src/libkriging/kriging_st_capi.f90 -----------------
+ * Shares the handle registry and the solve/getter entry points above
+ * (src/krigekit/kriging_st.py:105-212).  Coordinates are (x, y, z, t): the
+ * spatial part has ndim <= 3 rows (zero-padded to 3), time is the last row.
+ * Product-sum model only; sum-metric, gradients, blocks, drift and
+ * simulation return 1 with a message. */
+/* kriging_st_capi.f90:43 */
+int krige_st_create(int64_t *handle);
+/* kriging_st_capi.f90:61 */
+int krige_st_destroy(int64_t *handle);
+/* kriging_st_capi.f90:80-122 (ndim fixed at 3) */
+int krige_st_initialize(int64_t handle, int nvar, int ndrift, int unbias, int nsim, int anisotropic_search,
+                        int weight_correction, int use_old_weight, int store_weight, int cross_validation,
+                        int write_mat, int neglect_error, int verbose, const char *weight_file,
+                        const double bounds[2], int seed);
+/* kriging_st_capi.f90:128-163: model "product_sum" (k_ps) or "sum_metric" */
+int krige_st_set_st_model(int64_t handle, const char *model, const char *transform, double at,
+                          double time_nugget, double time_sill, double k_ps);
+/* kriging_st_capi.f90:165-192: coord (ndim+1, nobs), time in the last row; ndim must be 3 (the
+   space-time object has 3 spatial rows + time, kriging_st.F90:121-123) */
+int krige_st_set_obs(int64_t handle, int ivar, int nobs, int ndim, const double *coord, const double *value,
+                     const double *variance, int nmax, double maxdist, double sk_mean);
+/* kriging_st_capi.f90:201-239: returns 1, not provided */
+int krige_st_set_grad(int64_t handle, int ivar, int ngrad, int ndim, const double *coord1,
+                      const double *coord2, const double *grad_val, const double *variance, int ndrift_c,
+                      const double *drift_ext);
+/* kriging_st_capi.f90:244-258 */
+int krige_st_reset_vgm(int64_t handle, int ivar, int jvar);
+/* kriging_st_capi.f90:260-286: spatial structure */
+int krige_st_set_vgm(int64_t handle, int ivar, int jvar, const char *vtype, double nugget, double sill,
+                     double a_major, double a_minor1, double a_minor2, double azimuth, double dip, double plunge);
+/* kriging_st_capi.f90:288-307: temporal structure, isotropic range at_k */
+int krige_st_set_vgm_temporal(int64_t handle, int ivar, int jvar, const char *vtype, double nugget,
+                              double sill, double at_k);
+/* kriging_st_capi.f90:309-327: returns 1 (sum-metric not provided) */
+int krige_st_set_vgm_joint_sills(int64_t handle, int ivar, int jvar, int n, const double *sills);
+/* kriging_st_capi.f90:330-353: coord (3, ngrid), time (ngrid) */
+int krige_st_set_grid(int64_t handle, int ngrid, const double *coord, const double *time,
+                      const double *rangescale, const double *localnugget);
+/* kriging_st_capi.f90:355-388: returns 1, not provided */
+int krige_st_set_grid_block(int64_t handle, int nblocks, const double *coord, const double *time,
+                            const int *nblockpnt, int npnts_total, const double *blockcoord,
+                            const double *blocktime, const double *pointweight, const double *rangescale,
+                            const double *localnugget);
+/* kriging_st_capi.f90:390-406: returns 1, not provided */
+int krige_st_set_grid_drift(int64_t handle, int ndrift_c, int nblocks, const double *drift);
+/* kriging_st_capi.f90:414-453: returns 1, not provided */
+int krige_st_set_sim(int64_t handle, int nblocks, const void *randpath, int nsim_c, const void *sample);
+/* kriging_st_capi.f90:455-490: time_at scales time in the search metric */
+int krige_st_set_search(int64_t handle, int ivar, double time_at, double anis1, double anis2, double azimuth,
+                        double dip, double plunge, int sector_search);
+/* kriging_st_capi.f90:496-520 (this backend reports 1) */
+void krige_st_get_max_threads(int *n);
+void krige_st_get_num_threads(int *n);
+
 /* ---- extensions: no counterpart in the reference -------------------------- */
 /* precision 0 = FP64 solve, 1 = FP32 factorisation + FP64 refinement;
  * pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64;

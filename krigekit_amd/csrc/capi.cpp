@@ -345,11 +345,11 @@ struct TreeBuilder {
   HostTree& t;
   const double* data;
   int dim;
-  double lo(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + 3 + k]; }
-  double hi(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + 6 + k]; }
+  double lo(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + kk::NODE_LO + k]; }
+  double hi(int node, int k) const { return t.nd[(size_t)node * kk::NODE_D + kk::NODE_HI + k]; }
   void set_box(int node, int k, double l, double h) {
-    t.nd[(size_t)node * kk::NODE_D + 3 + k] = l;
-    t.nd[(size_t)node * kk::NODE_D + 6 + k] = h;
+    t.nd[(size_t)node * kk::NODE_D + kk::NODE_LO + k] = l;
+    t.nd[(size_t)node * kk::NODE_D + kk::NODE_HI + k] = h;
   }
   void spread(int c, int l, int u, double& mn, double& mx) const {
     mn = mx = data[c + dim * (size_t)t.ind[l]];
@@ -496,7 +496,8 @@ struct Obs {
   double sk_mean = 0.0;
   bool sector = false, search_set = false, need_search = false, aniso = false;
   double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  std::vector<double> rcoord;  // rotated coordinates (aniso search)
+  std::vector<double> rcoord;  // search coordinates: rotated (aniso search), space-time t * time_at
+  double time_at = 1.0;        // space-time search scaling of the time axis
   HostTree tree;
 };
 
@@ -509,6 +510,14 @@ struct DevVar {
 
 struct Krige {
   int ndim = 2, nvar = 1, ndrift = 0, unbias = 1, nsim = 0;
+  // space-time (krige_st_*): coordinates are (x, y, z, t), ndim = 4 rows,
+  // spatial structures in vgm, temporal ones in vgm_t, product-sum k_ps
+  bool st = false;
+  int st_model = 0;  // 1: product_sum
+  double k_ps = 0.0, st_cov0 = 0.0;
+  std::vector<VStruct> vgm_t;
+  Model tmodel{};
+  std::vector<double> ttab;
   bool aniso_search = false, weight_correction = false, use_old_weight = false, store_weight = false;
   bool cross_validation = false, write_mat = false, neglect_error = true, varying_vgm = false;
   bool std_ck = false, verbose = false, pf_cache = false;
@@ -540,7 +549,7 @@ struct Krige {
   bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz;
+  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_tmodel, d_ttab;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool perm_ok = false;
@@ -613,7 +622,7 @@ bool stage_inputs(Krige& K) {
   Obs& O = K.obs[0];
   const int nd = K.ndim;
   if (!K.model_ok) {
-    if (!build_model(K.vgm[0], nd, K.table_mode, K.model, K.tab)) {
+    if (!build_model(K.vgm[0], K.st ? 3 : nd, K.table_mode, K.model, K.tab)) {
       set_error(W, "invalid variogram model");
       return false;
     }
@@ -624,12 +633,28 @@ bool stage_inputs(Krige& K) {
       return false;
     }
     HIPCHK(hipMemcpy(K.d_model.p, &K.model, sizeof(Model), hipMemcpyHostToDevice), W);
+    if (K.st) {
+      // temporal structures: 1-D, isotropic range at_k; C(0, 0) of the
+      // product-sum model (variogram_st.f90:344-375)
+      if (!build_model(K.vgm_t, 1, K.table_mode, K.tmodel, K.ttab)) {
+        set_error(W, "invalid temporal variogram model");
+        return false;
+      }
+      if (!upload(K.d_ttab, K.ttab, W)) return false;
+      K.tmodel.tab = K.ttab.empty() ? nullptr : K.d_ttab.as<double>();
+      if (!K.d_tmodel.ensure(sizeof(Model))) {
+        set_error(W, "device allocation failed");
+        return false;
+      }
+      HIPCHK(hipMemcpy(K.d_tmodel.p, &K.tmodel, sizeof(Model), hipMemcpyHostToDevice), W);
+      K.st_cov0 = K.k_ps * K.model.cov0 * K.tmodel.cov0 + K.model.cov0 + K.tmodel.cov0;
+    }
     K.model_ok = true;
   }
   if (K.dev_inputs_ok) return true;
   if (!upload(K.d_ocoord, O.coord, W) || !upload(K.d_ovalue, O.value, W) || !upload(K.d_ovar, O.var, W))
     return false;
-  if (!upload(K.d_socoord, O.aniso ? O.rcoord : O.coord, W)) return false;
+  if (!upload(K.d_socoord, (O.aniso || K.st) ? O.rcoord : O.coord, W)) return false;
   if (O.need_search) {
     if (!upload(K.d_rdata, O.tree.rdata, W) || !upload(K.d_ind, O.tree.ind, W) ||
         !upload(K.d_ni, O.tree.ni, W) || !upload(K.d_nd, O.tree.nd, W))
@@ -638,7 +663,8 @@ bool stage_inputs(Krige& K) {
   // block centres, and their rotated copy for the search
   const std::vector<double>& bc = K.cross_validation ? O.coord : K.bcoord;
   if (!upload(K.d_bcoord, bc, W)) return false;
-  if (O.aniso) {
+  if (O.aniso || K.st) {
+    // queries in search coordinates (rotated; space-time t * time_at)
     std::vector<double> q(bc.size());
     kk::Rot3 R;
     std::memcpy(R.m, O.rot, sizeof R.m);
@@ -646,6 +672,7 @@ bool stage_inputs(Krige& K) {
       if (nd == 1) R.apply<1>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
       else if (nd == 2) R.apply<2>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
       else R.apply<3>(&bc[(size_t)nd * b], &q[(size_t)nd * b]);
+      if (K.st) q[(size_t)nd * b + 3] = bc[(size_t)nd * b + 3] * O.time_at;
     }
     if (!upload(K.d_qcoord, q, W)) return false;
   }
@@ -665,24 +692,27 @@ bool morton_perm(Krige& K) {
   const char* W = "t_kriging_base%solve_base";
   const std::vector<double>& bc = K.cross_validation ? K.obs[0].coord : K.bcoord;
   const int nd = K.ndim, nb = K.nblock;
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  // space-time queries are ordered in the search metric (t * time_at)
+  const double tsc = K.st ? K.obs[0].time_at : 1.0;
+  auto coordk = [&](int k, int b) { return bc[k + (size_t)nd * b] * (k == 3 ? tsc : 1.0); };
+  double lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
   for (int k = 0; k < nd; ++k) {
-    lo[k] = hi[k] = bc[k];
+    lo[k] = hi[k] = coordk(k, 0);
     for (int b = 1; b < nb; ++b) {
-      lo[k] = std::min(lo[k], bc[k + (size_t)nd * b]);
-      hi[k] = std::max(hi[k], bc[k + (size_t)nd * b]);
+      lo[k] = std::min(lo[k], coordk(k, b));
+      hi[k] = std::max(hi[k], coordk(k, b));
     }
   }
   double ext = 0.0;
   for (int k = 0; k < nd; ++k) ext = std::max(ext, hi[k] - lo[k]);
   if (!(ext > 0.0)) ext = 1.0;
-  const int bits = nd == 1 ? 30 : nd == 2 ? 21 : 20;
+  const int bits = nd == 1 ? 30 : nd == 2 ? 21 : nd == 3 ? 20 : 15;
   const double scale = (double)((1u << bits) - 1) / ext;
   std::vector<std::pair<uint64_t, int>> key(nb);
   for (int b = 0; b < nb; ++b) {
     uint64_t code = 0;
-    uint32_t c[3] = {0, 0, 0};
-    for (int k = 0; k < nd; ++k) c[k] = (uint32_t)((bc[k + (size_t)nd * b] - lo[k]) * scale);
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nd; ++k) c[k] = (uint32_t)((coordk(k, b) - lo[k]) * scale);
     for (int bit = bits - 1; bit >= 0; --bit)
       for (int k = 0; k < nd; ++k) code = (code << 1) | ((c[k] >> bit) & 1u);
     key[b] = {code, b};
@@ -703,7 +733,10 @@ bool run_pipeline(Krige& K) {
   const int nd = K.ndim, nb = K.nblock;
   int nmax = O.nmax;  // already normalised at set_search
   const bool sector = O.sector && O.need_search;
-  const int mmax = sector ? (1 << nd) * nmax : nmax;
+  // octants are over the spatial axes only: 2^3 for space-time
+  // (kriging_st.F90:587, obs_nmax in kriging_base.F90:3005-3011)
+  const int nsec = 1 << (K.st ? 3 : nd);
+  const int mmax = sector ? nsec * nmax : nmax;
   K.mmax = mmax;
   if (mmax > 64) {
     set_error(W, "neighbourhoods larger than 64 data rows are not supported by the GPU solver (mmax=" +
@@ -723,7 +756,7 @@ bool run_pipeline(Krige& K) {
   // ---- search -----------------------------------------------------------
   kk::SearchArgs sa{};
   sa.ndim = nd;
-  sa.qcoord = O.aniso ? K.d_qcoord.as<double>() : K.d_bcoord.as<double>();
+  sa.qcoord = (O.aniso || K.st) ? K.d_qcoord.as<double>() : K.d_bcoord.as<double>();
   sa.ocoord = K.d_socoord.as<double>();
   sa.nmax = nmax;
   sa.cv = K.cross_validation;
@@ -732,11 +765,18 @@ bool run_pipeline(Krige& K) {
   sa.nnear = K.d_nnear.as<int>();
   sa.inear = K.d_inear.as<int>();
   if (!O.need_search) {
+    // rotated_dists rotates (obs - query) of the unrotated coordinates;
+    // space-time adds the scaled time difference (kriging_st.F90:609-612)
     kk::Rot3 R;
     std::memcpy(R.m, O.rot, sizeof R.m);
     sa.nq = nb;
     sa.q0 = 0;
-    HIPCHK(kk::launch_search_all(sa, O.n, K.d_ocoord.as<double>(), R, st), W);
+    if (K.st && O.aniso) {
+      set_error(W, "space-time search anisotropy with nmax >= nobs is not supported by this backend");
+      return false;
+    }
+    sa.qcoord = K.st ? K.d_qcoord.as<double>() : K.d_bcoord.as<double>();
+    HIPCHK(kk::launch_search_all(sa, O.n, K.st ? K.d_socoord.as<double>() : K.d_ocoord.as<double>(), R, st), W);
   } else {
     sa.tree.dim = nd;
     sa.tree.n = O.n;
@@ -747,11 +787,12 @@ bool run_pipeline(Krige& K) {
     sa.tree.ni = K.d_ni.as<int>();
     sa.tree.nd = K.d_nd.as<double>();
     sa.sort = sector;
-    sa.k = sector ? std::min(O.n, (1 << nd) * nmax * 4) : nmax;
+    sa.k = sector ? std::min(O.n, nsec * nmax * 4) : nmax;
     const size_t lds = kk::fast_search_lds(nd, sector, sa.k, nmax);
     int nexact = nb;
     const int* exact_list = nullptr;
-    if (K.search_mode > 0 && lds <= 64 * 1024) {
+    // sectored space-time searches run the exact kernel for every query
+    if (K.search_mode > 0 && lds <= 64 * 1024 && !(K.st && sector)) {
       if (!K.d_sflag.ensure(sizeof(int) * nb)) {
         set_error(W, "device allocation failed (search flags)");
         return false;
@@ -819,8 +860,14 @@ bool run_pipeline(Krige& K) {
   va.bound_hi = K.bounds[1];
   va.nsub = 1;
   va.bcoord = K.d_bcoord.as<double>();
-  va.rangescale = K.rangescale.empty() ? nullptr : K.d_rs.as<double>();
+  // the space-time assembly does not range-scale lags (kriging_st.F90:805)
+  va.rangescale = (K.rangescale.empty() || K.st) ? nullptr : K.d_rs.as<double>();
   va.localnugget = K.localnugget.empty() ? nullptr : K.d_ln.as<double>();
+  if (K.st) {
+    va.tmodel = K.d_tmodel.as<Model>();
+    va.k_ps = K.k_ps;
+    va.st_cov0 = K.st_cov0;
+  }
   va.ocoord = K.d_ocoord.as<double>();
   va.ovalue = K.d_ovalue.as<double>();
   va.ovar = K.d_ovar.as<double>();
@@ -985,6 +1032,24 @@ bool validate_for_solve(Krige& K) {
   if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift != 0) {
     set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift=0");
     return false;
+  }
+  if (K.st) {
+    if (K.nvar != 1) {
+      set_error(W, "space-time kriging with nvar > 1 is not supported by this backend");
+      return false;
+    }
+    if (K.st_model != 1) {
+      set_error("t_kriging_st%prepare", "call set_st_model('product_sum', ...) before solve()");
+      return false;
+    }
+    if (K.vgm_t.empty()) {
+      set_error("t_kriging_st%prepare", "Invalid ST variogram: no temporal structure (set_vgm_temporal)");
+      return false;
+    }
+    if (!K.soff.empty()) {
+      set_error(W, "space-time block kriging is not supported by this backend");
+      return false;
+    }
   }
   if (K.nvar > 1 && K.cross_validation) {
     set_error(W, "cross validation with nvar > 1 is not supported by this backend");
@@ -1880,19 +1945,25 @@ int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azi
   O.rcoord.clear();
   std::fill(O.rot, O.rot + 9, 0.0);
   O.rot[0] = O.rot[4] = O.rot[8] = 1.0;
-  if (O.aniso) {
-    calc_rotmat(azimuth, dip, plunge, anis1, anis2, O.rot);
+  if (O.aniso) calc_rotmat(azimuth, dip, plunge, anis1, anis2, O.rot);
+  if (O.aniso || K->st) {
+    // search coordinates: spatial part rotated/scaled, space-time time axis
+    // scaled by time_at (kriging_st.F90:387-395)
     O.rcoord.resize(O.coord.size());
     kk::Rot3 R;
     std::memcpy(R.m, O.rot, sizeof R.m);
+    const int nd = K->ndim;
     for (int i = 0; i < O.n; ++i) {
-      if (K->ndim == 1) R.apply<1>(&O.coord[i], &O.rcoord[i]);
-      else if (K->ndim == 2) R.apply<2>(&O.coord[(size_t)2 * i], &O.rcoord[(size_t)2 * i]);
-      else R.apply<3>(&O.coord[(size_t)3 * i], &O.rcoord[(size_t)3 * i]);
+      const double* c = &O.coord[(size_t)nd * i];
+      double* r = &O.rcoord[(size_t)nd * i];
+      if (nd == 1) R.apply<1>(c, r);
+      else if (nd == 2) R.apply<2>(c, r);
+      else R.apply<3>(c, r);
+      if (K->st) r[3] = c[3] * O.time_at;
     }
   }
   if (O.need_search) {
-    if (!build_tree(O.tree, K->ndim, O.n, O.aniso ? O.rcoord.data() : O.coord.data())) {
+    if (!build_tree(O.tree, K->ndim, O.n, (O.aniso || K->st) ? O.rcoord.data() : O.coord.data())) {
       set_error("kdtree2_create", "kd-tree construction did not terminate (degenerate coordinates)");
       return ierr();
     }
@@ -2131,6 +2202,192 @@ int krige_set_weights(int64_t h, int, int, int, int, const int*, const int*, con
 // precision: 0 = FP64 solve, 1 = FP32 factorisation + FP64 refinement.
 // pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64.
 // table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic.
+// ---- space-time (kriging_st_capi.f90) --------------------------------------
+// The space-time object shares the handle registry; its coordinates are
+// (x, y, z, t): spatial ndim 3 (smaller ndim is zero-padded), time last.
+
+int krige_st_create(int64_t* handle) {
+  if (krige_create(handle)) return ierr();
+  Krige* K = checked(*handle);
+  K->st = true;
+  K->ndim = 4;
+  return 0;
+}
+
+int krige_st_destroy(int64_t* handle) { return krige_destroy(handle); }
+
+int krige_st_initialize(int64_t h, int nvar, int ndrift, int unbias, int nsim, int anisotropic_search,
+                        int weight_correction, int use_old_weight, int store_weight, int cross_validation,
+                        int write_mat, int neglect_error, int verbose, const char* weight_file,
+                        const double* bounds, int seed) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  if (krige_initialize(h, 3, nvar, ndrift, unbias, nsim, anisotropic_search, weight_correction, use_old_weight,
+                       store_weight, cross_validation, write_mat, neglect_error, 0, 1, verbose, 0, weight_file,
+                       bounds, seed))
+    return ierr();
+  Krige* K = checked(h);
+  K->st = true;
+  K->ndim = 4;
+  return 0;
+}
+
+int krige_st_set_st_model(int64_t h, const char* model, const char* transform, double at, double time_nugget,
+                          double time_sill, double k_ps) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  (void)transform;
+  (void)at;
+  (void)time_nugget;
+  (void)time_sill;
+  const std::string m = model ? model : "";
+  if (m == "product_sum") {
+    K->st_model = 1;
+    K->k_ps = k_ps;
+    K->model_ok = false;
+    K->solved = false;
+    return 0;
+  }
+  if (m == "sum_metric")
+    return unsupported("krige_st_set_st_model", "the sum-metric space-time model is not provided by this backend");
+  set_error("krige_st_set_st_model", "model must be \"sum_metric\" or \"product_sum\", got: " + m);
+  return ierr();
+}
+
+int krige_st_set_obs(int64_t h, int ivar, int nobs, int ndim, const double* coord, const double* value,
+                     const double* variance, int nmax, double maxdist, double sk_mean) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->st) {
+    set_error("t_kriging_base%set_obs", "space-time observations need a krige_st handle");
+    return ierr();
+  }
+  // coord is (ndim+1, nobs) with time last; the space-time object always has
+  // 3 spatial rows + time, so any other ndim fails the row-count check
+  // (kriging_st_capi.f90:167-189 -> set_obs_base)
+  return krige_set_obs(h, ivar, nobs, ndim + 1, coord, value, variance, nmax, maxdist, sk_mean);
+}
+
+int krige_st_set_grad(int64_t h, int ivar, int ngrad, int ndim, const double* coord1, const double* coord2,
+                      const double* grad_val, const double* variance, int ndrift_c, const double* drift_ext) {
+  (void)ivar, (void)ngrad, (void)ndim, (void)coord1, (void)coord2, (void)grad_val, (void)variance;
+  (void)ndrift_c, (void)drift_ext;
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_st_set_grad", "gradient constraints are not provided by this backend");
+}
+
+int krige_st_reset_vgm(int64_t h, int ivar, int jvar) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (krige_reset_vgm(h, ivar, jvar)) return ierr();
+  K->vgm_t.clear();
+  K->model_ok = false;
+  return 0;
+}
+
+int krige_st_set_vgm(int64_t h, int ivar, int jvar, const char* vtype, double nugget, double sill, double a_major,
+                     double a_minor1, double a_minor2, double azimuth, double dip, double plunge) {
+  return krige_set_vgm(h, ivar, jvar, vtype, nugget, sill, a_major, a_minor1, a_minor2, azimuth, dip, plunge, 0);
+}
+
+int krige_st_set_vgm_temporal(int64_t h, int ivar, int jvar, const char* vtype, double nugget, double sill,
+                              double at_k) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_st%set_vgm_temporal";
+  if (!K->st || ivar != 1 || jvar != 1) {
+    set_error(W, "temporal structures need a krige_st handle with nvar = 1");
+    return ierr();
+  }
+  const int id = vtype_from_str(vtype ? vtype : "sph");
+  if (id < 0) {
+    set_error("vtype_from_str", std::string("unknown vtype: ") + (vtype ? vtype : ""));
+    return ierr();
+  }
+  if (!(at_k > 0.0) || nugget < 0.0 || sill < 0.0 || (int)K->vgm_t.size() >= kk::MAXS) {
+    set_error(W, "invalid temporal structure (at_k > 0, nugget >= 0, sill >= 0)");
+    return ierr();
+  }
+  // isotropic 1-D geometry: a_major = a_minor1 = a_minor2 = at_k (variogram_st.f90:302-317)
+  K->vgm_t.push_back(VStruct{id, 0, nugget, sill, at_k, at_k, at_k, 0.0, 0.0, 0.0});
+  K->model_ok = false;
+  K->solved = false;
+  return 0;
+}
+
+int krige_st_set_vgm_joint_sills(int64_t h, int, int, int, const double*) {
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_st_set_vgm_joint_sills", "the sum-metric space-time model is not provided by this backend");
+}
+
+int krige_st_set_grid(int64_t h, int ngrid, const double* coord, const double* time, const double* rangescale,
+                      const double* localnugget) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->st) {
+    set_error("t_kriging_st%set_grid_point", "space-time grids need a krige_st handle");
+    return ierr();
+  }
+  std::vector<double> c((size_t)4 * ngrid);
+  for (int i = 0; i < ngrid; ++i) {
+    for (int k = 0; k < 3; ++k) c[k + (size_t)4 * i] = coord[k + (size_t)3 * i];
+    c[3 + (size_t)4 * i] = time[i];
+  }
+  return krige_set_grid(h, ngrid, 4, c.data(), rangescale, localnugget);
+}
+
+int krige_st_set_grid_block(int64_t h, int nblocks, const double* coord, const double* time, const int* nblockpnt,
+                            int npnts_total, const double* blockcoord, const double* blocktime,
+                            const double* pointweight, const double* rangescale, const double* localnugget) {
+  (void)nblocks, (void)coord, (void)time, (void)nblockpnt, (void)npnts_total, (void)blockcoord;
+  (void)blocktime, (void)pointweight, (void)rangescale, (void)localnugget;
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_st_set_grid_block", "space-time block kriging is not provided by this backend");
+}
+
+int krige_st_set_grid_drift(int64_t h, int ndrift_c, int nblocks, const double* drift) {
+  (void)ndrift_c, (void)nblocks, (void)drift;
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_st_set_grid_drift", "external drift is not provided by this backend");
+}
+
+int krige_st_set_sim(int64_t h, int nblocks, const void* randpath, int nsim_c, const void* sample) {
+  (void)nblocks, (void)randpath, (void)nsim_c, (void)sample;
+  clear_error();
+  if (!checked(h)) return ierr();
+  return unsupported("krige_st_set_sim", "sequential simulation is not provided by this backend");
+}
+
+int krige_st_set_search(int64_t h, int ivar, double time_at, double anis1, double anis2, double azimuth, double dip,
+                        double plunge, int sector_search) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->st || ivar < 1 || ivar > K->nvar) {
+    set_error("set_search", "space-time search needs a krige_st handle and a valid ivar");
+    return ierr();
+  }
+  if (!(time_at > 1.e-10)) {
+    set_error("set_search", "time_at must be positive.");
+    return ierr();
+  }
+  K->obs[ivar - 1].time_at = time_at;
+  K->perm_ok = false;
+  return krige_set_search(h, ivar, anis1, anis2, azimuth, dip, plunge, sector_search);
+}
+
+void krige_st_get_max_threads(int* n) { *n = 1; }
+void krige_st_get_num_threads(int* n) { *n = 1; }
+
 int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
   clear_error();
   Krige* K = checked(h);

@@ -11,7 +11,9 @@ namespace kk {
 constexpr int MAXS = 8;        // nested structures per variogram
 constexpr int MAXDIM = 3;
 constexpr int NODE_I = 6;      // ints per tree node: cut_dim, l, u, left, right, parent
-constexpr int NODE_D = 9;      // doubles per tree node: cut_val, cut_left, cut_right, lo[3], hi[3]
+constexpr int NODE_D = 11;     // doubles per tree node: cut_val, cut_left, cut_right, lo[4], hi[4]
+constexpr int NODE_LO = 3;     // box lower bounds (up to 4 search dimensions: space-time)
+constexpr int NODE_HI = 7;     // box upper bounds
 
 // variogram shape ids (src/libkriging/vgmfunc.f90:6-16)
 enum VType : int { NUG = 0, SPH = 1, EXP = 2, GAU = 3, HOL = 4, POW = 5, BSQ = 6,
@@ -126,7 +128,10 @@ struct MVArgs {
 };
 
 struct SolveArgs {
-  const Model* model;         // device copy of the covariance model
+  const Model* model;         // device copy of the covariance model (space-time: spatial part)
+  const Model* tmodel;        // space-time only: temporal structures (1-D)
+  double k_ps;                // space-time product-sum coefficient
+  double st_cov0;             // space-time C(0, 0) = k Cs0 Ct0 + Cs0 + Ct0
   int ndim;
   int unbias;                 // 1 ordinary, 0 simple
   double sk_mean;

@@ -177,7 +177,7 @@ __device__ inline void traverse(const Tree& T, const double* q, double& ball, Le
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
           if (i != cut && visit) {
-            dis = __dadd_rn(dis, dis2_bnd(q[i], nd[3 + i], nd[6 + i]));
+            dis = __dadd_rn(dis, dis2_bnd(q[i], nd[NODE_LO + i], nd[NODE_HI + i]));
             if (dis > ball) visit = false;
           }
         }
@@ -205,11 +205,13 @@ __device__ inline bool point_sd(const Tree& T, int i, const double* q, double ba
   return true;
 }
 
+// sector of a point relative to the query over the spatial dimensions (the
+// time coordinate of a space-time search is not sectored, kriging_st.F90:573-604)
 template <int DIM>
 __device__ inline int octant(const double* o, const double* q) {
   int sec = (__dsub_rn(o[0], q[0]) < 0.0);
   if (DIM >= 2) sec += 2 * (__dsub_rn(o[1], q[1]) < 0.0);
-  if (DIM == 3) sec += 4 * (__dsub_rn(o[2], q[2]) < 0.0);
+  if (DIM >= 3) sec += 4 * (__dsub_rn(o[2], q[2]) < 0.0);
   return sec;
 }
 
@@ -281,9 +283,10 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   if (a.sort) {
     heapsort(h, nn);
     // apply_sector_filter, kriging.F90:1093-1154
-    int counts[1 << DIM];
+    constexpr int NSEC = 1 << (DIM < 3 ? DIM : 3);
+    int counts[NSEC];
 #pragma unroll
-    for (int s = 0; s < (1 << DIM); ++s) counts[s] = 0;
+    for (int s = 0; s < NSEC; ++s) counts[s] = 0;
     const int skip_idx = a.cv ? qid : -1;
     for (int c = 1; c <= nn; ++c) {
       const int idx = h.I(c);
@@ -292,10 +295,10 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       const int sec = octant<DIM>(a.ocoord + DIM * (size_t)idx, q);
       int cs = 0;
 #pragma unroll
-      for (int s = 0; s < (1 << DIM); ++s) cs = (s == sec) ? counts[s] : cs;
+      for (int s = 0; s < NSEC; ++s) cs = (s == sec) ? counts[s] : cs;
       if (cs < a.nmax) {
 #pragma unroll
-        for (int s = 0; s < (1 << DIM); ++s) counts[s] += (s == sec);
+        for (int s = 0; s < NSEC; ++s) counts[s] += (s == sec);
         out[cnt++] = idx;
       }
     }
@@ -471,7 +474,7 @@ template <int DIM>
 __device__ inline double box_lb(const double* nd, const double* q) {
   double d = 0.0;
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) d = __dadd_rn(d, dis2_bnd(q[k], nd[3 + k], nd[6 + k]));
+  for (int k = 0; k < DIM; ++k) d = __dadd_rn(d, dis2_bnd(q[k], nd[NODE_LO + k], nd[NODE_HI + k]));
   return d;
 }
 
@@ -1129,11 +1132,13 @@ __global__ __launch_bounds__(256) void k_search_all(SearchArgs a, int nobs, cons
       removed = true;
       continue;
     }
-    // rotated_dists (src/libkriging/rotation.f90:193-218): rotate (obs - query)
-    double df[3] = {0.0, 0.0, 0.0}, r[3];
+    // rotated_dists (src/libkriging/rotation.f90:193-218): rotate (obs - query);
+    // space-time adds (t_obs * at - t_query * at)^2 (kriging_st.F90:609-612)
+    double df[4] = {0.0, 0.0, 0.0, 0.0}, r[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < DIM; ++k) df[k] = __dsub_rn(ocoord_orig[k + DIM * (size_t)i], q[k]);
-    srot.apply<DIM>(df, r);
+    srot.apply<(DIM < 3 ? DIM : 3)>(df, r);
+    if (DIM == 4) r[3] = df[3];
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < DIM; ++k) s = __dadd_rn(s, sq(r[k]));
@@ -1170,12 +1175,15 @@ hipError_t launch_search_wave(const SearchArgs& a, hipStream_t st) {
     return hipGetLastError();
   }
   if (a.sort) {
+    // sectored space-time searches go to the exact kernel
+    if (a.ndim == 4) return hipErrorInvalidValue;
     if (a.ndim == 2) hipLaunchKernelGGL((k_search_wave<2, true>), grd, blk, lds, st, a);
     else hipLaunchKernelGGL((k_search_wave<3, true>), grd, blk, lds, st, a);
   } else {
     if (a.ndim == 1) hipLaunchKernelGGL((k_search_wave<1, false>), grd, blk, lds, st, a);
     else if (a.ndim == 2) hipLaunchKernelGGL((k_search_wave<2, false>), grd, blk, lds, st, a);
-    else hipLaunchKernelGGL((k_search_wave<3, false>), grd, blk, lds, st, a);
+    else if (a.ndim == 3) hipLaunchKernelGGL((k_search_wave<3, false>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_wave<4, false>), grd, blk, lds, st, a);
   }
   return hipGetLastError();
 }
@@ -1184,12 +1192,14 @@ hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st) {
   const size_t lds = fast_search_lds(a.ndim, a.sort != 0, a.k, a.nmax);
   dim3 blk(64), grd((a.nq + 63) / 64);
   if (a.sort) {
+    if (a.ndim == 4) return hipErrorInvalidValue;
     if (a.ndim == 2) hipLaunchKernelGGL((k_search_fast<2, true>), grd, blk, lds, st, a);
     else hipLaunchKernelGGL((k_search_fast<3, true>), grd, blk, lds, st, a);
   } else {
     if (a.ndim == 1) hipLaunchKernelGGL((k_search_fast<1, false>), grd, blk, lds, st, a);
     else if (a.ndim == 2) hipLaunchKernelGGL((k_search_fast<2, false>), grd, blk, lds, st, a);
-    else hipLaunchKernelGGL((k_search_fast<3, false>), grd, blk, lds, st, a);
+    else if (a.ndim == 3) hipLaunchKernelGGL((k_search_fast<3, false>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search_fast<4, false>), grd, blk, lds, st, a);
   }
   return hipGetLastError();
 }
@@ -1202,13 +1212,15 @@ hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
     dim3 blk(32), grd((a.nq + 31) / 32);
     if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, true>), grd, blk, lds, st, a);
     else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, true>), grd, blk, lds, st, a);
-    else hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);
+    else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);
+    else hipLaunchKernelGGL((k_search<4, true>), grd, blk, lds, st, a);
     return hipGetLastError();
   }
   dim3 blk(256), grd((a.nq + 255) / 256);
   if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, false>), grd, blk, 0, st, a);
   else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, false>), grd, blk, 0, st, a);
-  else hipLaunchKernelGGL((k_search<3, false>), grd, blk, 0, st, a);
+  else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, false>), grd, blk, 0, st, a);
+  else hipLaunchKernelGGL((k_search<4, false>), grd, blk, 0, st, a);
   return hipGetLastError();
 }
 
@@ -1217,7 +1229,8 @@ hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord
   dim3 blk(256), grd((a.nq + 255) / 256);
   if (a.ndim == 1) hipLaunchKernelGGL(k_search_all<1>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
   else if (a.ndim == 2) hipLaunchKernelGGL(k_search_all<2>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
-  else hipLaunchKernelGGL(k_search_all<3>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
+  else if (a.ndim == 3) hipLaunchKernelGGL(k_search_all<3>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
+  else hipLaunchKernelGGL(k_search_all<4>, grd, blk, 0, st, a, nobs, ocoord_orig, r);
   return hipGetLastError();
 }
 

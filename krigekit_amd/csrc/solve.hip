@@ -45,6 +45,18 @@ __device__ inline int wmax(int v) {
   return v;
 }
 
+// Covariance of one pair of rows.  DIM 4 is space-time (x, y, z, t): the
+// product-sum k Cs(dx, dy, dz) Ct(|dt|) + Cs + Ct of the spatial model and
+// the 1-D temporal model (variogram_st.f90:222-282).
+template <int DIM, bool TB>
+__device__ inline double pair_cov(const SolveArgs& a, const Model& M, double dx, double dy, double dz,
+                                  double dt) {
+  if (DIM <= 3) return cov_eval<DIM, TB>(M, dx, dy, dz);
+  const double cs = cov_eval<3, TB>(M, dx, dy, dz);
+  const double ct = cov_general<1>(a.tmodel, fabs(dt), 0.0, 0.0);
+  return a.k_ps * cs * ct + cs + ct;
+}
+
 template <int G, int DIM, int PREC, bool TB>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
@@ -61,7 +73,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const int idx = live ? a.inear[(size_t)b * a.mmax + r] : 0;
   const int nw = wmax(n);  // largest system in this wave (uniform)
 
-  double x[3] = {0.0, 0.0, 0.0};
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
   if (live) {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) x[k] = a.ocoord[k + DIM * (size_t)idx];
@@ -73,6 +85,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const double rs = (act && a.rangescale) ? a.rangescale[b] : 1.0;
   const double ln = (act && a.localnugget) ? a.localnugget[b] : 0.0;
   const Model& M = *a.model;
+  const double cov0 = DIM == 4 ? a.st_cov0 : M.cov0;
   __syncthreads();
 
   // ---- assemble K by the fold (each pair once) --------------------------
@@ -84,15 +97,16 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     if (d < nw && live && c < n && mine) {
       const double dx = (x[0] - sx[s][0][c]) / rs;
       const double dy = DIM >= 2 ? (x[1] - sx[s][1][c]) / rs : 0.0;
-      const double dz = DIM == 3 ? (x[2] - sx[s][2][c]) / rs : 0.0;
-      v = cov_eval<DIM, TB>(M, dx, dy, dz);
+      const double dz = DIM >= 3 ? (x[2] - sx[s][2][c]) / rs : 0.0;
+      const double dt = DIM == 4 ? x[DIM - 1] - sx[s][DIM - 1][c] : 0.0;
+      v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
     }
     if (mine) {
       sm[s][r][c] = v;
       sm[s][c][r] = v;
     }
   }
-  sm[s][r][r] = live ? (M.cov0 + ov) + ln : 1.0;
+  sm[s][r][r] = live ? (cov0 + ov) + ln : 1.0;
   // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
   double rhs = 0.0;
   if (live) {
@@ -102,8 +116,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
       const double dx = (x[0] - g[0]) / rs;
       const double dy = DIM >= 2 ? (x[1] - g[1]) / rs : 0.0;
-      const double dz = DIM == 3 ? (x[2] - g[2]) / rs : 0.0;
-      rhs = rhs + cov_eval<DIM, TB>(M, dx, dy, dz) * w;
+      const double dz = DIM >= 3 ? (x[2] - g[2]) / rs : 0.0;
+      const double dt = DIM == 4 ? x[DIM - 1] - g[DIM - 1] : 0.0;
+      rhs = rhs + pair_cov<DIM, TB>(a, M, dx, dy, dz, dt) * w;
     }
   }
   // ones on the live rows: the unbias row F for ordinary kriging, and the
@@ -268,7 +283,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     return;
   }
   double var, est;
-  const double base = a.base_cov ? a.base_cov[b] : M.cov0;
+  const double base = a.base_cov ? a.base_cov[b] : (DIM == 4 ? a.st_cov0 : M.cov0);
   if (a.unbias) {
     const double lam = (uy - 1.0) / uu;
     var = (base - yy) + lam * (uy - 1.0);
@@ -337,6 +352,16 @@ __global__ void k_collect(const int* flag, int nblock, int code, int* count, int
 // (SSYTRF, then one retry with the data diagonal raised by 1e-6*mean(diag),
 // then NaN; kriging_base.F90:3299-3383).
 template <int DIM>
+__device__ inline double fb_cov(const SolveArgs& a, const Model& M, const double* xi, const double* xj, double rs) {
+  const double dx = (xi[0] - xj[0]) / rs, dy = DIM >= 2 ? (xi[1] - xj[1]) / rs : 0.0;
+  const double dz = DIM >= 3 ? (xi[2] - xj[2]) / rs : 0.0;
+  if (DIM <= 3) return cov<DIM>(M, dx, dy, dz);
+  const double cs = cov<3>(M, dx, dy, dz);
+  const double ct = cov<1>(*a.tmodel, fabs(xi[DIM - 1] - xj[DIM - 1]), 0.0, 0.0);
+  return a.k_ps * cs * ct + cs + ct;
+}
+
+template <int DIM>
 __global__ void k_fallback(SolveArgs a, double* scratch) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nlist) return;
@@ -354,11 +379,10 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   const Model& M = *a.model;
   for (int i = 0; i < n; ++i) {
     const double* xi = a.ocoord + DIM * (size_t)in[i];
-    K[i + ld * i] = (M.cov0 + a.ovar[in[i]]) + ln;
+    K[i + ld * i] = ((DIM == 4 ? a.st_cov0 : M.cov0) + a.ovar[in[i]]) + ln;
     for (int j = i + 1; j < n; ++j) {
       const double* xj = a.ocoord + DIM * (size_t)in[j];
-      const double c = cov<DIM>(M, (xi[0] - xj[0]) / rs, DIM >= 2 ? (xi[1] - xj[1]) / rs : 0.0,
-                                DIM == 3 ? (xi[2] - xj[2]) / rs : 0.0);
+      const double c = fb_cov<DIM>(a, M, xi, xj, rs);
       K[j + ld * i] = c;
       K[i + ld * j] = c;
     }
@@ -371,8 +395,7 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
       const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
                                                  : a.scoord + DIM * ((size_t)b * a.nsub + q);
       const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
-      tmp = tmp + cov<DIM>(M, (xi[0] - g[0]) / rs, DIM >= 2 ? (xi[1] - g[1]) / rs : 0.0,
-                           DIM == 3 ? (xi[2] - g[2]) / rs : 0.0) * w;
+      tmp = tmp + fb_cov<DIM>(a, M, xi, g, rs) * w;
     }
     rhs[i] = tmp;
   }
@@ -440,7 +463,7 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
     est = est + a.ovalue[in[i]] * xs[i];
     tw = tw + xs[i];
   }
-  const double base = a.base_cov ? a.base_cov[b] : M.cov0;
+  const double base = a.base_cov ? a.base_cov[b] : (DIM == 4 ? a.st_cov0 : M.cov0);
   double var = base - dot;
   if (!a.unbias && a.sk_mean != 0.0) est = est + (1.0 - tw) * a.sk_mean;
   var = var > 0.0 ? var : 0.0;
@@ -489,9 +512,12 @@ static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
   } else if (a.ndim == 2) {
     if (tb) hipLaunchKernelGGL((k_solve<G, 2, PREC, true>), grd, blk, 0, st, a);
     else hipLaunchKernelGGL((k_solve<G, 2, PREC, false>), grd, blk, 0, st, a);
-  } else {
+  } else if (a.ndim == 3) {
     if (tb) hipLaunchKernelGGL((k_solve<G, 3, PREC, true>), grd, blk, 0, st, a);
     else hipLaunchKernelGGL((k_solve<G, 3, PREC, false>), grd, blk, 0, st, a);
+  } else {  // space-time (x, y, z, t)
+    if (tb) hipLaunchKernelGGL((k_solve<G, 4, PREC, true>), grd, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve<G, 4, PREC, false>), grd, blk, 0, st, a);
   }
 }
 
@@ -527,7 +553,8 @@ hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st) 
   if (a.nlist <= 0) return hipSuccess;
   if (a.ndim == 1) hipLaunchKernelGGL(k_fallback<1>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
   else if (a.ndim == 2) hipLaunchKernelGGL(k_fallback<2>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
-  else hipLaunchKernelGGL(k_fallback<3>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
+  else if (a.ndim == 3) hipLaunchKernelGGL(k_fallback<3>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
+  else hipLaunchKernelGGL(k_fallback<4>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
   return hipGetLastError();
 }
 

@@ -82,6 +82,18 @@ def _load_lib():
     sig("kk_device_synchronize", [])
     sig("kk_copy_probe", [_c_double, _c_int, _pd])
     sig("kk_loo_grad", [_i64, _pd, _pd, _pd, _pd])
+    # space-time (krige_st_*)
+    sig("krige_st_create", [ctypes.POINTER(_i64)])
+    sig("krige_st_destroy", [ctypes.POINTER(_i64)])
+    sig("krige_st_initialize", [_i64] + [_c_int] * 12 + [ctypes.c_char_p, _pd, _c_int])
+    sig("krige_st_set_st_model", [_i64, ctypes.c_char_p, ctypes.c_char_p] + [_c_double] * 4)
+    sig("krige_st_set_obs", [_i64, _c_int, _c_int, _c_int, _pd, _pd, _pd, _c_int, _c_double, _c_double])
+    sig("krige_st_reset_vgm", [_i64, _c_int, _c_int])
+    sig("krige_st_set_vgm", [_i64, _c_int, _c_int, ctypes.c_char_p] + [_c_double] * 8)
+    sig("krige_st_set_vgm_temporal", [_i64, _c_int, _c_int, ctypes.c_char_p] + [_c_double] * 3)
+    sig("krige_st_set_vgm_joint_sills", [_i64, _c_int, _c_int, _c_int, _pd])
+    sig("krige_st_set_grid", [_i64, _c_int, _pd, _pd, _pd, _pd])
+    sig("krige_st_set_search", [_i64, _c_int] + [_c_double] * 6 + [_c_int])
     _lib = lib
     return lib
 
@@ -414,13 +426,20 @@ def device_synchronize():
 def from_spec(spec, **kw):
     """Builds a ``Kriging`` object from a workload description
     (``krigekit_amd.synth``) and returns it ready to ``solve()``."""
-    k = Kriging(ndim=spec["ndim"], unbias=spec.get("unbias", 1),
+    nvar = spec.get("nvar", 1)
+    if nvar > 1:
+        kw.setdefault("std_ck", spec.get("std_ck", True))
+    k = Kriging(ndim=spec["ndim"], nvar=nvar, unbias=spec.get("unbias", 1),
                 cross_validation=bool(spec.get("cross_validation", False)), **kw)
     for iv, o in enumerate(spec["obs"], start=1):
         k.set_obs(iv, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
     if spec.get("grid") is None:
         k.set_grid_cv()
+    elif spec.get("blocksize") is not None:
+        # Gauss-Legendre integration nodes, 4 per axis
+        k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]),
+                         rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
     else:
         k.set_grid(spec["grid"], rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
     for v in spec["vgm"]:

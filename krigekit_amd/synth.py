@@ -162,3 +162,39 @@ def ck2d(scale=1.0, seed=1003, grid_side=None, std_ck=True):
                 obs=[dict(coord=x1, value=z1, nmax=24), dict(coord=x2, value=z2, nmax=48)],
                 grid=centres, blocksize=bs, nsub=16, sub_coord=sub, sub_weight=w, vgm=vgm,
                 search=[dict(ivar=1, sector_search=False), dict(ivar=2, sector_search=False)])
+
+
+def st(nstation=500, ndays=365, side=500.0, grid_side=500, ntimes=30, nmax=48, maxdist=40.0, seed=1005):
+    """Space-time product-sum kriging (km and days).
+
+    `nstation` stations uniform in a `side` km square (z = 0) observed daily
+    for `ndays` days: value = sin(2 pi t / 365) + a seeded spatial sinusoid-sum
+    field (wavelengths 50-400 km) + a per-station AR(1) series (phi 0.7,
+    innovations N(0, 0.1^2)).  Targets: a grid_side x grid_side grid at
+    side/grid_side km spacing (cell centres) x `ntimes` times 6 + 12 k (integer
+    days, so time ties occur).  Model: Cs exponential, nugget 0.05, sill 1.0,
+    range 50 km; Ct exponential, sill 0.5, range 5 days; k_ps 0.5.  Search:
+    time_at 10 km/day, nmax 48, maxdist 40 km-equivalent (interior nodes get
+    48 neighbours, edges and corners fewer)."""
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(0.0, side, (nstation, 2))
+    field = _sinusoid_field_2d(np.random.default_rng(seed + 1), xy, nterm=8, lmin=50.0, lmax=400.0)
+    t = np.arange(ndays, dtype=np.float64)
+    ar = np.zeros((nstation, ndays))
+    for d in range(1, ndays):
+        ar[:, d] = 0.7 * ar[:, d - 1] + rng.normal(0.0, 0.1, nstation)
+    val = (np.sin(2 * np.pi * t / 365.0)[None, :] + field[:, None] + ar).ravel()
+    coord = np.zeros((nstation * ndays, 4))
+    coord[:, 0] = np.repeat(xy[:, 0], ndays)
+    coord[:, 1] = np.repeat(xy[:, 1], ndays)
+    coord[:, 3] = np.tile(t, nstation)
+    h = side / grid_side
+    ax = h * (0.5 + np.arange(grid_side))
+    times = 6.0 + 12.0 * np.arange(ntimes)
+    gx, gy, gt = np.meshgrid(ax, ax, times, indexing="ij")
+    grid = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size), gt.ravel()], axis=1)
+    return dict(name="st", st=True, ndim=3, k_ps=0.5,
+                obs=[dict(coord=coord, value=val, nmax=nmax, maxdist=maxdist)], grid=grid,
+                vgm=[dict(ivar=1, jvar=1, vtype="exp", nugget=0.05, sill=1.0, a_major=50.0)],
+                vgm_t=[dict(vtype="exp", sill=0.5, at_k=5.0)],
+                search=[dict(ivar=1, time_at=10.0, sector_search=False)])

@@ -767,7 +767,23 @@ typedef struct {
   double maxdist2;
   double srot[9];
   int have_srot;
+  /* space-time */
+  int st, nl;           /* nl: coordinate rows (ndim, or 4 for ST) */
+  const model_t *Mt;    /* temporal structures */
+  double k_ps, cov0;    /* cov0 of the (ST) model */
+  double time_at;
 } ctx_t;
+
+/* COV / COV_ST: cov_lag or, for space-time, the product-sum
+ * k_ps Cs(lag_s) Ct(|dt|) + Cs(lag_s) + Ct(|dt|) (variogram_st.f90:222-282;
+ * the temporal lag is the 1-D vector [|dt|, 0, 0]) */
+static double ctx_cov(const ctx_t *C, const double *lag) {
+  if (!C->st) return cov_lag(C->M, lag);
+  double ls[3] = {lag[0], lag[1], lag[2]};
+  double lt[3] = {fabs(lag[3]), 0.0, 0.0};
+  double cs = cov_lag(C->M, ls), ct = cov_lag(C->Mt, lt);
+  return C->k_ps * cs * ct + cs + ct;
+}
 
 static void rot_pt(const double *R, int ndim, const double *x, double *y) {
   if (ndim == 1) {
@@ -786,10 +802,12 @@ static void rot_pt(const double *R, int ndim, const double *x, double *y) {
 static int block_search(const ctx_t *C, int ib, const double *xloc, int *inear, double *dist,
                         kres_t *cand) {
   const okr_problem *P = C->P;
-  int nd = P->ndim, nnear = 0;
-  double newloc[3];
+  int nd = P->ndim, nl = C->nl, nnear = 0;
+  double newloc[4];
   if (C->have_srot) rot_pt(C->srot, nd, xloc, newloc);
   else for (int k = 0; k < nd; ++k) newloc[k] = xloc[k];
+  /* ST: search coordinate t * time_at (kriging_st.F90:90-95, 490-497) */
+  if (C->st) newloc[3] = xloc[3] * C->time_at;
   if (P->sector && C->need_search) {
     knn(C->tree, newloc, C->ncand, 1, cand);
     int cv_skip = P->cross_validation ? ib + 1 : 0;
@@ -799,7 +817,7 @@ static int block_search(const ctx_t *C, int ib, const double *xloc, int *inear, 
       int idx = cand[i].idx;
       if (idx == cv_skip) continue;
       if (cand[i].dis > C->maxdist2) continue;
-      const double *o = P->obs_coord + nd * (idx - 1);
+      const double *o = P->obs_coord + nl * (idx - 1);
       double pr[3];
       if (C->have_srot) rot_pt(C->srot, nd, o, pr);
       else for (int k = 0; k < nd; ++k) pr[k] = o[k];
@@ -823,13 +841,18 @@ static int block_search(const ctx_t *C, int ib, const double *xloc, int *inear, 
       /* rotated_dists rotation.f90:193-218 */
       nnear = P->nobs;
       for (int i = 0; i < nnear; ++i) {
-        const double *o = P->obs_coord + nd * i;
+        const double *o = P->obs_coord + nl * i;
         double df[3], r[3];
-        for (int k = 0; k < nd; ++k) df[k] = o[k] - newloc[k];
+        for (int k = 0; k < nd; ++k) df[k] = o[k] - xloc[k];
         if (C->have_srot) rot_pt(C->srot, nd, df, r);
         else for (int k = 0; k < nd; ++k) r[k] = df[k];
         double s = 0.0;
         for (int k = 0; k < nd; ++k) s = s + r[k] * r[k];
+        if (C->st) {
+          /* kriging_st.F90:609-612: + (t_obs * at - block_ht)^2 */
+          double dt = o[3] * C->time_at - newloc[3];
+          s = s + dt * dt;
+        }
         inear[i] = i + 1;
         dist[i] = s;
       }
@@ -854,8 +877,7 @@ static int block_search(const ctx_t *C, int ib, const double *xloc, int *inear, 
 static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double *work,
                      int *fail) {
   const okr_problem *P = C->P;
-  const model_t *M = C->M;
-  int nd = P->ndim;
+  int nd = C->nl;  /* coordinate rows */
   const double *xloc = P->cross_validation ? P->obs_coord + nd * ib : P->block_coord + nd * ib;
   int *inear = R->inear + (size_t)C->mmax * ib;
   double *dist = work;
@@ -879,7 +901,8 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
     R->variance[ib] = NAN;
     return 0;
   }
-  double rs = P->rangescale ? P->rangescale[ib] : 1.0;
+  /* the ST assembly does not range-scale lags (kriging_st.F90:805, 921) */
+  double rs = (P->rangescale && !C->st) ? P->rangescale[ib] : 1.0;
   double ln = P->localnugget ? P->localnugget[ib] : 0.0;
   int p = P->unbias ? 1 : 0, m = n + p;
   double *K = work + C->mmax;            /* m x m */
@@ -887,16 +910,16 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
   double *x = b + m;                      /* m */
   double *U = x + m;                      /* n x n */
   double *Y = U + (size_t)n * n;          /* n */
-  double lag[3] = {0, 0, 0};
+  double lag[4] = {0, 0, 0, 0};
   /* assemble_lhs kriging.F90:714-849 */
   for (int i = 0; i < n; ++i) {
     const double *xi = P->obs_coord + nd * (inear[i] - 1);
     double ov = P->obs_var ? P->obs_var[inear[i] - 1] : 0.0;
-    K[i + m * i] = M->cov0 + ov + ln;
+    K[i + m * i] = C->cov0 + ov + ln;
     for (int j = i + 1; j < n; ++j) {
       const double *xj = P->obs_coord + nd * (inear[j] - 1);
       for (int k = 0; k < nd; ++k) lag[k] = (xi[k] - xj[k]) / rs;
-      double c = cov_lag(M, lag);
+      double c = ctx_cov(C, lag);
       K[j + m * i] = c;
       K[i + m * j] = c;
     }
@@ -912,7 +935,7 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
       const double *g = (ns == 1 && !P->sub_coord) ? xloc : P->sub_coord + (size_t)nd * ((size_t)ib * ns + s);
       double w = (ns == 1 && !P->sub_weight) ? 1.0 : P->sub_weight[(size_t)ib * ns + s];
       for (int k = 0; k < nd; ++k) lag[k] = (xi[k] - g[k]) / rs;
-      tmp = tmp + cov_lag(M, lag) * w;
+      tmp = tmp + ctx_cov(C, lag) * w;
     }
     b[i] = tmp;
   }
@@ -969,18 +992,18 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
     return 0;
   }
   /* calc_variance kriging.F90:968-1028 */
-  double base = M->cov0;
+  double base = C->cov0;
   if (ns > 1) {
     base = 0.0;
     for (int i = 0; i < ns; ++i) {
       const double *gi = P->sub_coord + (size_t)nd * ((size_t)ib * ns + i);
       double wi = P->sub_weight[(size_t)ib * ns + i];
-      base = base + M->cov0 * wi * wi;
+      base = base + C->cov0 * wi * wi;
       for (int j = i + 1; j < ns; ++j) {
         const double *gj = P->sub_coord + (size_t)nd * ((size_t)ib * ns + j);
         double wj = P->sub_weight[(size_t)ib * ns + j];
         for (int k = 0; k < nd; ++k) lag[k] = gi[k] - gj[k];
-        base = base + cov_lag(M, lag) * wi * wj * 2.0;
+        base = base + ctx_cov(C, lag) * wi * wj * 2.0;
       }
     }
   }
@@ -1004,31 +1027,71 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
 }
 
 int okr_cov(const okr_problem *P, int n, const double *lags, double *out) {
-  model_t M;
+  model_t M, Mt;
   char err[512];
   if (model_build(&M, P, err)) return 1;
+  ctx_t C;
+  memset(&C, 0, sizeof C);
+  C.M = &M;
+  int nl = P->ndim;
+  if (P->st) {
+    okr_problem Pt = *P;
+    Pt.ndim = 1;
+    Pt.nstruct = P->nstruct_t;
+    Pt.structs = P->structs_t;
+    if (model_build(&Mt, &Pt, err)) {
+      model_free(&M);
+      return 1;
+    }
+    C.st = 1;
+    C.Mt = &Mt;
+    C.k_ps = P->k_ps;
+    nl = 4;
+  }
   for (int i = 0; i < n; ++i) {
-    double lag[3] = {0, 0, 0};
-    for (int k = 0; k < P->ndim; ++k) lag[k] = lags[k + P->ndim * i];
-    out[i] = cov_lag(&M, lag);
+    double lag[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nl; ++k) lag[k] = lags[k + nl * i];
+    out[i] = ctx_cov(&C, lag);
   }
   model_free(&M);
+  if (P->st) model_free(&Mt);
   return 0;
 }
 
 int okr_solve(const okr_problem *P, okr_result *R) {
   R->errmsg[0] = 0;
   memset(R->stats, 0, sizeof R->stats);
-  if (P->ndim < 1 || P->ndim > 3 || P->nobs < 1 || (P->ndim == 1 && P->sector)) {
-    snprintf(R->errmsg, sizeof R->errmsg, "bad ndim/nobs (sector search needs ndim >= 2)");
+  if (P->ndim < 1 || P->ndim > 3 || P->nobs < 1 || (P->ndim == 1 && P->sector) || (P->st && P->ndim != 3)) {
+    snprintf(R->errmsg, sizeof R->errmsg, "bad ndim/nobs (sector search needs ndim >= 2, ST ndim 3)");
     return 1;
   }
-  model_t M;
+  model_t M, Mt;
   if (model_build(&M, P, R->errmsg)) return 1;
   ctx_t C;
   memset(&C, 0, sizeof C);
+  memset(&Mt, 0, sizeof Mt);
   C.P = P;
   C.M = &M;
+  C.nl = P->ndim;
+  C.cov0 = M.cov0;
+  if (P->st) {
+    /* temporal model: 1-D, isotropic range at_k (variogram_st.f90:302-317);
+     * cov0 = k Cs0 Ct0 + Cs0 + Ct0 (variogram_st.f90:344-375) */
+    okr_problem Pt = *P;
+    Pt.ndim = 1;
+    Pt.nstruct = P->nstruct_t;
+    Pt.structs = P->structs_t;
+    if (model_build(&Mt, &Pt, R->errmsg)) {
+      model_free(&M);
+      return 1;
+    }
+    C.st = 1;
+    C.nl = 4;
+    C.Mt = &Mt;
+    C.k_ps = P->k_ps;
+    C.time_at = P->time_at;
+    C.cov0 = P->k_ps * M.cov0 * Mt.cov0 + M.cov0 + Mt.cov0;
+  }
   C.nmax = eff_nmax(P);
   C.need_search = P->nobs > C.nmax;
   C.maxdist2 = P->maxdist * P->maxdist;
@@ -1051,16 +1114,25 @@ int okr_solve(const okr_problem *P, okr_result *R) {
   if (C.have_srot) calc_rotmat(P->s_azimuth, P->s_dip, P->s_plunge, P->s_anis1, P->s_anis2, C.srot);
   if (C.need_search) {
     const double *src = P->obs_coord;
-    if (C.have_srot) {
-      rc = (double *)malloc(sizeof(double) * P->ndim * P->nobs);
-      for (int i = 0; i < P->nobs; ++i) rot_pt(C.srot, P->ndim, P->obs_coord + P->ndim * i, rc + P->ndim * i);
+    if (C.have_srot || C.st) {
+      /* search coordinates: rotated spatial part, ST time * time_at
+       * (kriging_st.F90:387-395) */
+      rc = (double *)malloc(sizeof(double) * C.nl * P->nobs);
+      for (int i = 0; i < P->nobs; ++i) {
+        const double *o = P->obs_coord + C.nl * i;
+        double *r = rc + C.nl * i;
+        if (C.have_srot) rot_pt(C.srot, P->ndim, o, r);
+        else for (int k = 0; k < P->ndim; ++k) r[k] = o[k];
+        if (C.st) r[3] = o[3] * C.time_at;
+      }
       src = rc;
     }
-    if (tree_build(&tree, P->ndim, P->nobs, src)) {
+    if (tree_build(&tree, C.nl, P->nobs, src)) {
       snprintf(R->errmsg, sizeof R->errmsg, "kd-tree build did not terminate");
       tree_free(&tree);
       free(rc);
       model_free(&M);
+      if (C.st) model_free(&Mt);
       return 1;
     }
     C.tree = &tree;
@@ -1101,5 +1173,6 @@ int okr_solve(const okr_problem *P, okr_result *R) {
   tree_free(&tree);
   free(rc);
   model_free(&M);
+  if (C.st) model_free(&Mt);
   return err ? 1 : 0;
 }

@@ -47,6 +47,16 @@ typedef struct {
   double bound_lo, bound_hi;
   int neglect_error;
   int nthread;
+  /* space-time kriging (kriging_st.F90): obs/block coordinates are 4 rows
+   * (x, y, z, t) with ndim = 3; the search runs on (x, y, z, t * time_at) and
+   * the covariance is the product-sum k_ps Cs Ct + Cs + Ct
+   * (variogram_st.f90:222-282) with the spatial structures above and the
+   * isotropic 1-D temporal structures below */
+  int st;
+  double time_at;
+  double k_ps;
+  int nstruct_t;
+  const okr_struct *structs_t;
 } okr_problem;
 
 typedef struct {

@@ -40,7 +40,9 @@ class Problem(ctypes.Structure):
         ("sub_coord", _pd), ("sub_weight", _pd), ("rangescale", _pd), ("localnugget", _pd),
         ("nstruct", ctypes.c_int), ("structs", ctypes.POINTER(Struct)),
         ("table_mode", ctypes.c_int), ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double),
-        ("neglect_error", ctypes.c_int), ("nthread", ctypes.c_int)]
+        ("neglect_error", ctypes.c_int), ("nthread", ctypes.c_int),
+        ("st", ctypes.c_int), ("time_at", ctypes.c_double), ("k_ps", ctypes.c_double),
+        ("nstruct_t", ctypes.c_int), ("structs_t", ctypes.POINTER(Struct))]
 
 
 class Result(ctypes.Structure):
@@ -143,6 +145,15 @@ def make_problem(spec, table_mode=True, nthread=8, bounds=None, neglect_error=Tr
     P.bound_lo, P.bound_hi = b
     P.neglect_error = int(neglect_error)
     P.nthread = nthread
+    if spec.get("st"):
+        # space-time: coordinates (n, 4) = (x, y, z, t), spatial ndim 3
+        P.st = 1
+        P.time_at = spec["search"][0].get("time_at", 1.0)
+        P.k_ps = spec.get("k_ps", 0.0)
+        h.stt = _structs([dict(v, a_major=v["at_k"], a_minor1=v["at_k"], a_minor2=v["at_k"])
+                          for v in spec["vgm_t"]])
+        P.nstruct_t = len(spec["vgm_t"])
+        P.structs_t = ctypes.cast(h.stt, ctypes.POINTER(Struct))
     return P, h
 
 
@@ -174,6 +185,7 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False, **kw):
 
 
 def cov(spec, lags, table_mode=True):
+    """covariance at lags (m, ndim), or (m, 4) space-time lags (dx, dy, dz, dt)"""
     P, h = make_problem(spec, table_mode, 1)
     lags = np.ascontiguousarray(lags, dtype=np.float64)
     out = np.empty(len(lags))

@@ -240,6 +240,27 @@ def ck_cases():
             ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small()]
 
 
+def st_small(nstation=60, ndays=60, side=170.0, grid_side=8, ntimes=3, nmax=24, maxdist=40.0, sector=False,
+             cv=False, name="st_small", seed=1005):
+    """space-time product-sum problem at C5 densities (60 stations x 60 days)"""
+    p = synth.st(nstation=nstation, ndays=ndays, side=side, grid_side=grid_side, ntimes=ntimes, nmax=nmax,
+                 maxdist=maxdist, seed=seed)
+    p["search"][0]["sector_search"] = sector
+    if sector:
+        p["obs"][0]["nmax"] = 3
+    if cv:
+        p["grid"] = None
+        p["cross_validation"] = True
+    p["name"] = name
+    return p
+
+
+def st_cases():
+    return [st_small(), st_small(sector=True, name="st_octant"), st_small(nmax=8, maxdist=25.0, name="st_maxdist"),
+            st_small(nstation=6, ndays=9, nmax=200, maxdist=500.0, name="st_allobs"),
+            st_small(nstation=30, ndays=20, cv=True, nmax=12, name="st_loo")]
+
+
 def _named(p, name):
     p["name"] = name
     return p

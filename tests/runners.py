@@ -173,3 +173,15 @@ def ck_errors(x, ref):
             np.array_equal(np.isnan(x["variance"]), np.isnan(ref["variance"]))):
         return np.inf, np.inf
     return float(np.max(e, initial=0.0)), float(np.max(v, initial=0.0))
+
+
+def run_gpu_st(spec, fast=True, **kw):
+    from krigekit_amd.kriging_st import from_st_spec
+    k = from_st_spec(spec, **kw)
+    k.set_search_mode(fast)
+    k.solve()
+    e, v = k.get_results()
+    nn, inn = k.neighbours()
+    out = dict(estimate=np.asarray(e), variance=np.asarray(v), nnear=nn, inear=inn, timing=k.timing)
+    k.close()
+    return out
