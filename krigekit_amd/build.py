@@ -24,12 +24,19 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-fvisibility=hidden", "-I", CSRC]
 
+# (source, extra flags, object name).  solve.hip is compiled five times: the
+# shared kernels and dispatch, then the k_solve instances of each coordinate
+# count (1-3 spatial, 4 space-time) in a unit of their own, in parallel.
 UNITS = [
-    ("search.hip", ["-ffp-contract=off"]),
-    ("solve.hip", []),
-    ("loograd.hip", []),
-    ("cokrige.hip", []),
-    ("capi.cpp", ["-x", "hip"]),
+    ("search.hip", ["-ffp-contract=off"], "search.o"),
+    ("solve.hip", [], "solve.o"),
+    ("solve.hip", ["-DKK_SOLVE_DIM=1"], "solve_d1.o"),
+    ("solve.hip", ["-DKK_SOLVE_DIM=2"], "solve_d2.o"),
+    ("solve.hip", ["-DKK_SOLVE_DIM=3"], "solve_d3.o"),
+    ("solve.hip", ["-DKK_SOLVE_DIM=4"], "solve_d4.o"),
+    ("loograd.hip", [], "loograd.o"),
+    ("cokrige.hip", [], "cokrige.o"),
+    ("capi.cpp", ["-x", "hip"], "capi.o"),
 ]
 
 
@@ -43,11 +50,11 @@ def _run(cmd, verbose):
     return r
 
 
-def build(verbose=False, jobs=3):
+def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     objs, procs = [], []
-    for src, extra in UNITS:
-        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
+    for src, extra, name in UNITS:
+        obj = os.path.join(OUT, name)
         cmd = [HIPCC, "-c"] + COMMON + extra + [os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -885,7 +885,9 @@ bool run_pipeline(Krige& K) {
     va.sweight = K.d_sweight.as<double>();
     va.base_cov = K.d_basecov.as<double>();
   }
-  const bool tb = K.model.mode == kk::COV_TABLE_B;
+  // the inline Level-B kernels need every model on one composite table: for
+  // space-time both the spatial and the temporal model
+  const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
   // ---- bucket by neighbour count, solve per bucket ---------------------------
   // round 0: every block the search has decided; round 1: the blocks the
   // exact search decided on the second stream

@@ -25,6 +25,10 @@
 #include "kk_device.hpp"
 #include "kk_types.hpp"
 
+#ifndef KK_SOLVE_DIM
+#define KK_SOLVE_DIM 0
+#endif
+
 namespace kk {
 
 template <int G>
@@ -39,6 +43,16 @@ __device__ inline int gmax(int v) {
   for (int m = G / 2; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, G));
   return v;
 }
+// 1/a to within an ulp or so: hardware estimate and two Newton steps (the
+// factorisation needs no correctly rounded division; a > 0 here or the
+// system is flagged)
+__device__ inline double recip(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
 __device__ inline int wmax(int v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
@@ -48,12 +62,13 @@ __device__ inline int wmax(int v) {
 // Covariance of one pair of rows.  DIM 4 is space-time (x, y, z, t): the
 // product-sum k Cs(dx, dy, dz) Ct(|dt|) + Cs + Ct of the spatial model and
 // the 1-D temporal model (variogram_st.f90:222-282).
+// With TB both models are single Level-B tables and both lookups are inline.
 template <int DIM, bool TB>
 __device__ inline double pair_cov(const SolveArgs& a, const Model& M, double dx, double dy, double dz,
                                   double dt) {
   if (DIM <= 3) return cov_eval<DIM, TB>(M, dx, dy, dz);
   const double cs = cov_eval<3, TB>(M, dx, dy, dz);
-  const double ct = cov_general<1>(a.tmodel, fabs(dt), 0.0, 0.0);
+  const double ct = cov_eval<1, TB>(*a.tmodel, fabs(dt), 0.0, 0.0);
   return a.k_ps * cs * ct + cs + ct;
 }
 
@@ -71,7 +86,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const int n = act ? a.nnear[b] : 0;
   const bool live = r < n;
   const int idx = live ? a.inear[(size_t)b * a.mmax + r] : 0;
-  const int nw = wmax(n);  // largest system in this wave (uniform)
+  // largest system in this wave; readfirstlane makes it a scalar, so the
+  // k/j loop guards below are scalar branches instead of exec-mask regions
+  const int nw = __builtin_amdgcn_readfirstlane(wmax(n));
 
   double x[4] = {0.0, 0.0, 0.0, 0.0};
   if (live) {
@@ -82,7 +99,10 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   for (int k = 0; k < DIM; ++k) sx[s][k][r] = x[k];
   const double z = live ? a.ovalue[idx] : 0.0;
   const double ov = live ? a.ovar[idx] : 0.0;
-  const double rs = (act && a.rangescale) ? a.rangescale[b] : 1.0;
+  // lags are divided by the block's range scale only when one is set (the
+  // host passes none when every block's scale is 1)
+  const bool scaled = a.rangescale != nullptr;
+  const double rs = (act && scaled) ? a.rangescale[b] : 1.0;
   const double ln = (act && a.localnugget) ? a.localnugget[b] : 0.0;
   const Model& M = *a.model;
   const double cov0 = DIM == 4 ? a.st_cov0 : M.cov0;
@@ -95,10 +115,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     double v = 0.0;
     const bool mine = (d < G / 2) || (r < G / 2);
     if (d < nw && live && c < n && mine) {
-      const double dx = (x[0] - sx[s][0][c]) / rs;
-      const double dy = DIM >= 2 ? (x[1] - sx[s][1][c]) / rs : 0.0;
-      const double dz = DIM >= 3 ? (x[2] - sx[s][2][c]) / rs : 0.0;
+      double dx = x[0] - sx[s][0][c];
+      double dy = DIM >= 2 ? x[1] - sx[s][1][c] : 0.0;
+      double dz = DIM >= 3 ? x[2] - sx[s][2][c] : 0.0;
       const double dt = DIM == 4 ? x[DIM - 1] - sx[s][DIM - 1][c] : 0.0;
+      if (scaled) {
+        dx = dx / rs;
+        dy = dy / rs;
+        dz = dz / rs;
+      }
       v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
     }
     if (mine) {
@@ -114,10 +139,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
                                                  : a.scoord + DIM * ((size_t)b * a.nsub + q);
       const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
-      const double dx = (x[0] - g[0]) / rs;
-      const double dy = DIM >= 2 ? (x[1] - g[1]) / rs : 0.0;
-      const double dz = DIM >= 3 ? (x[2] - g[2]) / rs : 0.0;
+      double dx = x[0] - g[0];
+      double dy = DIM >= 2 ? x[1] - g[1] : 0.0;
+      double dz = DIM >= 3 ? x[2] - g[2] : 0.0;
       const double dt = DIM == 4 ? x[DIM - 1] - g[DIM - 1] : 0.0;
+      if (scaled) {
+        dx = dx / rs;
+        dy = dy / rs;
+        dz = dz / rs;
+      }
       rhs = rhs + pair_cov<DIM, TB>(a, M, dx, dy, dz, dt) * w;
     }
   }
@@ -134,34 +164,34 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   double uu, uy, yy, vy, vu;
 
   if (PREC == PREC_FP64) {
-    // ---- L D L^T, row r in A[] ------------------------------------------
+    // ---- L D L^T, row r in A[], with b, F, z carried as three extra
+    // columns: eliminating them alongside K is the forward substitution
+    // y = L^-1 b, u = L^-1 F, v = L^-1 z ----------------------------------
     double D = 1.0;
+    double y = rhs, u = f, v = z;
+    double* ext = &sm[s][0][0];  // pivot row's (y, u, v); sm is free after the row loads
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < nw) {
         col[s][r] = A[k];
+        if (r == k) {
+          ext[0] = y;
+          ext[1] = u;
+          ext[2] = v;
+        }
         __syncthreads();
         const double akk = col[s][k];
         if (r == k) D = akk;
         fail |= (k < n) && !(akk > 0.0);
-        const double t = A[k] / akk;
+        const double t = (r > k) ? A[k] * recip(akk) : 0.0;
+        // columns past nw hold zeros (padding rows): updating them is
+        // harmless and keeps the unrolled loop branch-free
 #pragma unroll
-        for (int j = k + 1; j < G; ++j)
-          if (j < nw) A[j] = fma(-t, col[s][j], A[j]);
-        A[k] = (r > k) ? t : A[k];
+        for (int j = k + 1; j < G; ++j) A[j] = fma(-t, col[s][j], A[j]);
+        y = fma(-t, ext[0], y);
+        u = fma(-t, ext[1], u);
+        v = fma(-t, ext[2], v);
         __syncthreads();
-      }
-    }
-    // ---- forward substitution for b, F, z ---------------------------------
-    double y = rhs, u = f, v = z;
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      if (k < nw) {
-        const double yk = __shfl(y, k, G), uk = __shfl(u, k, G), vk = __shfl(v, k, G);
-        const double l = (r > k) ? A[k] : 0.0;
-        y = fma(-l, yk, y);
-        u = fma(-l, uk, u);
-        v = fma(-l, vk, v);
       }
     }
     const double id = live ? 1.0 / D : 0.0;
@@ -171,31 +201,46 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     vy = gsum<G>(v * y * id);
     vu = gsum<G>(v * u * id);
   } else {
-    // ---- FP32 L D L^T with FP64 residual refinement ----------------------
+    // ---- FP32 L D L^T; b, F, z carried as extra columns (fused forward
+    // substitution), one FP32 back substitution for xb ~ K^-1 b,
+    // x1 ~ K^-1 F, xz ~ K^-1 z.  Each quadratic form c'K^-1 d is then
+    // corrected with the FP64 residual of d:
+    //   c'K^-1 d = c'x_d + x_c'(d - K x_d) - e_c' K e_d,
+    // e = solve errors, so the dropped term is second order in the FP32
+    // error (one refinement step's accuracy without a second solve) --------
     float* col32 = reinterpret_cast<float*>(&col[s][0]);
+    float* ext32 = reinterpret_cast<float*>(&sm[s][0][0]);  // free after the row loads
     float L[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) L[j] = (float)A[j];
     float D = 1.0f;
     float pmin = 1.0f;
+    float py = (float)rhs, pu = (float)f, pv = (float)z;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < nw) {
         col32[r] = L[k];
+        if (r == k) {
+          ext32[0] = py;
+          ext32[1] = pu;
+          ext32[2] = pv;
+        }
         __syncthreads();
         const float akk = col32[k];
         if (r == k) D = akk;
         fail |= (k < n) && !(akk > 0.0f);
-        const float t = L[k] * __builtin_amdgcn_rcpf(akk);
+        const float t = (r > k) ? L[k] * __builtin_amdgcn_rcpf(akk) : 0.0f;
 #pragma unroll
-        for (int j = k + 1; j < G; ++j)
-          if (j < nw) L[j] = fmaf(-t, col32[j], L[j]);
+        for (int j = k + 1; j < G; ++j) L[j] = fmaf(-t, col32[j], L[j]);
+        py = fmaf(-t, ext32[0], py);
+        pu = fmaf(-t, ext32[1], pu);
+        pv = fmaf(-t, ext32[2], pv);
         L[k] = (r > k) ? t : L[k];
         __syncthreads();
       }
     }
     if (live) pmin = (float)((double)D / kdiag);
-    // transpose L through LDS for the backward sweeps
+    // transpose L through LDS for the back substitution
     float* sm32 = reinterpret_cast<float*>(&sm[s][0][0]);
 #pragma unroll
     for (int j = 0; j < G; ++j) sm32[r * (G + 1) + j] = L[j];
@@ -204,58 +249,38 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 #pragma unroll
     for (int j = 0; j < G; ++j) Lt[j] = sm32[j * (G + 1) + r];
     const float iD = 1.0f / D;
-    // solve K32 [xb x1] = [b f]
-    auto solve2 = [&](float p, float q, float& op, float& oq) {
+    py *= iD;
+    pu *= iD;
+    pv *= iD;
 #pragma unroll
-      for (int k = 0; k < G; ++k) {
-        if (k < nw) {
-          const float pk = __shfl(p, k, G), qk = __shfl(q, k, G);
-          const float l = (r > k) ? L[k] : 0.0f;
-          p = fmaf(-l, pk, p);
-          q = fmaf(-l, qk, q);
-        }
+    for (int k = G - 1; k >= 0; --k) {
+      if (k < nw) {
+        const float yk = __shfl(py, k, G), uk = __shfl(pu, k, G), vk = __shfl(pv, k, G);
+        const float l = (r < k) ? Lt[k] : 0.0f;
+        py = fmaf(-l, yk, py);
+        pu = fmaf(-l, uk, pu);
+        pv = fmaf(-l, vk, pv);
       }
-      p *= iD;
-      q *= iD;
-#pragma unroll
-      for (int k = G - 1; k >= 0; --k) {
-        if (k < nw) {
-          const float pk = __shfl(p, k, G), qk = __shfl(q, k, G);
-          const float l = (r < k) ? Lt[k] : 0.0f;
-          p = fmaf(-l, pk, p);
-          q = fmaf(-l, qk, q);
-        }
-      }
-      op = p;
-      oq = q;
-    };
-    float xb32, x132;
-    solve2((float)rhs, (float)f, xb32, x132);
-    double xb = xb32, x1 = x132;
-    // FP64 residual r = t - K x (K row r is A[], still FP64)
+    }
+    const double xb = py, x1 = pu, xz = pv;
+    // FP64 residuals rb = b - K xb, r1 = F - K x1 (row r of K is A[], FP64)
     double rb = rhs, r1 = f;
     double* xsh = &col[s][0];
     __syncthreads();
     xsh[r] = xb;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G; ++j)
-      if (j < nw) rb = fma(-A[j], xsh[j], rb);
+    for (int j = 0; j < G; ++j) rb = fma(-A[j], xsh[j], rb);
     __syncthreads();
     xsh[r] = x1;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G; ++j)
-      if (j < nw) r1 = fma(-A[j], xsh[j], r1);
-    float db, d1;
-    solve2((float)rb, (float)r1, db, d1);
-    xb += (double)db;
-    x1 += (double)d1;
-    uu = gsum<G>(f * x1);
-    uy = gsum<G>(f * xb);
-    yy = gsum<G>(rhs * xb);
-    vy = gsum<G>(z * xb);
-    vu = gsum<G>(z * x1);
+    for (int j = 0; j < G; ++j) r1 = fma(-A[j], xsh[j], r1);
+    uu = gsum<G>(fma(x1, r1, f * x1));
+    uy = gsum<G>(fma(x1, rb, f * xb));
+    yy = gsum<G>(fma(xb, rb, rhs * xb));
+    vy = gsum<G>(fma(xz, rb, z * xb));
+    vu = gsum<G>(fma(xz, r1, z * x1));
     // pivot ratio check: re-solve in FP64 below the threshold
     float pm = live ? pmin : 1.0f;
 #pragma unroll
@@ -301,6 +326,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   a.flag[b] = 0;
 }
 
+#if KK_SOLVE_DIM == 0
 // blocks without neighbours: NaN outputs (kriging_base.F90:3421-3425)
 __global__ void k_empty(SolveArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -502,27 +528,29 @@ hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int
   return hipGetLastError();
 }
 
+#endif  // KK_SOLVE_DIM == 0
+
+// k_solve launchers for one coordinate count, one per translation unit
+// (solve.hip compiled with -DKK_SOLVE_DIM=1..4) so the 16 instances of each
+// compile in parallel; the unit without KK_SOLVE_DIM holds the dispatch.
+hipError_t launch_solve_d1(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
+hipError_t launch_solve_d2(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
+hipError_t launch_solve_d3(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
+hipError_t launch_solve_d4(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
+
+#if KK_SOLVE_DIM > 0
 template <int G, int PREC>
 static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
   constexpr int S = 64 / G;
   dim3 grd((a.nlist + S - 1) / S), blk(64);
-  if (a.ndim == 1) {
-    if (tb) hipLaunchKernelGGL((k_solve<G, 1, PREC, true>), grd, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve<G, 1, PREC, false>), grd, blk, 0, st, a);
-  } else if (a.ndim == 2) {
-    if (tb) hipLaunchKernelGGL((k_solve<G, 2, PREC, true>), grd, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve<G, 2, PREC, false>), grd, blk, 0, st, a);
-  } else if (a.ndim == 3) {
-    if (tb) hipLaunchKernelGGL((k_solve<G, 3, PREC, true>), grd, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve<G, 3, PREC, false>), grd, blk, 0, st, a);
-  } else {  // space-time (x, y, z, t)
-    if (tb) hipLaunchKernelGGL((k_solve<G, 4, PREC, true>), grd, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve<G, 4, PREC, false>), grd, blk, 0, st, a);
-  }
+  if (tb) hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, true>), grd, blk, 0, st, a);
+  else hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, false>), grd, blk, 0, st, a);
 }
 
-hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st) {
-  if (a.nlist <= 0) return hipSuccess;
+#define KK_CAT2(x, y) x##y
+#define KK_CAT(x, y) KK_CAT2(x, y)
+hipError_t KK_CAT(launch_solve_d, KK_SOLVE_DIM)(const SolveArgs& a, int width, int prec, bool tableB,
+                                                hipStream_t st) {
   if (prec == PREC_FP64) {
     switch (width) {
       case 8: launch_g<8, PREC_FP64>(a, tableB, st); break;
@@ -542,6 +570,16 @@ hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hi
   }
   return hipGetLastError();
 }
+#else
+hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st) {
+  if (a.nlist <= 0) return hipSuccess;
+  switch (a.ndim) {
+    case 1: return launch_solve_d1(a, width, prec, tableB, st);
+    case 2: return launch_solve_d2(a, width, prec, tableB, st);
+    case 3: return launch_solve_d3(a, width, prec, tableB, st);
+    default: return launch_solve_d4(a, width, prec, tableB, st);  // space-time (x, y, z, t)
+  }
+}
 
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st) {
   if (a.nlist <= 0) return hipSuccess;
@@ -557,5 +595,7 @@ hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st) 
   else hipLaunchKernelGGL(k_fallback<4>, dim3((a.nlist + 63) / 64), dim3(64), 0, st, a, scratch);
   return hipGetLastError();
 }
+
+#endif  // KK_SOLVE_DIM
 
 }  // namespace kk
