@@ -752,8 +752,10 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 //   * jK = the lowest bin whose cumulative count reaches k, tracked as points
 //     arrive; its upper edge E bounds the k-th key, so boxes beyond E are
 //     pruned, and so are boxes beyond every sector's (nmax+1)-th key;
-//   * all counts are of distinct visited points, hence lower bounds of the
-//     true counts below any key, which is all the decisions below use:
+//   * all counts are of distinct visited points within the lane's bound at
+//     the time (the bound only shrinks), hence lower bounds of the true
+//     counts below any key, and exact below the final bound, where every
+//     kept entry lies; that is all the decisions below use:
 //       entry bin < jK, or bin == jK with cum(jK) <= k : #(d2 <= d2_e) <= k, in;
 //       entry bin > jK                                   : #(d2 <  d2_e) >= k, out;
 //       otherwise the rank of the entry straddles k: a walk over the
@@ -843,6 +845,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     double sd = 0.0;
 #pragma unroll
     for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(x[k], q[k])));
+    // beyond this lane's current bound (which only shrinks): not among its k
+    // nearest and not among the C nearest of its sector at the end, so it
+    // changes no decision; every point within the final bound is still
+    // counted, which is what the in/out rules below rely on
+    if (!(sd <= ball)) return;
     const float key = (float)sd;
     const int bin = key_bin(key, base);
     __hip_atomic_fetch_add(&hist[(bin >> 1) * 64 + lane], 1u << (16 * (bin & 1)), __ATOMIC_RELAXED,
