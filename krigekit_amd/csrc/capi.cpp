@@ -514,6 +514,7 @@ struct Krige {
   // spatial structures in vgm, temporal ones in vgm_t, product-sum k_ps
   bool st = false;
   int st_model = 0;  // 1: product_sum
+  int mv_simple = 0;  // cokriging models use only inline-evaluated shapes
   double k_ps = 0.0, st_cov0 = 0.0;
   std::vector<VStruct> vgm_t;
   Model tmodel{};
@@ -1340,6 +1341,13 @@ bool do_solve_mv(Krige& K, bool fetch) {
     if (!upload(K.d_mvtab, all, W)) return false;
     for (size_t i = 0; i < ms.size(); ++i) ms[i].tab = tabs[i].empty() ? nullptr : K.d_mvtab.as<double>() + off[i];
     if (!upload(K.d_models, ms, W)) return false;
+    // shapes with no trigonometric or power term evaluate inline
+    K.mv_simple = 1;
+    for (const Model& m : ms)
+      for (int i = 0; i < m.nstruct; ++i) {
+        const int t = m.vtype[i];
+        K.mv_simple &= (t == kk::NUG || t == kk::SPH || t == kk::EXP || t == kk::GAU || t == kk::BSQ || t == kk::LIN);
+      }
     K.model_ok = true;
   }
   if (!K.dev_inputs_ok) {
@@ -1381,9 +1389,10 @@ bool do_solve_mv(Krige& K, bool fetch) {
     rows += K.dv[v].mmax;
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
-  if (rows > 128) {
-    set_error(W, "cokriging neighbourhoods larger than 128 data rows are not supported by the GPU solver (" +
-                     std::to_string(rows) + ")");
+  const int cols = rows + kk::mv_extra_cols(nvar, K.std_ck, K.unbias);
+  if (cols > 128) {
+    set_error(W, "cokriging systems larger than 128 rows + right-hand sides are not supported by the GPU solver (" +
+                     std::to_string(cols) + ")");
     return false;
   }
   const int nsub = K.soff.empty() ? 1 : K.soff[1] - K.soff[0];
@@ -1399,7 +1408,8 @@ bool do_solve_mv(Krige& K, bool fetch) {
   a.std_ck = K.std_ck;
   a.unbias = K.unbias;
   a.nsub = nsub;
-  a.ld = rows + 1;
+  a.rows = rows;
+  a.simple = K.mv_simple;
   for (int v = 0; v < nvar; ++v) a.sk_mean[v] = K.obs[v].sk_mean;
   a.bound_lo = K.bounds[0];
   a.bound_hi = K.bounds[1];

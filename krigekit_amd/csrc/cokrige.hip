@@ -9,7 +9,7 @@
 //              otherwise a single row of ones; none for simple kriging;
 //   b_q        per target variable q: sum_k w_k C_{vgm(v_r, q)}((x_r - g_k)/rs)
 //              over the block's integration nodes; d_q = e_q or 1;
-//   solution   Cholesky K = L L', S = F' K^-1 F, mu_q = S^-1 (F' K^-1 b_q - d_q),
+//   solution   K = L L', S = F' K^-1 F, mu_q = S^-1 (F' K^-1 b_q - d_q),
 //              w_q = K^-1 b_q - K^-1 F mu_q (the bordered system's solution);
 //   estimate   w_q' z, plus the Isaaks-Srivastava shift (non-standard
 //              cokriging) and the simple-kriging mean term, clipped to bounds;
@@ -20,8 +20,15 @@
 // with a <= b -- the pairs the reference builds tables for -- and analytic
 // models for the others, so every evaluation matches the reference's.
 //
-// One 128-thread workgroup per block: thread r owns row r (rows <= 128), K
-// and the solve columns live in LDS, FP64 throughout.
+// One 256-thread workgroup per block, a 16 x 16 thread grid.  The bordered
+// matrix [K | b_0.. | F_0.. | z | (I_0.. | 1)] is held in registers, 2-D
+// block-cyclic: thread (tr, tc) owns rows tr + 16a and columns tc + 16b.
+// Right-looking Cholesky over the n data columns eliminates the extra
+// columns alongside (Y = L^-1 X), one LDS broadcast vector per step.  Every
+// output is a dot product of two columns of Y (c'K^-1 d = y_c' y_d), so no
+// back substitution is needed: the Gram matrix of the extra columns gives
+// S, F'K^-1 b, z'K^-1 b, ..., and one thread finishes the small algebra.
+// FP64 throughout.
 #include <hip/hip_runtime.h>
 
 #include "kk_cov.hpp"
@@ -30,41 +37,54 @@
 
 namespace kk {
 
-constexpr int MVT = 128;           // threads per workgroup = max rows
-constexpr int MAXCOL = 2 * MAXVAR; // right-hand sides + constraint columns
+constexpr int MVT = 256;            // threads per workgroup (16 x 16)
+constexpr int MVR = 128;            // max rows + extra columns
+constexpr int MAXE = 3 * MAXVAR + 1;  // extra columns: b, F, z, indicators or ones
 constexpr int MAXSUB = 64;
 
-// sum over the workgroup (two waves); every thread gets the total
-__device__ inline double block_sum(double v, double* part) {
+template <int DIM, bool SIMPLE>
+__device__ inline double mv_cov(const Model* models, int a, int b, int nvar, double dx, double dy, double dz) {
+  const Model* M = models + a + nvar * b;
+  if (SIMPLE) return cov_simple<DIM>(*M, dx, dy, dz);
+  return cov_general<DIM>(M, dx, dy, dz);
+}
+
+// sum over the workgroup; every thread gets the total
+__device__ inline double wg_sum(double v, double* part) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6;
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) part[w] = v;
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
   __syncthreads();
-  return part[0] + part[1];
+  return (part[0] + part[1]) + (part[2] + part[3]);
 }
 
-template <int DIM>
-__device__ inline double mv_cov(const Model* models, int a, int b, int nvar, const double* l) {
-  return cov_general<DIM>(models + a + nvar * b, l[0], DIM > 1 ? l[1] : 0.0, DIM > 2 ? l[2] : 0.0);
-}
-
-template <int DIM, int NV>
+template <int DIM, int NV, int RB, bool SIMPLE>
 __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
-  extern __shared__ double lds[];
-  __shared__ double sx[DIM][MVT];
-  __shared__ double sz[MVT];
-  __shared__ int svr[MVT];
+  __shared__ double sx[DIM][MVR];
+  __shared__ double sz[MVR];
+  __shared__ double sov[MVR];
+  __shared__ int svr[MVR];
   __shared__ double sg[DIM][MAXSUB];
   __shared__ double sgw[MAXSUB];
-  __shared__ double part[2];
+  __shared__ double sb[2][NV][MVR];   // right-hand sides, two partial sums
+  __shared__ double vv[MVR];          // step broadcast: L(:, c) and Y(c, :)
+  // Y of the extra columns after the elimination, [MAXE][MVR]; during the
+  // assembly each thread's RB values of one row slot, [RB][MVT]
+  constexpr int SBUF = MAXE * MVR > RB * MVT ? MAXE * MVR : RB * MVT;
+  __shared__ double sbuf[SBUF];
+  __shared__ double gram[MAXE][MAXE];
+  __shared__ double sbase[NV * NV];
+  __shared__ double part[4];
+  __shared__ double spiv;
+
   const int b = blockIdx.x, t = threadIdx.x;
+  const int tr = t >> 4, tc = t & 15;
   constexpr int nvar = NV;
   int nv[NV], st[NV], n = 0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    nv[v] = v < nvar ? a.nnear[v][b] : 0;
+    nv[v] = a.nnear[v][b];
     st[v] = n;
     n += nv[v];
   }
@@ -78,28 +98,29 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     return;
   }
   const int p = a.unbias ? (a.std_ck ? nvar : 1) : 0;
-  const int ncol = nvar + p;
-  const int ld = a.ld;
-  double* Kd = lds;                         // rows x ld, lower triangle by rows
-  double* cols = lds + (size_t)ld * ld;     // MAXCOL x ld
+  const bool isa = !a.std_ck && a.unbias && nvar > 1;  // Isaaks-Srivastava shift
+  // extra column layout (offsets from n)
+  const int eb = 0, ef = nvar, ez = nvar + p, ei = ez + 1;
+  const int ne = ez + 1 + (isa ? nvar : 0) + (a.unbias ? 0 : 1);
+  const int nx = n + ne;
   const double rs = a.rangescale ? a.rangescale[b] : 1.0;
   const double ln = a.localnugget ? a.localnugget[b] : 0.0;
+  const bool scaled = a.rangescale != nullptr;
   double x0[DIM];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) x0[k] = a.bcoord[k + DIM * (size_t)b];
-  // rows: variable, coordinates relative to the block centre, value
-  const bool row = t < n;
-  int vr = 0, idx = 0;
-  double ovar = 0.0;
-  if (row) {
+
+  // ---- stage rows (coordinates relative to the block centre) and nodes -------
+  if (t < n) {
+    int vr = 0;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
-      if (v < nvar && t >= st[v] && t < st[v] + nv[v]) vr = v;
-    idx = a.inear[vr][(size_t)b * a.mmax[vr] + (t - st[vr])];
+      if (t >= st[v]) vr = v;
+    const int idx = a.inear[vr][(size_t)b * a.mmax[vr] + (t - st[vr])];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) sx[k][t] = a.ocoord[vr][k + DIM * (size_t)idx] - x0[k];
     sz[t] = a.ovalue[vr][idx];
-    ovar = a.ovar[vr][idx];
+    sov[t] = a.ovar[vr][idx];
     svr[t] = vr;
   }
   const int ns = a.nsub;
@@ -109,53 +130,166 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int k = 0; k < DIM; ++k) sg[k][t] = ns > 1 ? a.scoord[k + DIM * s] - x0[k] : 0.0;
     sgw[t] = ns > 1 ? a.sweight[s] : 1.0;
   }
+  if (t < MVR) vv[t] = 0.0;
   __syncthreads();
-  // ---- assembly ---------------------------------------------------------------
-  double bq[NV];
-  if (row) {
-    for (int m = 0; m < t; ++m) {
-      const int vm = svr[m];
-      double l[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) l[k] = (sx[k][t] - sx[k][m]) / rs;
-      Kd[t * ld + m] = mv_cov<DIM>(a.models, vr > vm ? vr : vm, vr > vm ? vm : vr, nvar, l);
-    }
-    Kd[t * ld + t] = a.models[vr + nvar * vr].cov0 + ovar + ln;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
+
+  // ---- right-hand sides: items (row, target, half of the nodes) ---------------
+  {
+    const int h1 = (ns + 1) / 2;
+    for (int it = t; it < 2 * n * nvar; it += MVT) {
+      const int h = it / (n * nvar), rq = it - h * n * nvar;
+      const int q = rq / n, r = rq - q * n;
+      const int k0 = h ? h1 : 0, k1 = h ? ns : h1;
+      const int vr = svr[r];
       double s = 0.0;
-      if (q < nvar) {
-        for (int k = 0; k < ns; ++k) {
-          double l[DIM];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) l[d] = (sx[d][t] - sg[d][k]) / rs;
-          s += mv_cov<DIM>(a.models, vr, q, nvar, l) * sgw[k];
+      for (int k = k0; k < k1; ++k) {
+        double dx = sx[0][r] - sg[0][k];
+        double dy = DIM > 1 ? sx[1][r] - sg[1][k] : 0.0;
+        double dz = DIM > 2 ? sx[2][r] - sg[2][k] : 0.0;
+        if (scaled) {
+          dx = dx / rs;
+          dy = dy / rs;
+          dz = dz / rs;
         }
-        cols[q * ld + t] = s;
+        s += mv_cov<DIM, SIMPLE>(a.models, vr, q, nvar, dx, dy, dz) * sgw[k];
       }
-      bq[q] = s;
+      sb[h][q][r] = s;
     }
-    for (int c = 0; c < p; ++c) cols[(nvar + c) * ld + t] = (a.std_ck ? (vr == c) : true) ? 1.0 : 0.0;
+  }
+  // ---- block base covariances sum_k sum_l w_k w_l C_ij(g_k - g_l) -------------
+  if (ns > 1) {
+    const int npair = ns * (ns + 1) / 2;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        if (j < i) continue;
+        const Model& M = a.models[i + nvar * j];
+        double s = 0.0;
+        for (int e = t; e < npair; e += MVT) {
+          // e -> (k, l), k <= l, row-major upper triangle
+          int k = 0, rem = e;
+          while (rem >= ns - k) {
+            rem -= ns - k;
+            ++k;
+          }
+          const int l = k + rem;
+          if (l == k) {
+            s += M.cov0 * sgw[k] * sgw[k];
+          } else {
+            const double dx = sg[0][k] - sg[0][l];
+            const double dy = DIM > 1 ? sg[1][k] - sg[1][l] : 0.0;
+            const double dz = DIM > 2 ? sg[2][k] - sg[2][l] : 0.0;
+            s += mv_cov<DIM, SIMPLE>(a.models, i, j, nvar, dx, dy, dz) * sgw[k] * sgw[l] * 2.0;
+          }
+        }
+        s = wg_sum(s, part);
+        if (t == 0) sbase[i + NV * j] = s;
+      }
   }
   __syncthreads();
-  // ---- Cholesky K = L L' ------------------------------------------------------
+
+  // ---- assemble the owned elements --------------------------------------------
+  // one row slot at a time: the values go through the thread's own LDS slots
+  // (a loop that is not unrolled evaluates them), then into registers with
+  // static indices
+  double A[RB][RB];
+#pragma unroll
+  for (int ra = 0; ra < RB; ++ra) {
+    const int i = tr + 16 * ra;
+    const bool ri = i < n;
+    const int vi = ri ? svr[i] : 0;
+    double xi[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) xi[k] = ri ? sx[k][i] : 0.0;
+#pragma unroll 1
+    for (int cb = 0; cb < RB; ++cb) {
+      const int j = tc + 16 * cb;
+      double v = 0.0;
+      if (ri && j < n) {
+        if (j == i) {
+          v = a.models[vi + nvar * vi].cov0 + sov[i] + ln;
+        } else if (j < i) {
+          const int vj = svr[j];
+          double dx = xi[0] - sx[0][j];
+          double dy = DIM > 1 ? xi[1] - sx[1][j] : 0.0;
+          double dz = DIM > 2 ? xi[2] - sx[2][j] : 0.0;
+          if (scaled) {
+            dx = dx / rs;
+            dy = dy / rs;
+            dz = dz / rs;
+          }
+          v = mv_cov<DIM, SIMPLE>(a.models, vi > vj ? vi : vj, vi > vj ? vj : vi, nvar, dx, dy, dz);
+        }
+      } else if (ri && j < nx) {
+        const int e = j - n;
+        if (e < ef) v = sb[0][e][i] + sb[1][e][i];
+        else if (e < ez) v = (a.std_ck ? (vi == e - ef) : true) ? 1.0 : 0.0;
+        else if (e == ez) v = sz[i];
+        else if (isa) v = (vi == e - ei) ? 1.0 : 0.0;
+        else v = 1.0;  // simple kriging: the ones column for the mean term
+      }
+      sbuf[cb * MVT + t] = v;
+    }
+#pragma unroll
+    for (int cb = 0; cb < RB; ++cb) A[ra][cb] = sbuf[cb * MVT + t];
+  }
+  if (t == 0) spiv = a.models[svr[0] + nvar * svr[0]].cov0 + sov[0] + ln;
+  __syncthreads();
+
+  // ---- Cholesky of K with the extra columns eliminated alongside ---------------
   bool ok = true;
-  for (int c = 0; c < n; ++c) {
-    const double piv = Kd[c * ld + c];
-    if (!(piv > 0.0)) {
-      ok = false;
-      break;
+#pragma unroll
+  for (int cb = 0; cb < RB; ++cb) {
+    for (int cc = 0; cc < 16; ++cc) {
+      const int c = 16 * cb + cc;
+      if (c >= n || !ok) break;
+      const double piv = spiv;
+      if (!(piv > 0.0)) {
+        ok = false;
+        break;
+      }
+      const double d = sqrt(piv);
+      // column c below the diagonal: L(i, c) = A(i, c) / d (owners of column c)
+      if (tc == cc) {
+#pragma unroll
+        for (int ra = 0; ra < RB; ++ra) {
+          const int i = tr + 16 * ra;
+          if (i < n) vv[i] = i > c ? A[ra][cb] / d : 0.0;
+        }
+      }
+      // row c of the extra columns: Y(c, j) = X(c, j) / d (owners of row c)
+      if (tr == cc) {
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+          const int j = tc + 16 * k;
+          if (j >= n && j < nx) {
+            const double y = A[cb][k] / d;
+            A[cb][k] = y;
+            vv[j] = y;
+          }
+        }
+      }
+      __syncthreads();
+      double li[RB], lj[RB];
+#pragma unroll
+      for (int ra = 0; ra < RB; ++ra) li[ra] = (tr + 16 * ra < n) ? vv[tr + 16 * ra] : 0.0;
+#pragma unroll
+      for (int k = 0; k < RB; ++k) lj[k] = vv[tc + 16 * k];
+      // trailing update; entries with i <= c or j <= c see a zero factor
+#pragma unroll
+      for (int ra = 0; ra < RB; ++ra)
+#pragma unroll
+        for (int k = 0; k < RB; ++k) A[ra][k] = fma(-li[ra], lj[k], A[ra][k]);
+      // the next pivot, from its owner
+      const int c1 = c + 1;
+      if (c1 < n && tr == (c1 & 15) && tc == (c1 & 15)) {
+#pragma unroll
+        for (int ra = 0; ra < RB; ++ra)
+          if (ra == (c1 >> 4)) spiv = A[ra][ra];
+      }
+      __syncthreads();
     }
-    const double d = sqrt(piv);
-    __syncthreads();
-    if (t == c) Kd[c * ld + c] = d;
-    if (t > c && row) Kd[t * ld + c] = Kd[t * ld + c] / d;
-    __syncthreads();
-    if (t > c && row) {
-      const double ltc = Kd[t * ld + c];
-      for (int m = c + 1; m <= t; ++m) Kd[t * ld + m] -= ltc * Kd[m * ld + c];
-    }
-    __syncthreads();
   }
   if (!ok) {
     if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
@@ -163,58 +297,62 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     if (t == 0) a.flag[b] = 3;
     return;
   }
-  // ---- K^-1 [b_0 .. b_{nvar-1}, F_0 .. F_{p-1}] -----------------------------------
-  for (int c = 0; c < n; ++c) {
-    if (t < ncol) cols[t * ld + c] = cols[t * ld + c] / Kd[c * ld + c];
-    __syncthreads();
-    if (t > c && row) {
-      const double l = Kd[t * ld + c];
-      for (int j = 0; j < ncol; ++j) cols[j * ld + t] -= l * cols[j * ld + c];
-    }
-    __syncthreads();
-  }
-  for (int c = n - 1; c >= 0; --c) {
-    if (t < ncol) cols[t * ld + c] = cols[t * ld + c] / Kd[c * ld + c];
-    __syncthreads();
-    if (t < c) {
-      const double l = Kd[c * ld + t];
-      for (int j = 0; j < ncol; ++j) cols[j * ld + t] -= l * cols[j * ld + c];
-    }
-    __syncthreads();
-  }
-  double kb[NV], kf[NV];
+
+  // ---- Gram matrix of the eliminated extra columns -----------------------------
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    kb[q] = (row && q < nvar) ? cols[q * ld + t] : 0.0;
-    kf[q] = (row && q < p) ? cols[(nvar + q) * ld + t] : 0.0;
-  }
-  // ---- Schur complement S = F' K^-1 F, G = F' K^-1 b, multipliers ---------------
-  double S[NV][NV], G[NV][NV];
+  for (int ra = 0; ra < RB; ++ra) {
+    const int i = tr + 16 * ra;
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const double fi = row && i < p ? ((a.std_ck ? (vr == i) : true) ? 1.0 : 0.0) : 0.0;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      S[i][j] = (i < p && j < p) ? block_sum(fi * kf[j], part) : 0.0;
-      G[i][j] = (i < p && j < nvar) ? block_sum(fi * kb[j], part) : 0.0;
+    for (int k = 0; k < RB; ++k) {
+      const int j = tc + 16 * k;
+      if (i < n && j >= n && j < nx) sbuf[(j - n) * MVR + i] = A[ra][k];
     }
   }
-  // mu_q = S^-1 (G_q - d_q): Gaussian elimination with partial pivoting on the
-  // small p x p system, every thread the same
-  double mu[NV][NV];  // mu[q][a]
+  __syncthreads();
+  {
+    const int npair = ne * (ne + 1) / 2;
+    const int g = t >> 4, lid = t & 15;
+    for (int pp = g; pp < npair; pp += MVT / 16) {
+      int e1 = 0, rem = pp;
+      while (rem >= ne - e1) {
+        rem -= ne - e1;
+        ++e1;
+      }
+      const int e2 = e1 + rem;
+      double s = 0.0;
+      for (int i = lid; i < n; i += 16) s = fma(sbuf[e1 * MVR + i], sbuf[e2 * MVR + i], s);
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+      if (lid == 0) gram[e1][e2] = gram[e2][e1] = s;
+    }
+  }
+  // group means of the neighbour values (Isaaks-Srivastava)
+  double gm[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    gm[v] = 0.0;
+    if (isa && nv[v] > 0) gm[v] = wg_sum((t < n && svr[t] == v) ? sz[t] : 0.0, part) / nv[v];
+  }
+  __syncthreads();
+  if (t != 0) return;
+
+  // ---- multipliers, estimates and variances (one thread) -----------------------
+  // mu_q = S^-1 (G_q - d_q), S = F'K^-1 F, G = F'K^-1 b: Gaussian elimination
+  // with partial pivoting on the small p x p system
+  double mu[NV][NV];  // mu[q][c]
 #pragma unroll
   for (int q = 0; q < NV; ++q)
 #pragma unroll
-    for (int i = 0; i < NV; ++i) mu[q][i] = 0.0;
+    for (int c = 0; c < NV; ++c) mu[q][c] = 0.0;
   if (p > 0) {
-    double A[NV][NV], R[NV][NV];
+    double S[NV][NV], R[NV][NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        A[i][j] = S[i][j];
+        S[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
         const double d = a.std_ck ? (i == j ? 1.0 : 0.0) : 1.0;
-        R[i][j] = G[i][j] - d;  // row i (constraint), column j (target)
+        R[i][j] = (i < p) ? gram[ef + i][eb + j] - d : 0.0;  // row i (constraint), column j (target)
       }
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
@@ -222,12 +360,12 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
       int pr = c;
 #pragma unroll
       for (int r = 0; r < NV; ++r)
-        if (r > c && r < p && fabs(A[r][c]) > fabs(A[pr][c])) pr = r;
+        if (r > c && r < p && fabs(S[r][c]) > fabs(S[pr][c])) pr = r;
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        double x = A[c][j];
-        A[c][j] = A[pr][j];
-        A[pr][j] = x;
+        double x = S[c][j];
+        S[c][j] = S[pr][j];
+        S[pr][j] = x;
         x = R[c][j];
         R[c][j] = R[pr][j];
         R[pr][j] = x;
@@ -235,10 +373,10 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
 #pragma unroll
       for (int r = 0; r < NV; ++r) {
         if (r <= c || r >= p) continue;
-        const double f = A[r][c] / A[c][c];
+        const double f = S[r][c] / S[c][c];
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-          A[r][j] -= f * A[c][j];
+          S[r][j] -= f * S[c][j];
           R[r][j] -= f * R[c][j];
         }
       }
@@ -251,96 +389,78 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
         double s = R[c][q];
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-          if (j > c && j < p) s -= A[c][j] * mu[q][j];
-        mu[q][c] = s / A[c][c];
+          if (j > c && j < p) s -= S[c][j] * mu[q][j];
+        mu[q][c] = s / S[c][c];
       }
     }
   }
-  // ---- weights, estimates -------------------------------------------------------
-  double w[NV];
+  // c' w_q = c'K^-1 b_q - sum_f mu_q[f] c'K^-1 F_f for an extra column c
+  auto cw = [&](int e, int q) {
+    double s = gram[e][eb + q];
+#pragma unroll
+    for (int f = 0; f < NV; ++f)
+      if (f < p) s -= mu[q][f] * gram[e][ef + f];
+    return s;
+  };
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = kb[q];
-#pragma unroll
-    for (int c = 0; c < NV; ++c) s -= mu[q][c] * kf[c];
-    w[q] = (row && q < nvar) ? s : 0.0;
-  }
-  const double zr = row ? sz[t] : 0.0;
-  const bool isa = !a.std_ck && a.unbias && nvar > 1;
-  double gm[NV];  // group means of the neighbour values (Isaaks-Srivastava)
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    gm[v] = 0.0;
-    if (isa && v < nvar && nv[v] > 0) gm[v] = block_sum((row && vr == v) ? zr : 0.0, part) / nv[v];
-  }
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    if (q >= nvar) break;
-    double e = block_sum(w[q] * zr, part);
+    double e = cw(ez, q);
     if (isa) {
 #pragma unroll
       for (int v = 0; v < NV; ++v)
-        if (v < nvar && v != q && nv[v] > 0) e += (gm[q] - gm[v]) * block_sum((row && vr == v) ? w[q] : 0.0, part);
+        if (v != q && nv[v] > 0) e += (gm[q] - gm[v]) * cw(ei + v, q);
     }
-    if (!a.unbias && a.sk_mean[q] != 0.0) e += (1.0 - block_sum(w[q], part)) * a.sk_mean[q];
+    if (!a.unbias && a.sk_mean[q] != 0.0) e += (1.0 - cw(ez + 1, q)) * a.sk_mean[q];
     e = e < a.bound_lo ? a.bound_lo : e;
     e = e > a.bound_hi ? a.bound_hi : e;
-    if (t == 0) a.est[b + (size_t)a.nb * q] = e;
+    a.est[b + (size_t)a.nb * q] = e;
   }
-  // ---- variance -------------------------------------------------------------------
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    if (i >= nvar) break;
+  for (int i = 0; i < NV; ++i)
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      if (j < i || j >= nvar) continue;
-      const Model& M = a.models[i + nvar * j];
-      double base;
-      if (ns == 1) {
-        base = M.cov0;
-      } else {
-        double s = 0.0;
-        for (int e = t; e < ns * ns; e += MVT) {
-          const int k = e / ns, l = e % ns;
-          if (l < k) continue;
-          if (l == k) {
-            s += M.cov0 * sgw[k] * sgw[k];
-          } else {
-            double lag[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) lag[d] = sg[d][k] - sg[d][l];
-            s += mv_cov<DIM>(a.models, i, j, nvar, lag) * sgw[k] * sgw[l] * 2.0;
-          }
-        }
-        base = block_sum(s, part);
-      }
+      if (j < i) continue;
+      const double base = ns > 1 ? sbase[i + NV * j] : a.models[i + nvar * j].cov0;
       double mud = 0.0;
 #pragma unroll
       for (int c = 0; c < NV; ++c)
         if (c < p) mud += mu[i][c] * (a.std_ck ? (c == j ? 1.0 : 0.0) : 1.0);
-      double v = base - (block_sum(w[i] * bq[j], part) + mud);
+      double v = base - (cw(eb + j, i) + mud);
       v = v > 0.0 ? v : 0.0;  // diagonal and off-diagonal clamped at 0
-      if (t == 0) {
-        a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
-        a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
-      }
+      a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
+      a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
     }
-  }
-  if (t == 0) a.flag[b] = 0;
+  a.flag[b] = 0;
 }
 
-size_t mv_lds_bytes(int ld, int ncol) { return sizeof(double) * ((size_t)ld * ld + (size_t)MAXCOL * ld); }
+int mv_extra_cols(int nvar, int std_ck, int unbias) {
+  const int p = unbias ? (std_ck ? nvar : 1) : 0;
+  const bool isa = !std_ck && unbias && nvar > 1;
+  return nvar + p + 1 + (isa ? nvar : 0) + (unbias ? 0 : 1);
+}
+
+template <int DIM, int NV>
+static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
+  dim3 g(a.nb), blk(MVT);
+  if (rb == 5) {
+    if (a.simple) hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, false>), g, blk, 0, st, a);
+  } else {
+    if (a.simple) hipLaunchKernelGGL((k_solve_mv<DIM, NV, 8, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<DIM, NV, 8, false>), g, blk, 0, st, a);
+  }
+}
 
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
   if (a.nb <= 0) return hipSuccess;
-  const size_t lds = mv_lds_bytes(a.ld, MAXCOL);
-  if (a.ld > MVT + 1 || a.nsub > MAXSUB || a.nvar < 2 || a.nvar > MAXVAR) return hipErrorInvalidValue;
-  dim3 g(a.nb), blk(MVT);
-#define KK_MV(D)                                                                      \
-  do {                                                                               \
-    if (a.nvar == 2) hipLaunchKernelGGL((k_solve_mv<D, 2>), g, blk, lds, st, a);     \
-    else if (a.nvar == 3) hipLaunchKernelGGL((k_solve_mv<D, 3>), g, blk, lds, st, a); \
-    else hipLaunchKernelGGL((k_solve_mv<D, 4>), g, blk, lds, st, a);                 \
+  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
+  if (a.rows > MVR || cols > MVR || a.nsub > MAXSUB || a.nvar < 2 || a.nvar > MAXVAR) return hipErrorInvalidValue;
+  const int rb = cols <= 80 ? 5 : 8;
+#define KK_MV(D)                                  \
+  do {                                           \
+    if (a.nvar == 2) launch_nv<D, 2>(a, rb, st); \
+    else if (a.nvar == 3) launch_nv<D, 3>(a, rb, st); \
+    else launch_nv<D, 4>(a, rb, st);             \
   } while (0)
   if (a.ndim == 1) KK_MV(1);
   else if (a.ndim == 2) KK_MV(2);

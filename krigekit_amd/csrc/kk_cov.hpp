@@ -137,6 +137,76 @@ __device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy
   return cov<DIM>(*Mp, dx, dy, dz);
 }
 
+// Inline evaluation of any mode for models whose shapes are all "simple"
+// (nugget, spherical, exponential, Gaussian, bounded-square, linear): the
+// same arithmetic as cov<DIM>, without the trigonometric and power shapes
+// whose FP64 library code would inflate every caller's registers.
+__device__ inline bool simple_shape(int t) { return t == NUG || t == SPH || t == EXP || t == GAU || t == BSQ || t == LIN; }
+
+__device__ inline double corefunc_simple(int t, double r) {
+  switch (t) {
+    case SPH: return (r < 1.0) ? (1.0 - 1.5 * r) + 0.5 * (r * r * r) : 0.0;
+    case EXP: return exp(-3.0 * r);
+    case GAU: return exp(-3.0625 * (r * r));
+    case BSQ: { double a = 1.0 - r * r; return (r < 1.0) ? a * a : 0.0; }
+    case LIN: return (r < 1.0) ? 1.0 - r : 0.0;
+    default: return 0.0;
+  }
+}
+
+__device__ inline double comp_cov_h_simple(const Model& M, int c, double h) {
+  const double eps = 2.2250738585072014e-308 * 1.0e3;
+  return (h > eps) ? M.sill[c] * corefunc_simple(M.vtype[c], h) : M.sill[c] + M.nugget[c];
+}
+
+template <int DIM>
+__device__ inline double cov_simple(const Model& M, double dx, double dy, double dz) {
+  if (M.mode == COV_TABLE_B) {
+    const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
+    if (h >= M.zb[3]) {
+      if (!M.b_tail) return 0.0;
+      double res = 0.0;
+      int iv = 0;
+      while (iv < M.nstruct) {
+        double g = comp_cov_h_simple(M, iv, h);
+        while (iv + 1 < M.nstruct && M.is_product[iv + 1]) {
+          ++iv;
+          g = g * comp_cov_h_simple(M, iv, h);
+        }
+        res = res + g;
+        ++iv;
+      }
+      return res;
+    }
+    if (h <= 0.0) return M.tab[0];
+    return table_interp(M, M.tab, h);
+  }
+  double res = 0.0;
+  int iv = 0;
+  while (iv < M.nstruct) {
+    double g = 1.0;
+    bool first = true;
+    do {
+      if (!first) ++iv;
+      double h = aniso_h<DIM>(M.mat[iv], dx, dy, dz);
+      double c;
+      if (M.mode == COV_TABLE_A) {
+        const double* tab = M.tab + (size_t)iv * (M.ntab + 1);
+        if (h >= M.zb[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h_simple(M, iv, h) : 0.0;
+        else if (h <= 0.0) c = tab[0];
+        else c = table_interp(M, tab, h);
+      } else {
+        c = comp_cov_h_simple(M, iv, h);
+      }
+      g = first ? c : g * c;
+      first = false;
+    } while (iv + 1 < M.nstruct && M.is_product[iv + 1]);
+    res = res + g;
+    ++iv;
+  }
+  return res;
+}
+
 // kernels specialise on the model mode: TB = Level-B table inline,
 // otherwise the general path out of line
 template <int DIM, bool TB>

@@ -49,8 +49,8 @@ hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
 hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
-// multi-variable block solve; lds bytes from mv_lds_bytes
-size_t mv_lds_bytes(int ld, int ncol);
+// multi-variable block solve; rows + mv_extra_cols(...) must not exceed 128
+int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,

@@ -108,7 +108,8 @@ struct MVArgs {
   int std_ck;                 // one unbiasedness row per variable (else one in total)
   int unbias;                 // 0: simple kriging, no constraint rows
   int nsub;                   // integration nodes per block (1 = point)
-  int ld;                     // leading dimension of the LDS system (> rows)
+  int rows;                   // largest number of data rows (sum of the variables' mmax)
+  int simple;                 // every structure of every model has a simple shape
   double sk_mean[MAXVAR];
   double bound_lo, bound_hi;
   const double* bcoord;       // (ndim, nb) block centres
