@@ -20,15 +20,15 @@
 // with a <= b -- the pairs the reference builds tables for -- and analytic
 // models for the others, so every evaluation matches the reference's.
 //
-// One 256-thread workgroup per block, a 16 x 16 thread grid.  The bordered
-// matrix [K | b_0.. | F_0.. | z | (I_0.. | 1)] is held in registers, 2-D
-// block-cyclic: thread (tr, tc) owns rows tr + 16a and columns tc + 16b.
-// Right-looking Cholesky over the n data columns eliminates the extra
-// columns alongside (Y = L^-1 X), one LDS broadcast vector per step.  Every
-// output is a dot product of two columns of Y (c'K^-1 d = y_c' y_d), so no
-// back substitution is needed: the Gram matrix of the extra columns gives
-// S, F'K^-1 b, z'K^-1 b, ..., and one thread finishes the small algebra.
-// FP64 throughout.
+// One 256-thread workgroup per block, a 16 x 16 thread grid.  The symmetric
+// bordered matrix [K X'; X 0], X = (b_0.., F_0.., z, (I_0.. | 1)) as extra
+// rows, is held in registers, lower triangle, 2-D block-cyclic: thread
+// (tr, tc) owns rows tr + 16a and columns tc + 16b.  Right-looking Cholesky
+// over the n pivots of K leaves -X K^-1 X' in the trailing block, i.e. minus
+// the Gram matrix of Y = L^-1 X' (c'K^-1 d = y_c' y_d), so no substitution is
+// needed: it gives S, F'K^-1 b, z'K^-1 b, ..., and one thread finishes the
+// small algebra.  One LDS broadcast vector and one barrier per step.  The
+// variable pairs' models are staged in LDS.  FP64 throughout.
 #include <hip/hip_runtime.h>
 
 #include "kk_cov.hpp"
@@ -38,7 +38,7 @@
 namespace kk {
 
 constexpr int MVT = 256;            // threads per workgroup (16 x 16)
-constexpr int MVR = 128;            // max rows + extra columns
+constexpr int MVR = 128;            // max rows + extra rows
 constexpr int MAXE = 3 * MAXVAR + 1;  // extra columns: b, F, z, indicators or ones
 constexpr int MAXSUB = 64;
 
@@ -47,6 +47,13 @@ __device__ inline double mv_cov(const Model* models, int a, int b, int nvar, dou
   const Model* M = models + a + nvar * b;
   if (SIMPLE) return cov_simple<DIM>(*M, dx, dy, dz);
   return cov_general<DIM>(M, dx, dy, dz);
+}
+
+__device__ inline double readlane_d(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // sum over the workgroup; every thread gets the total
@@ -68,18 +75,19 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   __shared__ double sg[DIM][MAXSUB];
   __shared__ double sgw[MAXSUB];
   __shared__ double sb[2][NV][MVR];   // right-hand sides, two partial sums
-  __shared__ double vv[MVR];          // step broadcast: L(:, c) and Y(c, :)
-  // Y of the extra columns after the elimination, [MAXE][MVR]; during the
-  // assembly each thread's RB values of one row slot, [RB][MVT]
-  constexpr int SBUF = MAXE * MVR > RB * MVT ? MAXE * MVR : RB * MVT;
-  __shared__ double sbuf[SBUF];
+  __shared__ double sbuf[RB * MVT];   // assembly: each thread's RB values of one row slot
   __shared__ double gram[MAXE][MAXE];
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
-  __shared__ double spiv;
 
+  __shared__ Model smodels[NV * NV];
+  __shared__ double vv2[2][MVR];      // step broadcast L(:, c), double-buffered
+  __shared__ int sfail;
   const int b = blockIdx.x, t = threadIdx.x;
-  const int tr = t >> 4, tc = t & 15;
+  // column class tc = t / 16: the 16 threads of a column class (all row
+  // classes) sit in one wave, so a column's pivot and scaling never leave it
+  const int tc = t >> 4, tr = t & 15;
+  const int wave = t >> 6;
   constexpr int nvar = NV;
   int nv[NV], st[NV], n = 0;
 #pragma unroll
@@ -130,7 +138,15 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int k = 0; k < DIM; ++k) sg[k][t] = ns > 1 ? a.scoord[k + DIM * s] - x0[k] : 0.0;
     sgw[t] = ns > 1 ? a.sweight[s] : 1.0;
   }
-  if (t < MVR) vv[t] = 0.0;
+  if (t < MVR) vv2[0][t] = vv2[1][t] = 0.0;
+  if (t == 0) sfail = 0;
+  {
+    // the variable pairs' models, read by every covariance evaluation
+    static_assert(sizeof(Model) % 8 == 0, "Model copied as 8-byte words");
+    const long long* src = reinterpret_cast<const long long*>(a.models);
+    long long* dst = reinterpret_cast<long long*>(smodels);
+    for (int w = t; w < (int)(NV * NV * sizeof(Model) / 8); w += MVT) dst[w] = src[w];
+  }
   __syncthreads();
 
   // ---- right-hand sides: items (row, target, half of the nodes) ---------------
@@ -151,7 +167,7 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
           dy = dy / rs;
           dz = dz / rs;
         }
-        s += mv_cov<DIM, SIMPLE>(a.models, vr, q, nvar, dx, dy, dz) * sgw[k];
+        s += mv_cov<DIM, SIMPLE>(smodels, vr, q, nvar, dx, dy, dz) * sgw[k];
       }
       sb[h][q][r] = s;
     }
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         if (j < i) continue;
-        const Model& M = a.models[i + nvar * j];
+        const Model& M = smodels[i + nvar * j];
         double s = 0.0;
         for (int e = t; e < npair; e += MVT) {
           // e -> (k, l), k <= l, row-major upper triangle
@@ -180,7 +196,7 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
             const double dx = sg[0][k] - sg[0][l];
             const double dy = DIM > 1 ? sg[1][k] - sg[1][l] : 0.0;
             const double dz = DIM > 2 ? sg[2][k] - sg[2][l] : 0.0;
-            s += mv_cov<DIM, SIMPLE>(a.models, i, j, nvar, dx, dy, dz) * sgw[k] * sgw[l] * 2.0;
+            s += mv_cov<DIM, SIMPLE>(smodels, i, j, nvar, dx, dy, dz) * sgw[k] * sgw[l] * 2.0;
           }
         }
         s = wg_sum(s, part);
@@ -189,10 +205,12 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   }
   __syncthreads();
 
-  // ---- assemble the owned elements --------------------------------------------
-  // one row slot at a time: the values go through the thread's own LDS slots
-  // (a loop that is not unrolled evaluates them), then into registers with
-  // static indices
+  // ---- assemble the owned elements of the bordered matrix ------------------------
+  // M = [K X'; X 0] (lower triangle), X = the extra columns as rows n..nx-1:
+  // eliminating the n pivots of K leaves -X K^-1 X' (minus the Gram matrix of
+  // the eliminated extra columns) in the trailing block.  One row slot at a
+  // time, the values go through the thread's own LDS slots (a loop that is
+  // not unrolled evaluates them), then into registers with static indices.
   double A[RB][RB];
 #pragma unroll
   for (int ra = 0; ra < RB; ++ra) {
@@ -206,10 +224,10 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int cb = 0; cb < RB; ++cb) {
       const int j = tc + 16 * cb;
       double v = 0.0;
-      if (ri && j < n) {
+      if (ri && j <= i) {
         if (j == i) {
-          v = a.models[vi + nvar * vi].cov0 + sov[i] + ln;
-        } else if (j < i) {
+          v = smodels[vi + nvar * vi].cov0 + sov[i] + ln;
+        } else {
           const int vj = svr[j];
           double dx = xi[0] - sx[0][j];
           double dy = DIM > 1 ? xi[1] - sx[1][j] : 0.0;
@@ -219,14 +237,15 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
             dy = dy / rs;
             dz = dz / rs;
           }
-          v = mv_cov<DIM, SIMPLE>(a.models, vi > vj ? vi : vj, vi > vj ? vj : vi, nvar, dx, dy, dz);
+          v = mv_cov<DIM, SIMPLE>(smodels, vi > vj ? vi : vj, vi > vj ? vj : vi, nvar, dx, dy, dz);
         }
-      } else if (ri && j < nx) {
-        const int e = j - n;
-        if (e < ef) v = sb[0][e][i] + sb[1][e][i];
-        else if (e < ez) v = (a.std_ck ? (vi == e - ef) : true) ? 1.0 : 0.0;
-        else if (e == ez) v = sz[i];
-        else if (isa) v = (vi == e - ei) ? 1.0 : 0.0;
+      } else if (i >= n && i < nx && j < n) {
+        // extra row e = i - n, data column j
+        const int e = i - n, vj = svr[j];
+        if (e < ef) v = sb[0][e][j] + sb[1][e][j];
+        else if (e < ez) v = (a.std_ck ? (vj == e - ef) : true) ? 1.0 : 0.0;
+        else if (e == ez) v = sz[j];
+        else if (isa) v = (vj == e - ei) ? 1.0 : 0.0;
         else v = 1.0;  // simple kriging: the ones column for the mean term
       }
       sbuf[cb * MVT + t] = v;
@@ -234,61 +253,53 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
 #pragma unroll
     for (int cb = 0; cb < RB; ++cb) A[ra][cb] = sbuf[cb * MVT + t];
   }
-  if (t == 0) spiv = a.models[svr[0] + nvar * svr[0]].cov0 + sov[0] + ln;
-  __syncthreads();
 
-  // ---- Cholesky of K with the extra columns eliminated alongside ---------------
+  // ---- Cholesky of the n pivots ------------------------------------------------
+  // step c: the wave holding column class c % 16 reads the pivot from its
+  // owner lane, scales column c by 1 / sqrt(pivot) (as LAPACK's unblocked
+  // Cholesky does, DSCAL by ONE / AJJ) into the broadcast buffer; after one
+  // barrier every thread applies A(i, j) -= L(i, c) L(j, c).  The buffer is
+  // double-buffered, so the next step's writer cannot overtake a reader.
   bool ok = true;
 #pragma unroll
   for (int cb = 0; cb < RB; ++cb) {
     for (int cc = 0; cc < 16; ++cc) {
       const int c = 16 * cb + cc;
       if (c >= n || !ok) break;
-      const double piv = spiv;
-      if (!(piv > 0.0)) {
-        ok = false;
-        break;
-      }
-      const double d = sqrt(piv);
-      // column c below the diagonal: L(i, c) = A(i, c) / d (owners of column c)
-      if (tc == cc) {
+      double* vb = vv2[c & 1];
+      if (wave == (cc >> 2)) {
+        const int owner = (cc & 3) * 16 + cc;  // lane of (tr = cc, tc = cc)
+        const double piv = readlane_d(A[cb][cb], owner);
+        if (!(piv > 0.0)) {
+          if (t == owner + 64 * wave) sfail = 1;
+        } else if (tc == cc) {
+          const double rd = 1.0 / sqrt(piv);
 #pragma unroll
-        for (int ra = 0; ra < RB; ++ra) {
-          const int i = tr + 16 * ra;
-          if (i < n) vv[i] = i > c ? A[ra][cb] / d : 0.0;
-        }
-      }
-      // row c of the extra columns: Y(c, j) = X(c, j) / d (owners of row c)
-      if (tr == cc) {
-#pragma unroll
-        for (int k = 0; k < RB; ++k) {
-          const int j = tc + 16 * k;
-          if (j >= n && j < nx) {
-            const double y = A[cb][k] / d;
-            A[cb][k] = y;
-            vv[j] = y;
+          for (int ra = 0; ra < RB; ++ra) {
+            const int i = tr + 16 * ra;
+            if (16 * ra < nx && i < nx) vb[i] = i > c ? A[ra][cb] * rd : 0.0;
           }
         }
       }
       __syncthreads();
-      double li[RB], lj[RB];
-#pragma unroll
-      for (int ra = 0; ra < RB; ++ra) li[ra] = (tr + 16 * ra < n) ? vv[tr + 16 * ra] : 0.0;
-#pragma unroll
-      for (int k = 0; k < RB; ++k) lj[k] = vv[tc + 16 * k];
-      // trailing update; entries with i <= c or j <= c see a zero factor
-#pragma unroll
-      for (int ra = 0; ra < RB; ++ra)
-#pragma unroll
-        for (int k = 0; k < RB; ++k) A[ra][k] = fma(-li[ra], lj[k], A[ra][k]);
-      // the next pivot, from its owner
-      const int c1 = c + 1;
-      if (c1 < n && tr == (c1 & 15) && tc == (c1 & 15)) {
-#pragma unroll
-        for (int ra = 0; ra < RB; ++ra)
-          if (ra == (c1 >> 4)) spiv = A[ra][ra];
+      if (sfail) {
+        ok = false;
+        break;
       }
-      __syncthreads();
+      // trailing update; entries with i <= c or j <= c see a zero factor.
+      // Slots whose rows or columns are all <= c, slots past nx and slots
+      // above the diagonal are skipped (scalar conditions).
+#pragma unroll
+      for (int ra = 0; ra < RB; ++ra) {
+        if (16 * ra + 15 <= c || 16 * ra >= nx) continue;
+        const int i = tr + 16 * ra;
+        const double li = i < nx ? vb[i] : 0.0;
+#pragma unroll
+        for (int k = 0; k <= ra; ++k) {
+          if (16 * k + 15 <= c) continue;
+          A[ra][k] = fma(-li, vb[tc + 16 * k], A[ra][k]);
+        }
+      }
     }
   }
   if (!ok) {
@@ -298,32 +309,14 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     return;
   }
 
-  // ---- Gram matrix of the eliminated extra columns -----------------------------
+  // ---- Gram matrix of the eliminated extra columns: minus the trailing block ----
 #pragma unroll
   for (int ra = 0; ra < RB; ++ra) {
     const int i = tr + 16 * ra;
 #pragma unroll
-    for (int k = 0; k < RB; ++k) {
+    for (int k = 0; k <= ra; ++k) {
       const int j = tc + 16 * k;
-      if (i < n && j >= n && j < nx) sbuf[(j - n) * MVR + i] = A[ra][k];
-    }
-  }
-  __syncthreads();
-  {
-    const int npair = ne * (ne + 1) / 2;
-    const int g = t >> 4, lid = t & 15;
-    for (int pp = g; pp < npair; pp += MVT / 16) {
-      int e1 = 0, rem = pp;
-      while (rem >= ne - e1) {
-        rem -= ne - e1;
-        ++e1;
-      }
-      const int e2 = e1 + rem;
-      double s = 0.0;
-      for (int i = lid; i < n; i += 16) s = fma(sbuf[e1 * MVR + i], sbuf[e2 * MVR + i], s);
-#pragma unroll
-      for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-      if (lid == 0) gram[e1][e2] = gram[e2][e1] = s;
+      if (i >= n && i < nx && j >= n && j <= i) gram[i - n][j - n] = gram[j - n][i - n] = -A[ra][k];
     }
   }
   // group means of the neighbour values (Isaaks-Srivastava)
