@@ -76,7 +76,10 @@ template <int G, int DIM, int PREC, bool TB>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
   __shared__ double sx[S][DIM][G];
-  __shared__ double sm[S][G][G + 1];
+  // fold exchange: sp[d][r] = C(row r, row (r + d) mod G) for d = 1..G/2 as
+  // lane r computed it, sp[0][r] = diagonal; later reused for the pivot row's
+  // extra columns and (as floats) the transposed FP32 factor
+  __shared__ double sp[S][G / 2 + 1][G];
   __shared__ double col[S][G];
 
   const int lane = threadIdx.x, s = lane / G, r = lane % G;
@@ -126,12 +129,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       }
       v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
     }
-    if (mine) {
-      sm[s][r][c] = v;
-      sm[s][c][r] = v;
-    }
+    if (mine) sp[s][d][r] = v;
   }
-  sm[s][r][r] = live ? (cov0 + ov) + ln : 1.0;
+  sp[s][0][r] = live ? (cov0 + ov) + ln : 1.0;
   // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
   double rhs = 0.0;
   if (live) {
@@ -155,10 +155,16 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   // weight sum 1^T K^-1 b that simple kriging needs for its mean term
   const double f = live ? 1.0 : 0.0;
   __syncthreads();
+  // row r: entry j is the pair at offset d = (j - r) mod G, computed by lane
+  // r when d <= G/2 (d == G/2 only by the lower lane), else by lane j
   double A[G];
 #pragma unroll
-  for (int j = 0; j < G; ++j) A[j] = sm[s][r][j];
-  const double kdiag = sm[s][r][r];
+  for (int j = 0; j < G; ++j) {
+    const int d = (j - r) & (G - 1);
+    const bool own = d < G / 2 || (d == G / 2 && r < G / 2);
+    A[j] = own ? sp[s][d][r] : sp[s][G - d][j];
+  }
+  const double kdiag = sp[s][0][r];
 
   int fail = 0;
   double uu, uy, yy, vy, vu;
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     // y = L^-1 b, u = L^-1 F, v = L^-1 z ----------------------------------
     double D = 1.0;
     double y = rhs, u = f, v = z;
-    double* ext = &sm[s][0][0];  // pivot row's (y, u, v); sm is free after the row loads
+    double* ext = &sp[s][0][0];  // pivot row's (y, u, v); sp is free after the row loads
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       if (k < nw) {
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     // e = solve errors, so the dropped term is second order in the FP32
     // error (one refinement step's accuracy without a second solve) --------
     float* col32 = reinterpret_cast<float*>(&col[s][0]);
-    float* ext32 = reinterpret_cast<float*>(&sm[s][0][0]);  // free after the row loads
+    float* ext32 = reinterpret_cast<float*>(&sp[s][0][0]);  // free after the row loads
     float L[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) L[j] = (float)A[j];
@@ -241,7 +247,8 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     }
     if (live) pmin = (float)((double)D / kdiag);
     // transpose L through LDS for the back substitution
-    float* sm32 = reinterpret_cast<float*>(&sm[s][0][0]);
+    static_assert(sizeof(float) * G * (G + 1) <= sizeof(double) * (G / 2 + 1) * G, "transpose fits");
+    float* sm32 = reinterpret_cast<float*>(&sp[s][0][0]);
 #pragma unroll
     for (int j = 0; j < G; ++j) sm32[r * (G + 1) + j] = L[j];
     __syncthreads();
