@@ -839,7 +839,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   unsigned base = 0;
   bool have_base = false;
   int jK = NBIN - 1, tot = 0, nleaf = 0, nvisit = 0;
-  double ball = BIG;
+  // before any point is counted: the deepest node on the first query's path
+  // that still holds >= k points has them all inside its box, so every lane's
+  // k-th squared distance is at most the squared distance to that box's
+  // farthest corner (inflated like the box test).  Points beyond it rank
+  // below k for this lane, so it caps the bound from the start.
+  double ubound = BIG;
+  {
+    int node = T.root;
+    for (;;) {
+      const int* ni = T.ni + (size_t)node * NODE_I;
+      const int left = ni[3], right = ni[4];
+      if (left < 0 || right < 0) break;
+      const int child = (q0[ni[0]] < T.nd[(size_t)node * NODE_D]) ? left : right;
+      const int* nc = T.ni + (size_t)child * NODE_I;
+      if (nc[2] - nc[1] + 1 < K) break;
+      node = __builtin_amdgcn_readfirstlane(child);
+    }
+    const int* ni = T.ni + (size_t)node * NODE_I;
+    if (ni[2] - ni[1] + 1 >= K) {
+      const double* nd = T.nd + (size_t)node * NODE_D;
+      double u = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        const double dl = fabs(q[k] - nd[NODE_LO + k]), dh = fabs(nd[NODE_HI + k] - q[k]);
+        const double m = dl > dh ? dl : dh;
+        u = u + m * m;
+      }
+      ubound = u * (1.0 + 1e-12);
+    }
+  }
+  double ball = ubound;
 
   auto point = [&](const double* x, int idx, int pos) {
     double sd = 0.0;
@@ -893,11 +923,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         base = mb > (32u << BSH) ? ((mb - (32u << BSH)) >> BSH) << BSH : 0u;
         have_base = true;
       }
+      // every lane takes the same point: uniform (scalar) loads
       for (int p = 0; p < m; ++p) {
+        const int pp = __builtin_amdgcn_readfirstlane(l + p);
         double x[DIM];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) x[k] = readlane_d(v, DIM * p + k);
-        point(x, __builtin_amdgcn_readlane(iv, p), l + p);
+        for (int k = 0; k < DIM; ++k) x[k] = T.rdata[DIM * (size_t)pp + k];
+        point(x, T.ind[pp], pp);
       }
     }
     // lowest bin reaching k, and the ball: min(upper edge of that bin,
@@ -918,6 +950,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int s = 0; s < NOCT; ++s) mo = fmaxf(mo, ok[s][C - 1]);
     const double bO = mo < FINF ? (double)f_next(mo) : BIG;
     ball = bE < bO ? bE : bO;
+    ball = ball < ubound ? ball : ubound;
   });
 
   // ---- k-nearest membership of every kept entry ------------------------------
