@@ -350,20 +350,37 @@ __global__ void k_empty(SolveArgs a) {
 // A negative count marks a block whose neighbours are still being searched.
 __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* sel, int nblock, int* counts,
                                                  int* lists, int cap) {
-  __shared__ int lcount[NBUCKET + 1];
+  // order-preserving within the workgroup: ballot ranks within a wave, an
+  // exclusive scan over the 16 waves, one global atomic per bucket, so a
+  // workgroup's blocks keep their order (and their run of the list is the
+  // same on every pass)
+  __shared__ int wcount[16][NBUCKET + 1];
   __shared__ int lbase[NBUCKET + 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (threadIdx.x <= NBUCKET) lcount[threadIdx.x] = 0;
-  __syncthreads();
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int b = t < nblock ? (sel ? sel[t] : t) : -1;
   const int n = b >= 0 ? nnear[b] : -1;
   const int bk = n < 0 ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
-  const int local = bk >= 0 ? atomicAdd(&lcount[bk], 1) : 0;
+  int local = 0;
+#pragma unroll
+  for (int k = 0; k <= NBUCKET; ++k) {
+    const unsigned long long m = __ballot(bk == k);
+    if (bk == k) local = __popcll(m & ((1ull << ln) - 1ull));
+    if (ln == 0) wcount[wv][k] = __popcll(m);
+  }
   __syncthreads();
-  if (threadIdx.x <= NBUCKET)
-    lbase[threadIdx.x] = lcount[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], lcount[threadIdx.x]) : 0;
+  if (threadIdx.x <= NBUCKET) {
+    const int k = threadIdx.x;
+    int acc = 0;
+    for (int w = 0; w < 16; ++w) {
+      const int c = wcount[w][k];
+      wcount[w][k] = acc;
+      acc += c;
+    }
+    lbase[k] = acc ? atomicAdd(&counts[k], acc) : 0;
+  }
   __syncthreads();
-  if (bk >= 0) lists[(size_t)bk * cap + lbase[bk] + local] = b;
+  if (bk >= 0) lists[(size_t)bk * cap + lbase[bk] + wcount[wv][bk] + local] = b;
 }
 
 __global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
