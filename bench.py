@@ -213,6 +213,8 @@ def main():
                     help="skip the space-time and cokriging workload lines")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
+    ap.add_argument("--only", choices=["space-time", "cokriging"], default=None,
+                    help="only time this secondary workload and print its line")
     args = ap.parse_args()
     if args.copy_probe > 0:
         from krigekit_amd.kriging import copy_probe
@@ -223,6 +225,10 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
                  "(nodes shard independently; see DESIGN.md).")
+    if args.only:
+        out = space_time() if args.only == "space-time" else cokriging()
+        print(json.dumps({"workload": args.only, **out}), flush=True)
+        return
     from krigekit_amd import synth
     spec = synth.ok3d()
     nb = len(spec["grid"])

@@ -228,19 +228,14 @@ bool build_model(const std::vector<VStruct>& vs, int ndim, bool table_mode, Mode
   M.mode = kk::COV_ANALYTIC;
   tab.clear();
   if (!table_mode) return true;
-  const double hb[4] = {0.0, 0.1, 0.5, 3.5};
-  const double dh[3] = {1e-5, 1e-4, 1e-3};
-  int off = 0;
+  const double* hb = kk::TAB_HB;
   for (int k = 0; k < 3; ++k) {
-    int n = (int)std::lround((hb[k + 1] - hb[k]) / dh[k]);
-    n = std::max(1, n);
-    M.zn[k] = n;
-    M.zoff[k] = off;
-    M.zinv[k] = (double)n / (hb[k + 1] - hb[k]);
-    off += n;
+    M.zn[k] = kk::tab_zn(k);
+    M.zoff[k] = kk::tab_zoff(k);
+    M.zinv[k] = kk::tab_zinv(k);
   }
   for (int k = 0; k < 4; ++k) M.zb[k] = hb[k];
-  M.ntab = off;
+  M.ntab = kk::TAB_N;
   // Level B if all non-nugget structures share one matrix (MAT_TOL 1e-6;
   // product members only need equal angles, ANG_TOL 1e-4)
   int ref = -1;

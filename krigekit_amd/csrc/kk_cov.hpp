@@ -99,6 +99,58 @@ __device__ inline double cov_tableB(const Model& M, double dx, double dy, double
   return table_interp(M, M.tab, h);
 }
 
+// Level-B lookup split into index, load and pick, for loops that evaluate
+// many lags: the zone parameters are compile-time constants chosen by selects
+// (no per-lane gathers), the index is kept inside the table for every h, so
+// the two entry loads are unconditional and several lookups can be in flight.
+// Lags beyond the table are reported, not evaluated: the caller re-evaluates
+// them with cov_tableB.  In-table values are those of cov_tableB.
+struct TabB {
+  double m[9];
+  double c0;
+  int tail;
+  const __attribute__((address_space(1))) double* tab;
+};
+
+__device__ inline TabB tabB_view(const Model& M) {
+  TabB T;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) T.m[k] = M.bmat[k];
+  T.tail = M.b_tail;
+  T.tab = (const __attribute__((address_space(1))) double*)M.tab;
+  T.c0 = M.tab[0];
+  return T;
+}
+
+// entry index and fraction of h (table_interp's arithmetic on the zones of
+// TAB_HB); the index stays in [0, TAB_N - 1] for every h, NaN and h beyond
+// the table included
+__device__ inline int tabB_index(double h, double& fr) {
+  constexpr double zb1 = TAB_HB[1], zb2 = TAB_HB[2];
+  constexpr double zi0 = tab_zinv(0), zi1 = tab_zinv(1), zi2 = tab_zinv(2);
+  constexpr int zo0 = tab_zoff(0), zo1 = tab_zoff(1), zo2 = tab_zoff(2);
+  constexpr int zn0 = tab_zn(0), zn1 = tab_zn(1), zn2 = tab_zn(2);
+  const bool k1 = h >= zb1, k2 = h >= zb2;
+  const double zb = k2 ? zb2 : k1 ? zb1 : TAB_HB[0];
+  const double zi = k2 ? zi2 : k1 ? zi1 : zi0;
+  const int zo = k2 ? zo2 : k1 ? zo1 : zo0;
+  const int zn = k2 ? zn2 : k1 ? zn1 : zn0;
+  double fi = (h - zb) * zi;
+  const int ifi = (int)fmin(fi, 1073741824.0);
+  fr = fi - (double)ifi;
+  return max(zo + min(ifi, zn - 1), 0);
+}
+
+// C(h) from the two loaded entries.  Beyond the table it returns 0, which is
+// final when the model has no analytic tail; otherwise `out` is set and the
+// caller evaluates the tail.
+__device__ inline double tabB_pick(const TabB& T, double h, double fr, double e0, double e1, bool& out) {
+  const bool beyond = h >= TAB_HB[3];
+  out = beyond && T.tail;
+  const double v = e0 + fr * (e1 - e0);
+  return beyond ? 0.0 : (h <= 0.0 ? T.c0 : v);
+}
+
 template <int DIM>
 __device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy, double dz);
 

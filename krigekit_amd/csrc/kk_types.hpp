@@ -22,6 +22,20 @@ enum VType : int { NUG = 0, SPH = 1, EXP = 2, GAU = 3, HOL = 4, POW = 5, BSQ = 6
 // covariance evaluation mode
 enum CovMode : int { COV_ANALYTIC = 0, COV_TABLE_B = 1, COV_TABLE_A = 2 };
 
+// Covariance table zones: bounds h = 0, .1, .5, 3.5 with steps 1e-5, 1e-4,
+// 1e-3; entries per zone max(1, nint(width/step)), 1/step = entries/width
+// (src/libkriging/variogram.f90:223-255).  The host table builder and the
+// device lookups both take the zone parameters from here.
+constexpr double TAB_HB[4] = {0.0, 0.1, 0.5, 3.5};
+constexpr double TAB_DH[3] = {1e-5, 1e-4, 1e-3};
+constexpr int tab_zn(int k) {
+  const int n = (int)((TAB_HB[k + 1] - TAB_HB[k]) / TAB_DH[k] + 0.5);
+  return n < 1 ? 1 : n;
+}
+constexpr int tab_zoff(int k) { return k == 0 ? 0 : tab_zoff(k - 1) + tab_zn(k - 1); }
+constexpr double tab_zinv(int k) { return (double)tab_zn(k) / (TAB_HB[k + 1] - TAB_HB[k]); }
+constexpr int TAB_N = tab_zoff(3);  // entries 1..TAB_N follow C(0) at entry 0
+
 // Covariance model of one variable pair, passed by value to kernels.
 struct Model {
   int ndim;

@@ -72,6 +72,30 @@ __device__ inline double pair_cov(const SolveArgs& a, const Model& M, double dx,
   return a.k_ps * cs * ct + cs + ct;
 }
 
+// pair_cov of the TB kernels through the split Level-B lookup: the spatial
+// and temporal entry loads are issued together.  `out` reports a lag beyond a
+// table with an analytic tail; the caller then evaluates the pair with
+// pair_cov, which includes the tail.
+template <int DIM>
+__device__ inline double pair_cov_tb(const SolveArgs& a, const TabB& Ts, const TabB& Tt, double dx, double dy,
+                                     double dz, double dt, bool& out) {
+  constexpr int SD = DIM == 4 ? 3 : DIM;
+  const double hs = aniso_h<SD>(Ts.m, dx, dy, dz);
+  double frs;
+  const int is = tabB_index(hs, frs);
+  const double s0 = Ts.tab[is], s1 = Ts.tab[is + 1];
+  if (DIM <= 3) return tabB_pick(Ts, hs, frs, s0, s1, out);
+  const double ht = aniso_h<1>(Tt.m, fabs(dt), 0.0, 0.0);
+  double frt;
+  const int it = tabB_index(ht, frt);
+  const double t0 = Tt.tab[it], t1 = Tt.tab[it + 1];
+  bool os, ot;
+  const double cs = tabB_pick(Ts, hs, frs, s0, s1, os);
+  const double ct = tabB_pick(Tt, ht, frt, t0, t1, ot);
+  out = os || ot;
+  return a.k_ps * cs * ct + cs + ct;
+}
+
 template <int G, int DIM, int PREC, bool TB>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
@@ -109,27 +133,58 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const double ln = (act && a.localnugget) ? a.localnugget[b] : 0.0;
   const Model& M = *a.model;
   const double cov0 = DIM == 4 ? a.st_cov0 : M.cov0;
+  // TB: the tables' parameters, read once (kernel-uniform)
+  TabB Ts{}, Tt{};
+  if constexpr (TB) {
+    Ts = tabB_view(M);
+    if constexpr (DIM == 4) Tt = tabB_view(*a.tmodel);
+  }
   __syncthreads();
 
+  // lag from this lane's point to (u[0], u[DIM-1]), range-scaled when set
+  auto lag = [&](const double* u, int stride, double& dx, double& dy, double& dz, double& dt) {
+    dx = x[0] - u[0];
+    dy = DIM >= 2 ? x[1] - u[stride] : 0.0;
+    dz = DIM >= 3 ? x[2] - u[2 * stride] : 0.0;
+    dt = DIM == 4 ? x[DIM - 1] - u[(DIM - 1) * stride] : 0.0;
+    if (scaled) {
+      dx = dx / rs;
+      dy = dy / rs;
+      dz = dz / rs;
+    }
+  };
+
   // ---- assemble K by the fold (each pair once) --------------------------
+  // TB: fold offsets whose lag lies beyond a table with an analytic tail are
+  // recorded and evaluated after the loop, keeping the loop free of calls
+  unsigned long long redo = 0;
 #pragma unroll 1
   for (int d = 1; d <= G / 2; ++d) {
     const int c = (r + d) & (G - 1);
     double v = 0.0;
     const bool mine = (d < G / 2) || (r < G / 2);
     if (d < nw && live && c < n && mine) {
-      double dx = x[0] - sx[s][0][c];
-      double dy = DIM >= 2 ? x[1] - sx[s][1][c] : 0.0;
-      double dz = DIM >= 3 ? x[2] - sx[s][2][c] : 0.0;
-      const double dt = DIM == 4 ? x[DIM - 1] - sx[s][DIM - 1][c] : 0.0;
-      if (scaled) {
-        dx = dx / rs;
-        dy = dy / rs;
-        dz = dz / rs;
+      double dx, dy, dz, dt;
+      lag(&sx[s][0][c], G, dx, dy, dz, dt);
+      if (TB) {
+        bool out;
+        v = pair_cov_tb<DIM>(a, Ts, Tt, dx, dy, dz, dt, out);
+        if (out) redo |= 1ull << d;
+      } else {
+        v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
       }
-      v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
     }
     if (mine) sp[s][d][r] = v;
+  }
+  if (TB) {
+    while (redo) {
+      const int d = __ffsll((long long)redo) - 1;
+      redo &= redo - 1;
+      const int c = (r + d) & (G - 1);
+      double dx, dy, dz, dt;
+      lag(&sx[s][0][c], G, dx, dy, dz, dt);
+      sp[s][d][r] = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
+    }
   }
   sp[s][0][r] = live ? (cov0 + ov) + ln : 1.0;
   // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
@@ -139,16 +194,17 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
                                                  : a.scoord + DIM * ((size_t)b * a.nsub + q);
       const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
-      double dx = x[0] - g[0];
-      double dy = DIM >= 2 ? x[1] - g[1] : 0.0;
-      double dz = DIM >= 3 ? x[2] - g[2] : 0.0;
-      const double dt = DIM == 4 ? x[DIM - 1] - g[DIM - 1] : 0.0;
-      if (scaled) {
-        dx = dx / rs;
-        dy = dy / rs;
-        dz = dz / rs;
+      double dx, dy, dz, dt;
+      lag(g, 1, dx, dy, dz, dt);
+      double v;
+      if (TB) {
+        bool out;
+        v = pair_cov_tb<DIM>(a, Ts, Tt, dx, dy, dz, dt, out);
+        if (out) v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
+      } else {
+        v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
       }
-      rhs = rhs + pair_cov<DIM, TB>(a, M, dx, dy, dz, dt) * w;
+      rhs = rhs + v * w;
     }
   }
   // ones on the live rows: the unbias row F for ordinary kriging, and the
