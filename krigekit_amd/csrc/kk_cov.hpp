@@ -107,7 +107,6 @@ __device__ inline double cov_tableB(const Model& M, double dx, double dy, double
 // them with cov_tableB.  In-table values are those of cov_tableB.
 struct TabB {
   double m[9];
-  double c0;
   int tail;
   const __attribute__((address_space(1))) double* tab;
 };
@@ -118,7 +117,6 @@ __device__ inline TabB tabB_view(const Model& M) {
   for (int k = 0; k < 9; ++k) T.m[k] = M.bmat[k];
   T.tail = M.b_tail;
   T.tab = (const __attribute__((address_space(1))) double*)M.tab;
-  T.c0 = M.tab[0];
   return T;
 }
 
@@ -141,14 +139,17 @@ __device__ inline int tabB_index(double h, double& fr) {
   return max(zo + min(ifi, zn - 1), 0);
 }
 
-// C(h) from the two loaded entries.  Beyond the table it returns 0, which is
-// final when the model has no analytic tail; otherwise `out` is set and the
-// caller evaluates the tail.
+// C(h) from the two loaded entries.  At h = 0 the index is entry 0 with
+// fraction 0, so the interpolation returns C(0) itself.  Beyond the table it
+// returns 0, which is final when the model has no analytic tail; otherwise
+// `out` is set and the caller evaluates the tail.  The select is marked
+// unpredictable so that it stays a select: as a branch it would sink the
+// entry loads into it and serialise consecutive lookups.
 __device__ inline double tabB_pick(const TabB& T, double h, double fr, double e0, double e1, bool& out) {
   const bool beyond = h >= TAB_HB[3];
   out = beyond && T.tail;
   const double v = e0 + fr * (e1 - e0);
-  return beyond ? 0.0 : (h <= 0.0 ? T.c0 : v);
+  return __builtin_unpredictable(beyond) ? 0.0 : v;
 }
 
 template <int DIM>

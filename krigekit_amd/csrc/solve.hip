@@ -21,6 +21,8 @@
 // kriging_base.F90:3299-3366).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kk_cov.hpp"
 #include "kk_device.hpp"
 #include "kk_types.hpp"
@@ -142,40 +144,57 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   __syncthreads();
 
   // lag from this lane's point to (u[0], u[DIM-1]), range-scaled when set
-  auto lag = [&](const double* u, int stride, double& dx, double& dy, double& dz, double& dt) {
+  auto lag_sc = [&](bool sc, const double* u, int stride, double& dx, double& dy, double& dz, double& dt) {
     dx = x[0] - u[0];
     dy = DIM >= 2 ? x[1] - u[stride] : 0.0;
     dz = DIM >= 3 ? x[2] - u[2 * stride] : 0.0;
     dt = DIM == 4 ? x[DIM - 1] - u[(DIM - 1) * stride] : 0.0;
-    if (scaled) {
+    if (sc) {
       dx = dx / rs;
       dy = dy / rs;
       dz = dz / rs;
     }
   };
+  auto lag = [&](const double* u, int stride, double& dx, double& dy, double& dz, double& dt) {
+    lag_sc(scaled, u, stride, dx, dy, dz, dt);
+  };
 
   // ---- assemble K by the fold (each pair once) --------------------------
   // TB: fold offsets whose lag lies beyond a table with an analytic tail are
   // recorded and evaluated after the loop, keeping the loop free of calls
+  // offsets d >= nw pair no live rows: their slots are zero.  A slot
+  // (G/2, r >= G/2) is never read, so every lane stores unconditionally; with
+  // TB the loop body is branch-free and consecutive offsets' loads overlap
+  // (the loop is instantiated with and without range scaling, so that no
+  // branch splits its unrolled body)
   unsigned long long redo = 0;
-#pragma unroll 1
-  for (int d = 1; d <= G / 2; ++d) {
-    const int c = (r + d) & (G - 1);
-    double v = 0.0;
-    const bool mine = (d < G / 2) || (r < G / 2);
-    if (d < nw && live && c < n && mine) {
-      double dx, dy, dz, dt;
-      lag(&sx[s][0][c], G, dx, dy, dz, dt);
+  const int dmax = min(G / 2, nw - 1);
+  auto fold = [&](auto sc) {
+#pragma unroll 2
+    for (int d = 1; d <= dmax; ++d) {
+      const int c = (r + d) & (G - 1);
+      const bool use = live && c < n && ((d < G / 2) || (r < G / 2));
+      double v = 0.0;
       if (TB) {
+        double dx, dy, dz, dt;
+        lag_sc(decltype(sc)::value, &sx[s][0][c], G, dx, dy, dz, dt);
         bool out;
         v = pair_cov_tb<DIM>(a, Ts, Tt, dx, dy, dz, dt, out);
-        if (out) redo |= 1ull << d;
-      } else {
+        if (use && out) redo |= 1ull << d;
+        v = use ? v : 0.0;
+      } else if (use) {
+        double dx, dy, dz, dt;
+        lag_sc(decltype(sc)::value, &sx[s][0][c], G, dx, dy, dz, dt);
         v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
       }
+      sp[s][d][r] = v;
     }
-    if (mine) sp[s][d][r] = v;
-  }
+  };
+  if (scaled)
+    fold(std::true_type{});
+  else
+    fold(std::false_type{});
+  for (int d = max(dmax, 0) + 1; d <= G / 2; ++d) sp[s][d][r] = 0.0;
   if (TB) {
     while (redo) {
       const int d = __ffsll((long long)redo) - 1;
