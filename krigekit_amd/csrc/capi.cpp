@@ -925,7 +925,7 @@ bool run_pipeline(Krige& K) {
     if (nref > 0) {
       va.list = K.d_lists.as<int>();
       va.nlist = nref;
-      HIPCHK(kk::launch_solve(va, mmax <= 8 ? 8 : mmax <= 16 ? 16 : mmax <= 32 ? 32 : 64, kk::PREC_FP64, tb, st), W);
+      HIPCHK(kk::launch_solve(va, kk::solve_width(mmax), kk::PREC_FP64, tb, st), W);
     }
   }
   HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);

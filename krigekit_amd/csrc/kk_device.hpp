@@ -37,12 +37,13 @@ hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord
 enum Precision : int { PREC_FP64 = 0, PREC_FP32R = 1 };
 
 // bucket blocks by neighbour count: counts[NB] and lists (device), returns via counts
-constexpr int NBUCKET = 5;  // n==0, <=8, <=16, <=32, <=64 ; larger -> overflow bucket
+constexpr int NBUCKET = 6;  // n==0, <=8, <=16, <=32, <=48, <=64 ; larger -> overflow bucket
 // blocks 0..nblock-1, or the nblock blocks listed in sel; blocks with a
 // negative neighbour count are left out
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st);
-int bucket_width(int b);  // lanes per system of bucket b (0 for the empty bucket)
+int bucket_width(int b);  // solve width (matrix rows) of bucket b (0 for the empty bucket)
+int solve_width(int n);   // smallest solve width holding n rows: 8, 16, 32, 48 or 64
 
 hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st);

@@ -98,17 +98,27 @@ __device__ inline double pair_cov_tb(const SolveArgs& a, const TabB& Ts, const T
   return a.k_ps * cs * ct + cs + ct;
 }
 
-template <int G, int DIM, int PREC, bool TB>
+// G lanes per system, R <= G matrix rows (R < G: lanes R..G-1 hold no row;
+// the 64-lane, 48-row instance takes systems of 33 to 48 rows with 48-column
+// rows, a quarter fewer registers and trailing updates than 64)
+template <int G, int DIM, int PREC, bool TB, int R = G>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
+  static_assert(R <= G && 2 * G <= 3 * R, "lanes past R store into the unread fold slots");
   __shared__ double sx[S][DIM][G];
-  // fold exchange: sp[d][r] = C(row r, row (r + d) mod G) for d = 1..G/2 as
+  // fold exchange: sp[d][r] = C(row r, row (r + d) mod R) for d = 1..R/2 as
   // lane r computed it, sp[0][r] = diagonal; later reused for the pivot row's
   // extra columns and (as floats) the transposed FP32 factor
-  __shared__ double sp[S][G / 2 + 1][G];
+  __shared__ double sp[S][R / 2 + 1][R];
   __shared__ double col[S][G];
 
   const int lane = threadIdx.x, s = lane / G, r = lane % G;
+  // lane r holds row r; the slots (R/2, R/2..R-1) are never read, so lanes
+  // past R store there instead of branching around their stores
+  const bool row = R == G || r < R;
+  const int rr = row ? r : 0;
+  auto slot = [&](int d) -> double* { return row ? &sp[s][d][r] : &sp[s][R / 2][r - R / 2]; };
+  auto wrap = [](int c) { return R == G ? (c & (G - 1)) : (c >= R ? c - R : c); };
   const int sys = blockIdx.x * S + s;
   const bool act = sys < a.nlist;
   const int b = act ? a.list[sys] : 0;
@@ -163,17 +173,19 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   // TB: fold offsets whose lag lies beyond a table with an analytic tail are
   // recorded and evaluated after the loop, keeping the loop free of calls
   // offsets d >= nw pair no live rows: their slots are zero.  A slot
-  // (G/2, r >= G/2) is never read, so every lane stores unconditionally; with
+  // (R/2, r >= R/2) is never read, so every lane stores unconditionally; with
   // TB the loop body is branch-free and consecutive offsets' loads overlap
   // (the loop is instantiated with and without range scaling, so that no
   // branch splits its unrolled body)
   unsigned long long redo = 0;
-  const int dmax = min(G / 2, nw - 1);
+  const int dmax = min(R / 2, nw - 1);
+  // unrolled by 4 (by 2 where that would cost the FP32 variant occupancy)
+  constexpr int FOLD_UNROLL = (PREC != PREC_FP64 && G >= 32) ? 2 : 4;
   auto fold = [&](auto sc) {
-#pragma unroll 2
+#pragma unroll FOLD_UNROLL
     for (int d = 1; d <= dmax; ++d) {
-      const int c = (r + d) & (G - 1);
-      const bool use = live && c < n && ((d < G / 2) || (r < G / 2));
+      const int c = wrap(r + d);
+      const bool use = live && c < n && ((d < R / 2) || (r < R / 2));
       double v = 0.0;
       if (TB) {
         double dx, dy, dz, dt;
@@ -187,25 +199,25 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         lag_sc(decltype(sc)::value, &sx[s][0][c], G, dx, dy, dz, dt);
         v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
       }
-      sp[s][d][r] = v;
+      *slot(d) = v;
     }
   };
   if (scaled)
     fold(std::true_type{});
   else
     fold(std::false_type{});
-  for (int d = max(dmax, 0) + 1; d <= G / 2; ++d) sp[s][d][r] = 0.0;
+  for (int d = max(dmax, 0) + 1; d <= R / 2; ++d) *slot(d) = 0.0;
   if (TB) {
     while (redo) {
       const int d = __ffsll((long long)redo) - 1;
       redo &= redo - 1;
-      const int c = (r + d) & (G - 1);
+      const int c = wrap(r + d);
       double dx, dy, dz, dt;
       lag(&sx[s][0][c], G, dx, dy, dz, dt);
       sp[s][d][r] = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
     }
   }
-  sp[s][0][r] = live ? (cov0 + ov) + ln : 1.0;
+  if (row) sp[s][0][r] = live ? (cov0 + ov) + ln : 1.0;
   // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
   double rhs = 0.0;
   if (live) {
@@ -230,16 +242,18 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   // weight sum 1^T K^-1 b that simple kriging needs for its mean term
   const double f = live ? 1.0 : 0.0;
   __syncthreads();
-  // row r: entry j is the pair at offset d = (j - r) mod G, computed by lane
-  // r when d <= G/2 (d == G/2 only by the lower lane), else by lane j
-  double A[G];
+  // row r: entry j is the pair at offset d = (j - r) mod R, computed by lane
+  // r when d <= R/2 (d == R/2 only by the lower lane), else by lane j; lanes
+  // past R hold a zero row
+  double A[R];
 #pragma unroll
-  for (int j = 0; j < G; ++j) {
-    const int d = (j - r) & (G - 1);
-    const bool own = d < G / 2 || (d == G / 2 && r < G / 2);
-    A[j] = own ? sp[s][d][r] : sp[s][G - d][j];
+  for (int j = 0; j < R; ++j) {
+    const int d = wrap(j - rr + R);
+    const bool own = d < R / 2 || (d == R / 2 && rr < R / 2);
+    const double e = own ? sp[s][d][rr] : sp[s][R - d][j];
+    A[j] = row ? e : 0.0;
   }
-  const double kdiag = sp[s][0][r];
+  const double kdiag = sp[s][0][rr];
 
   int fail = 0;
   double uu, uy, yy, vy, vu;
@@ -252,7 +266,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     double y = rhs, u = f, v = z;
     double* ext = &sp[s][0][0];  // pivot row's (y, u, v); sp is free after the row loads
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
+    for (int k = 0; k < R; ++k) {
       if (k < nw) {
         col[s][r] = A[k];
         if (r == k) {
@@ -268,7 +282,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         // columns past nw hold zeros (padding rows): updating them is
         // harmless and keeps the unrolled loop branch-free
 #pragma unroll
-        for (int j = k + 1; j < G; ++j) A[j] = fma(-t, col[s][j], A[j]);
+        for (int j = k + 1; j < R; ++j) A[j] = fma(-t, col[s][j], A[j]);
         y = fma(-t, ext[0], y);
         u = fma(-t, ext[1], u);
         v = fma(-t, ext[2], v);
@@ -291,14 +305,14 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     // error (one refinement step's accuracy without a second solve) --------
     float* col32 = reinterpret_cast<float*>(&col[s][0]);
     float* ext32 = reinterpret_cast<float*>(&sp[s][0][0]);  // free after the row loads
-    float L[G];
+    float L[R];
 #pragma unroll
-    for (int j = 0; j < G; ++j) L[j] = (float)A[j];
+    for (int j = 0; j < R; ++j) L[j] = (float)A[j];
     float D = 1.0f;
     float pmin = 1.0f;
     float py = (float)rhs, pu = (float)f, pv = (float)z;
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
+    for (int k = 0; k < R; ++k) {
       if (k < nw) {
         col32[r] = L[k];
         if (r == k) {
@@ -312,7 +326,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         fail |= (k < n) && !(akk > 0.0f);
         const float t = (r > k) ? L[k] * __builtin_amdgcn_rcpf(akk) : 0.0f;
 #pragma unroll
-        for (int j = k + 1; j < G; ++j) L[j] = fmaf(-t, col32[j], L[j]);
+        for (int j = k + 1; j < R; ++j) L[j] = fmaf(-t, col32[j], L[j]);
         py = fmaf(-t, ext32[0], py);
         pu = fmaf(-t, ext32[1], pu);
         pv = fmaf(-t, ext32[2], pv);
@@ -322,20 +336,24 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     }
     if (live) pmin = (float)((double)D / kdiag);
     // transpose L through LDS for the back substitution
-    static_assert(sizeof(float) * G * (G + 1) <= sizeof(double) * (G / 2 + 1) * G, "transpose fits");
+    // (lanes past R read column r < G of the row-major R x (R+1) block, in
+    // bounds, and never use it)
+    static_assert(sizeof(float) * ((R - 1) * (R + 1) + G) <= sizeof(double) * (R / 2 + 1) * R, "transpose fits");
     float* sm32 = reinterpret_cast<float*>(&sp[s][0][0]);
+    if (row) {
 #pragma unroll
-    for (int j = 0; j < G; ++j) sm32[r * (G + 1) + j] = L[j];
+      for (int j = 0; j < R; ++j) sm32[r * (R + 1) + j] = L[j];
+    }
     __syncthreads();
-    float Lt[G];
+    float Lt[R];
 #pragma unroll
-    for (int j = 0; j < G; ++j) Lt[j] = sm32[j * (G + 1) + r];
+    for (int j = 0; j < R; ++j) Lt[j] = sm32[j * (R + 1) + r];
     const float iD = 1.0f / D;
     py *= iD;
     pu *= iD;
     pv *= iD;
 #pragma unroll
-    for (int k = G - 1; k >= 0; --k) {
+    for (int k = R - 1; k >= 0; --k) {
       if (k < nw) {
         const float yk = __shfl(py, k, G), uk = __shfl(pu, k, G), vk = __shfl(pv, k, G);
         const float l = (r < k) ? Lt[k] : 0.0f;
@@ -352,12 +370,12 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     xsh[r] = xb;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G; ++j) rb = fma(-A[j], xsh[j], rb);
+    for (int j = 0; j < R; ++j) rb = fma(-A[j], xsh[j], rb);
     __syncthreads();
     xsh[r] = x1;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < G; ++j) r1 = fma(-A[j], xsh[j], r1);
+    for (int j = 0; j < R; ++j) r1 = fma(-A[j], xsh[j], r1);
     uu = gsum<G>(fma(x1, r1, f * x1));
     uy = gsum<G>(fma(x1, rb, f * xb));
     yy = gsum<G>(fma(xb, rb, rhs * xb));
@@ -435,7 +453,8 @@ __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* se
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int b = t < nblock ? (sel ? sel[t] : t) : -1;
   const int n = b >= 0 ? nnear[b] : -1;
-  const int bk = n < 0 ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 64 ? 4 : NBUCKET;
+  const int bk = n < 0 ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 48 ? 4 : n <= 64 ? 5
+                                                                                                  : NBUCKET;
   int local = 0;
 #pragma unroll
   for (int k = 0; k <= NBUCKET; ++k) {
@@ -610,7 +629,9 @@ hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st)
   return hipGetLastError();
 }
 
-int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 64 : 0; }
+int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 48 : bk == 5 ? 64 : 0; }
+
+int solve_width(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : n <= 48 ? 48 : 64; }
 
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st) {
@@ -638,12 +659,12 @@ hipError_t launch_solve_d3(const SolveArgs& a, int width, int prec, bool tableB,
 hipError_t launch_solve_d4(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 
 #if KK_SOLVE_DIM > 0
-template <int G, int PREC>
+template <int G, int PREC, int R = G>
 static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
   constexpr int S = 64 / G;
   dim3 grd((a.nlist + S - 1) / S), blk(64);
-  if (tb) hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, true>), grd, blk, 0, st, a);
-  else hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, false>), grd, blk, 0, st, a);
+  if (tb) hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, true, R>), grd, blk, 0, st, a);
+  else hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, false, R>), grd, blk, 0, st, a);
 }
 
 #define KK_CAT2(x, y) x##y
@@ -655,6 +676,7 @@ hipError_t KK_CAT(launch_solve_d, KK_SOLVE_DIM)(const SolveArgs& a, int width, i
       case 8: launch_g<8, PREC_FP64>(a, tableB, st); break;
       case 16: launch_g<16, PREC_FP64>(a, tableB, st); break;
       case 32: launch_g<32, PREC_FP64>(a, tableB, st); break;
+      case 48: launch_g<64, PREC_FP64, 48>(a, tableB, st); break;
       case 64: launch_g<64, PREC_FP64>(a, tableB, st); break;
       default: return hipErrorInvalidValue;
     }
@@ -663,6 +685,7 @@ hipError_t KK_CAT(launch_solve_d, KK_SOLVE_DIM)(const SolveArgs& a, int width, i
       case 8: launch_g<8, PREC_FP32R>(a, tableB, st); break;
       case 16: launch_g<16, PREC_FP32R>(a, tableB, st); break;
       case 32: launch_g<32, PREC_FP32R>(a, tableB, st); break;
+      case 48: launch_g<64, PREC_FP32R, 48>(a, tableB, st); break;
       case 64: launch_g<64, PREC_FP32R>(a, tableB, st); break;
       default: return hipErrorInvalidValue;
     }

@@ -34,6 +34,20 @@ def test_st_fp32r(spec):
     assert ee < 1e-6 and ev < 1e-6, (ee, ev)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32r"])
+def test_st_48_rows(precision):
+    """the full-size workload's system size: 33-48 rows (64 lanes, 48-row instance)"""
+    spec = problems.st_small(nmax=44, name="st_rows44")
+    print("case st_rows44", precision, flush=True)
+    o = run_oracle(spec)
+    g = run_gpu_st(spec, precision=precision)
+    assert same_lists(g, o)
+    assert 32 < g["nnear"].max() <= 48
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    ee, ev = max_errors(g, o)
+    assert ee < tol and ev < tol, (ee, ev)
+
+
 @pytest.mark.parametrize("fast", [2, 1, 0])
 def test_st_search_modes_agree(fast):
     spec = problems.st_small(nstation=80, ndays=40, side=200.0, grid_side=12, ntimes=4, name="st_modes")
