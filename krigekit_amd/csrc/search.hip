@@ -745,8 +745,10 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 // key RN_fp32(d^2) and tree position), after the maxdist and leave-one-out
 // filters, and decides membership in the k-nearest set from a per-lane
 // histogram of the keys of every point it has seen:
-//   * 64 quarter-octave bins of the key's float bits relative to a per-lane
-//     base (bin 0 and bin 63 catch everything below and above), 16-bit
+//   * 64 eighth-octave bins of the key's float bits relative to a per-lane
+//     base: the top bin holds the key of the lane's initial bound, so the
+//     bins span the 8 octaves below it (bin 0 catches everything below;
+//     without an initial bound bin 63 catches everything above), 16-bit
 //     counters packed two to a word (a wave that visits more than 65000
 //     points leaves its queries to the exact kernel);
 //   * jK = the lowest bin whose cumulative count reaches k, tracked as points
@@ -768,8 +770,9 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 // Points of a leaf are read with one vector load (a leaf is contiguous in
 // tree order) and broadcast with readlane.
 // ---------------------------------------------------------------------------
-constexpr int NBIN = 64;      // quarter-octave bins, 16-bit counters
-constexpr int BSH = 21;       // key bits >> BSH = exponent and 2 mantissa bits
+constexpr int NBIN = 64;      // eighth-octave bins, 16-bit counters
+constexpr int BSH = 20;       // key bits >> BSH = exponent and 3 mantissa bits
+constexpr int SUBSH = BSH - 5;  // 32 sub-bins of a bin: the next 5 key bits
 constexpr int SLEAFCAP = 512;
 constexpr int MAXVISIT = 65000;
 constexpr int NAMB = 4;  // straddling entries resolved per lane
@@ -870,6 +873,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
   }
   double ball = ubound;
+  // with that bound, the 64 bins (8 octaves) end at the bin of its key: every
+  // counted point lies at or below it, so no key is clamped at the top
+  if (ubound < BIG) {
+    const unsigned kb = __float_as_uint((float)ubound) >> BSH;
+    base = kb >= (unsigned)(NBIN - 1) ? (kb - (NBIN - 1)) << BSH : 0u;
+    have_base = true;
+  }
 
   auto point = [&](const double* x, int idx, int pos) {
     double sd = 0.0;
@@ -910,8 +920,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const double v = lane < DIM * m ? T.rdata[DIM * (size_t)l + lane] : 0.0;
       const int iv = lane < m ? T.ind[l + lane] : 0;
       if (!have_base) {
-        // bins start 8 octaves below the largest key of the first leaf (64
-        // quarter-octave bins span 16 octaves)
+        // no initial bound: bins start 4 octaves below the largest key of the
+        // first leaf (keys past the top bin count in the top bin)
         float mk = 0.0f;
         for (int p = 0; p < m; ++p) {
           double sd = 0.0;
@@ -1051,7 +1061,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         }
     };
-    // walk 1: sub-histogram of bin jK (bits 16..20 of the key: 1/128
+    // walk 1: sub-histogram of bin jK (the 5 key bits below the bin's: 1/256
     // octave, 32-bit counters) plus exact ranks for the first NAMB undecided
     // distances
     const int hK = hcount(jK);
@@ -1069,7 +1079,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     walk(b1, [&](double sd) {
       const float key = (float)sd;
       if (subok && key_bin(key, base) == jK)
-        __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> 16) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> SUBSH) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
       for (int z = 0; z < NAMB; ++z) {
@@ -1094,7 +1104,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int j = 0; j < C; ++j) {
           if (!(und_m & bit(s, j))) continue;
-          const int b = (int)((__float_as_uint(ok[s][j]) >> 16) & 31);
+          const int b = (int)((__float_as_uint(ok[s][j]) >> SUBSH) & 31);
           if (b < jK2 || (b == jK2 && tot2 <= K)) {
             in_m |= bit(s, j);
             und_m &= ~bit(s, j);

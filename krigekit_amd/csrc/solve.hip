@@ -179,8 +179,8 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   // branch splits its unrolled body)
   unsigned long long redo = 0;
   const int dmax = min(R / 2, nw - 1);
-  // unrolled by 4 (by 2 where that would cost the FP32 variant occupancy)
-  constexpr int FOLD_UNROLL = (PREC != PREC_FP64 && G >= 32) ? 2 : 4;
+  // unrolled by 2 (by 4 in the 64-lane FP64 instances)
+  constexpr int FOLD_UNROLL = (PREC == PREC_FP64 && G == 64) ? 4 : 2;
   auto fold = [&](auto sc) {
 #pragma unroll FOLD_UNROLL
     for (int d = 1; d <= dmax; ++d) {
