@@ -745,12 +745,12 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 // key RN_fp32(d^2) and tree position), after the maxdist and leave-one-out
 // filters, and decides membership in the k-nearest set from a per-lane
 // histogram of the keys of every point it has seen:
-//   * 64 eighth-octave bins of the key's float bits relative to a per-lane
-//     base: the top bin holds the key of the lane's initial bound, so the
-//     bins span the 8 octaves below it (bin 0 catches everything below;
-//     without an initial bound bin 63 catches everything above), 16-bit
-//     counters packed two to a word (a wave that visits more than 65000
-//     points leaves its queries to the exact kernel);
+//   * 64 bins of the key's float bits relative to a per-lane base: eighth-
+//     octave bins ending at the key of the lane's initial bound when that
+//     bound is tight, otherwise quarter-octave bins (bin 0 catches everything
+//     below, and bin 63 everything above unless it holds the bound's key),
+//     16-bit counters packed two to a word (a wave that visits more than
+//     65000 points leaves its queries to the exact kernel);
 //   * jK = the lowest bin whose cumulative count reaches k, tracked as points
 //     arrive; its upper edge E bounds the k-th key, so boxes beyond E are
 //     pruned, and so are boxes beyond every sector's (nmax+1)-th key;
@@ -771,8 +771,9 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 // tree order) and broadcast with readlane.
 // ---------------------------------------------------------------------------
 constexpr int NBIN = 64;      // eighth-octave bins, 16-bit counters
-constexpr int BSH = 20;       // key bits >> BSH = exponent and 3 mantissa bits
-constexpr int SUBSH = BSH - 5;  // 32 sub-bins of a bin: the next 5 key bits
+// per-lane bin width: key bits >> bsh = exponent and 3 mantissa bits (bsh 20,
+// eighth-octave bins, 8 octaves) or 2 (bsh 21, quarter-octave, 16 octaves);
+// the 32 sub-bins of a bin are the next 5 key bits
 constexpr int SLEAFCAP = 512;
 constexpr int MAXVISIT = 65000;
 constexpr int NAMB = 4;  // straddling entries resolved per lane
@@ -802,8 +803,8 @@ __device__ inline void sector_insert(float (&k)[C], int (&p)[C], float key, int 
   }
 }
 
-__device__ inline int key_bin(float key, unsigned base) {
-  const int d = (int)(__float_as_uint(key) >> BSH) - (int)(base >> BSH);
+__device__ inline int key_bin(float key, unsigned base, unsigned bsh) {
+  const int d = (int)(__float_as_uint(key) >> bsh) - (int)(base >> bsh);
   return d < 0 ? 0 : (d >= NBIN ? NBIN - 1 : d);
 }
 
@@ -842,8 +843,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   unsigned base = 0;
   bool have_base = false;
   int jK = NBIN - 1, tot = 0, nleaf = 0, nvisit = 0;
-  // before any point is counted: the deepest node on the first query's path
-  // that still holds >= k points has them all inside its box, so every lane's
+  // before any point is counted: the deepest node on this lane's query path
+  // that still holds >= k points has them all inside its box, so the lane's
   // k-th squared distance is at most the squared distance to that box's
   // farthest corner (inflated like the box test).  Points beyond it rank
   // below k for this lane, so it caps the bound from the start.
@@ -854,10 +855,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const int* ni = T.ni + (size_t)node * NODE_I;
       const int left = ni[3], right = ni[4];
       if (left < 0 || right < 0) break;
-      const int child = (q0[ni[0]] < T.nd[(size_t)node * NODE_D]) ? left : right;
+      const int child = (q[ni[0]] < T.nd[(size_t)node * NODE_D]) ? left : right;
       const int* nc = T.ni + (size_t)child * NODE_I;
       if (nc[2] - nc[1] + 1 < K) break;
-      node = __builtin_amdgcn_readfirstlane(child);
+      node = child;
     }
     const int* ni = T.ni + (size_t)node * NODE_I;
     if (ni[2] - ni[1] + 1 >= K) {
@@ -873,13 +874,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
   }
   double ball = ubound;
-  // with that bound, the 64 bins (8 octaves) end at the bin of its key: every
-  // counted point lies at or below it, so no key is clamped at the top
-  if (ubound < BIG) {
-    const unsigned kb = __float_as_uint((float)ubound) >> BSH;
-    base = kb >= (unsigned)(NBIN - 1) ? (kb - (NBIN - 1)) << BSH : 0u;
-    have_base = true;
-  }
+  // Bin range, set at the first leaf before any point is counted, from mk =
+  // the largest key of that leaf:
+  //   * the bound within 5 octaves above mk: eighth-octave bins ending at the
+  //     bin of the bound's key (every counted point lies at or below it, so
+  //     the top bin is exact);
+  //   * within 10 octaves: quarter-octave bins ending there (16 octaves);
+  //   * otherwise (a loose bound, or none): quarter-octave bins from 8
+  //     octaves below mk, the top bin also counting every larger key.
+  const unsigned ukey = ubound < BIG ? __float_as_uint((float)ubound) : 0xffffffffu;
+  bool anchored = false;
+  unsigned bsh = 21;
 
   auto point = [&](const double* x, int idx, int pos) {
     double sd = 0.0;
@@ -891,7 +896,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // counted, which is what the in/out rules below rely on
     if (!(sd <= ball)) return;
     const float key = (float)sd;
-    const int bin = key_bin(key, base);
+    const int bin = key_bin(key, base, bsh);
     __hip_atomic_fetch_add(&hist[(bin >> 1) * 64 + lane], 1u << (16 * (bin & 1)), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     tot += (bin <= jK);
@@ -920,8 +925,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const double v = lane < DIM * m ? T.rdata[DIM * (size_t)l + lane] : 0.0;
       const int iv = lane < m ? T.ind[l + lane] : 0;
       if (!have_base) {
-        // no initial bound: bins start 4 octaves below the largest key of the
-        // first leaf (keys past the top bin count in the top bin)
         float mk = 0.0f;
         for (int p = 0; p < m; ++p) {
           double sd = 0.0;
@@ -929,8 +932,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
           for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(readlane_d(v, DIM * p + k), q[k])));
           mk = fmaxf(mk, (float)sd);
         }
-        const unsigned mb = __float_as_uint(mk) > (1u << 23) ? __float_as_uint(mk) : (1u << 23);
-        base = mb > (32u << BSH) ? ((mb - (32u << BSH)) >> BSH) << BSH : 0u;
+        const unsigned mkey = __float_as_uint(mk);
+        if (ukey != 0xffffffffu && (ukey >> 20) <= (mkey >> 20) + 40u) {
+          bsh = 20;
+          anchored = true;
+        } else if (ukey != 0xffffffffu && (ukey >> 21) <= (mkey >> 21) + 40u) {
+          anchored = true;
+        }
+        if (anchored) {
+          const unsigned top = ukey >> bsh;
+          base = top >= (unsigned)(NBIN - 1) ? (top - (NBIN - 1)) << bsh : 0u;
+        } else {
+          const unsigned mb = mkey > (1u << 23) ? mkey : (1u << 23);
+          base = mb > (32u << 21) ? ((mb - (32u << 21)) >> 21) << 21 : 0u;
+        }
         have_base = true;
       }
       // every lane takes the same point: uniform (scalar) loads
@@ -952,7 +967,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     double bE = BIG;
     if (tot >= K && jK < NBIN - 1) {
-      const unsigned eb = base + ((unsigned)(jK + 1) << BSH);
+      const unsigned eb = base + ((unsigned)(jK + 1) << bsh);
       if (eb < 0x7f800000u) bE = (double)__uint_as_float(eb);
     }
     float mo = 0.0f;
@@ -980,7 +995,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       if (op[s][j] < 0) continue;
-      const int b = key_bin(ok[s][j], base);
+      const int b = key_bin(ok[s][j], base, bsh);
       if (b < jK || (b == jK && tot <= K)) in_m |= bit(s, j);
       else if (b == jK) und_m |= bit(s, j);
     }
@@ -1062,12 +1077,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     };
     // walk 1: sub-histogram of bin jK (the 5 key bits below the bin's: 1/256
-    // octave, 32-bit counters) plus exact ranks for the first NAMB undecided
+    // or 1/128 octave, 32-bit counters) plus exact ranks for the first NAMB undecided
     // distances
     const int hK = hcount(jK);
     const int cb = tot - hK;  // counted points in bins below jK
-    const bool subok = jK > 0 && jK < NBIN - 1;
-    const unsigned ekb = base + ((unsigned)(jK + 1) << BSH);
+    // (bin 0 also holds every smaller key; the top bin is an exact bin when
+    // the bins end at the initial bound, otherwise it holds every larger key)
+    const bool subok = jK > 0 && (jK < NBIN - 1 || anchored);
+    const unsigned ekb = base + ((unsigned)(jK + 1) << bsh);
     const double eK = !subok ? 0.0 : (ekb < 0x7f800000u ? (double)__uint_as_float(ekb) : BIG);
     __syncthreads();
 #pragma unroll
@@ -1078,8 +1095,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int z = 0; z < NAMB; ++z) b1 = fmax(b1, amb[z]);
     walk(b1, [&](double sd) {
       const float key = (float)sd;
-      if (subok && key_bin(key, base) == jK)
-        __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> SUBSH) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
+      // unclamped bin: the walk also sees points beyond the bound
+      if (subok && (int)(__float_as_uint(key) >> bsh) - (int)(base >> bsh) == jK)
+        __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> (bsh - 5)) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
       for (int z = 0; z < NAMB; ++z) {
@@ -1104,7 +1122,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int j = 0; j < C; ++j) {
           if (!(und_m & bit(s, j))) continue;
-          const int b = (int)((__float_as_uint(ok[s][j]) >> SUBSH) & 31);
+          const int b = (int)((__float_as_uint(ok[s][j]) >> (bsh - 5)) & 31);
           if (b < jK2 || (b == jK2 && tot2 <= K)) {
             in_m |= bit(s, j);
             und_m &= ~bit(s, j);
