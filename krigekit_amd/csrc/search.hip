@@ -1151,8 +1151,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   if (!valid) return;
   // per sector: the in-entries must be a prefix of the key order (only equal
   // keys could break that), take the first nmax, and a key tie at the cut
-  // between two in-entries is left to the exact kernel
+  // between two in-entries is left to the exact kernel.
+  // The list is sorted in this lane's column of the (now free) histogram
+  // and written once with streaming stores, so the output rows neither
+  // round-trip through L2 for the sort nor evict the tree from it.
+  constexpr bool LDS_OUT = NOCT * (C - 1) <= NBIN / 2;
   int* out = a.inear + (size_t)qid * a.mmax;
+  int* col = reinterpret_cast<int*>(hist) + lane;
   int cnt = 0;
 #pragma unroll
   for (int s = 0; s < NOCT; ++s) {
@@ -1170,14 +1175,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (nin == C && ok[s][C - 2] == ok[s][C - 1]) flag = true;
 #pragma unroll
     for (int j = 0; j < C - 1; ++j)
-      if (j < nin) out[cnt++] = T.ind[op[s][j]];
+      if (j < nin) {
+        const int v = T.ind[op[s][j]];
+        if (LDS_OUT)
+          col[64 * cnt] = v;
+        else
+          out[cnt] = v;
+        ++cnt;
+      }
   }
   if (flag) {
     a.qflag[qid] = 1;
     a.nnear[qid] = -1;  // decided by the exact kernel
     return;
   }
-  isort(out, cnt);
+  if (LDS_OUT) {
+    for (int i = 1; i < cnt; ++i) {
+      const int t = col[64 * i];
+      int j = i - 1;
+      while (j >= 0 && col[64 * j] > t) {
+        col[64 * (j + 1)] = col[64 * j];
+        --j;
+      }
+      col[64 * (j + 1)] = t;
+    }
+    for (int i = 0; i < cnt; ++i) __builtin_nontemporal_store(col[64 * i], &out[i]);
+  } else {
+    isort(out, cnt);
+  }
   a.nnear[qid] = cnt;
   a.qflag[qid] = 0;
 }
