@@ -637,6 +637,9 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
     for (int s = 0; s < NOCT; ++s) ocnt[s * 64 + lane] = 0;
   }
   int* out = a.inear + (size_t)qid * a.mmax;
+  // plain lists are collected in this lane's column of the pass-1 heap (k >=
+  // mmax words, free now), sorted there and written once with streaming stores
+  int* col = reinterpret_cast<int*>(lds_d) + lane;
   int nin = 0, cnt = 0;
   bool flag = false;
   Boundary B;
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
       ii[(size_t)j * 64] = idx;
       ocnt[s * 64 + lane] = c < C ? c + 1 : C;
     } else {
-      if (cnt < a.mmax) out[cnt++] = idx;
+      if (cnt < a.mmax) col[64 * cnt++] = idx;
       else flag = true;
     }
   };
@@ -728,7 +731,20 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
     a.nnear[qid] = -1;  // decided by the exact kernel
     return;
   }
-  isort(out, cnt);
+  if (SECTOR) {
+    isort(out, cnt);
+  } else {
+    for (int i = 1; i < cnt; ++i) {
+      const int t = col[64 * i];
+      int j = i - 1;
+      while (j >= 0 && col[64 * j] > t) {
+        col[64 * (j + 1)] = col[64 * j];
+        --j;
+      }
+      col[64 * (j + 1)] = t;
+    }
+    for (int i = 0; i < cnt; ++i) __builtin_nontemporal_store(col[64 * i], &out[i]);
+  }
   a.nnear[qid] = cnt;
   a.qflag[qid] = 0;
 }
