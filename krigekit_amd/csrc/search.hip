@@ -279,7 +279,14 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     }
   });
 
+  // with the heap in LDS the list is compacted into the heap's index slots
+  // (position <= the slot being read), sorted there and written once with
+  // streaming stores; otherwise it is built and sorted in the output row
   int cnt = 0;
+  auto emit = [&](int idx) {
+    if (LDSHEAP) h.I(++cnt) = idx;
+    else out[cnt++] = idx;
+  };
   if (a.sort) {
     heapsort(h, nn);
     // apply_sector_filter, kriging.F90:1093-1154
@@ -299,7 +306,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       if (cs < a.nmax) {
 #pragma unroll
         for (int s = 0; s < NSEC; ++s) counts[s] += (s == sec);
-        out[cnt++] = idx;
+        emit(idx);
       }
     }
   } else {
@@ -310,10 +317,23 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         removed = true;
         continue;
       }
-      if (h.D(c) <= a.maxdist2) out[cnt++] = idx;
+      if (h.D(c) <= a.maxdist2) emit(idx);
     }
   }
-  isort(out, cnt);
+  if (LDSHEAP) {
+    for (int i = 2; i <= cnt; ++i) {
+      const int t = h.I(i);
+      int j = i - 1;
+      while (j >= 1 && h.I(j) > t) {
+        h.I(j + 1) = h.I(j);
+        --j;
+      }
+      h.I(j + 1) = t;
+    }
+    for (int i = 1; i <= cnt; ++i) __builtin_nontemporal_store(h.I(i), &out[i - 1]);
+  } else {
+    isort(out, cnt);
+  }
   a.nnear[qid] = cnt;
 }
 
