@@ -1184,14 +1184,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       flag |= und_m != 0;
     }
   }
-  if (!valid) return;
   // per sector: the in-entries must be a prefix of the key order (only equal
   // keys could break that), take the first nmax, and a key tie at the cut
   // between two in-entries is left to the exact kernel.
-  // The list is sorted in this lane's column of the (now free) histogram
-  // and written once with streaming stores, so the output rows neither
-  // round-trip through L2 for the sort nor evict the tree from it.
+  // The list is sorted in this lane's column of the (now free) histogram;
+  // the wave then writes its rows with streaming stores, two rows per store
+  // (32 lanes per row), so the output rows neither round-trip through L2 for
+  // the sort nor evict the tree from it, and each row leaves as one
+  // contiguous write.
   constexpr bool LDS_OUT = NOCT * (C - 1) <= NBIN / 2;
+  static_assert(!LDS_OUT || NOCT * (C - 1) <= 32, "a row fits half a wave");
+  if (!LDS_OUT && !valid) return;
   int* out = a.inear + (size_t)qid * a.mmax;
   int* col = reinterpret_cast<int*>(hist) + lane;
   int cnt = 0;
@@ -1220,10 +1223,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         ++cnt;
       }
   }
-  if (flag) {
+  if (valid && flag) {
     a.qflag[qid] = 1;
     a.nnear[qid] = -1;  // decided by the exact kernel
-    return;
   }
   if (LDS_OUT) {
     for (int i = 1; i < cnt; ++i) {
@@ -1235,12 +1237,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
       col[64 * (j + 1)] = t;
     }
-    for (int i = 0; i < cnt; ++i) __builtin_nontemporal_store(col[64 * i], &out[i]);
-  } else {
+    const int wc = (valid && !flag) ? cnt : 0;
+    const int* cols = reinterpret_cast<const int*>(hist);
+    __syncthreads();  // other lanes' columns are read below
+    for (int r = 0; r < 64; r += 2) {
+      const int rr = r + (lane >> 5), j = lane & 31;
+      const int c = __shfl(wc, rr, 64);
+      const int qr = __shfl(qid, rr, 64);
+      if (j < c) __builtin_nontemporal_store(cols[64 * j + rr], a.inear + (size_t)qr * a.mmax + j);
+    }
+  } else if (!flag) {
     isort(out, cnt);
   }
-  a.nnear[qid] = cnt;
-  a.qflag[qid] = 0;
+  if (valid && !flag) {
+    a.nnear[qid] = cnt;
+    a.qflag[qid] = 0;
+  }
 }
 
 // every observation is a neighbour (nmax >= nobs): kriging.F90:640-657
