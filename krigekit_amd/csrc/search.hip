@@ -937,13 +937,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     tot += (bin <= jK);
     if ((a.cv && idx == qid) || !(sd <= a.maxdist2)) return;
-    int oct = (__dsub_rn(x[0], q[0]) < 0.0);
-    if (DIM >= 2) oct += 2 * (__dsub_rn(x[1], q[1]) < 0.0);
-    if (DIM == 3) oct += 4 * (__dsub_rn(x[2], q[2]) < 0.0);
-    float thr = ok[0][C - 1];
+    // the octant's (nmax+1)-th key by a select tree on the three sign tests
+    // (octant 1 + ix + 2 iy + 4 iz with ix = (dx < 0)); the octant index
+    // itself is only formed for an insert
+    const bool b0 = __dsub_rn(x[0], q[0]) < 0.0;
+    const bool b1 = DIM >= 2 && __dsub_rn(x[1], q[1]) < 0.0;
+    const bool b2 = DIM == 3 && __dsub_rn(x[2], q[2]) < 0.0;
+    float thr;
+    if (DIM == 1) {
+      thr = b0 ? ok[1][C - 1] : ok[0][C - 1];
+    } else {
+      float t[NOCT / 2];
 #pragma unroll
-    for (int s = 1; s < NOCT; ++s) thr = oct == s ? ok[s][C - 1] : thr;
+      for (int s = 0; s < NOCT / 2; ++s) t[s] = b0 ? ok[2 * s + 1][C - 1] : ok[2 * s][C - 1];
+      if (DIM == 2) {
+        thr = b1 ? t[1] : t[0];
+      } else {
+        const float lo = b1 ? t[1] : t[0], hi = b1 ? t[3 % (NOCT / 2)] : t[2 % (NOCT / 2)];
+        thr = b2 ? hi : lo;
+      }
+    }
     if (!(key < thr)) return;
+    const int oct = (int)b0 + 2 * (int)b1 + 4 * (int)b2;
 #pragma unroll
     for (int s = 0; s < NOCT; ++s)
       if (oct == s) sector_insert<C>(ok[s], op[s], key, pos);
