@@ -922,7 +922,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   bool anchored = false;
   unsigned bsh = 21;
 
-  auto point = [&](const double* x, int idx, int pos) {
+  // pos = tree position; the observation index is read only for the
+  // leave-one-out exclusion
+  auto point = [&](const double* x, int pos) {
     double sd = 0.0;
 #pragma unroll
     for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(x[k], q[k])));
@@ -936,7 +938,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __hip_atomic_fetch_add(&hist[(bin >> 1) * 64 + lane], 1u << (16 * (bin & 1)), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
     tot += (bin <= jK);
-    if ((a.cv && idx == qid) || !(sd <= a.maxdist2)) return;
+    if (!(sd <= a.maxdist2)) return;
+    if (a.cv && T.ind[pos] == qid) return;
     // the octant's (nmax+1)-th key by a select tree on the three sign tests
     // (octant 1 + ix + 2 iy + 4 iz with ix = (dx < 0)); the octant index
     // itself is only formed for an insert
@@ -1005,7 +1008,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         double x[DIM];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) x[k] = T.rdata[DIM * (size_t)pp + k];
-        point(x, T.ind[pp], pp);
+        point(x, pp);
       }
     }
     // lowest bin reaching k, and the ball: min(upper edge of that bin,
