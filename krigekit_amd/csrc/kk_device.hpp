@@ -36,14 +36,21 @@ hipError_t launch_search_all(const SearchArgs& a, int nobs, const double* ocoord
 // precision of the per-node solve
 enum Precision : int { PREC_FP64 = 0, PREC_FP32R = 1 };
 
+// neighbour-count buckets: 0 (no neighbours), then one per solve width
+// (rows) below; larger counts go to the overflow bucket NBUCKET
+constexpr int NBUCKET = 8;
+constexpr int BUCKET_ROWS[NBUCKET - 1] = {8, 12, 16, 24, 32, 48, 64};
+__host__ __device__ constexpr int bucket_of(int n) {  // 1 <= n <= 64
+  return n <= 8 ? 1 : n <= 12 ? 2 : n <= 16 ? 3 : n <= 24 ? 4 : n <= 32 ? 5 : n <= 48 ? 6 : 7;
+}
+
 // bucket blocks by neighbour count: counts[NB] and lists (device), returns via counts
-constexpr int NBUCKET = 6;  // n==0, <=8, <=16, <=32, <=48, <=64 ; larger -> overflow bucket
 // blocks 0..nblock-1, or the nblock blocks listed in sel; blocks with a
 // negative neighbour count are left out
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st);
 int bucket_width(int b);  // solve width (matrix rows) of bucket b (0 for the empty bucket)
-int solve_width(int n);   // smallest solve width holding n rows: 8, 16, 32, 48 or 64
+int solve_width(int n);   // smallest solve width holding n rows (BUCKET_ROWS)
 
 hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st);

@@ -99,8 +99,9 @@ __device__ inline double pair_cov_tb(const SolveArgs& a, const TabB& Ts, const T
 }
 
 // G lanes per system, R <= G matrix rows (R < G: lanes R..G-1 hold no row;
-// the 64-lane, 48-row instance takes systems of 33 to 48 rows with 48-column
-// rows, a quarter fewer registers and trailing updates than 64)
+// the 16-lane 12-row, 32-lane 24-row and 64-lane 48-row instances take the
+// systems just above the next smaller width with rows a quarter shorter:
+// fewer registers, fold offsets and trailing updates)
 template <int G, int DIM, int PREC, bool TB, int R = G>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
@@ -453,8 +454,7 @@ __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* se
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int b = t < nblock ? (sel ? sel[t] : t) : -1;
   const int n = b >= 0 ? nnear[b] : -1;
-  const int bk = n < 0 ? -1 : n == 0 ? 0 : n <= 8 ? 1 : n <= 16 ? 2 : n <= 32 ? 3 : n <= 48 ? 4 : n <= 64 ? 5
-                                                                                                  : NBUCKET;
+  const int bk = n < 0 ? -1 : n == 0 ? 0 : n > 64 ? NBUCKET : bucket_of(n);
   int local = 0;
 #pragma unroll
   for (int k = 0; k <= NBUCKET; ++k) {
@@ -629,9 +629,9 @@ hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st)
   return hipGetLastError();
 }
 
-int bucket_width(int bk) { return bk == 1 ? 8 : bk == 2 ? 16 : bk == 3 ? 32 : bk == 4 ? 48 : bk == 5 ? 64 : 0; }
+int bucket_width(int bk) { return bk >= 1 && bk < NBUCKET ? BUCKET_ROWS[bk - 1] : 0; }
 
-int solve_width(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : n <= 48 ? 48 : 64; }
+int solve_width(int n) { return BUCKET_ROWS[bucket_of(n) - 1]; }
 
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
                          hipStream_t st) {
@@ -674,7 +674,9 @@ hipError_t KK_CAT(launch_solve_d, KK_SOLVE_DIM)(const SolveArgs& a, int width, i
   if (prec == PREC_FP64) {
     switch (width) {
       case 8: launch_g<8, PREC_FP64>(a, tableB, st); break;
+      case 12: launch_g<16, PREC_FP64, 12>(a, tableB, st); break;
       case 16: launch_g<16, PREC_FP64>(a, tableB, st); break;
+      case 24: launch_g<32, PREC_FP64, 24>(a, tableB, st); break;
       case 32: launch_g<32, PREC_FP64>(a, tableB, st); break;
       case 48: launch_g<64, PREC_FP64, 48>(a, tableB, st); break;
       case 64: launch_g<64, PREC_FP64>(a, tableB, st); break;
@@ -683,7 +685,9 @@ hipError_t KK_CAT(launch_solve_d, KK_SOLVE_DIM)(const SolveArgs& a, int width, i
   } else {
     switch (width) {
       case 8: launch_g<8, PREC_FP32R>(a, tableB, st); break;
+      case 12: launch_g<16, PREC_FP32R, 12>(a, tableB, st); break;
       case 16: launch_g<16, PREC_FP32R>(a, tableB, st); break;
+      case 24: launch_g<32, PREC_FP32R, 24>(a, tableB, st); break;
       case 32: launch_g<32, PREC_FP32R>(a, tableB, st); break;
       case 48: launch_g<64, PREC_FP32R, 48>(a, tableB, st); break;
       case 64: launch_g<64, PREC_FP32R>(a, tableB, st); break;

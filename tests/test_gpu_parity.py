@@ -93,13 +93,14 @@ def test_closed_forms_1d():
     np.testing.assert_allclose(g["variance"], 0.0, atol=1e-10)
 
 
-@pytest.mark.parametrize("ndim,nmax,maxdist", [(2, 40, None), (2, 48, None), (2, 60, 600.0), (3, 44, None),
-                                               (3, 56, None)])
+@pytest.mark.parametrize("ndim,nmax,maxdist", [(2, 10, None), (3, 20, None), (2, 24, None), (2, 40, None),
+                                               (2, 48, None), (2, 60, 600.0), (3, 44, None), (3, 56, None)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32r"])
 @pytest.mark.parametrize("table", [True, False], ids=["table", "analytic"])
 def test_solve_widths(ndim, nmax, maxdist, precision, table):
-    """systems of 33-48 rows (64 lanes, 48-row instance) and 49-64 rows, with
-    the 32-row bucket alongside when maxdist spreads the neighbour counts"""
+    """every solve width: the 12-, 24- and 48-row instances (16, 32 and 64
+    lanes) and the full-width ones; the maxdist case spreads the neighbour
+    counts over every bucket in one solve"""
     kw = dict(maxdist=maxdist) if maxdist else {}
     spec = problems.random_problem(ndim, nobs=800, nq=400, nmax=nmax, seed=40 + nmax, **kw)
     print("case widths", ndim, nmax, maxdist, precision, table, flush=True)
@@ -107,7 +108,10 @@ def test_solve_widths(ndim, nmax, maxdist, precision, table):
     o = run_oracle(spec, table_mode=table)
     assert same_lists(g, o)
     nn = g["nnear"]
-    assert nn.max() > 32 and (nn.max() > 48 or nmax <= 48)
+    if maxdist:
+        assert nn.min() <= 8 and nn.max() > 48
+    else:
+        assert (nn == nmax).all()
     tol = FP64_TOL if precision == "fp64" else FP32R_TOL
     ee, ev = max_errors(g, o)
     assert ee < tol and ev < tol, (ee, ev)
