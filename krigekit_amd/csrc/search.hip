@@ -1367,8 +1367,11 @@ bool exact_search_fits_lds(int k) { return (size_t)k * 12 * 32 <= 64 * 1024; }
 
 hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
   if (a.qlist && exact_search_fits_lds(a.k)) {
-    const size_t lds = (size_t)a.k * 12 * 32;
-    dim3 blk(32), grd((a.nq + 31) / 32);
+    // a few flagged queries: one per workgroup, so none waits on another's
+    // divergent path; many: 32 per workgroup
+    const int lanes = a.nq <= 4096 ? 1 : 32;
+    const size_t lds = (size_t)a.k * 12 * lanes;
+    dim3 blk(lanes), grd((a.nq + lanes - 1) / lanes);
     if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, true>), grd, blk, lds, st, a);
     else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, true>), grd, blk, lds, st, a);
     else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);
