@@ -558,12 +558,14 @@ struct Krige {
   std::vector<double> est_mv, var_mv;  // [nb * nvar], [nb * nvar * nvar]
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // exact search of flagged queries, overlapped with the solve
-  hipEvent_t ev[12] = {nullptr};
+  hipStream_t stream3 = nullptr;  // solve of the first search chunk, overlapped with the second
+  hipEvent_t ev[14] = {nullptr};
   ~Krige() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream3) (void)hipStreamDestroy(stream3);
   }
 };
 
@@ -606,6 +608,7 @@ bool ensure_stream(Krige& K) {
   if (!K.stream) {
     HIPCHK(hipStreamCreateWithFlags(&K.stream, hipStreamNonBlocking), "solve");
     HIPCHK(hipStreamCreateWithFlags(&K.stream2, hipStreamNonBlocking), "solve");
+    HIPCHK(hipStreamCreateWithFlags(&K.stream3, hipStreamNonBlocking), "solve");
     for (auto& e : K.ev) HIPCHK(hipEventCreate(&e), "solve");
   }
   return true;
@@ -741,14 +744,85 @@ bool run_pipeline(Krige& K) {
   }
   if (!K.d_nnear.ensure(sizeof(int) * nb) || !K.d_inear.ensure(sizeof(int) * (size_t)nb * mmax) ||
       !K.d_est.ensure(sizeof(double) * nb) || !K.d_var.ensure(sizeof(double) * nb) ||
-      !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 16) ||
-      !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb + 64)) {
+      !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 32) ||
+      !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb * 2 + 64)) {
     set_error(W, "device allocation failed");
     return false;
   }
+  // ---- per-node solve arguments ------------------------------------------------
+  kk::SolveArgs va{};
+  va.model = K.d_model.as<Model>();
+  va.ndim = nd;
+  va.unbias = K.unbias;
+  va.sk_mean = O.sk_mean;
+  va.bound_lo = K.bounds[0];
+  va.bound_hi = K.bounds[1];
+  va.nsub = 1;
+  va.bcoord = K.d_bcoord.as<double>();
+  // the space-time assembly does not range-scale lags (kriging_st.F90:805)
+  va.rangescale = (K.rangescale.empty() || K.st) ? nullptr : K.d_rs.as<double>();
+  va.localnugget = K.localnugget.empty() ? nullptr : K.d_ln.as<double>();
+  if (K.st) {
+    va.tmodel = K.d_tmodel.as<Model>();
+    va.k_ps = K.k_ps;
+    va.st_cov0 = K.st_cov0;
+  }
+  va.ocoord = K.d_ocoord.as<double>();
+  va.ovalue = K.d_ovalue.as<double>();
+  va.ovar = K.d_ovar.as<double>();
+  va.nnear = K.d_nnear.as<int>();
+  va.inear = K.d_inear.as<int>();
+  va.mmax = mmax;
+  va.pivot_tol = K.pivot_tol;
+  va.est = K.d_est.as<double>();
+  va.var = K.d_var.as<double>();
+  va.flag = K.d_flag.as<int>();
+  if (!K.soff.empty()) {
+    // uniform sub-point count per block (Gauss-Legendre blocks)
+    va.nsub = K.soff[1] - K.soff[0];
+    va.scoord = K.d_scoord.as<double>();
+    va.sweight = K.d_sweight.as<double>();
+    va.base_cov = K.d_basecov.as<double>();
+  }
+  // the inline Level-B kernels need every model on one composite table: for
+  // space-time both the spatial and the temporal model
+  const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
+  // bucket the blocks (all, or the nsel listed in sel) by neighbour count and
+  // solve each bucket on stream s, with lists of capacity cap in their own area
+  float bucket_ms = 0.0f;
+  int nempty = 0;
+  auto bucket_solve = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* lists, int cap) -> bool {
+    HIPCHK(hipEventRecord(K.ev[5], s), W);
+    HIPCHK(hipMemsetAsync(dcounts, 0, sizeof(int) * 16, s), W);
+    HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), sel, nsel, dcounts, lists, cap, s), W);
+    int counts[16];
+    HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
+    HIPCHK(hipEventRecord(K.ev[6], s), W);
+    HIPCHK(hipStreamSynchronize(s), W);
+    float bms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
+    bucket_ms += bms;
+    if (counts[kk::NBUCKET] > 0) {
+      set_error(W, "neighbourhood larger than 64 rows");
+      return false;
+    }
+    nempty += counts[0];
+    kk::SolveArgs v = va;
+    for (int bk = 0; bk < kk::NBUCKET; ++bk) {
+      v.list = lists + (size_t)bk * cap;
+      v.nlist = counts[bk];
+      if (bk == 0) HIPCHK(kk::launch_empty(v, s), W);
+      else HIPCHK(kk::launch_solve(v, kk::bucket_width(bk), K.precision, tb, s), W);
+    }
+    return true;
+  };
   HIPCHK(hipEventRecord(K.ev[0], st), W);
   bool overlap = false;  // exact search of flagged queries runs on stream2
   int nx = 0;            // blocks solved in the second round
+  // with a Morton order and enough blocks the search runs in two chunks, and
+  // the first chunk's blocks are bucketed and solved on stream3 while the
+  // second chunk is searched
+  int split = 0;
   // ---- search -----------------------------------------------------------
   kk::SearchArgs sa{};
   sa.ndim = nd;
@@ -799,13 +873,32 @@ bool run_pipeline(Krige& K) {
       HIPCHK(hipEventRecord(K.ev[8], st), W);
       if (K.search_mode == 2) {
         if (K.reorder && !morton_perm(K)) return false;
-        sa.qlist = K.reorder ? K.d_perm.as<int>() : nullptr;
+        const int* perm = K.reorder ? K.d_perm.as<int>() : nullptr;
+        split = (perm && nb >= (1 << 17)) ? (nb / 2 + 63) / 64 * 64 : 0;
+        sa.qlist = perm;
+        if (split) {
+          sa.nq = split;
+          HIPCHK(kk::launch_search_wave(sa, st), W);
+          HIPCHK(hipEventRecord(K.ev[12], st), W);
+          sa.qlist = perm + split;
+          sa.nq = nb - split;
+        }
         HIPCHK(kk::launch_search_wave(sa, st), W);
         sa.qlist = nullptr;
+        sa.nq = nb;
       } else {
         HIPCHK(kk::launch_search_fast(sa, st), W);
       }
       HIPCHK(hipEventRecord(K.ev[9], st), W);
+      if (split) {
+        // first chunk: decided blocks (flagged ones carry a negative count)
+        // bucketed into the second half of the list area
+        HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[12], 0), W);
+        if (!bucket_solve(K.stream3, K.d_perm.as<int>(), split, K.d_counts.as<int>() + 16,
+                          K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split))
+          return false;
+        HIPCHK(hipEventRecord(K.ev[13], K.stream3), W);
+      }
       if (!K.d_xlist.ensure(sizeof(int) * nb)) {
         set_error(W, "device allocation failed (search flags)");
         return false;
@@ -846,74 +939,18 @@ bool run_pipeline(Krige& K) {
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
   HIPCHK(hipEventRecord(K.ev[2], st), W);
-  // ---- per-node solve ---------------------------------------------------------
-  kk::SolveArgs va{};
-  va.model = K.d_model.as<Model>();
-  va.ndim = nd;
-  va.unbias = K.unbias;
-  va.sk_mean = O.sk_mean;
-  va.bound_lo = K.bounds[0];
-  va.bound_hi = K.bounds[1];
-  va.nsub = 1;
-  va.bcoord = K.d_bcoord.as<double>();
-  // the space-time assembly does not range-scale lags (kriging_st.F90:805)
-  va.rangescale = (K.rangescale.empty() || K.st) ? nullptr : K.d_rs.as<double>();
-  va.localnugget = K.localnugget.empty() ? nullptr : K.d_ln.as<double>();
-  if (K.st) {
-    va.tmodel = K.d_tmodel.as<Model>();
-    va.k_ps = K.k_ps;
-    va.st_cov0 = K.st_cov0;
-  }
-  va.ocoord = K.d_ocoord.as<double>();
-  va.ovalue = K.d_ovalue.as<double>();
-  va.ovar = K.d_ovar.as<double>();
-  va.nnear = K.d_nnear.as<int>();
-  va.inear = K.d_inear.as<int>();
-  va.mmax = mmax;
-  va.pivot_tol = K.pivot_tol;
-  va.est = K.d_est.as<double>();
-  va.var = K.d_var.as<double>();
-  va.flag = K.d_flag.as<int>();
-  if (!K.soff.empty()) {
-    // uniform sub-point count per block (Gauss-Legendre blocks)
-    va.nsub = K.soff[1] - K.soff[0];
-    va.scoord = K.d_scoord.as<double>();
-    va.sweight = K.d_sweight.as<double>();
-    va.base_cov = K.d_basecov.as<double>();
-  }
-  // the inline Level-B kernels need every model on one composite table: for
-  // space-time both the spatial and the temporal model
-  const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
   // ---- bucket by neighbour count, solve per bucket ---------------------------
-  // round 0: every block the search has decided; round 1: the blocks the
-  // exact search decided on the second stream
-  int nempty = 0;
-  float bucket_ms = 0.0f;
+  // round 0: every block the search has decided (the second chunk's, when
+  // the first was solved on stream3); round 1: the blocks the exact search
+  // decided on the second stream
+  const float bucket_ms_first = bucket_ms;  // the first chunk's, before ev[2]
   for (int round = 0; round < (nx > 0 ? 2 : 1); ++round) {
     if (round == 1) HIPCHK(hipStreamWaitEvent(st, K.ev[11], 0), W);
-    HIPCHK(hipEventRecord(K.ev[5], st), W);
-    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-    HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), round ? K.d_xlist.as<int>() : nullptr, round ? nx : nb,
-                             K.d_counts.as<int>(), K.d_lists.as<int>(), nb, st), W);
-    int counts[16];
-    HIPCHK(hipMemcpyAsync(counts, K.d_counts.p, sizeof(int) * 16, hipMemcpyDeviceToHost, st), W);
-    HIPCHK(hipEventRecord(K.ev[6], st), W);
-    HIPCHK(hipStreamSynchronize(st), W);
-    float bms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
-    bucket_ms += bms;
-    if (counts[kk::NBUCKET] > 0) {
-      set_error(W, "neighbourhood larger than 64 rows");
-      return false;
-    }
-    nempty += counts[0];
-    for (int bk = 0; bk < kk::NBUCKET; ++bk) {
-      va.list = K.d_lists.as<int>() + (size_t)bk * nb;
-      va.nlist = counts[bk];
-      if (bk == 0) HIPCHK(kk::launch_empty(va, st), W);
-      else HIPCHK(kk::launch_solve(va, kk::bucket_width(bk), K.precision, tb, st), W);
-    }
+    const int* sel = round ? K.d_xlist.as<int>() : (split ? K.d_perm.as<int>() + split : nullptr);
+    const int nsel = round ? nx : nb - split;
+    if (!bucket_solve(st, sel, nsel, K.d_counts.as<int>(), K.d_lists.as<int>(), nb)) return false;
   }
+  if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[13], 0), W);
   HIPCHK(hipEventRecord(K.ev[3], st), W);
   // ---- FP64 re-solve of flagged FP32 systems, LU fallback ----------------
   int nref = 0, nfb = 0;
@@ -952,7 +989,7 @@ bool run_pipeline(Krige& K) {
   K.timing[0] = ms;
   K.timing[1] = bucket_ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[2], K.ev[3]), W);
-  K.timing[2] = ms - bucket_ms;
+  K.timing[2] = ms - (bucket_ms - bucket_ms_first);
   HIPCHK(hipEventElapsedTime(&ms, K.ev[3], K.ev[4]), W);
   K.timing[3] = ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
