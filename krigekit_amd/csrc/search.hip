@@ -1053,6 +1053,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       if (b < jK || (b == jK && tot <= K)) in_m |= bit(s, j);
       else if (b == jK) und_m |= bit(s, j);
     }
+  // only the first nmax entries of an octant are output; the (nmax+1)-th
+  // matters only for the key-tie check against the nmax-th, so when their
+  // keys differ it is left out unresolved (out: the prefix and tie checks
+  // then come out as they would for either membership)
+#pragma unroll
+  for (int s = 0; s < NOCT; ++s)
+    if ((und_m & bit(s, C - 1)) && ok[s][C - 2] != ok[s][C - 1]) und_m &= ~bit(s, C - 1);
   if (__ballot(und_m != 0)) {
     // walks over the points of the recorded leaves (every point with a key
     // below the final ball) whose box is within some lane's bound
@@ -1144,7 +1151,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
     for (int b = 0; b < 32; ++b) hist[b * 64 + lane] = 0u;
     pick();
-    double b1 = eK;
+    // a lane with nothing undecided takes no leaf into the walk (bound -1)
+    double b1 = und_m ? eK : -1.0;
 #pragma unroll
     for (int z = 0; z < NAMB; ++z) b1 = fmax(b1, amb[z]);
     walk(b1, [&](double sd) {
@@ -1188,7 +1196,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // walk 2: exact ranks for what is still undecided
     if (__ballot(und_m != 0)) {
       pick();
-      double b2 = 0.0;
+      double b2 = -1.0;
 #pragma unroll
       for (int z = 0; z < NAMB; ++z) b2 = fmax(b2, amb[z]);
       walk(b2, [&](double sd) {
