@@ -498,6 +498,19 @@ __device__ inline double box_lb(const double* nd, const double* q) {
   return d;
 }
 
+// squared distance from q to the farthest corner of a node's box
+template <int DIM>
+__device__ inline double box_ub(const double* nd, const double* q) {
+  double d = 0.0;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    const double a = fabs(q[k] - nd[NODE_LO + k]), b = fabs(nd[NODE_HI + k] - q[k]);
+    const double m = a > b ? a : b;
+    d = d + m * m;
+  }
+  return d;
+}
+
 template <int DIM, class Leaf, class Ball>
 __device__ inline void traverse_wave(const Tree& T, const double* q, const double* q0, Ball&& ball,
                                      Leaf&& leaf) {
@@ -1063,23 +1076,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   if (__ballot(und_m != 0)) {
     // walks over the points of the recorded leaves (every point with a key
     // below the final ball) whose box is within some lane's bound
-    auto walk = [&](double bound, auto&& fn) {
+    // A recorded leaf is walked for a lane when its box reaches within the
+    // lane's bound and is not wholly inside the lane's inner radius (inner
+    // > 0: every point of such a leaf has a key below the bin being
+    // resolved, so it is already in the count below that bin; the 1e-6
+    // margin keeps a point whose key would round up to the bin's edge).
+    auto walk = [&](double bound, double inner, auto&& fn) {
       auto leaf = [&](int l0, int mm) {
-        for (int l = l0; l < l0 + mm; l += CHK) {
-          const int m = l0 + mm - l < CHK ? l0 + mm - l : CHK;
-          const double v = lane < DIM * m ? T.rdata[DIM * (size_t)l + lane] : 0.0;
-          for (int p = 0; p < m; ++p) {
-            double sd = 0.0;
+        for (int p = 0; p < mm; ++p) {
+          const int pp = __builtin_amdgcn_readfirstlane(l0 + p);
+          double sd = 0.0;
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(readlane_d(v, DIM * p + k), q[k])));
-            fn(sd);
-          }
+          for (int k = 0; k < DIM; ++k) sd = __dadd_rn(sd, sq(__dsub_rn(T.rdata[DIM * (size_t)pp + k], q[k])));
+          fn(sd);
         }
       };
       if (nleaf <= SLEAFCAP) {
         for (int j = 0; j < nleaf; ++j) {
           const int node = __builtin_amdgcn_readfirstlane(leaves[j]);
-          if (__ballot(box_lb<DIM>(T.nd + (size_t)node * NODE_D, q) <= bound * (1.0 + 1e-12)) == 0) continue;
+          const double* nd = T.nd + (size_t)node * NODE_D;
+          const bool need = box_lb<DIM>(nd, q) <= bound * (1.0 + 1e-12) && !(box_ub<DIM>(nd, q) * (1.0 + 1e-6) < inner);
+          if (__ballot(need) == 0) continue;
           const int* ni = T.ni + (size_t)node * NODE_I;
           leaf(ni[1], ni[2] - ni[1] + 1);
         }
@@ -1138,8 +1155,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     };
     // walk 1: sub-histogram of bin jK (the 5 key bits below the bin's: 1/256
-    // or 1/128 octave, 32-bit counters) plus exact ranks for the first NAMB undecided
-    // distances
+    // or 1/128 octave, 32-bit counters) plus exact ranks for the first NAMB
+    // undecided distances.  Undecided entries lie in bin jK, so with an exact
+    // bin (subok) their rank is cb plus a count over bin jK alone, and leaves
+    // wholly below the bin's lower edge are skipped.
     const int hK = hcount(jK);
     const int cb = tot - hK;  // counted points in bins below jK
     // (bin 0 also holds every smaller key; the top bin is an exact bin when
@@ -1155,18 +1174,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     double b1 = und_m ? eK : -1.0;
 #pragma unroll
     for (int z = 0; z < NAMB; ++z) b1 = fmax(b1, amb[z]);
-    walk(b1, [&](double sd) {
+    const unsigned lkb = base + ((unsigned)jK << bsh);
+    const double inner = (subok && und_m) ? (double)__uint_as_float(lkb) : 0.0;
+    walk(b1, inner, [&](double sd) {
       const float key = (float)sd;
       // unclamped bin: the walk also sees points beyond the bound
-      if (subok && (int)(__float_as_uint(key) >> bsh) - (int)(base >> bsh) == jK)
+      const bool inbin = (int)(__float_as_uint(key) >> bsh) - (int)(base >> bsh) == jK;
+      if (subok && inbin)
         __hip_atomic_fetch_add(&hist[((__float_as_uint(key) >> (bsh - 5)) & 31) * 64 + lane], 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (!subok || inbin) {
 #pragma unroll
-      for (int z = 0; z < NAMB; ++z) {
-        clt[z] += (sd < amb[z]);
-        ceq[z] += (sd == amb[z]);
+        for (int z = 0; z < NAMB; ++z) {
+          clt[z] += (sd < amb[z]);
+          ceq[z] += (sd == amb[z]);
+        }
       }
     });
+    if (subok) {
+#pragma unroll
+      for (int z = 0; z < NAMB; ++z) clt[z] += cb;
+    }
     settle();
     if (subok) {
       int c = cb, jK2 = 31, tot2 = cb;
@@ -1199,7 +1227,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       double b2 = -1.0;
 #pragma unroll
       for (int z = 0; z < NAMB; ++z) b2 = fmax(b2, amb[z]);
-      walk(b2, [&](double sd) {
+      walk(b2, 0.0, [&](double sd) {
 #pragma unroll
         for (int z = 0; z < NAMB; ++z) {
           clt[z] += (sd < amb[z]);
