@@ -205,11 +205,14 @@ int kk_solve_resident(int64_t handle);
 int kk_fetch(int64_t handle);
 /* out[12]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
  * nodes, LU-fallback nodes, queries decided by the exact search kernel, fast
- * search used, fast search kernel ms, exact search kernel ms, reserved.
+ * search launches (0: none), fast search kernel ms (all launches), exact
+ * search kernel ms, LOO-gradient kernel ms.
  * Times come from HIP events on the stream the kernels run on.  When the fast
  * search leaves queries to the exact kernel, that kernel runs on a second
  * stream while the other blocks are solved: it is outside "search" and its
- * wait, if any, is inside "solve". */
+ * wait, if any, is inside "solve".  With more than one fast search launch
+ * (Morton chunks), the blocks of the first chunk are solved on a third stream
+ * while the second is searched, so part of the solve lies inside "search". */
 int kk_get_timing(int64_t handle, double *out);
 /* Leave-one-out cross-validation MSE and its gradient with respect to the
  * variogram parameters (nugget, sill, a_major, a_minor1, a_minor2, azimuth,

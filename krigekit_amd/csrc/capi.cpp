@@ -559,8 +559,10 @@ struct Krige {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // exact search of flagged queries, overlapped with the solve
   hipStream_t stream3 = nullptr;  // solve of the first search chunk, overlapped with the second
+  int* h_counts = nullptr;        // pinned: bucket counts read while the GPU works on
   hipEvent_t ev[14] = {nullptr};
   ~Krige() {
+    if (h_counts) (void)hipHostFree(h_counts);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -609,6 +611,7 @@ bool ensure_stream(Krige& K) {
     HIPCHK(hipStreamCreateWithFlags(&K.stream, hipStreamNonBlocking), "solve");
     HIPCHK(hipStreamCreateWithFlags(&K.stream2, hipStreamNonBlocking), "solve");
     HIPCHK(hipStreamCreateWithFlags(&K.stream3, hipStreamNonBlocking), "solve");
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&K.h_counts), sizeof(int) * 32, hipHostMallocDefault), "solve");
     for (auto& e : K.ev) HIPCHK(hipEventCreate(&e), "solve");
   }
   return true;
@@ -789,19 +792,21 @@ bool run_pipeline(Krige& K) {
   const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
   // bucket the blocks (all, or the nsel listed in sel) by neighbour count and
   // solve each bucket on stream s, with lists of capacity cap in their own area
-  float bucket_ms = 0.0f;
+  float bucket_ms = 0.0f, bucket_ms_first = 0.0f;  // all bucketing; the first chunk's (inside "search")
   int nempty = 0;
-  auto bucket_solve = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* lists, int cap) -> bool {
-    HIPCHK(hipEventRecord(K.ev[5], s), W);
+  // bucketing: counts reset, bucket kernel, counts copied to pinned host
+  // memory, between events e0 and e1 on stream s
+  auto bucket = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* hcounts, int* lists, int cap,
+                    hipEvent_t e0, hipEvent_t e1) -> bool {
+    HIPCHK(hipEventRecord(e0, s), W);
     HIPCHK(hipMemsetAsync(dcounts, 0, sizeof(int) * 16, s), W);
     HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), sel, nsel, dcounts, lists, cap, s), W);
-    int counts[16];
-    HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
-    HIPCHK(hipEventRecord(K.ev[6], s), W);
-    HIPCHK(hipStreamSynchronize(s), W);
-    float bms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
-    bucket_ms += bms;
+    HIPCHK(hipMemcpyAsync(hcounts, dcounts, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
+    HIPCHK(hipEventRecord(e1, s), W);
+    return true;
+  };
+  // once the counts are on the host: one solve launch per bucket on stream s
+  auto solve_buckets = [&](hipStream_t s, const int* counts, int* lists, int cap) -> bool {
     if (counts[kk::NBUCKET] > 0) {
       set_error(W, "neighbourhood larger than 64 rows");
       return false;
@@ -815,6 +820,14 @@ bool run_pipeline(Krige& K) {
       else HIPCHK(kk::launch_solve(v, kk::bucket_width(bk), K.precision, tb, s), W);
     }
     return true;
+  };
+  auto bucket_solve = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* lists, int cap) -> bool {
+    if (!bucket(s, sel, nsel, dcounts, K.h_counts, lists, cap, K.ev[5], K.ev[6])) return false;
+    HIPCHK(hipEventSynchronize(K.ev[6]), W);
+    float bms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
+    bucket_ms += bms;
+    return solve_buckets(s, K.h_counts, lists, cap);
   };
   HIPCHK(hipEventRecord(K.ev[0], st), W);
   bool overlap = false;  // exact search of flagged queries runs on stream2
@@ -879,7 +892,13 @@ bool run_pipeline(Krige& K) {
         if (split) {
           sa.nq = split;
           HIPCHK(kk::launch_search_wave(sa, st), W);
-          HIPCHK(hipEventRecord(K.ev[12], st), W);
+          // the first chunk's decided blocks (flagged ones carry a negative
+          // count) are bucketed right behind its search, into the second half
+          // of the list area: queued behind the next search they would wait
+          // for its waves to retire
+          if (!bucket(st, perm, split, K.d_counts.as<int>() + 16, K.h_counts + 16,
+                      K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split, K.ev[12], K.ev[13]))
+            return false;
           sa.qlist = perm + split;
           sa.nq = nb - split;
         }
@@ -891,13 +910,14 @@ bool run_pipeline(Krige& K) {
       }
       HIPCHK(hipEventRecord(K.ev[9], st), W);
       if (split) {
-        // first chunk: decided blocks (flagged ones carry a negative count)
-        // bucketed into the second half of the list area
-        HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[12], 0), W);
-        if (!bucket_solve(K.stream3, K.d_perm.as<int>(), split, K.d_counts.as<int>() + 16,
-                          K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split))
+        // the first chunk's solves run on stream3 beside the second search
+        HIPCHK(hipEventSynchronize(K.ev[13]), W);
+        HIPCHK(hipEventElapsedTime(&bucket_ms_first, K.ev[12], K.ev[13]), W);
+        bucket_ms += bucket_ms_first;
+        HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[13], 0), W);
+        if (!solve_buckets(K.stream3, K.h_counts + 16, K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split))
           return false;
-        HIPCHK(hipEventRecord(K.ev[13], K.stream3), W);
+        HIPCHK(hipEventRecord(K.ev[12], K.stream3), W);
       }
       if (!K.d_xlist.ensure(sizeof(int) * nb)) {
         set_error(W, "device allocation failed (search flags)");
@@ -910,7 +930,7 @@ bool run_pipeline(Krige& K) {
       exact_list = K.d_xlist.as<int>();
     }
     K.timing[7] = exact_list ? nexact : -1;
-    K.timing[8] = exact_list ? 1.0 : 0.0;  // marks that ev[8]/ev[9] hold the fast kernel
+    K.timing[8] = exact_list ? (split ? 2 : 1) : 0;  // fast-search launches between ev[8] and ev[9]
     // the queries the fast kernel left undecided run on the second stream,
     // overlapped with the solve of every other block
     overlap = exact_list != nullptr && nexact > 0;
@@ -943,14 +963,13 @@ bool run_pipeline(Krige& K) {
   // round 0: every block the search has decided (the second chunk's, when
   // the first was solved on stream3); round 1: the blocks the exact search
   // decided on the second stream
-  const float bucket_ms_first = bucket_ms;  // the first chunk's, before ev[2]
   for (int round = 0; round < (nx > 0 ? 2 : 1); ++round) {
     if (round == 1) HIPCHK(hipStreamWaitEvent(st, K.ev[11], 0), W);
     const int* sel = round ? K.d_xlist.as<int>() : (split ? K.d_perm.as<int>() + split : nullptr);
     const int nsel = round ? nx : nb - split;
     if (!bucket_solve(st, sel, nsel, K.d_counts.as<int>(), K.d_lists.as<int>(), nb)) return false;
   }
-  if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[13], 0), W);
+  if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
   HIPCHK(hipEventRecord(K.ev[3], st), W);
   // ---- FP64 re-solve of flagged FP32 systems, LU fallback ----------------
   int nref = 0, nfb = 0;
@@ -997,7 +1016,7 @@ bool run_pipeline(Krige& K) {
   K.timing[9] = K.timing[10] = 0.0;
   if (K.timing[8] > 0.0) {
     HIPCHK(hipEventElapsedTime(&ms, K.ev[8], K.ev[9]), W);
-    K.timing[9] = ms;
+    K.timing[9] = ms - bucket_ms_first;  // the first chunk's bucketing runs between the two launches
   }
   if (O.need_search) {
     HIPCHK(hipEventElapsedTime(&ms, K.ev[10], K.ev[11]), W);

@@ -292,7 +292,8 @@ class Kriging:
         _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
                     total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]),
-                    exact_search_queries=int(out[7]), fast_search_kernel_ms=out[9],
+                    exact_search_queries=int(out[7]), fast_search_launches=int(out[8]),
+                    fast_search_kernel_ms=out[9],
                     exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11])
 
     GRAD_PARAMS = ("nugget", "sill", "a_major", "a_minor1", "a_minor2", "azimuth", "dip", "plunge")
