@@ -43,9 +43,13 @@ def solve_flops(nnear, c=30, p=1, q=1):
     return float(np.sum(f))
 
 
-def time_path(spec, precision, steps, warmup):
+def time_path(spec, precision, steps, warmup, reorder=1):
+    """reorder 1: the default pipeline (two search chunks, the first chunk's
+    solve overlapping the second search); 2: one search launch, so the solve
+    phase is timed on its own"""
     from krigekit_amd.kriging import device_synchronize, from_spec
     k = from_spec(spec, precision=precision)
+    k.set_search_mode(True, reorder)
     k.solve()  # uploads inputs, first full pass
     for _ in range(warmup):
         k.solve_resident()
@@ -234,6 +238,9 @@ def main():
     nb = len(spec["grid"])
     w64, t64, e64, v64, nn = time_path(spec, "fp64", args.steps, args.warmup)
     w32, t32, e32, v32, _ = time_path(spec, "fp32r", args.steps, args.warmup)
+    # the solves timed apart from the search (no overlapped chunk)
+    _, t64s, _, _, _ = time_path(spec, "fp64", 2, 1, reorder=2)
+    _, t32s, _, _, _ = time_path(spec, "fp32r", 2, 1, reorder=2)
     value = nb / w32
     search_ms = t32["fast_search_kernel_ms"]
     # the search runs as fast_search_launches launches over equal query
@@ -264,7 +271,7 @@ def main():
         "fp64_nodes_per_s": nb / w64,
         "fp64_ms_per_step": w64 * 1e3,
         "speedup_fp32r_vs_fp64": w64 / w32,
-        "solve_kernel_speedup_fp32r_vs_fp64": t64["solve_ms"] / t32["solve_ms"],
+        "solve_kernel_speedup_fp32r_vs_fp64": t64s["solve_ms"] / t32s["solve_ms"],
         "err_fp32r_vs_fp64": {"estimate": rel_stats(e32, e64, nn), "variance": rel_stats(v32, v64, nn),
                               "floor": 1e-4},
         "phase_ms_fp32r": {k: round(v, 4) for k, v in t32.items()},
@@ -284,10 +291,12 @@ def main():
         "solve_roofline": {
             "kernel": "k_solve (assembly + LDL^T + forward solves, FP64)",
             "bound": "valu_fp64",
-            "achieved": flops / (t64["solve_ms"] * 1e-3) / 1e12,
+            "achieved": flops / (t64s["solve_ms"] * 1e-3) / 1e12,
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": flops / (t64["solve_ms"] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "frac": flops / (t64s["solve_ms"] * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "solve_ms": t64s["solve_ms"],
+            "note": "solve timed in a pass with one search launch (no overlapped chunk)",
         },
     }
     if not args.no_loo_gradient:

@@ -197,7 +197,10 @@ int kk_set_options(int64_t handle, int precision, double pivot_tol, int table_mo
 /* mode 2 (default): wave-cooperative two-pass LDS search, mode 1: per-lane
  * two-pass LDS search (both run the exact kernel only for the queries they
  * flag), mode 0: exact-emulation search kernel for every query.
- * reorder = 1: the wave-cooperative search visits queries in Morton order. */
+ * reorder: 0 queries in the given order; 1 (default) Morton order, searched
+ * in two chunks, the first chunk's blocks solved while the second chunk is
+ * searched; 2 Morton order in one search launch, so that search and solve
+ * are timed apart. */
 int kk_set_search_mode(int64_t handle, int mode, int reorder);
 /* solve with inputs and results left resident in HBM (no result copy-back) */
 int kk_solve_resident(int64_t handle);

@@ -548,6 +548,7 @@ struct Krige {
   DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_tmodel, d_ttab;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
+  bool chunked = true;          // with Morton order: two search chunks, the first one's solve overlapped
   bool perm_ok = false;
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
   Model model{};
@@ -887,7 +888,7 @@ bool run_pipeline(Krige& K) {
       if (K.search_mode == 2) {
         if (K.reorder && !morton_perm(K)) return false;
         const int* perm = K.reorder ? K.d_perm.as<int>() : nullptr;
-        split = (perm && nb >= (1 << 17)) ? (nb / 2 + 63) / 64 * 64 : 0;
+        split = (perm && K.chunked && nb >= (1 << 17)) ? (nb / 2 + 63) / 64 * 64 : 0;
         sa.qlist = perm;
         if (split) {
           sa.nq = split;
@@ -2468,8 +2469,10 @@ int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
 }
 
 // mode 0: exact-emulation search kernel for every query; 1: per-lane two-pass
-// search; 2: wave-cooperative two-pass search (default).  reorder: process
-// queries of the wave-cooperative search in Morton order.
+// search; 2: wave-cooperative two-pass search (default).  reorder: 0 query
+// order as given, 1 Morton order in two chunks with the first chunk's solve
+// overlapping the second chunk's search (default), 2 Morton order in one
+// launch (every phase timed on its own).
 int kk_set_search_mode(int64_t h, int mode, int reorder) {
   clear_error();
   Krige* K = checked(h);
@@ -2480,6 +2483,7 @@ int kk_set_search_mode(int64_t h, int mode, int reorder) {
   }
   K->search_mode = mode;
   K->reorder = reorder != 0;
+  K->chunked = reorder == 1;
   K->solved = false;
   return 0;
 }

@@ -321,9 +321,11 @@ class Kriging:
             return mse.value, g, zhat, dz
         return mse.value, g
 
-    def set_search_mode(self, fast=True, reorder: bool = True):
+    def set_search_mode(self, fast=True, reorder=True):
         """fast: True/2 wave-cooperative two-pass search (default), 1 per-lane
-        two-pass search, False/0 exact-emulation kernel for every query."""
+        two-pass search, False/0 exact-emulation kernel for every query.
+        reorder: True/1 Morton order in two overlapped chunks (default), 2
+        Morton order in one launch (phases timed apart), False/0 as given."""
         mode = 2 if fast is True else 0 if fast is False else int(fast)
         _check(_load_lib().kk_set_search_mode(self._handle, mode, int(reorder)), "kk_set_search_mode")
 
