@@ -816,10 +816,10 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 // Anything the reference's visit order could decide -- equal FP64 distances
 // straddling rank k, equal keys at a sector's nmax / nmax+1 cut -- flags the
 // query for the exact kernel.
-// Points of a leaf are read with one vector load (a leaf is contiguous in
-// tree order) and broadcast with readlane.
+// Every lane takes the same point of a leaf (contiguous in tree order), so
+// points are read with uniform (scalar) loads.
 // ---------------------------------------------------------------------------
-constexpr int NBIN = 64;      // eighth-octave bins, 16-bit counters
+constexpr int NBIN = 64;      // bins per lane, 16-bit counters
 // per-lane bin width: key bits >> bsh = exponent and 3 mantissa bits (bsh 20,
 // eighth-octave bins, 8 octaves) or 2 (bsh 21, quarter-octave, 16 octaves);
 // the 32 sub-bins of a bin are the next 5 key bits

@@ -1,9 +1,10 @@
 // Per-node kriging solve on the GPU.
 //
-// One system per group of G lanes (G = 8/16/32/64 by neighbour-count
-// bucket), one matrix row per lane, held in registers:
+// One system per group of G lanes (G = 8/16/32/64) with R <= G matrix rows
+// (by neighbour-count bucket: R = 8, 12, 16, 24, 32, 48 or 64), one matrix
+// row per lane, held in registers:
 //   1. assemble K (kriging.F90:714-849) and b (kriging.F90:864-939): each
-//      lane evaluates G/2 pairs of a balanced "fold" (r, r+d mod G) and the
+//      lane evaluates R/2 pairs of a balanced "fold" (r, r+d mod R) and the
 //      pair values are exchanged through LDS, so every covariance is
 //      evaluated once;
 //   2. factor K = L D L^T right-looking; column k is broadcast through LDS;
