@@ -888,7 +888,9 @@ bool run_pipeline(Krige& K) {
       if (K.search_mode == 2) {
         if (K.reorder && !morton_perm(K)) return false;
         const int* perm = K.reorder ? K.d_perm.as<int>() : nullptr;
-        split = (perm && K.chunked && nb >= (1 << 17)) ? (nb / 2 + 63) / 64 * 64 : 0;
+        // first chunk 60% of the blocks (measured: 11.15 ms at 60/40, about
+        // 11.35 ms at 50/50 and 70/30 on the 3-D workload)
+        split = (perm && K.chunked && nb >= (1 << 17)) ? (int)((size_t)nb * 3 / 5 + 63) / 64 * 64 : 0;
         sa.qlist = perm;
         if (split) {
           sa.nq = split;
