@@ -545,7 +545,7 @@ struct Krige {
   bool model_ok = false;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_tmodel, d_ttab;
+  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_tmodel, d_ttab;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool chunked = true;          // with Morton order: two search chunks, the first one's solve overlapped
@@ -2532,7 +2532,8 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
   }
   if (!do_solve(*K, false)) return ierr();
   const int nb = K->nblock, nd = K->ndim;
-  if (!K->d_gz.ensure(sizeof(double) * nb) || !K->d_gdz.ensure(sizeof(double) * (size_t)nb * kk::NGRAD)) {
+  if (!K->d_gz.ensure(sizeof(double) * nb) || !K->d_gdz.ensure(sizeof(double) * (size_t)nb * kk::NGRAD) ||
+      !K->d_gred.ensure(sizeof(double) * kk::loo_reduce_words())) {
     set_error(W, "device allocation failed");
     return ierr();
   }
@@ -2554,27 +2555,24 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
   HIPCHK_I(hipEventRecord(K->ev[5], st), W);
   HIPCHK_I(kk::launch_loo_grad(ga, st), W);
   HIPCHK_I(hipEventRecord(K->ev[6], st), W);
-  std::vector<double> z(nb), d((size_t)nb * kk::NGRAD);
-  HIPCHK_I(hipMemcpyAsync(z.data(), K->d_gz.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st), W);
-  HIPCHK_I(hipMemcpyAsync(d.data(), K->d_gdz.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, st), W);
+  // squared error and gradient sums on the device; the per-block values
+  // travel only when the caller asks for them
+  constexpr int NS = kk::NGRAD + 2;
+  double sums[NS];
+  HIPCHK_I(kk::launch_loo_reduce(K->d_gz.as<double>(), K->d_gdz.as<double>(), K->d_ovalue.as<double>(), nb,
+                                 K->d_gred.as<double>(), st), W);
+  HIPCHK_I(hipMemcpyAsync(sums, K->d_gred.as<double>() + (kk::loo_reduce_words() - NS), sizeof sums,
+                          hipMemcpyDeviceToHost, st), W);
+  if (zhat) HIPCHK_I(hipMemcpyAsync(zhat, K->d_gz.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st), W);
+  if (dz)
+    HIPCHK_I(hipMemcpyAsync(dz, K->d_gdz.p, sizeof(double) * (size_t)nb * kk::NGRAD, hipMemcpyDeviceToHost, st), W);
   HIPCHK_I(hipStreamSynchronize(st), W);
   float ms = 0.0f;
   HIPCHK_I(hipEventElapsedTime(&ms, K->ev[5], K->ev[6]), W);
   K->timing[11] = ms;
-  const std::vector<double>& val = K->obs[0].value;
-  double s = 0.0, g[kk::NGRAD] = {0.0};
-  int cnt = 0;
-  for (int b = 0; b < nb; ++b) {
-    if (!std::isfinite(z[b])) continue;
-    const double r = z[b] - val[b];
-    s += r * r;
-    for (int p = 0; p < kk::NGRAD; ++p) g[p] += r * d[(size_t)b * kk::NGRAD + p];
-    ++cnt;
-  }
-  *mse = cnt ? s / cnt : std::nan("");
-  for (int p = 0; p < kk::NGRAD; ++p) grad[p] = cnt ? 2.0 * g[p] / cnt : std::nan("");
-  if (zhat) std::memcpy(zhat, z.data(), sizeof(double) * nb);
-  if (dz) std::memcpy(dz, d.data(), sizeof(double) * d.size());
+  const double cnt = sums[NS - 1];
+  *mse = cnt > 0.0 ? sums[0] / cnt : std::nan("");
+  for (int p = 0; p < kk::NGRAD; ++p) grad[p] = cnt > 0.0 ? 2.0 * sums[1 + p] / cnt : std::nan("");
   return 0;
 }
 

@@ -57,6 +57,12 @@ hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
 hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
+// device reduction of the LOO squared error and gradient: work holds
+// loo_reduce_words() doubles; the NGRAD + 2 sums (sum r^2, sum r dz_p, count)
+// are left at its end, work + loo_reduce_words() - (NGRAD + 2)
+int loo_reduce_words();
+hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double* val, int nb, double* work,
+                             hipStream_t st);
 // multi-variable block solve; rows + mv_extra_cols(...) must not exceed 128
 int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);

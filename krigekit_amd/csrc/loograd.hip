@@ -256,6 +256,58 @@ __global__ __launch_bounds__(64) void k_loo_grad(GradArgs a) {
     if (j == p) a.dz[(size_t)b * NGRAD + p] = g[p];
 }
 
+// LOO squared error and gradient sums reduced on the device in a fixed order
+// (no atomics), so repeated evaluations give identical sums: RED_BLOCKS
+// workgroups each sum a strided share of the blocks, then one workgroup adds
+// the partials in workgroup order.  out[0] = sum r^2, out[1 + p] = sum r dz_p,
+// out[NGRAD + 1] = number of finite estimates, r = zhat - z.
+constexpr int RED_BLOCKS = 256, RED_T = 256, RED_N = NGRAD + 2;
+
+__global__ __launch_bounds__(RED_T) void k_loo_reduce(const double* zhat, const double* dz, const double* val,
+                                                      int nb, double* part) {
+  __shared__ double sh[RED_N][RED_T];
+  double acc[RED_N];
+#pragma unroll
+  for (int i = 0; i < RED_N; ++i) acc[i] = 0.0;
+  for (int b = blockIdx.x * RED_T + threadIdx.x; b < nb; b += RED_BLOCKS * RED_T) {
+    const double z = zhat[b];
+    if (!isfinite(z)) continue;
+    const double r = z - val[b];
+    acc[0] += r * r;
+#pragma unroll
+    for (int p = 0; p < NGRAD; ++p) acc[1 + p] += r * dz[(size_t)b * NGRAD + p];
+    acc[NGRAD + 1] += 1.0;
+  }
+#pragma unroll
+  for (int i = 0; i < RED_N; ++i) sh[i][threadIdx.x] = acc[i];
+  __syncthreads();
+  for (int s = RED_T / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int i = 0; i < RED_N; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < RED_N) part[(size_t)blockIdx.x * RED_N + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_loo_reduce_final(const double* part, double* out) {
+  const int i = threadIdx.x;
+  if (i >= RED_N) return;
+  double s = 0.0;
+  for (int k = 0; k < RED_BLOCKS; ++k) s += part[(size_t)k * RED_N + i];
+  out[i] = s;
+}
+
+int loo_reduce_words() { return (RED_BLOCKS + 1) * RED_N; }
+
+hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double* val, int nb, double* work,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_loo_reduce, dim3(RED_BLOCKS), dim3(RED_T), 0, st, zhat, dz, val, nb, work);
+  hipLaunchKernelGGL(k_loo_reduce_final, dim3(1), dim3(64), 0, st, work, work + (size_t)RED_BLOCKS * RED_N);
+  return hipGetLastError();
+}
+
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st) {
   if (a.nb <= 0) return hipSuccess;
   if (a.mmax > GMAX) return hipErrorInvalidValue;
