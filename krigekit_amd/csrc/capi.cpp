@@ -927,7 +927,10 @@ bool run_pipeline(Krige& K) {
         return false;
       }
       HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_xlist.as<int>(), st), W);
+      // in Morton order when the search used it: the exact kernel's lanes
+      // then take neighbouring queries and follow similar tree paths
+      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_xlist.as<int>(), st,
+                                K.search_mode == 2 && K.reorder ? K.d_perm.as<int>() : nullptr), W);
       HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
       HIPCHK(hipStreamSynchronize(st), W);
       exact_list = K.d_xlist.as<int>();

@@ -67,7 +67,9 @@ hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double*
 int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
-hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
-                          hipStream_t st);
+// blocks whose flag equals code into list (count: device counter); with
+// perm, taken in the order perm lists them
+hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list, hipStream_t st,
+                          const int* perm = nullptr);
 
 }  // namespace kk

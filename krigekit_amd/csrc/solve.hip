@@ -478,10 +478,13 @@ __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* se
   if (bk >= 0) lists[(size_t)bk * cap + lbase[bk] + wcount[wv][bk] + local] = b;
 }
 
-__global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// blocks with flag == code into list (atomic count); with perm, in the
+// order perm lists the blocks
+__global__ void k_collect(const int* flag, int nblock, int code, int* count, int* list, const int* perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const bool hit = t < nblock && flag[t] == code;
+  const int t = i < nblock ? (perm ? perm[i] : i) : 0;
+  const bool hit = i < nblock && flag[t] == code;
   const unsigned long long m = __ballot(hit);
   if (m == 0) return;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -643,9 +646,9 @@ hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* coun
 }
 
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list,
-                          hipStream_t st) {
+                          hipStream_t st, const int* perm) {
   hipLaunchKernelGGL(k_collect, dim3((nblock + 255) / 256), dim3(256), 0, st, flag, nblock, code,
-                     count, list);
+                     count, list, perm);
   return hipGetLastError();
 }
 
