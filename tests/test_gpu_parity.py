@@ -161,6 +161,30 @@ def test_resident_solve_and_determinism():
     k.close()
 
 
+def test_pipeline_orders_agree():
+    """the default pipeline (Morton order, two search launches, the first
+    chunk's blocks solved while the second is searched), one Morton-ordered
+    launch, and the given query order: identical lists and bitwise identical
+    results (every block is solved on its own)"""
+    from krigekit_amd import synth
+    spec = synth.ok3d(scale=1.0 / 16)  # 262,144 nodes, above the two-chunk threshold
+    runs = []
+    for reorder in (1, 2, 0):
+        print("case pipeline order", reorder, flush=True)
+        k = make_gpu(spec)
+        k.set_search_mode(True, reorder)
+        k.solve()
+        e, v = k.get_results(copy=True)
+        nn, inn = k.neighbours()
+        k.close()
+        inn = np.where(np.arange(inn.shape[1])[None, :] < nn[:, None], inn, 0)  # valid entries only
+        runs.append((e, v, nn, inn))
+    e0, v0, nn0, inn0 = runs[0]
+    for e, v, nn, inn in runs[1:]:
+        assert np.array_equal(nn, nn0) and np.array_equal(inn, inn0)
+        assert np.array_equal(e, e0, equal_nan=True) and np.array_equal(v, v0, equal_nan=True)
+
+
 def test_full_size_3d_octant():
     """4,194,304-node 3-D octant workload: a strided subset of nodes against
     the oracle (lists bit-exact, 1e-10), the FP32+refine path against FP64 at
