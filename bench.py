@@ -230,7 +230,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
-                 "(nodes shard independently; see DESIGN.md).")
+                 "(nodes are independent: each GPU would take a contiguous range of the Morton-ordered nodes).")
     if args.only:
         out = space_time() if args.only == "space-time" else cokriging()
         print(json.dumps({"workload": args.only, **out}), flush=True)
