@@ -511,6 +511,27 @@ __device__ inline double box_ub(const double* nd, const double* q) {
   return d;
 }
 
+// The deepest node on the query's path that still holds >= k points has them
+// all inside its box, so the k-th squared distance is at most the squared
+// distance to that box's farthest corner (inflated like the box test);
+// huge when the tree holds fewer than k points.
+template <int DIM>
+__device__ inline double kpoint_bound(const Tree& T, const double* q, int K) {
+  int node = T.root;
+  for (;;) {
+    const int* ni = T.ni + (size_t)node * NODE_I;
+    const int left = ni[3], right = ni[4];
+    if (left < 0 || right < 0) break;
+    const int child = (q[ni[0]] < T.nd[(size_t)node * NODE_D]) ? left : right;
+    const int* nc = T.ni + (size_t)child * NODE_I;
+    if (nc[2] - nc[1] + 1 < K) break;
+    node = child;
+  }
+  const int* ni = T.ni + (size_t)node * NODE_I;
+  if (ni[2] - ni[1] + 1 < K) return 1.7976931348623157e308;
+  return box_ub<DIM>(T.nd + (size_t)node * NODE_D, q) * (1.0 + 1e-12);
+}
+
 template <int DIM, class Leaf, class Ball>
 __device__ inline void traverse_wave(const Tree& T, const double* q, const double* q0, Ball&& ball,
                                      Leaf&& leaf) {
@@ -609,8 +630,11 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
 
   // ---- pass 1: k-th smallest FP32 key; records the leaves it visits --------
   // The ball is next(next(max key)) so the recorded leaves cover every point
-  // pass 2 needs (its bound is next(next(f))).
-  double ball = 1.7976931348623157e308;
+  // pass 2 needs (its bound is next(next(f))).  From the start it is capped
+  // by the lane's k-point node bound, inflated by 1e-6 so that the cap also
+  // covers next(next(f)) (f = RN_fp32 of the k-th squared distance).
+  const double ucap = kpoint_bound<DIM>(a.tree, q, a.k) * (1.0 + 1e-6);
+  double ball = ucap;
   int nf = 0, nleaf = 0;
   traverse_wave<DIM>(T, q, q0, [&]() { return ball; }, [&](int l, int u, int node) {
     if (nleaf < LEAFCAP && lane == 0) leaves[nleaf] = node;
@@ -633,7 +657,7 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
           }
         }
         HK(c) = key;
-        if (nf == K) ball = (double)f_next(f_next(HK(1)));
+        if (nf == K) ball = fmin((double)f_next(f_next(HK(1))), ucap);
       } else if (key < HK(1)) {
         int par = 1, ch = 2;
         while (ch <= K) {
@@ -651,7 +675,7 @@ __global__ __launch_bounds__(64) void k_search_wave(SearchArgs a) {
           ch = 2 * par;
         }
         HK(par) = key;
-        ball = (double)f_next(f_next(HK(1)));
+        ball = fmin((double)f_next(f_next(HK(1))), ucap);
       }
     }
   });
@@ -892,36 +916,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   unsigned base = 0;
   bool have_base = false;
   int jK = NBIN - 1, tot = 0, nleaf = 0, nvisit = 0;
-  // before any point is counted: the deepest node on this lane's query path
-  // that still holds >= k points has them all inside its box, so the lane's
-  // k-th squared distance is at most the squared distance to that box's
-  // farthest corner (inflated like the box test).  Points beyond it rank
-  // below k for this lane, so it caps the bound from the start.
-  double ubound = BIG;
-  {
-    int node = T.root;
-    for (;;) {
-      const int* ni = T.ni + (size_t)node * NODE_I;
-      const int left = ni[3], right = ni[4];
-      if (left < 0 || right < 0) break;
-      const int child = (q[ni[0]] < T.nd[(size_t)node * NODE_D]) ? left : right;
-      const int* nc = T.ni + (size_t)child * NODE_I;
-      if (nc[2] - nc[1] + 1 < K) break;
-      node = child;
-    }
-    const int* ni = T.ni + (size_t)node * NODE_I;
-    if (ni[2] - ni[1] + 1 >= K) {
-      const double* nd = T.nd + (size_t)node * NODE_D;
-      double u = 0.0;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        const double dl = fabs(q[k] - nd[NODE_LO + k]), dh = fabs(nd[NODE_HI + k] - q[k]);
-        const double m = dl > dh ? dl : dh;
-        u = u + m * m;
-      }
-      ubound = u * (1.0 + 1e-12);
-    }
-  }
+  // before any point is counted: the lane's k-point node bound.  Points
+  // beyond it rank below k for this lane, so it caps the bound from the start.
+  const double ubound = kpoint_bound<DIM>(T, q, K);
   double ball = ubound;
   // Bin range, set at the first leaf before any point is counted, from mk =
   // the largest key of that leaf:
