@@ -1339,10 +1339,18 @@ bool search_var(Krige& K, int v, hipStream_t st) {
     }
     sa.qflag = D.sflag.as<int>();
     sa.nq = nb;
-    if (K.search_mode == 2) HIPCHK(kk::launch_search_wave(sa, st), W);
-    else HIPCHK(kk::launch_search_fast(sa, st), W);
+    if (K.search_mode == 2) {
+      // queries in Morton order: a wave's queries are close together
+      if (K.reorder && !morton_perm(K)) return false;
+      sa.qlist = K.reorder ? K.d_perm.as<int>() : nullptr;
+      HIPCHK(kk::launch_search_wave(sa, st), W);
+      sa.qlist = nullptr;
+    } else {
+      HIPCHK(kk::launch_search_fast(sa, st), W);
+    }
     HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-    HIPCHK(kk::launch_collect(D.sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, D.xlist.as<int>(), st), W);
+    HIPCHK(kk::launch_collect(D.sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, D.xlist.as<int>(), st,
+                              K.search_mode == 2 && K.reorder ? K.d_perm.as<int>() : nullptr), W);
     HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
     HIPCHK(hipStreamSynchronize(st), W);
     list = D.xlist.as<int>();
