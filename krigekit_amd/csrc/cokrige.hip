@@ -56,6 +56,18 @@ __device__ inline double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// 1 / sqrt(x) for x > 0: the hardware estimate and two Newton steps
+// (relative error e -> 1.5 e^2 per step), a few ulp from the rounded value
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
 // sum over the workgroup; every thread gets the total
 __device__ inline double wg_sum(double v, double* part) {
 #pragma unroll
@@ -260,20 +272,20 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   // Cholesky does, DSCAL by ONE / AJJ) into the broadcast buffer; after one
   // barrier every thread applies A(i, j) -= L(i, c) L(j, c).  The buffer is
   // double-buffered, so the next step's writer cannot overtake a reader.
-  bool ok = true;
+  // A pivot that is not positive sets the failure flag, which is read once
+  // after the loop: the steps after it compute values that are discarded.
 #pragma unroll
   for (int cb = 0; cb < RB; ++cb) {
     for (int cc = 0; cc < 16; ++cc) {
       const int c = 16 * cb + cc;
-      if (c >= n || !ok) break;
+      if (c >= n) break;
       double* vb = vv2[c & 1];
       if (wave == (cc >> 2)) {
         const int owner = (cc & 3) * 16 + cc;  // lane of (tr = cc, tc = cc)
         const double piv = readlane_d(A[cb][cb], owner);
-        if (!(piv > 0.0)) {
-          if (t == owner + 64 * wave) sfail = 1;
-        } else if (tc == cc) {
-          const double rd = 1.0 / sqrt(piv);
+        if (!(piv > 0.0) && t == owner + 64 * wave) sfail = 1;
+        if (tc == cc) {
+          const double rd = rsqrt_nr(piv);
 #pragma unroll
           for (int ra = 0; ra < RB; ++ra) {
             const int i = tr + 16 * ra;
@@ -282,10 +294,6 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
         }
       }
       __syncthreads();
-      if (sfail) {
-        ok = false;
-        break;
-      }
       // trailing update; entries with i <= c or j <= c see a zero factor.
       // Slots whose rows or columns are all <= c, slots past nx and slots
       // above the diagonal are skipped (scalar conditions).
@@ -302,7 +310,8 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
       }
     }
   }
-  if (!ok) {
+  __syncthreads();
+  if (sfail) {
     if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
     if (t < nvar * nvar) a.var[b + (size_t)a.nb * t] = qnan;
     if (t == 0) a.flag[b] = 3;

@@ -54,3 +54,31 @@ def test_full_size_cokriging():
     o = ckrige.solve(sub)
     ee, ev = ck_errors(dict(estimate=e[idx], variance=v[idx]), o)
     assert ee < 1e-9 and ev < 1e-9, (ee, ev)
+
+
+def test_indefinite_coregionalisation_fails_the_block():
+    """a cross sill above the geometric mean of the sills, with collocated
+    data, makes the block's system indefinite: the Cholesky meets a negative
+    pivot (the reference's spotrf returns info > 0, solver.f90:224-243), the
+    block's results are NaN, and without neglect_error the solve reports a
+    singular matrix.  A block elsewhere, whose variables' data lie beyond the
+    range of each other, is solved as usual."""
+    a = np.array([[0.0, 0.0], [30.0, 0.0], [0.0, 30.0]])
+    b1 = a + [10000.0, 0.0]
+    b2 = b1 + [0.0, 500.0]
+    x1, x2 = np.vstack([a, b1]), np.vstack([a, b2])
+    spec = dict(ndim=2, nvar=2, std_ck=True, unbias=1,
+                obs=[dict(coord=x1, value=np.arange(6.0), nmax=3), dict(coord=x2, value=np.arange(6.0)[::-1], nmax=3)],
+                grid=np.array([[10.0, 10.0], [10010.0, 250.0]]),
+                vgm=[dict(ivar=i, jvar=j, vtype="sph", sill=s, a_major=100.0) for (i, j), s in
+                     (((1, 1), 1.0), ((1, 2), 2.0), ((2, 2), 1.0))],
+                search=[dict(ivar=1, sector_search=False), dict(ivar=2, sector_search=False)])
+    print("case indefinite LMC", flush=True)
+    g = run_gpu_ck(spec)
+    assert np.all(np.isnan(g["estimate"][0])) and np.all(np.isnan(g["variance"][0]))
+    assert np.all(np.isfinite(g["estimate"][1])) and np.all(g["variance"][1] >= 0.0)
+    from runners import make_gpu_ck
+    k = make_gpu_ck(spec, neglect_error=False)
+    with pytest.raises(RuntimeError, match="Singular matrix"):
+        k.solve()
+    k.close()
