@@ -27,8 +27,9 @@
 // over the n pivots of K leaves -X K^-1 X' in the trailing block, i.e. minus
 // the Gram matrix of Y = L^-1 X' (c'K^-1 d = y_c' y_d), so no substitution is
 // needed: it gives S, F'K^-1 b, z'K^-1 b, ..., and one thread finishes the
-// small algebra.  One LDS broadcast vector and one barrier per step.  The
-// variable pairs' models are staged in LDS.  FP64 throughout.
+// small algebra.  The factorisation takes four pivots per barrier (a panel
+// factored by the one wave that holds its columns).  The variable pairs'
+// models are staged in LDS.  FP64 throughout.
 #include <hip/hip_runtime.h>
 
 #include "kk_cov.hpp"
@@ -68,6 +69,14 @@ __device__ inline double rsqrt_nr(double x) {
   return y;
 }
 
+// orders one wave's LDS writes before its later LDS reads (and the reverse):
+// a wave's LDS operations complete in issue order, the fence keeps the
+// compiler from moving them across
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // sum over the workgroup; every thread gets the total
 __device__ inline double wg_sum(double v, double* part) {
 #pragma unroll
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   __shared__ double part[4];
 
   __shared__ Model smodels[NV * NV];
-  __shared__ double vv2[2][MVR];      // step broadcast L(:, c), double-buffered
+  __shared__ __align__(16) double vpan[2][MVR][4];  // panel broadcast L(:, c0..c0+3), double-buffered
   __shared__ int sfail;
   const int b = blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
@@ -150,7 +159,7 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int k = 0; k < DIM; ++k) sg[k][t] = ns > 1 ? a.scoord[k + DIM * s] - x0[k] : 0.0;
     sgw[t] = ns > 1 ? a.sweight[s] : 1.0;
   }
-  if (t < MVR) vv2[0][t] = vv2[1][t] = 0.0;
+  for (int w = t; w < 2 * MVR * 4; w += MVT) (&vpan[0][0][0])[w] = 0.0;
   if (t == 0) sfail = 0;
   {
     // the variable pairs' models, read by every covariance evaluation
@@ -266,46 +275,84 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int cb = 0; cb < RB; ++cb) A[ra][cb] = sbuf[cb * MVT + t];
   }
 
-  // ---- Cholesky of the n pivots ------------------------------------------------
-  // step c: the wave holding column class c % 16 reads the pivot from its
-  // owner lane, scales column c by 1 / sqrt(pivot) (as LAPACK's unblocked
-  // Cholesky does, DSCAL by ONE / AJJ) into the broadcast buffer; after one
-  // barrier every thread applies A(i, j) -= L(i, c) L(j, c).  The buffer is
-  // double-buffered, so the next step's writer cannot overtake a reader.
+  // ---- Cholesky of the n pivots, four columns per barrier ------------------------
+  // Columns c0..c0+3 (c0 a multiple of 4) are column classes c0 % 16 .. +3,
+  // which all sit in wave (c0 % 16) / 4.  That wave factors the panel alone:
+  // for each column it reads the pivot from its owner lane, scales the column
+  // by 1 / sqrt(pivot) (as LAPACK's unblocked Cholesky does, DSCAL by
+  // ONE / AJJ) into the panel buffer, and updates the panel's later columns
+  // from it, exchanging through LDS within the wave.  It then clears the
+  // panel's own rows in the buffer, and after one barrier every thread
+  // applies the rank-4 update A(i, j) -= sum_k L(i, c0+k) L(j, c0+k) to the
+  // trailing entries, in the order of the unblocked algorithm.  The buffer is
+  // double-buffered, so the next panel's writer cannot overtake a reader.
   // A pivot that is not positive sets the failure flag, which is read once
   // after the loop: the steps after it compute values that are discarded.
 #pragma unroll
   for (int cb = 0; cb < RB; ++cb) {
-    for (int cc = 0; cc < 16; ++cc) {
-      const int c = 16 * cb + cc;
-      if (c >= n) break;
-      double* vb = vv2[c & 1];
-      if (wave == (cc >> 2)) {
-        const int owner = (cc & 3) * 16 + cc;  // lane of (tr = cc, tc = cc)
-        const double piv = readlane_d(A[cb][cb], owner);
-        if (!(piv > 0.0) && t == owner + 64 * wave) sfail = 1;
-        if (tc == cc) {
+    for (int pp = 0; pp < 4; ++pp) {
+      const int c0 = 16 * cb + 4 * pp;
+      if (c0 >= n) break;
+      const int np = n - c0 < 4 ? n - c0 : 4;  // pivots in this panel
+      const int last = c0 + np - 1;
+      double(*vp)[4] = vpan[pp & 1];
+      if (wave == pp) {
+        const int kl = (t & 63) >> 4;  // this lane's panel column: tc = 4 pp + kl
+        for (int k = 0; k < np; ++k) {
+          const int c = c0 + k;
+          const double piv = readlane_d(A[cb][cb], 16 * k + (c & 15));
+          if (!(piv > 0.0) && (t & 63) == 0) sfail = 1;
           const double rd = rsqrt_nr(piv);
+          if (kl == k) {
+#pragma unroll
+            for (int ra = 0; ra < RB; ++ra) {
+              const int i = tr + 16 * ra;
+              if (16 * ra < nx && i < nx) vp[i][k] = i > c ? A[ra][cb] * rd : 0.0;
+            }
+          }
+          wave_lds_sync();
+          // the panel's later pivot columns; columns past the last pivot
+          // (extra columns) take the whole panel in the trailing update
+          if (kl > k && kl < np) {
+            const double f = vp[c0 + kl][k];  // L(c0 + kl, c)
+#pragma unroll
+            for (int ra = 0; ra < RB; ++ra) {
+              if (16 * ra + 15 <= c || 16 * ra >= nx) continue;
+              A[ra][cb] = fma(-vp[tr + 16 * ra][k], f, A[ra][cb]);
+            }
+          }
+        }
+        wave_lds_sync();
+        // columns past the last pivot: zero factors
+        if (kl >= np) {
 #pragma unroll
           for (int ra = 0; ra < RB; ++ra) {
             const int i = tr + 16 * ra;
-            if (16 * ra < nx && i < nx) vb[i] = i > c ? A[ra][cb] * rd : 0.0;
+            if (16 * ra < nx && i < nx) vp[i][kl] = 0.0;
           }
         }
+        // the panel's pivot rows take no part in the trailing update
+        if ((t & 63) < 16 && (t & 3) < np) vp[c0 + (t & 3)][(t & 15) >> 2] = 0.0;
       }
       __syncthreads();
-      // trailing update; entries with i <= c or j <= c see a zero factor.
-      // Slots whose rows or columns are all <= c, slots past nx and slots
-      // above the diagonal are skipped (scalar conditions).
 #pragma unroll
       for (int ra = 0; ra < RB; ++ra) {
-        if (16 * ra + 15 <= c || 16 * ra >= nx) continue;
-        const int i = tr + 16 * ra;
-        const double li = i < nx ? vb[i] : 0.0;
+        if (16 * ra + 15 <= last || 16 * ra >= nx) continue;
+        const double* li = vp[tr + 16 * ra];
+        const double2 l01 = *reinterpret_cast<const double2*>(li);
+        const double2 l23 = *reinterpret_cast<const double2*>(li + 2);
 #pragma unroll
-        for (int k = 0; k <= ra; ++k) {
-          if (16 * k + 15 <= c) continue;
-          A[ra][k] = fma(-li, vb[tc + 16 * k], A[ra][k]);
+        for (int kk = 0; kk <= ra; ++kk) {
+          if (16 * kk + 15 <= last) continue;
+          const double* lj = vp[tc + 16 * kk];
+          const double2 m01 = *reinterpret_cast<const double2*>(lj);
+          const double2 m23 = *reinterpret_cast<const double2*>(lj + 2);
+          double v = A[ra][kk];
+          v = fma(-l01.x, m01.x, v);
+          v = fma(-l01.y, m01.y, v);
+          v = fma(-l23.x, m23.x, v);
+          v = fma(-l23.y, m23.y, v);
+          A[ra][kk] = v;
         }
       }
     }

@@ -82,3 +82,18 @@ def test_indefinite_coregionalisation_fails_the_block():
     with pytest.raises(RuntimeError, match="Singular matrix"):
         k.solve()
     k.close()
+
+
+@pytest.mark.parametrize("nmax", [(23, 46), (22, 44), (21, 41)], ids=["69rows", "66rows", "62rows"])
+def test_cokriging_row_counts(nmax):
+    """data row counts that are not multiples of the four-pivot panel (the
+    last panel holds 1-3 pivots and the extra columns start inside it),
+    against the oracle"""
+    spec = problems.ck_small(True, True, nb=120, seed=11)
+    for o, m in zip(spec["obs"], nmax):
+        o["nmax"] = m
+    print("case ck rows", nmax, flush=True)
+    g = run_gpu_ck(spec)
+    o = ckrige.solve(spec)
+    ee, ev = ck_errors(g, o)
+    assert ee < 1e-9 and ev < 1e-9, (ee, ev)
