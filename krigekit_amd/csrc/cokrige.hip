@@ -95,11 +95,14 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   __shared__ int svr[MVR];
   __shared__ double sg[DIM][MAXSUB];
   __shared__ double sgw[MAXSUB];
-  __shared__ double sb[2][NV][MVR];   // right-hand sides, two partial sums
+  // right-hand sides as NPART partial sums over the integration nodes
+  constexpr int NPART = NV == 2 ? 4 : 2;
+  __shared__ double sb[NPART][NV][MVR];
   __shared__ double sbuf[RB * MVT];   // assembly: each thread's RB values of one row slot
   __shared__ double gram[MAXE][MAXE];
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
+  __shared__ double partb[4][NV * NV];
 
   __shared__ Model smodels[NV * NV];
   __shared__ __align__(16) double vpan[2][MVR][4];  // panel broadcast L(:, c0..c0+3), double-buffered
@@ -170,13 +173,12 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   }
   __syncthreads();
 
-  // ---- right-hand sides: items (row, target, half of the nodes) ---------------
+  // ---- right-hand sides: items (row, target, part of the nodes) ---------------
   {
-    const int h1 = (ns + 1) / 2;
-    for (int it = t; it < 2 * n * nvar; it += MVT) {
+    for (int it = t; it < NPART * n * nvar; it += MVT) {
       const int h = it / (n * nvar), rq = it - h * n * nvar;
       const int q = rq / n, r = rq - q * n;
-      const int k0 = h ? h1 : 0, k1 = h ? ns : h1;
+      const int k0 = h * ns / NPART, k1 = (h + 1) * ns / NPART;
       const int vr = svr[r];
       double s = 0.0;
       for (int k = k0; k < k1; ++k) {
@@ -194,35 +196,47 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     }
   }
   // ---- block base covariances sum_k sum_l w_k w_l C_ij(g_k - g_l) -------------
+  // Each thread takes node pairs (k, l), k <= l, for every variable pair;
+  // the sums are reduced together: one barrier for all variable pairs.
   if (ns > 1) {
     const int npair = ns * (ns + 1) / 2;
+    double bs[NV][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < NV; ++j) bs[i][j] = 0.0;
+    for (int e = t; e < npair; e += MVT) {
+      // e -> (k, l), k <= l, row-major upper triangle
+      int k = 0, rem = e;
+      while (rem >= ns - k) {
+        rem -= ns - k;
+        ++k;
+      }
+      const int l = k + rem;
+      const double dx = sg[0][k] - sg[0][l];
+      const double dy = DIM > 1 ? sg[1][k] - sg[1][l] : 0.0;
+      const double dz = DIM > 2 ? sg[2][k] - sg[2][l] : 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          if (j < i) continue;
+          if (l == k) bs[i][j] += smodels[i + nvar * j].cov0 * sgw[k] * sgw[k];
+          else bs[i][j] += mv_cov<DIM, SIMPLE>(smodels, i, j, nvar, dx, dy, dz) * sgw[k] * sgw[l] * 2.0;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         if (j < i) continue;
-        const Model& M = smodels[i + nvar * j];
-        double s = 0.0;
-        for (int e = t; e < npair; e += MVT) {
-          // e -> (k, l), k <= l, row-major upper triangle
-          int k = 0, rem = e;
-          while (rem >= ns - k) {
-            rem -= ns - k;
-            ++k;
-          }
-          const int l = k + rem;
-          if (l == k) {
-            s += M.cov0 * sgw[k] * sgw[k];
-          } else {
-            const double dx = sg[0][k] - sg[0][l];
-            const double dy = DIM > 1 ? sg[1][k] - sg[1][l] : 0.0;
-            const double dz = DIM > 2 ? sg[2][k] - sg[2][l] : 0.0;
-            s += mv_cov<DIM, SIMPLE>(smodels, i, j, nvar, dx, dy, dz) * sgw[k] * sgw[l] * 2.0;
-          }
-        }
-        s = wg_sum(s, part);
-        if (t == 0) sbase[i + NV * j] = s;
+        double v = bs[i][j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((t & 63) == 0) partb[wave][i + NV * j] = v;
       }
+    __syncthreads();
+    if (t < NV * NV) sbase[t] = (partb[0][t] + partb[1][t]) + (partb[2][t] + partb[3][t]);
   }
   __syncthreads();
 
@@ -263,7 +277,10 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
       } else if (i >= n && i < nx && j < n) {
         // extra row e = i - n, data column j
         const int e = i - n, vj = svr[j];
-        if (e < ef) v = sb[0][e][j] + sb[1][e][j];
+        if (e < ef) {
+          v = sb[0][e][j] + sb[1][e][j];
+          if (NPART == 4) v += sb[2][e][j] + sb[3][e][j];
+        }
         else if (e < ez) v = (a.std_ck ? (vj == e - ef) : true) ? 1.0 : 0.0;
         else if (e == ez) v = sz[j];
         else if (isa) v = (vj == e - ei) ? 1.0 : 0.0;
