@@ -88,7 +88,7 @@ __device__ inline double wg_sum(double v, double* part) {
 }
 
 template <int DIM, int NV, int RB, bool SIMPLE>
-__global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
+__global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   __shared__ double sx[DIM][MVR];
   __shared__ double sz[MVR];
   __shared__ double sov[MVR];
@@ -98,14 +98,17 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
   // right-hand sides as NPART partial sums over the integration nodes
   constexpr int NPART = NV == 2 ? 4 : 2;
   __shared__ double sb[NPART][NV][MVR];
-  __shared__ double sbuf[RB * MVT];   // assembly: each thread's RB values of one row slot
+  // assembly: each thread's RB values of one row slot; then, for the
+  // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered
+  static_assert(RB * MVT >= 2 * MVR * 4, "panel buffer fits the assembly slots");
+  __shared__ __align__(16) double sbuf[RB * MVT];
+  double(*const vpan)[MVR][4] = reinterpret_cast<double(*)[MVR][4]>(sbuf);
   __shared__ double gram[MAXE][MAXE];
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
   __shared__ double partb[4][NV * NV];
 
   __shared__ Model smodels[NV * NV];
-  __shared__ __align__(16) double vpan[2][MVR][4];  // panel broadcast L(:, c0..c0+3), double-buffered
   __shared__ int sfail;
   const int b = blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
@@ -162,7 +165,6 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
     for (int k = 0; k < DIM; ++k) sg[k][t] = ns > 1 ? a.scoord[k + DIM * s] - x0[k] : 0.0;
     sgw[t] = ns > 1 ? a.sweight[s] : 1.0;
   }
-  for (int w = t; w < 2 * MVR * 4; w += MVT) (&vpan[0][0][0])[w] = 0.0;
   if (t == 0) sfail = 0;
   {
     // the variable pairs' models, read by every covariance evaluation
@@ -291,6 +293,11 @@ __global__ __launch_bounds__(MVT) void k_solve_mv(MVArgs a) {
 #pragma unroll
     for (int cb = 0; cb < RB; ++cb) A[ra][cb] = sbuf[cb * MVT + t];
   }
+  // the panel buffer takes the assembly slots: cleared once every thread has
+  // read its own (rows past nx keep zero factors throughout)
+  __syncthreads();
+  for (int w = t; w < 2 * MVR * 4; w += MVT) sbuf[w] = 0.0;
+  __syncthreads();
 
   // ---- Cholesky of the n pivots, four columns per barrier ------------------------
   // Columns c0..c0+3 (c0 a multiple of 4) are column classes c0 % 16 .. +3,
