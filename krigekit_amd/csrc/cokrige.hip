@@ -89,20 +89,22 @@ __device__ inline double wg_sum(double v, double* part) {
 
 template <int DIM, int NV, int RB, bool SIMPLE>
 __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
-  __shared__ double sx[DIM][MVR];
-  __shared__ double sz[MVR];
-  __shared__ double sov[MVR];
-  __shared__ int svr[MVR];
+  // rows + extra rows fit 16 RB (the instance is chosen by the column count)
+  constexpr int ROWS = 16 * RB;
+  __shared__ double sx[DIM][ROWS];
+  __shared__ double sz[ROWS];
+  __shared__ double sov[ROWS];
+  __shared__ int svr[ROWS];
   __shared__ double sg[DIM][MAXSUB];
   __shared__ double sgw[MAXSUB];
   // right-hand sides as NPART partial sums over the integration nodes
   constexpr int NPART = NV == 2 ? 4 : 2;
-  __shared__ double sb[NPART][NV][MVR];
+  __shared__ double sb[NPART][NV][ROWS];
   // assembly: each thread's RB values of one row slot; then, for the
   // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered
-  static_assert(RB * MVT >= 2 * MVR * 4, "panel buffer fits the assembly slots");
+  static_assert(RB * MVT >= 2 * ROWS * 4, "panel buffer fits the assembly slots");
   __shared__ __align__(16) double sbuf[RB * MVT];
-  double(*const vpan)[MVR][4] = reinterpret_cast<double(*)[MVR][4]>(sbuf);
+  double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
   __shared__ double gram[MAXE][MAXE];
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
@@ -147,15 +149,27 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
 
   // ---- stage rows (coordinates relative to the block centre) and nodes -------
   if (t < n) {
-    int vr = 0;
+    // the row's variable, by selects (a runtime index into the argument
+    // arrays would copy them to scratch)
+    int vr = 0, s0 = 0, mm = a.mmax[0];
+    const int* inr = a.inear[0];
+    const double *ocr = a.ocoord[0], *ovl = a.ovalue[0], *ovr = a.ovar[0];
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
-      if (t >= st[v]) vr = v;
-    const int idx = a.inear[vr][(size_t)b * a.mmax[vr] + (t - st[vr])];
+    for (int v = 1; v < NV; ++v)
+      if (t >= st[v]) {
+        vr = v;
+        s0 = st[v];
+        mm = a.mmax[v];
+        inr = a.inear[v];
+        ocr = a.ocoord[v];
+        ovl = a.ovalue[v];
+        ovr = a.ovar[v];
+      }
+    const int idx = inr[(size_t)b * mm + (t - s0)];
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) sx[k][t] = a.ocoord[vr][k + DIM * (size_t)idx] - x0[k];
-    sz[t] = a.ovalue[vr][idx];
-    sov[t] = a.ovar[vr][idx];
+    for (int k = 0; k < DIM; ++k) sx[k][t] = ocr[k + DIM * (size_t)idx] - x0[k];
+    sz[t] = ovl[idx];
+    sov[t] = ovr[idx];
     svr[t] = vr;
   }
   const int ns = a.nsub;
@@ -296,7 +310,7 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   // the panel buffer takes the assembly slots: cleared once every thread has
   // read its own (rows past nx keep zero factors throughout)
   __syncthreads();
-  for (int w = t; w < 2 * MVR * 4; w += MVT) sbuf[w] = 0.0;
+  for (int w = t; w < 2 * ROWS * 4; w += MVT) sbuf[w] = 0.0;
   __syncthreads();
 
   // ---- Cholesky of the n pivots, four columns per barrier ------------------------
