@@ -75,15 +75,34 @@ __device__ inline double struct_cov_h(const Model& M, double h) {
   return res;
 }
 
-// in-table part of cov_struct_tab_h / comp_cov_tab_h; h in (0, zb[3])
-__device__ inline double table_interp(const Model& M, const double* tab, double h) {
-  int k = (h >= M.zb[1]) + (h >= M.zb[2]);
-  double fi = (h - M.zb[k]) * M.zinv[k];
-  int ifi = (int)fi;
-  int i = M.zoff[k] + min(ifi, M.zn[k] - 1);
-  fi = fi - (double)ifi;
-  double t0 = tab[i], t1 = tab[i + 1];
-  return t0 + fi * (t1 - t0);
+// Table entry index and fraction of h, as cov_struct_tab_h computes them
+// (variogram.f90:682-710): zone k = (h >= .1) + (h >= .5), fi = (h - zb_k) /
+// step_k, index zoff_k + min(int(fi), n_k - 1), fraction fi - int(fi).  The
+// zones are those of TAB_HB, which every table is built on, so they are
+// constants chosen by selects.  The index stays in [0, TAB_N - 1] for every
+// h, NaN and h beyond the table included.
+__device__ inline int tabB_index(double h, double& fr) {
+  constexpr double zb1 = TAB_HB[1], zb2 = TAB_HB[2];
+  constexpr double zi0 = tab_zinv(0), zi1 = tab_zinv(1), zi2 = tab_zinv(2);
+  constexpr int zo0 = tab_zoff(0), zo1 = tab_zoff(1), zo2 = tab_zoff(2);
+  constexpr int zn0 = tab_zn(0), zn1 = tab_zn(1), zn2 = tab_zn(2);
+  const bool k1 = h >= zb1, k2 = h >= zb2;
+  const double zb = k2 ? zb2 : k1 ? zb1 : TAB_HB[0];
+  const double zi = k2 ? zi2 : k1 ? zi1 : zi0;
+  const int zo = k2 ? zo2 : k1 ? zo1 : zo0;
+  const int zn = k2 ? zn2 : k1 ? zn1 : zn0;
+  double fi = (h - zb) * zi;
+  const int ifi = (int)fmin(fi, 1073741824.0);
+  fr = fi - (double)ifi;
+  return max(zo + min(ifi, zn - 1), 0);
+}
+
+// in-table part of cov_struct_tab_h / comp_cov_tab_h; h in (0, 3.5)
+__device__ inline double table_interp_c(const double* tab, double h) {
+  double fr;
+  const int i = tabB_index(h, fr);
+  const double t0 = tab[i], t1 = tab[i + 1];
+  return t0 + fr * (t1 - t0);
 }
 
 // analytic tail beyond the table (h >= 3.5), kept out of line: it is rare
@@ -94,9 +113,9 @@ __device__ __noinline__ double struct_cov_h_tail(const Model* M, double h) { ret
 template <int DIM>
 __device__ inline double cov_tableB(const Model& M, double dx, double dy, double dz) {
   const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
-  if (h >= M.zb[3]) return M.b_tail ? struct_cov_h_tail(&M, h) : 0.0;
+  if (h >= TAB_HB[3]) return M.b_tail ? struct_cov_h_tail(&M, h) : 0.0;
   if (h <= 0.0) return M.tab[0];
-  return table_interp(M, M.tab, h);
+  return table_interp_c(M.tab, h);
 }
 
 // Level-B lookup split into index, load and pick, for loops that evaluate
@@ -118,25 +137,6 @@ __device__ inline TabB tabB_view(const Model& M) {
   T.tail = M.b_tail;
   T.tab = (const __attribute__((address_space(1))) double*)M.tab;
   return T;
-}
-
-// entry index and fraction of h (table_interp's arithmetic on the zones of
-// TAB_HB); the index stays in [0, TAB_N - 1] for every h, NaN and h beyond
-// the table included
-__device__ inline int tabB_index(double h, double& fr) {
-  constexpr double zb1 = TAB_HB[1], zb2 = TAB_HB[2];
-  constexpr double zi0 = tab_zinv(0), zi1 = tab_zinv(1), zi2 = tab_zinv(2);
-  constexpr int zo0 = tab_zoff(0), zo1 = tab_zoff(1), zo2 = tab_zoff(2);
-  constexpr int zn0 = tab_zn(0), zn1 = tab_zn(1), zn2 = tab_zn(2);
-  const bool k1 = h >= zb1, k2 = h >= zb2;
-  const double zb = k2 ? zb2 : k1 ? zb1 : TAB_HB[0];
-  const double zi = k2 ? zi2 : k1 ? zi1 : zi0;
-  const int zo = k2 ? zo2 : k1 ? zo1 : zo0;
-  const int zn = k2 ? zn2 : k1 ? zn1 : zn0;
-  double fi = (h - zb) * zi;
-  const int ifi = (int)fmin(fi, 1073741824.0);
-  fr = fi - (double)ifi;
-  return max(zo + min(ifi, zn - 1), 0);
 }
 
 // C(h) from the two loaded entries.  At h = 0 the index is entry 0 with
@@ -168,10 +168,10 @@ __device__ inline double cov(const Model& M, double dx, double dy, double dz) {
       double h = aniso_h<DIM>(M.mat[iv], dx, dy, dz);
       double c;
       if (M.mode == COV_TABLE_A) {
-        const double* tab = M.tab + (size_t)iv * (M.ntab + 1);
-        if (h >= M.zb[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h(M, iv, h) : 0.0;
+        const double* tab = M.tab + (size_t)iv * (TAB_N + 1);
+        if (h >= TAB_HB[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h(M, iv, h) : 0.0;
         else if (h <= 0.0) c = tab[0];
-        else c = table_interp(M, tab, h);
+        else c = table_interp_c(tab, h);
       } else {
         c = comp_cov_h(M, iv, h);
       }
@@ -216,7 +216,7 @@ template <int DIM>
 __device__ inline double cov_simple(const Model& M, double dx, double dy, double dz) {
   if (M.mode == COV_TABLE_B) {
     const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
-    if (h >= M.zb[3]) {
+    if (h >= TAB_HB[3]) {
       if (!M.b_tail) return 0.0;
       double res = 0.0;
       int iv = 0;
@@ -232,7 +232,7 @@ __device__ inline double cov_simple(const Model& M, double dx, double dy, double
       return res;
     }
     if (h <= 0.0) return M.tab[0];
-    return table_interp(M, M.tab, h);
+    return table_interp_c(M.tab, h);
   }
   double res = 0.0;
   int iv = 0;
@@ -244,10 +244,10 @@ __device__ inline double cov_simple(const Model& M, double dx, double dy, double
       double h = aniso_h<DIM>(M.mat[iv], dx, dy, dz);
       double c;
       if (M.mode == COV_TABLE_A) {
-        const double* tab = M.tab + (size_t)iv * (M.ntab + 1);
-        if (h >= M.zb[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h_simple(M, iv, h) : 0.0;
+        const double* tab = M.tab + (size_t)iv * (TAB_N + 1);
+        if (h >= TAB_HB[3]) c = has_tail(M.vtype[iv]) ? comp_cov_h_simple(M, iv, h) : 0.0;
         else if (h <= 0.0) c = tab[0];
-        else c = table_interp(M, tab, h);
+        else c = table_interp_c(tab, h);
       } else {
         c = comp_cov_h_simple(M, iv, h);
       }
