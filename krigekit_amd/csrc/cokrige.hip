@@ -87,6 +87,10 @@ __device__ inline double wg_sum(double v, double* part) {
   return (part[0] + part[1]) + (part[2] + part[3]);
 }
 
+// The second launch bound is the minimum number of waves per SIMD the
+// registers must allow: five for the 80-column instance (its 25 KB of LDS
+// admits five workgroups per CU; measured faster than four despite a few
+// spills), unconstrained for the 128-column one.
 template <int DIM, int NV, int RB, bool SIMPLE>
 __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   // rows + extra rows fit 16 RB (the instance is chosen by the column count)
