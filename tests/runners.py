@@ -144,10 +144,11 @@ def make_gpu_ck(spec, **kw):
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   sk_mean=o.get("sk_mean", 0.0))
+    rl = dict(rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
     if spec.get("sub_coord") is not None:
-        k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]))
+        k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]), **rl)
     else:
-        k.set_grid(spec["grid"])
+        k.set_grid(spec["grid"], **rl)
     for v in spec["vgm"]:
         k.set_vgm(**v)
     for s in spec["search"]:

@@ -97,3 +97,19 @@ def test_cokriging_row_counts(nmax):
     o = ckrige.solve(spec)
     ee, ev = ck_errors(g, o)
     assert ee < 1e-9 and ev < 1e-9, (ee, ev)
+
+
+@pytest.mark.parametrize("gq", [True, False], ids=["blocks", "points"])
+def test_cokriging_range_scale_and_local_nugget(gq):
+    """per-block range scale (lags divided by it in the matrix and the
+    right-hand sides, not in the block base covariance) and local nugget
+    (added to every data diagonal), against the oracle"""
+    spec = problems.ck_small(True, gq, nb=150, seed=13)
+    rng = np.random.default_rng(14)
+    spec["rangescale"] = rng.uniform(0.7, 1.4, len(spec["grid"]))
+    spec["localnugget"] = rng.uniform(0.0, 0.1, len(spec["grid"]))
+    print("case ck rangescale", gq, flush=True)
+    g = run_gpu_ck(spec)
+    o = ckrige.solve(spec)
+    ee, ev = ck_errors(g, o)
+    assert ee < 1e-9 and ev < 1e-9, (ee, ev)
