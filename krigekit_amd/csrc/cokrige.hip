@@ -102,7 +102,7 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   __shared__ double sg[DIM][MAXSUB];
   __shared__ double sgw[MAXSUB];
   // right-hand sides as NPART partial sums over the integration nodes
-  constexpr int NPART = NV == 2 ? 4 : 2;
+  constexpr int NPART = NV == 2 ? 8 : 2;
   __shared__ double sb[NPART][NV][ROWS];
   // assembly: each thread's RB values of one row slot; then, for the
   // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered
@@ -298,8 +298,15 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
         // extra row e = i - n, data column j
         const int e = i - n, vj = svr[j];
         if (e < ef) {
-          v = sb[0][e][j] + sb[1][e][j];
-          if (NPART == 4) v += sb[2][e][j] + sb[3][e][j];
+          // pairwise sums of the partial sums, in a fixed order
+          double w[NPART];
+#pragma unroll
+          for (int h = 0; h < NPART; ++h) w[h] = sb[h][e][j];
+#pragma unroll
+          for (int step = 1; step < NPART; step *= 2)
+#pragma unroll
+            for (int h = 0; h + step < NPART; h += 2 * step) w[h] += w[h + step];
+          v = w[0];
         }
         else if (e < ez) v = (a.std_ck ? (vj == e - ef) : true) ? 1.0 : 0.0;
         else if (e == ez) v = sz[j];
