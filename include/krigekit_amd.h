@@ -207,7 +207,7 @@ int kk_solve_resident(int64_t handle);
 /* copy the results of the last kk_solve_resident to the host */
 int kk_fetch(int64_t handle);
 /* out[12]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
- * nodes, LU-fallback nodes, queries decided by the exact search kernel, fast
+ * nodes, Bunch-Kaufman fallback nodes, queries decided by the exact search kernel, fast
  * search launches (0: none), fast search kernel ms (all launches), exact
  * search kernel ms, LOO-gradient kernel ms.
  * Times come from HIP events on the stream the kernels run on.  When the fast

@@ -977,7 +977,7 @@ bool run_pipeline(Krige& K) {
   }
   if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
   HIPCHK(hipEventRecord(K.ev[3], st), W);
-  // ---- FP64 re-solve of flagged FP32 systems, LU fallback ----------------
+  // ---- FP64 re-solve of flagged FP32 systems, Bunch-Kaufman fallback -------
   int nref = 0, nfb = 0;
   if (K.precision == kk::PREC_FP32R) {
     HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
@@ -996,8 +996,7 @@ bool run_pipeline(Krige& K) {
   HIPCHK(hipStreamSynchronize(st), W);
   if (nfb > 0) {
     const int fchunk = 4096;
-    const size_t ld = (size_t)mmax + 1;
-    if (!K.d_fb.ensure(sizeof(double) * fchunk * (2 * ld * ld + 3 * ld))) {
+    if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::fallback_words(mmax))) {
       set_error(W, "device allocation failed (fallback scratch)");
       return false;
     }
@@ -1045,9 +1044,10 @@ bool fetch_results(Krige& K) {
   K.var.resize(nb);
   HIPCHK(hipMemcpy(K.est.data(), K.d_est.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
   HIPCHK(hipMemcpy(K.var.data(), K.d_var.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
-  // LU fallback failures: NaN results, counted in n_fail (kriging_base.F90:3369-3383)
+  // fallback failures: NaN estimates, counted in n_fail (kriging_base.F90:3369-3383)
+  std::vector<int> flag;
   if (K.stats[3] > 0) {
-    std::vector<int> flag(nb);
+    flag.resize(nb);
     HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
     int nf = 0;
     long first = -1;
@@ -1068,6 +1068,10 @@ bool fetch_results(Krige& K) {
     K.inear.resize((size_t)nb * K.mmax);
     HIPCHK(hipMemcpy(K.nnear.data(), K.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
     HIPCHK(hipMemcpy(K.inear.data(), K.d_inear.p, sizeof(int) * (size_t)nb * K.mmax, hipMemcpyDeviceToHost), W);
+    // a failed block keeps one neighbour in the store, as the reference sets
+    // ctx%nnear = 1 for it (kriging_base.F90:3369-3383)
+    for (size_t b = 0; b < flag.size(); ++b)
+      if (flag[b] == 3) K.nnear[b] = 1;
   }
   // blocks without neighbours are an error unless neglect_error
   // (kriging_base.F90:3198-3207)
@@ -1495,6 +1499,27 @@ bool do_solve_mv(Krige& K, bool fetch) {
   a.var = K.d_mvvar.as<double>();
   a.flag = K.d_flag.as<int>();
   HIPCHK(kk::launch_solve_mv(a, st), W);
+  // blocks whose K or Schur complement is not positive definite: the
+  // Bunch-Kaufman fallback on the full augmented system
+  int nfb = 0;
+  if (!K.d_lists.ensure(sizeof(int) * (size_t)nb)) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+  HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 2, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
+  HIPCHK(hipMemcpyAsync(&nfb, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  if (nfb > 0) {
+    const int fchunk = 256;
+    if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::mv_fallback_words(rows))) {
+      set_error(W, "device allocation failed (fallback scratch)");
+      return false;
+    }
+    for (int f0 = 0; f0 < nfb; f0 += fchunk)
+      HIPCHK(kk::launch_fallback_mv(a, K.d_lists.as<int>() + f0, std::min(fchunk, nfb - f0), K.d_fb.as<double>(), st),
+             W);
+  }
   HIPCHK(hipEventRecord(K.ev[4], st), W);
   HIPCHK(hipStreamSynchronize(st), W);
   float ms = 0.0f;
@@ -1523,8 +1548,8 @@ bool do_solve_mv(Krige& K, bool fetch) {
     if (flag[b] == 3) {
       ++nfail;
       if (first < 0) first = b + 1;
-    } else if (tot[b] > 0) {
-      ++nsolved;
+    } else if (tot[b] > 1) {
+      ++nsolved;  // a single neighbour is not solved (kriging_base.F90:2127)
     }
     if (tot[b] == 0 && !K.neglect_error) {
       set_error("t_kriging_base%assemble_linear_system", "not enough neighbors for kriging at block", b + 1);
@@ -1532,8 +1557,9 @@ bool do_solve_mv(Krige& K, bool fetch) {
     }
   }
   K.stats[0] = nfail;
-  K.stats[1] = nsolved;
-  K.stats[2] = K.stats[3] = K.stats[4] = 0;
+  K.stats[1] = nsolved - (nfb - nfail);
+  K.stats[2] = K.stats[4] = 0;
+  K.stats[3] = nfb - nfail;
   if (nfail > 0 && !K.neglect_error) {
     set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
     return false;
@@ -2588,7 +2614,7 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
 }
 
 // out[0..11]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
-// FP64 re-solved nodes, LU-fallback nodes, queries decided by the exact
+// FP64 re-solved nodes, symmetric-indefinite fallback nodes, queries decided by the exact
 // search kernel (-1 when the fast search was not used), fast search used
 // (1/0), fast search kernel ms, exact search kernel ms, LOO gradient kernel ms
 int kk_get_timing(int64_t h, double* out) {

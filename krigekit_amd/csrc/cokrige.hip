@@ -32,6 +32,7 @@
 // models are staged in LDS.  FP64 throughout.
 #include <hip/hip_runtime.h>
 
+#include "kk_bk.hpp"
 #include "kk_cov.hpp"
 #include "kk_device.hpp"
 #include "kk_types.hpp"
@@ -408,9 +409,8 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   }
   __syncthreads();
   if (sfail) {
-    if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
-    if (t < nvar * nvar) a.var[b + (size_t)a.nb * t] = qnan;
-    if (t == 0) a.flag[b] = 3;
+    // K not positive definite: the Bunch-Kaufman fallback solves the block
+    if (t == 0) a.flag[b] = 2;
     return;
   }
 
@@ -435,60 +435,68 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   if (t != 0) return;
 
   // ---- multipliers, estimates and variances (one thread) -----------------------
-  // mu_q = S^-1 (G_q - d_q), S = F'K^-1 F, G = F'K^-1 b: Gaussian elimination
-  // with partial pivoting on the small p x p system
+  // mu_q = S^-1 (G_q - d_q), S = F'K^-1 F, G = F'K^-1 b, through the Cholesky
+  // factor S = U'U as the reference's kriging_setup / kriging_solve_prepared
+  // (solver.f90:224-300); a pivot that is not positive (a variable without
+  // neighbours leaves a zero row in S under std_ck) sends the block to the
+  // Bunch-Kaufman fallback like the reference's failed SPOTRF of S
   double mu[NV][NV];  // mu[q][c]
 #pragma unroll
   for (int q = 0; q < NV; ++q)
 #pragma unroll
     for (int c = 0; c < NV; ++c) mu[q][c] = 0.0;
   if (p > 0) {
-    double S[NV][NV], R[NV][NV];
+    double U[NV][NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        S[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
-        const double d = a.std_ck ? (i == j ? 1.0 : 0.0) : 1.0;
-        R[i][j] = (i < p) ? gram[ef + i][eb + j] - d : 0.0;  // row i (constraint), column j (target)
-      }
+      for (int j = 0; j < NV; ++j) U[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
+    bool spd = true;
 #pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      if (c >= p) break;
-      int pr = c;
+    for (int j = 0; j < NV; ++j) {
+      if (j >= p) break;
+      double d = U[j][j];
 #pragma unroll
-      for (int r = 0; r < NV; ++r)
-        if (r > c && r < p && fabs(S[r][c]) > fabs(S[pr][c])) pr = r;
+      for (int k = 0; k < NV; ++k)
+        if (k < j) d -= U[k][j] * U[k][j];
+      spd = spd && d > 0.0;
+      d = sqrt(d);
+      U[j][j] = d;
 #pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        double x = S[c][j];
-        S[c][j] = S[pr][j];
-        S[pr][j] = x;
-        x = R[c][j];
-        R[c][j] = R[pr][j];
-        R[pr][j] = x;
-      }
+      for (int i = 0; i < NV; ++i) {
+        if (i <= j || i >= p) continue;
+        double v = U[j][i];
 #pragma unroll
-      for (int r = 0; r < NV; ++r) {
-        if (r <= c || r >= p) continue;
-        const double f = S[r][c] / S[c][c];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-          S[r][j] -= f * S[c][j];
-          R[r][j] -= f * R[c][j];
-        }
+        for (int k = 0; k < NV; ++k)
+          if (k < j) v -= U[k][j] * U[k][i];
+        U[j][i] = v / d;
       }
     }
+    if (!spd) {
+      a.flag[b] = 2;
+      return;
+    }
 #pragma unroll
-    for (int c = NV - 1; c >= 0; --c) {
-      if (c >= p) continue;
+    for (int q = 0; q < NV; ++q) {
+      double y[NV];
 #pragma unroll
-      for (int q = 0; q < NV; ++q) {
-        double s = R[c][q];
+      for (int i = 0; i < NV; ++i) {
+        if (i >= p) continue;
+        const double d = a.std_ck ? (i == q ? 1.0 : 0.0) : 1.0;
+        double v = gram[ef + i][eb + q] - d;
 #pragma unroll
-        for (int j = 0; j < NV; ++j)
-          if (j > c && j < p) s -= S[c][j] * mu[q][j];
-        mu[q][c] = s / S[c][c];
+        for (int k = 0; k < NV; ++k)
+          if (k < i) v -= U[k][i] * y[k];
+        y[i] = v / U[i][i];
+      }
+#pragma unroll
+      for (int i = NV - 1; i >= 0; --i) {
+        if (i >= p) continue;
+        double v = y[i];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          if (k > i && k < p) v -= U[i][k] * mu[q][k];
+        mu[q][i] = v / U[i][i];
       }
     }
   }
@@ -529,6 +537,161 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
       a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
     }
   a.flag[b] = 0;
+}
+
+// Bunch-Kaufman fallback for the blocks k_solve_mv flags (K or S not positive
+// definite), one thread per block, matrices in HBM scratch: the reference's
+// SSYTRF/SSYTRS on the full augmented system with one jittered retry, then
+// NaN (kriging_base.F90:3299-3383).  The system is assembled in the
+// reference's layout -- data rows by variable group, then the constraint
+// rows -- with the right-hand sides of every target variable
+// (kriging.F90:714-939); the variance and estimate follow calc_variance
+// (kriging.F90:968-1028) and estimate_block_base (kriging_base.F90:3407-3503)
+// in their summation order.
+template <int DIM>
+__global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int b = list[t];
+  const int nvar = a.nvar;
+  const int ld = a.rows + MAXVAR;
+  double* K = scratch + (size_t)t * mv_fallback_words(a.rows);
+  double* A = K + (size_t)ld * ld;
+  double* B = A + (size_t)ld * ld;       // right-hand sides, then solutions
+  double* B0 = B + (size_t)ld * MAXVAR;  // right-hand sides kept for the variance
+  int* ipiv = reinterpret_cast<int*>(B0 + (size_t)ld * MAXVAR);
+  int nv[MAXVAR], st[MAXVAR], n = 0;
+  for (int v = 0; v < MAXVAR; ++v) {
+    nv[v] = v < nvar ? a.nnear[v][b] : 0;
+    st[v] = n;
+    n += nv[v];
+  }
+  const int p = a.unbias ? (a.std_ck ? nvar : 1) : 0, m = n + p;
+  const double rs = a.rangescale ? a.rangescale[b] : 1.0;
+  const double ln = a.localnugget ? a.localnugget[b] : 0.0;
+  const int ns = a.nsub;
+  const double* x0 = a.bcoord + DIM * (size_t)b;
+  // row r: variable, observation index
+  auto rowvar = [&](int r) {
+    int v = 0;
+    for (int u = 1; u < nvar; ++u)
+      if (r >= st[u]) v = u;
+    return v;
+  };
+  auto rowx = [&](int r, int v) { return a.ocoord[v] + DIM * (size_t)a.inear[v][(size_t)b * a.mmax[v] + (r - st[v])]; };
+  for (int c = 0; c < n; ++c) {
+    const int vc = rowvar(c);
+    const double* xc = rowx(c, vc);
+    const int ic = a.inear[vc][(size_t)b * a.mmax[vc] + (c - st[vc])];
+    K[c + ld * c] = (a.models[vc + nvar * vc].cov0 + a.ovar[vc][ic]) + ln;
+    for (int r = 0; r < c; ++r) {
+      const int vr = rowvar(r);
+      const double* xr = rowx(r, vr);
+      const double dx = (xr[0] - xc[0]) / rs;
+      const double dy = DIM > 1 ? (xr[1] - xc[1]) / rs : 0.0;
+      const double dz = DIM > 2 ? (xr[2] - xc[2]) / rs : 0.0;
+      const double v = mv_cov<DIM, false>(a.models, vc, vr, nvar, dx, dy, dz);  // vgm(max, min)
+      K[r + ld * c] = v;
+      K[c + ld * r] = v;
+    }
+    for (int f = 0; f < p; ++f) {
+      const double e = a.std_ck ? (vc == f ? 1.0 : 0.0) : 1.0;
+      K[(n + f) + ld * c] = e;
+      K[c + ld * (n + f)] = e;
+    }
+    // right-hand sides: sum over the integration nodes of w_k C_{vgm(v_c, q)}
+    for (int q = 0; q < nvar; ++q) {
+      double tmp = 0.0;
+      for (int k = 0; k < ns; ++k) {
+        const double* g = ns > 1 ? a.scoord + DIM * ((size_t)b * ns + k) : x0;
+        const double w = ns > 1 ? a.sweight[(size_t)b * ns + k] : 1.0;
+        const double dx = (xc[0] - g[0]) / rs;
+        const double dy = DIM > 1 ? (xc[1] - g[1]) / rs : 0.0;
+        const double dz = DIM > 2 ? (xc[2] - g[2]) / rs : 0.0;
+        tmp = tmp + mv_cov<DIM, false>(a.models, vc, q, nvar, dx, dy, dz) * w;
+      }
+      B[c + ld * q] = tmp;
+    }
+  }
+  for (int f = 0; f < p; ++f) {
+    for (int g = 0; g < p; ++g) K[(n + f) + ld * (n + g)] = 0.0;
+    for (int q = 0; q < nvar; ++q) B[(n + f) + ld * q] = a.std_ck ? (f == q ? 1.0 : 0.0) : 1.0;
+  }
+  for (int q = 0; q < nvar; ++q)
+    for (int r = 0; r < m; ++r) B0[r + ld * q] = B[r + ld * q];
+  const int info = bk_fallback(K, A, ipiv, n, m, ld, B, nvar);
+  if (info) {
+    // x = NaN: NaN estimates, variances max(NaN, 0) = 0 (kriging.F90:1019-1025)
+    for (int q = 0; q < nvar; ++q) a.est[b + (size_t)a.nb * q] = __builtin_nan("");
+    for (int q = 0; q < nvar * nvar; ++q) a.var[b + (size_t)a.nb * q] = 0.0;
+    a.flag[b] = 3;
+    return;
+  }
+  // variances: base_ij - x_i . rhs_j over all m rows, i <= j, clamped at 0
+  for (int i = 0; i < nvar; ++i)
+    for (int j = i; j < nvar; ++j) {
+      const Model& Mij = a.models[i + nvar * j];
+      double base = Mij.cov0;
+      if (ns > 1) {
+        base = 0.0;
+        for (int k = 0; k < ns; ++k) {
+          const double* gk = a.scoord + DIM * ((size_t)b * ns + k);
+          const double wk = a.sweight[(size_t)b * ns + k];
+          base = base + Mij.cov0 * wk * wk;
+          for (int l = k + 1; l < ns; ++l) {
+            const double* gl = a.scoord + DIM * ((size_t)b * ns + l);
+            const double dx = gk[0] - gl[0];
+            const double dy = DIM > 1 ? gk[1] - gl[1] : 0.0;
+            const double dz = DIM > 2 ? gk[2] - gl[2] : 0.0;
+            base = base + mv_cov<DIM, false>(a.models, i, j, nvar, dx, dy, dz) * wk * a.sweight[(size_t)b * ns + l] * 2.0;
+          }
+        }
+      }
+      double dot = 0.0;
+      for (int r = 0; r < m; ++r) dot = dot + B[r + ld * i] * B0[r + ld * j];
+      double v = base - dot;
+      v = v > 0.0 ? v : 0.0;
+      a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
+      a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
+    }
+  // estimates: per target q, the groups in order (Isaaks-Srivastava shift for
+  // non-standard cokriging, simple-kriging mean term), clipped to the bounds
+  const bool isa = !a.std_ck && a.unbias && nvar > 1;
+  double tm[MAXVAR];
+  for (int v = 0; v < nvar; ++v) {
+    tm[v] = 0.0;
+    if (!isa) continue;
+    for (int k = 0; k < nv[v]; ++k) tm[v] = tm[v] + a.ovalue[v][a.inear[v][(size_t)b * a.mmax[v] + k]];
+    if (nv[v] > 0) tm[v] = tm[v] / nv[v];
+  }
+  for (int q = 0; q < nvar; ++q) {
+    double avg = 0.0, twsum = 0.0;
+    for (int v = 0; v < nvar; ++v) {
+      double tw = 0.0, dp = 0.0;
+      for (int k = 0; k < nv[v]; ++k) {
+        const double w = B[(st[v] + k) + ld * q];
+        tw = tw + w;
+        dp = dp + a.ovalue[v][a.inear[v][(size_t)b * a.mmax[v] + k]] * w;
+      }
+      avg = avg + dp;
+      if (isa && v != q && nv[v] > 0) avg = avg + (tm[q] - tm[v]) * tw;
+      twsum = twsum + tw;
+    }
+    if (!a.unbias && a.sk_mean[q] != 0.0) avg = avg + (1.0 - twsum) * a.sk_mean[q];
+    avg = avg < a.bound_lo ? a.bound_lo : avg;
+    avg = avg > a.bound_hi ? a.bound_hi : avg;
+    a.est[b + (size_t)a.nb * q] = avg;
+  }
+  a.flag[b] = 0;
+}
+
+hipError_t launch_fallback_mv(const MVArgs& a, const int* list, int nlist, double* scratch, hipStream_t st) {
+  if (nlist <= 0) return hipSuccess;
+  dim3 g((nlist + 63) / 64), blk(64);
+  if (a.ndim == 1) hipLaunchKernelGGL(k_fallback_mv<1>, g, blk, 0, st, a, list, nlist, scratch);
+  else if (a.ndim == 2) hipLaunchKernelGGL(k_fallback_mv<2>, g, blk, 0, st, a, list, nlist, scratch);
+  else hipLaunchKernelGGL(k_fallback_mv<3>, g, blk, 0, st, a, list, nlist, scratch);
+  return hipGetLastError();
 }
 
 int mv_extra_cols(int nvar, int std_ck, int unbias) {

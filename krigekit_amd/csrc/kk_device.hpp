@@ -54,7 +54,11 @@ int solve_width(int n);   // smallest solve width holding n rows (BUCKET_ROWS)
 
 hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
+// Bunch-Kaufman fallback: scratch holds fallback_words(mmax) doubles per listed block
 hipError_t launch_fallback(const SolveArgs& a, double* scratch, hipStream_t st);
+__host__ __device__ constexpr size_t fallback_words(int mmax) {
+  return 2 * (size_t)(mmax + 1) * (mmax + 1) + 3 * (size_t)(mmax + 1);
+}
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
 // device reduction of the LOO squared error and gradient: work holds
@@ -66,6 +70,13 @@ hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double*
 // multi-variable block solve; rows + mv_extra_cols(...) must not exceed 128
 int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
+// Bunch-Kaufman fallback of the blocks listed (flag 2 after launch_solve_mv):
+// scratch holds mv_fallback_words(a.rows) doubles per listed block; flag 0
+// when solved, 3 when the system stays singular (NaN estimates)
+hipError_t launch_fallback_mv(const MVArgs& a, const int* list, int nlist, double* scratch, hipStream_t st);
+__host__ __device__ constexpr size_t mv_fallback_words(int rows) {
+  return 2 * (size_t)(rows + MAXVAR) * (rows + MAXVAR) + 2 * (size_t)(rows + MAXVAR) * MAXVAR + (rows + MAXVAR);
+}
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
 // blocks whose flag equals code into list (count: device counter); with
 // perm, taken in the order perm lists them

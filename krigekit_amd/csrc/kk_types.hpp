@@ -139,7 +139,7 @@ struct MVArgs {
   int mmax[MAXVAR];
   double* est;                // est[b + nb * q]
   double* var;                // var[b + nb * (i + nvar * j)]
-  int* flag;                  // 3: factorisation failed (NaN results)
+  int* flag;                  // 2: K or S not positive definite (fallback); 3: fallback failed (NaN)
 };
 
 struct SolveArgs {

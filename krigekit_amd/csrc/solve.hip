@@ -18,12 +18,13 @@
 // K^-1 F once against the FP64 matrix (residual in FP64).  A system whose
 // FP32 pivot ratio D_k / K_kk falls below pivot_tol is flagged and solved
 // again by the FP64 kernel.  A system whose FP64 pivot is not positive is
-// flagged for the LU fallback (the reference's SSYTRF fallback,
+// flagged for the Bunch-Kaufman fallback (the reference's SSYTRF fallback,
 // kriging_base.F90:3299-3366).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
+#include "kk_bk.hpp"
 #include "kk_cov.hpp"
 #include "kk_device.hpp"
 #include "kk_types.hpp"
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     return;
   }
   if (fail & 1) {
-    a.flag[b] = 2;  // not positive definite: LU fallback
+    a.flag[b] = 2;  // not positive definite: Bunch-Kaufman fallback
     return;
   }
   if (fail & 4) {
@@ -495,10 +496,11 @@ __global__ void k_collect(const int* flag, int nblock, int code, int* count, int
   if (hit) list[base + __popcll(m & below)] = t;
 }
 
-// Dense LU with partial pivoting on the full augmented system, one thread per
-// system, matrix in HBM scratch: the reference's fallback when Cholesky fails
-// (SSYTRF, then one retry with the data diagonal raised by 1e-6*mean(diag),
-// then NaN; kriging_base.F90:3299-3383).
+// The reference's fallback when the Cholesky path fails, one thread per
+// system with the matrices in HBM scratch: Bunch-Kaufman factorisation of the
+// full augmented system (SSYTRF/SSYTRS), one retry with the data diagonal
+// raised by 1e-6*mean(diag) when it is exactly singular, then NaN
+// (kriging_base.F90:3299-3383).
 template <int DIM>
 __device__ inline double fb_cov(const SolveArgs& a, const Model& M, const double* xi, const double* xj, double rs) {
   const double dx = (xi[0] - xj[0]) / rs, dy = DIM >= 2 ? (xi[1] - xj[1]) / rs : 0.0;
@@ -517,10 +519,11 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   const int n = a.nnear[b];
   const int p = a.unbias ? 1 : 0, m = n + p;
   const int ld = a.mmax + 1;
-  double* K = scratch + (size_t)t * (2 * ld * ld + 3 * ld);
+  double* K = scratch + (size_t)t * fallback_words(a.mmax);
   double* A = K + ld * ld;
   double* rhs = A + ld * ld;
   double* xs = rhs + ld;
+  int* ipiv = reinterpret_cast<int*>(xs + ld);
   const int* in = a.inear + (size_t)b * a.mmax;
   const double rs = a.rangescale ? a.rangescale[b] : 1.0;
   const double ln = a.localnugget ? a.localnugget[b] : 0.0;
@@ -551,58 +554,14 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
     K[n + ld * n] = 0.0;
     rhs[n] = 1.0;
   }
-  int info = 1;
-  for (int attempt = 0; attempt < 2 && info; ++attempt) {
-    if (attempt == 1) {
-      double ds = 0.0;
-      for (int i = 0; i < n; ++i) ds = ds + K[i + ld * i];
-      if (!(ds > 0.0)) break;
-      ds = 1.0e-6 * (ds / (double)n);
-      for (int i = 0; i < n; ++i) K[i + ld * i] = K[i + ld * i] + ds;
-    }
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) A[i + ld * j] = K[i + ld * j];
-    for (int i = 0; i < m; ++i) xs[i] = rhs[i];
-    info = 0;
-    for (int k = 0; k < m && !info; ++k) {
-      int pv = k;
-      double mx = fabs(A[k + ld * k]);
-      for (int i = k + 1; i < m; ++i)
-        if (fabs(A[i + ld * k]) > mx) {
-          mx = fabs(A[i + ld * k]);
-          pv = i;
-        }
-      if (mx == 0.0) {
-        info = k + 1;
-        break;
-      }
-      if (pv != k) {
-        for (int j = 0; j < m; ++j) {
-          const double tt = A[k + ld * j];
-          A[k + ld * j] = A[pv + ld * j];
-          A[pv + ld * j] = tt;
-        }
-        const double tt = xs[k];
-        xs[k] = xs[pv];
-        xs[pv] = tt;
-      }
-      for (int i = k + 1; i < m; ++i) {
-        const double fct = A[i + ld * k] / A[k + ld * k];
-        for (int j = k + 1; j < m; ++j) A[i + ld * j] -= fct * A[k + ld * j];
-        xs[i] -= fct * xs[k];
-      }
-    }
-    if (!info)
-      for (int i = m - 1; i >= 0; --i) {
-        double v = xs[i];
-        for (int j = i + 1; j < m; ++j) v -= A[i + ld * j] * xs[j];
-        xs[i] = v / A[i + ld * i];
-      }
-  }
+  for (int i = 0; i < m; ++i) xs[i] = rhs[i];
+  const int info = bk_fallback(K, A, ipiv, n, m, ld, xs, 1);
   a.flag[b] = info ? 3 : 0;
   if (info) {
+    // x = NaN (kriging_base.F90:3369-3383): NaN estimate; the variance is
+    // max(NaN, 0) = 0 after calc_variance's clamp (kriging.F90:1019-1025)
     a.est[b] = __builtin_nan("");
-    a.var[b] = __builtin_nan("");
+    a.var[b] = 0.0;
     return;
   }
   double dot = 0.0, est = 0.0, tw = 0.0;

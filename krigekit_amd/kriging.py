@@ -291,7 +291,7 @@ class Kriging:
         out = np.zeros(12)
         _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
-                    total_ms=out[4], fp64_resolved=int(out[5]), lu_fallback=int(out[6]),
+                    total_ms=out[4], fp64_resolved=int(out[5]), bk_fallback=int(out[6]),
                     exact_search_queries=int(out[7]), fast_search_launches=int(out[8]),
                     fast_search_kernel_ms=out[9],
                     exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11])

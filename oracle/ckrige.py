@@ -27,10 +27,16 @@ Follows the reference's per-block computation for nvar > 1:
     multipliers, base = cov0 (points) or the double sum over integration
     nodes (lag not range-scaled); symmetrised; diagonal and off-diagonal
     clamped at 0.
-The bordered system is solved with numpy.linalg.solve (any exact solver gives
-the reference's solution to rounding).
+The bordered system is solved as the reference solves it (solver.f90:224-300,
+kriging_base.F90:3244-3398): Cholesky K = U'U, K^-1 F, the Schur complement
+S = F'K^-1 F and its Cholesky factor, mu = S^-1 (F'K^-1 b - d),
+w = K^-1 b - K^-1 F mu; when either factorisation meets a pivot that is not
+positive, the Bunch-Kaufman factorisation of the full augmented system with
+one jittered retry (okrige.bk_fallback, the C restatement of SSYTRF/SSYTRS),
+and NaN weights when that fails too (estimate NaN, variance max(NaN, 0) = 0).
 """
 import numpy as np
+import scipy.linalg as sla
 
 from oracle import okrige
 
@@ -62,6 +68,24 @@ def neighbour_lists(spec, nthread=8):
         o = okrige.solve(sv, table_mode=True, nthread=nthread)
         out.append((o["nnear"], o["inear"] - 1))
     return out
+
+
+def _solve_like_reference(K, R, n, p):
+    """solution of the augmented system [K F; F' 0] x = R as the reference
+    computes it, or None when it fails (NaN weights)"""
+    try:
+        U = sla.cholesky(K[:n, :n], lower=False, check_finite=False)
+        xb = sla.cho_solve((U, False), R[:n], check_finite=False)
+        if p:
+            F = K[:n, n:]
+            kf = sla.cho_solve((U, False), F, check_finite=False)
+            Us = sla.cholesky(F.T @ kf, lower=False, check_finite=False)
+            lam = sla.cho_solve((Us, False), kf.T @ R[:n] - R[n:], check_finite=False)
+            return np.vstack([xb - kf @ lam, lam])
+        return xb
+    except np.linalg.LinAlgError:
+        info, x = okrige.bk_fallback(K, n, R)
+        return None if info else x
 
 
 def solve(spec, table_mode=True, nthread=8):
@@ -142,7 +166,11 @@ def solve(spec, table_mode=True, nthread=8):
             # the reference skips the solve of a one-neighbour block
             # (kriging_base.F90:2127): NaN, as the library reports
             continue
-        x = np.linalg.solve(K, R)
+        x = _solve_like_reference(K, R, n, p)
+        if x is None:
+            est[b] = np.nan
+            var[b] = 0.0
+            continue
         w = x[:n]
         for q in range(nvar):
             e = w[:, q] @ zr

@@ -693,37 +693,187 @@ static void chol_solve(const double *U, int n, int ld, double *b) {
     b[i] = v / U[i + ld * i];
   }
 }
-/* LU with partial pivoting for the full augmented system (stands in for the
- * Bunch-Kaufman SSYTRF/SSYTRS fallback, solver.f90:337-406: both factor the
- * same symmetric indefinite matrix and fail only on an exactly zero pivot). */
-static int lu_solve(double *A, int n, double *b) {
-  int *piv = (int *)malloc(sizeof(int) * n);
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    double mx = fabs(A[k + n * k]);
-    for (int i = k + 1; i < n; ++i)
-      if (fabs(A[i + n * k]) > mx) { mx = fabs(A[i + n * k]); p = i; }
-    piv[k] = p;
-    if (mx == 0.0) { free(piv); return k + 1; }
-    if (p != k) {
-      for (int j = 0; j < n; ++j) { double t = A[k + n * j]; A[k + n * j] = A[p + n * j]; A[p + n * j] = t; }
-      double t = b[k]; b[k] = b[p]; b[p] = t;
-    }
-    for (int i = k + 1; i < n; ++i) {
-      double f = A[i + n * k] / A[k + n * k];
-      A[i + n * k] = f;
-      for (int j = k + 1; j < n; ++j) A[i + n * j] -= f * A[k + n * j];
-      b[i] -= f * b[k];
-    }
+/* Bunch-Kaufman factorisation and solve of the full augmented system: the
+ * reference's SSYTRF/SSYTRS fallback (solver.f90:337-406), restated from the
+ * vendored LAPACK, src/libkriging/lapack.f: SSYTF2 upper (:6213-6445; SSYTRF
+ * runs it for every system of up to 64 rows, :6811-6900), ISAMAX (:669-720),
+ * SSYR, SGER, SGEMV and SSCAL in their reference loop orders, and SSYTRS
+ * upper (:7478-7600).  Column-major, leading dimension ld, 0-based indices;
+ * ipiv holds LAPACK's 1-based codes (k+1 for a 1x1 pivot, -(kp+1) on both
+ * columns of a 2x2 pivot).  The return value is INFO: 0, or the 1-based
+ * column whose pivot block is exactly zero. */
+static int iamax(int n, const double *x, int inc) {
+  int im = 0;
+  double mx = fabs(x[0]);
+  for (int i = 1; i < n; ++i) {
+    double v = fabs(x[(size_t)i * inc]);
+    if (v > mx) { mx = v; im = i; }
   }
-  for (int i = n - 1; i >= 0; --i) {
-    double v = b[i];
-    for (int j = i + 1; j < n; ++j) v -= A[i + n * j] * b[j];
-    b[i] = v / A[i + n * i];
-  }
-  free(piv);
-  return 0;
+  return im;
 }
+static int bk_factor(double *A, int n, int ld, int *ipiv) {
+  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+  int info = 0;
+  int k = n - 1;
+  while (k >= 0) {
+    int kstep = 1, kp, imax = 0;
+    double absakk = fabs(A[k + ld * k]), colmax = 0.0;
+    if (k > 0) {
+      imax = iamax(k, A + (size_t)ld * k, 1);
+      colmax = fabs(A[imax + ld * k]);
+    }
+    double mx = absakk > colmax ? absakk : colmax;
+    if (mx == 0.0 || isnan(absakk)) {
+      if (info == 0) info = k + 1;
+      kp = k;
+    } else {
+      if (absakk >= alpha * colmax) {
+        kp = k;
+      } else {
+        int jmax = imax + 1 + iamax(k - imax, A + imax + (size_t)ld * (imax + 1), ld);
+        double rowmax = fabs(A[imax + ld * jmax]);
+        if (imax > 0) {
+          jmax = iamax(imax, A + (size_t)ld * imax, 1);
+          double v = fabs(A[jmax + ld * imax]);
+          rowmax = rowmax > v ? rowmax : v;
+        }
+        if (absakk >= alpha * colmax * (colmax / rowmax)) kp = k;
+        else if (fabs(A[imax + ld * imax]) >= alpha * rowmax) kp = imax;
+        else { kp = imax; kstep = 2; }
+      }
+      int kk = k - kstep + 1;
+      if (kp != kk) {
+        for (int i = 0; i < kp; ++i) { double t = A[i + ld * kk]; A[i + ld * kk] = A[i + ld * kp]; A[i + ld * kp] = t; }
+        for (int i = kp + 1; i < kk; ++i) { double t = A[i + ld * kk]; A[i + ld * kk] = A[kp + ld * i]; A[kp + ld * i] = t; }
+        double t = A[kk + ld * kk]; A[kk + ld * kk] = A[kp + ld * kp]; A[kp + ld * kp] = t;
+        if (kstep == 2) { t = A[(k - 1) + ld * k]; A[(k - 1) + ld * k] = A[kp + ld * k]; A[kp + ld * k] = t; }
+      }
+      if (kstep == 1) {
+        /* SSYR upper with alpha = -1/d, then SSCAL of the column */
+        double r1 = 1.0 / A[k + ld * k];
+        for (int j = 0; j < k; ++j) {
+          double xj = A[j + ld * k];
+          if (xj != 0.0) {
+            double tmp = -r1 * xj;
+            for (int i = 0; i <= j; ++i) A[i + ld * j] = A[i + ld * j] + A[i + ld * k] * tmp;
+          }
+        }
+        for (int i = 0; i < k; ++i) A[i + ld * k] = r1 * A[i + ld * k];
+      } else if (k > 1) {
+        double d12 = A[(k - 1) + ld * k];
+        double d22 = A[(k - 1) + ld * (k - 1)] / d12;
+        double d11 = A[k + ld * k] / d12;
+        double t = 1.0 / (d11 * d22 - 1.0);
+        d12 = t / d12;
+        for (int j = k - 2; j >= 0; --j) {
+          double wkm1 = d12 * (d11 * A[j + ld * (k - 1)] - A[j + ld * k]);
+          double wk = d12 * (d22 * A[j + ld * k] - A[j + ld * (k - 1)]);
+          for (int i = j; i >= 0; --i) A[i + ld * j] = A[i + ld * j] - A[i + ld * k] * wk - A[i + ld * (k - 1)] * wkm1;
+          A[j + ld * k] = wk;
+          A[j + ld * (k - 1)] = wkm1;
+        }
+      }
+    }
+    if (kstep == 1) ipiv[k] = kp + 1;
+    else { ipiv[k] = -(kp + 1); ipiv[k - 1] = -(kp + 1); }
+    k -= kstep;
+  }
+  return info;
+}
+/* SSYTRS upper for nrhs right-hand sides B (column-major, leading dimension ldb) */
+static void bk_solve(const double *A, int n, int ld, const int *ipiv, double *B, int nrhs, int ldb) {
+#define SWAPROW(r1, r2) for (int j_ = 0; j_ < nrhs; ++j_) { double t_ = B[(r1) + ldb * j_]; B[(r1) + ldb * j_] = B[(r2) + ldb * j_]; B[(r2) + ldb * j_] = t_; }
+  int k = n - 1;
+  while (k >= 0) {
+    if (ipiv[k] > 0) {
+      int kp = ipiv[k] - 1;
+      if (kp != k) SWAPROW(k, kp);
+      for (int j = 0; j < nrhs; ++j) {  /* SGER */
+        double y = B[k + ldb * j];
+        if (y != 0.0) {
+          double tmp = -y;
+          for (int i = 0; i < k; ++i) B[i + ldb * j] = B[i + ldb * j] + A[i + ld * k] * tmp;
+        }
+      }
+      double r = 1.0 / A[k + ld * k];  /* SSCAL */
+      for (int j = 0; j < nrhs; ++j) B[k + ldb * j] = r * B[k + ldb * j];
+      k -= 1;
+    } else {
+      int kp = -ipiv[k] - 1;
+      if (kp != k - 1) SWAPROW(k - 1, kp);
+      for (int c = 0; c < 2; ++c) {  /* SGER with columns k, then k-1 */
+        int col = k - c;
+        for (int j = 0; j < nrhs; ++j) {
+          double y = B[col + ldb * j];
+          if (y != 0.0) {
+            double tmp = -y;
+            for (int i = 0; i < k - 1; ++i) B[i + ldb * j] = B[i + ldb * j] + A[i + ld * col] * tmp;
+          }
+        }
+      }
+      double akm1k = A[(k - 1) + ld * k];
+      double akm1 = A[(k - 1) + ld * (k - 1)] / akm1k;
+      double ak = A[k + ld * k] / akm1k;
+      double denom = akm1 * ak - 1.0;
+      for (int j = 0; j < nrhs; ++j) {
+        double bkm1 = B[(k - 1) + ldb * j] / akm1k;
+        double bk = B[k + ldb * j] / akm1k;
+        B[(k - 1) + ldb * j] = (ak * bkm1 - bk) / denom;
+        B[k + ldb * j] = (akm1 * bk - bkm1) / denom;
+      }
+      k -= 2;
+    }
+  }
+  k = 0;
+  while (k < n) {
+    int two = ipiv[k] < 0;
+    for (int c = 0; c <= two; ++c) {  /* SGEMV 'T', alpha -1, beta 1 */
+      int col = k + c;
+      for (int j = 0; j < nrhs; ++j) {
+        double tmp = 0.0;
+        for (int i = 0; i < k; ++i) tmp = tmp + B[i + ldb * j] * A[i + ld * col];
+        if (k > 0) B[col + ldb * j] = B[col + ldb * j] + (-1.0) * tmp;
+      }
+    }
+    int kp = (two ? -ipiv[k] : ipiv[k]) - 1;
+    if (kp != k) SWAPROW(k, kp);
+    k += two ? 2 : 1;
+  }
+#undef SWAPROW
+}
+/* the reference's fallback for one augmented system of m rows (n data rows):
+ * SSYTRF + SSYTRS, and when SSYTRF reports an exactly singular block one
+ * retry with 1e-6 * mean(data diagonal) added to the data diagonal
+ * (kriging_base.F90:3299-3345).  K (ld = m) is modified by the jitter as the
+ * reference's matA is; B holds nrhs right-hand sides (ldb = m) on entry and
+ * the solutions on success.  Returns INFO. */
+static int bk_fallback(double *K, int n, int m, double *B, int nrhs) {
+  double *A = (double *)malloc(sizeof(double) * m * m);
+  double *B0 = (double *)malloc(sizeof(double) * m * nrhs);
+  int *ipiv = (int *)malloc(sizeof(int) * m);
+  memcpy(B0, B, sizeof(double) * m * nrhs);
+  memcpy(A, K, sizeof(double) * m * m);
+  int info = bk_factor(A, m, m, ipiv);
+  if (info != 0 && n > 0) {
+    double ds = 0.0;
+    for (int i = 0; i < n; ++i) ds = ds + K[i + m * i];
+    if (ds > 0.0) {
+      ds = 1.0e-6 * (ds / (double)n);
+      for (int i = 0; i < n; ++i) K[i + m * i] = K[i + m * i] + ds;
+      memcpy(A, K, sizeof(double) * m * m);
+      info = bk_factor(A, m, m, ipiv);
+    }
+  }
+  if (info == 0) bk_solve(A, m, m, ipiv, B, nrhs, m);
+  free(A);
+  free(B0);
+  free(ipiv);
+  return info;
+}
+
+/* exported for the numpy cokriging restatement: the same fallback on one
+ * column-major m x m system K (modified by the jitter) with nrhs columns B */
+int okr_bk_fallback(double *K, int n, int m, double *B, int nrhs) { return bk_fallback(K, n, m, B, nrhs); }
 
 /* insertion sort: kriging_base.F90:3510-3524 */
 static void isort(int *a, int n) {
@@ -963,30 +1113,17 @@ static int run_block(const ctx_t *C, int ib, okr_result *R, kres_t *cand, double
   }
   if (info != 0) {
     /* SSYTRF fallback with a single jittered retry: kriging_base.F90:3299-3366 */
-    double *A = U;  /* reuse: needs m*m, U has n*n + n of room plus Y... allocate */
-    A = (double *)malloc(sizeof(double) * m * m);
-    memcpy(A, K, sizeof(double) * m * m);
     memcpy(x, b, sizeof(double) * m);
-    info = lu_solve(A, m, x);
-    if (info != 0) {
-      double ds = 0.0;
-      for (int i = 0; i < n; ++i) ds = ds + K[i + m * i];
-      if (ds > 0.0) {
-        ds = 1.0e-6 * (ds / (double)n);
-        for (int i = 0; i < n; ++i) K[i + m * i] = K[i + m * i] + ds;
-        memcpy(A, K, sizeof(double) * m * m);
-        memcpy(x, b, sizeof(double) * m);
-        info = lu_solve(A, m, x);
-      }
-    }
-    free(A);
+    info = bk_fallback(K, n, m, x, 1);
     if (info == 0) R->stats[3] += 1;
   }
   if (info != 0) {
     if (!P->neglect_error) return 3;
+    /* x = NaN (kriging_base.F90:3369-3383): the estimate is NaN; the variance
+     * max(NaN, 0) of calc_variance (kriging.F90:1019-1025) is 0 */
     *fail += 1;
     R->estimate[ib] = NAN;
-    R->variance[ib] = NAN;
+    R->variance[ib] = 0.0;
     if (R->weights)
       for (int i = 0; i < n; ++i) R->weights[(size_t)C->mmax * ib + i] = NAN;
     return 0;

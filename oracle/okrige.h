@@ -84,6 +84,12 @@ int okr_cov(const okr_problem *P, int n, const double *lags, double *out);
 int okr_knn(int ndim, int n, const double *coord, const double *q, int k, int sort,
             int *idx, double *dis);
 
+/* the reference's symmetric-indefinite fallback (SSYTRF/SSYTRS with one
+ * jittered retry) on one column-major m x m augmented system K of n data rows
+ * (K is modified by the jitter); B holds nrhs right-hand sides (ld m) and
+ * receives the solutions.  Returns INFO (0 = solved). */
+int okr_bk_fallback(double *K, int n, int m, double *B, int nrhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,8 @@ def lib():
         _lib.okr_knn.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pd, ctypes.c_int, ctypes.c_int,
                                  _pi, _pd]
         _lib.okr_knn.restype = ctypes.c_int
+        _lib.okr_bk_fallback.argtypes = [_pd, ctypes.c_int, ctypes.c_int, _pd, ctypes.c_int]
+        _lib.okr_bk_fallback.restype = ctypes.c_int
     return _lib
 
 
@@ -192,6 +194,16 @@ def cov(spec, lags, table_mode=True):
     if lib().okr_cov(ctypes.byref(P), len(lags), _d(lags), _d(out)):
         raise RuntimeError("okr_cov failed")
     return out
+
+
+def bk_fallback(K, n, B):
+    """the reference's Bunch-Kaufman fallback with the jittered retry
+    (okrige.c: bk_fallback) on the augmented system K (m x m, n data rows)
+    for the right-hand sides B (m x nrhs): (info, solutions)"""
+    Kf = np.asfortranarray(np.array(K, dtype=np.float64))
+    Bf = np.asfortranarray(np.array(B, dtype=np.float64).reshape(len(Kf), -1))
+    info = lib().okr_bk_fallback(_d(Kf), int(n), len(Kf), _d(Bf), Bf.shape[1])
+    return info, Bf
 
 
 def knn(coord, q, k, sort=False):

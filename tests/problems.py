@@ -235,9 +235,68 @@ def ck3_small(nb=150, seed=9):
     return s
 
 
+def ck_empty_var(std=True):
+    """a maxdist on the secondary variable leaves some blocks without
+    secondary neighbours: under std_ck its constraint row is empty, the Schur
+    complement singular, and the reference's fallback ends in NaN estimates,
+    variances 0 and n_fail; non-standard cokriging solves those blocks"""
+    s = ck_small(std=std, gq=False, nb=80, seed=15)
+    s["obs"][1]["maxdist"] = 25.0
+    s["name"] = f"ck_empty_var_std{int(std)}"
+    return s
+
+
+def ck_indefinite():
+    """a cross sill above the geometric mean of the sills with collocated data
+    makes the first block's K indefinite (negative Cholesky pivot): the
+    reference solves it with the symmetric-indefinite factorisation of the
+    augmented system; the second block's variables lie beyond each other's
+    range and factor by Cholesky"""
+    a = np.array([[0.0, 0.0], [30.0, 0.0], [0.0, 30.0]])
+    b1 = a + [10000.0, 0.0]
+    b2 = b1 + [0.0, 500.0]
+    x1, x2 = np.vstack([a, b1]), np.vstack([a, b2])
+    return dict(name="ck_indefinite", ndim=2, nvar=2, std_ck=True, unbias=1,
+                obs=[dict(coord=x1, value=np.arange(6.0), nmax=3),
+                     dict(coord=x2, value=np.arange(6.0)[::-1], nmax=3)],
+                grid=np.array([[10.0, 10.0], [10010.0, 250.0]]),
+                vgm=[dict(ivar=i, jvar=j, vtype="sph", sill=s, a_major=100.0) for (i, j), s in
+                     (((1, 1), 1.0), ((1, 2), 2.0), ((2, 2), 1.0))],
+                search=[dict(ivar=1, sector_search=False), dict(ivar=2, sector_search=False)])
+
+
+def ck_wide():
+    """two variables with 40 + 60 neighbours: 100 data rows, the 128-column
+    instance of the block solver"""
+    s = ck_small(std=True, gq=True, nb=60, seed=17)
+    s["obs"][0]["nmax"] = 40
+    s["obs"][1]["nmax"] = 60
+    s["name"] = "ck_wide_100rows"
+    return s
+
+
+def ck4_small(nb=100, seed=19):
+    """four variables (the third and fourth noisy mixes of the first two),
+    point support, standard cokriging"""
+    s = ck3_small(nb=nb, seed=seed)
+    rng = np.random.default_rng(seed + 2)
+    x4 = rng.uniform(0.0, 1600.0, (500, 2))
+    z4 = np.cos(x4[:, 1] / 250.0) + rng.normal(0.0, 0.2, len(x4))
+    s["nvar"] = 4
+    s["obs"].append(dict(coord=x4, value=z4, nmax=10))
+    s["search"].append(dict(ivar=4, sector_search=False))
+    sills = {(1, 1): 1.0, (1, 2): 0.6, (1, 3): 0.3, (1, 4): 0.2, (2, 2): 0.8, (2, 3): 0.2, (2, 4): 0.1,
+             (3, 3): 0.7, (3, 4): 0.15, (4, 4): 0.6}
+    s["vgm"] = [dict(ivar=i, jvar=j, vtype="exp", sill=v, a_major=700.0, nugget=0.05 if i == j else 0.0)
+                for (i, j), v in sills.items()]
+    s["name"] = "ck4_point"
+    return s
+
+
 def ck_cases():
     return [ck_small(True, True), ck_small(True, False), ck_small(False, True), ck_small(False, False),
-            ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small()]
+            ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small(), ck_empty_var(True), ck_empty_var(False),
+            ck_indefinite(), ck_wide(), ck4_small()]
 
 
 def st_small(nstation=60, ndays=60, side=170.0, grid_side=8, ntimes=3, nmax=24, maxdist=40.0, sector=False,
@@ -261,6 +320,77 @@ def st_cases():
             st_small(nstation=30, ndays=20, cv=True, nmax=12, name="st_loo")]
 
 
+def ladder():
+    """the reference's failure ladder (kriging_base.F90:3244-3398) on exact
+    zeros: observations on a 10 m lattice, a spherical model whose range
+    (4 m) is below the spacing, so every off-diagonal covariance is exactly 0,
+    and a local nugget of -1 cancelling C(0) = 1, so the diagonal is the
+    observation variance (powers of two, so the factorisations are exact).
+    Every block takes the 4 corners of one lattice cell; by quadrant of the
+    lattice, a cell has 4 positive variances (Cholesky), one zero (Cholesky
+    fails, the symmetric-indefinite factorisation of the bordered system
+    succeeds), one positive (that factorisation is exactly singular; the
+    1e-6 mean-diagonal jitter repairs it) or none (no retry when the diagonal
+    sums to 0: NaN estimate, variance max(NaN, 0) = 0, n_fail)."""
+    ax = np.arange(8) * 10.0
+    obs = np.stack([m.ravel() for m in np.meshgrid(ax, ax, indexing="ij")], 1)
+    rng = np.random.default_rng(71)
+    val = rng.normal(size=len(obs))
+    i, j = np.meshgrid(np.arange(8), np.arange(8), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    q = (i >= 4).astype(int) + 2 * (j >= 4).astype(int)
+    odd = (i % 2 == 1) & (j % 2 == 1)
+    even = (i % 2 == 0) & (j % 2 == 0)
+    pos = np.where(q == 0, True, np.where(q == 1, ~odd, np.where(q == 2, even, False)))
+    var = np.where(pos, rng.choice([0.25, 0.5], len(obs)), 0.0)
+    # queries at cell centres (zero right-hand sides) and 2.9 m from a lattice
+    # node (one non-zero right-hand side), both seeing that cell's 4 corners
+    c = np.arange(7) * 10.0 + 5.0
+    g1 = np.stack([m.ravel() for m in np.meshgrid(c, c, indexing="ij")], 1)
+    g2 = np.stack([m.ravel() for m in np.meshgrid(c - 3.5, c + 2.5, indexing="ij")], 1)
+    grid = np.concatenate([g1, g2])
+    n = len(grid)
+    return _base("ladder", 2, obs, val, grid, [dict(ivar=1, jvar=1, vtype="sph", nugget=0.0, sill=1.0, a_major=4.0)],
+                 nmax=4, variance=var, localnugget=np.full(n, -1.0))
+
+
+def bounds_case():
+    """estimates clipped to the bounds (kriging_base.F90:3497-3498)"""
+    p = random_problem(seed=81, nq=1500)
+    p["bounds"] = (-0.4, 0.9)
+    p["name"] = "bounds"
+    return p
+
+
+def aniso_search_case(ndim=2, sector=False):
+    """anisotropic_search: the kd-tree on rotated, axis-scaled coordinates and
+    rotated query lags (kriging.F90:354-357, 371-381, 518-522)"""
+    p = random_problem(ndim=ndim, seed=83 + ndim, nq=1200, nmax=3 if sector else 10, sector=sector)
+    p["anisotropic_search"] = True
+    if ndim == 2:
+        p["search"][0].update(anis1=0.4, azimuth=35.0)
+    else:
+        p["search"][0].update(anis1=0.5, anis2=0.1, azimuth=35.0, dip=10.0, plunge=5.0)
+    p["name"] = f"aniso_search{ndim}d_s{int(sector)}"
+    return p
+
+
+def near_duplicates(nmax=12):
+    """jittered duplicate observations (0.01 m apart) and a 1e-4 nugget: the
+    FP32 pivot ratio of such a pair falls far below 1e-3, so the
+    FP32+refine path re-solves those systems in FP64"""
+    rng = np.random.default_rng(87)
+    base = rng.uniform(0, 3000.0, (700, 2))
+    dup = base[rng.choice(len(base), 40, replace=False)]
+    ang = rng.uniform(0, 2 * np.pi, len(dup))
+    dup = dup + 0.01 * np.stack([np.cos(ang), np.sin(ang)], 1)
+    obs = np.concatenate([base, dup])
+    val = np.sin(obs[:, 0] / 500.0) + np.cos(obs[:, 1] / 700.0) + rng.normal(0, 0.05, len(obs))
+    grid = rng.uniform(0, 3000.0, (1500, 2))
+    vgm = [dict(ivar=1, jvar=1, vtype="sph", nugget=1e-4, sill=1.0, a_major=1500.0, a_minor1=900.0, azimuth=20.0)]
+    return _base("near_duplicates", 2, obs, val, grid, vgm, nmax=nmax)
+
+
 def _named(p, name):
     p["name"] = name
     return p
@@ -273,5 +403,7 @@ def parity_cases():
              lattice(3, nside=12, qoff=2.5, sector=True, nmax=2), lattice(2, qoff=0, qstep=1, cv=True, nmax=7),
              lattice(3, nside=10, qoff=0, qstep=1, cv=True, sector=True, nmax=2),
              _named(random_problem(3, sector=True, nmax=3), "random3d_octant"), nested_levelA(), product_structure(),
-             local_params(), obs_error(), simple_kriging(), maxdist_problem()]
+             local_params(), obs_error(), simple_kriging(), maxdist_problem(),
+             ladder(), bounds_case(), aniso_search_case(2), aniso_search_case(3),
+             aniso_search_case(3, sector=True), near_duplicates()]
     return cases

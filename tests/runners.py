@@ -20,7 +20,8 @@ def run_oracle(spec, table_mode=True, nthread=8, weights=False):
 def run_ref(spec, mode="table", nthread=4):
     from oracle.refdrv import RefKriging
     k = RefKriging(mode=mode, ndim=spec["ndim"], store_weight=True, unbias=spec.get("unbias", 1),
-                   cross_validation=bool(spec.get("cross_validation", False)))
+                   cross_validation=bool(spec.get("cross_validation", False)), bounds=spec.get("bounds"),
+                   anisotropic_search=bool(spec.get("anisotropic_search", False)))
     o = spec["obs"][0]
     k.set_obs(1, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
               maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
@@ -46,6 +47,8 @@ def run_ref(spec, mode="table", nthread=4):
 
 def make_gpu(spec, **kw):
     from krigekit_amd.kriging import Kriging
+    kw.setdefault("bounds", spec.get("bounds"))
+    kw.setdefault("anisotropic_search", bool(spec.get("anisotropic_search", False)))
     k = Kriging(ndim=spec["ndim"], unbias=spec.get("unbias", 1),
                 cross_validation=bool(spec.get("cross_validation", False)), **kw)
     o = spec["obs"][0]
@@ -90,8 +93,29 @@ def rel_err(a, b, floor=1e-4):
     return e
 
 
+def _store_view(d):
+    """(nnear, inear) as the reference's weight store reports them: a block
+    whose solve failed (NaN estimate, variance max(NaN, 0) = 0) keeps one
+    neighbour (ctx%nnear = 1, kriging_base.F90:3369-3383)"""
+    nn, inn = np.asarray(d["nnear"]), np.asarray(d["inear"])
+    if "estimate" not in d or "variance" not in d:
+        return nn, inn
+    failed = np.isnan(d["estimate"]) & (np.asarray(d["variance"]) == 0.0)
+    if not failed.any():
+        return nn, inn
+    nn = np.where(failed, 1, nn)
+    inn = inn.copy()
+    inn[failed, 1:] = 0
+    return nn, inn
+
+
 def same_lists(x, y):
-    """neighbour counts and ascending id lists identical"""
+    """neighbour counts and ascending id lists identical (failed blocks as the
+    reference's weight store reports them)"""
+    xn, xi = _store_view(x)
+    yn, yi = _store_view(y)
+    x = dict(nnear=xn, inear=xi)
+    y = dict(nnear=yn, inear=yi)
     if not np.array_equal(x["nnear"], y["nnear"]):
         return False
     w = min(x["inear"].shape[1], y["inear"].shape[1])
@@ -122,7 +146,7 @@ def run_ref_ck(spec, mode="table", nthread=4):
                    unbias=spec.get("unbias", 1))
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
-                  sk_mean=o.get("sk_mean", 0.0))
+                  maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
     if spec.get("sub_coord") is not None:
         k.set_grid_block_gq(spec["grid"], spec["blocksize"])
     else:
@@ -133,8 +157,9 @@ def run_ref_ck(spec, mode="table", nthread=4):
         k.set_search(**s)
     k.solve(nthread=nthread)
     e, v = k.results()
+    stats = k.solver_stats()
     k.close()
-    return dict(estimate=e, variance=v)
+    return dict(estimate=e, variance=v, stats=stats)
 
 
 def make_gpu_ck(spec, **kw):
@@ -143,7 +168,7 @@ def make_gpu_ck(spec, **kw):
                 unbias=spec.get("unbias", 1), **kw)
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
-                  sk_mean=o.get("sk_mean", 0.0))
+                  maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
     rl = dict(rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
     if spec.get("sub_coord") is not None:
         k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]), **rl)

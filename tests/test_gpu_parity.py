@@ -28,7 +28,19 @@ def ref_vectors():
 def _ref(vec, key):
     nn = vec[f"{key}/nnear"]
     inn = vec[f"{key}/inear"].astype(np.int32)
-    return dict(estimate=vec[f"{key}/estimate"], variance=vec[f"{key}/variance"], nnear=nn, inear=inn)
+    return dict(estimate=vec[f"{key}/estimate"], variance=vec[f"{key}/variance"], nnear=nn, inear=inn,
+                stats=vec[f"{key}/stats"])
+
+
+def _check_counters(g, ref):
+    """failures and symmetric-indefinite solves as the reference counts them;
+    Cholesky solves too when no block has a single neighbour (the reference
+    skips those solves, kriging_base.F90:2127)"""
+    st = g["stats"]
+    f, c, s = (int(x) for x in ref["stats"])
+    assert st["fail"] == f and st["ssytrf_fact"] == s, (st, ref["stats"])
+    if not np.any(ref["nnear"] == 1):
+        assert st["chol_fact"] == c, (st, ref["stats"])
 
 
 @pytest.mark.parametrize("spec", CASES, ids=IDS)
@@ -40,6 +52,7 @@ def test_fp64_matches_reference_and_oracle(spec, fast, ref_vectors):
     assert same_lists(g, ref), "neighbour lists differ from the compiled reference"
     ee, ev = max_errors(g, ref)
     assert ee < FP64_TOL and ev < FP64_TOL, (ee, ev)
+    _check_counters(g, ref)
     o = run_oracle(spec)
     assert same_lists(g, o)
     ee, ev = max_errors(g, o)
@@ -47,13 +60,32 @@ def test_fp64_matches_reference_and_oracle(spec, fast, ref_vectors):
 
 
 @pytest.mark.parametrize("spec", CASES, ids=IDS)
-def test_fp32_refined_within_1e6(spec):
+def test_fp32_refined_within_1e6(spec, ref_vectors):
+    """FP32 factorisation + FP64 refinement within 1e-6 of the reference and
+    the oracle; the near-duplicate problem's systems fall below the FP32
+    pivot-ratio threshold (1e-3) and are re-solved in FP64"""
     print("case", spec["name"], "fp32r", flush=True)
     g = run_gpu(spec, precision="fp32r")
+    ref = _ref(ref_vectors, spec["name"])
+    assert same_lists(g, ref)
+    ee, ev = max_errors(g, ref)
+    assert ee < FP32R_TOL and ev < FP32R_TOL, (ee, ev)
+    _check_counters(g, ref)
     o = run_oracle(spec)
     assert same_lists(g, o)
     ee, ev = max_errors(g, o)
     assert ee < FP32R_TOL and ev < FP32R_TOL, (ee, ev)
+    if spec["name"] == "near_duplicates":
+        assert g["timing"]["fp64_resolved"] > 0
+
+
+def test_failure_without_neglect_error():
+    """a system that stays singular after the jittered retry is an error
+    unless neglect_error (kriging_base.F90:3369-3383)"""
+    k = make_gpu(problems.ladder(), neglect_error=False)
+    with pytest.raises(RuntimeError, match="Singular matrix"):
+        k.solve()
+    k.close()
 
 
 @pytest.mark.parametrize("spec", problems.all_shapes(), ids=lambda s: s["name"])

@@ -207,11 +207,42 @@ def test_committed_reference_vectors():
         o = run_oracle(spec)
         nn = vec[f"{key}/nnear"]
         inn = vec[f"{key}/inear"].astype(np.int32)
-        assert np.array_equal(o["nnear"], nn), key
-        assert np.array_equal(o["inear"][:, :inn.shape[1]], inn), key
-        ref = dict(estimate=vec[f"{key}/estimate"], variance=vec[f"{key}/variance"], nnear=nn)
+        ref = dict(estimate=vec[f"{key}/estimate"], variance=vec[f"{key}/variance"], nnear=nn, inear=inn)
+        assert same_lists(o, ref), key
         ee, ev = max_errors(o, ref)
         assert ee < 1e-10 and ev < 1e-10, key
+        # failures, Cholesky solves and symmetric-indefinite solves
+        st = o["stats"]
+        got = [st["fail"], st["chol_fact"] + st["chol_reuse"], st["ssytrf_fact"] + st["ssytrf_reuse"]]
+        assert got == list(vec[f"{key}/stats"]), (key, got, list(vec[f"{key}/stats"]))
+
+
+def test_failure_ladder_is_reached():
+    """the ladder problem reaches every step of the reference's fallback
+    (kriging_base.F90:3244-3398): Cholesky, the symmetric-indefinite
+    factorisation, its jittered retry, and failure"""
+    spec = problems.ladder()
+    o = run_oracle(spec)
+    v = spec["obs"][0]["variance"]
+    npos = np.array([(v[o["inear"][b, :o["nnear"][b]] - 1] > 0).sum() for b in range(len(o["nnear"]))])
+    assert set(np.unique(npos)) >= {0, 1, 3, 4}
+    assert np.all(np.isnan(o["estimate"][npos == 0])) and np.all(o["variance"][npos == 0] == 0.0)
+    assert np.all(np.isfinite(o["estimate"][npos > 0]))
+    assert o["stats"]["fail"] == (npos == 0).sum()
+    assert o["stats"]["ssytrf_fact"] == ((npos > 0) & (npos < 4)).sum()
+    with pytest.raises(RuntimeError):
+        okrige.solve(spec, neglect_error=False)
+
+
+def test_bounds_clip_estimates():
+    spec = problems.bounds_case()
+    o = run_oracle(spec)
+    lo, hi = spec["bounds"]
+    assert (o["estimate"] == lo).sum() > 50 and (o["estimate"] == hi).sum() > 50
+    free = dict(spec)
+    free.pop("bounds")
+    f = run_oracle(free)
+    np.testing.assert_array_equal(o["estimate"], np.clip(f["estimate"], lo, hi))
 
 
 def test_planted_mistake_is_caught():
