@@ -225,6 +225,19 @@ int kk_get_timing(int64_t handle, double *out);
  * cross_validation=1 and set_grid_cv.  grad[8]; zhat[nblock] and
  * dz[nblock*8] (block-major) may be NULL. */
 int kk_loo_grad(int64_t handle, double *mse, double *grad, double *zhat, double *dz);
+/* No counterpart in the reference: kk_solve_resident for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree: the
+ * pass's neighbour lists, flags, bucket lists, search heaps and fallback
+ * matrices live in a fixed 64 MiB device array (one per GPU, held until the
+ * process ends); a problem whose pass needs more (about 4*mmax + 90 bytes per
+ * block plus the search-heap and fallback areas) is rejected with an error.
+ * est and var are caller-owned device buffers of nblock doubles.  The handle
+ * must hold a finished single-variable solve of its current inputs (its
+ * inputs, streams and query order stay resident); its own results are left
+ * unchanged.  All calls on one GPU share the array: calls that overlap on one
+ * GPU race. */
+int kk_solve_resident_noalloc(int64_t handle, double *est, double *var);
 /* hipDeviceSynchronize */
 int kk_device_synchronize(void);
 /* plain device copy of nbytes, reps times; *ms = mean kernel time.  A kernel

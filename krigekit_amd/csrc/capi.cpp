@@ -728,9 +728,34 @@ bool morton_perm(Krige& K) {
   return true;
 }
 
-bool run_pipeline(Krige& K) {
+// Device memory one pass of run_pipeline works in.  By default the handle's
+// buffers, grown as needed; the allocation-free entry point passes views of
+// its fixed scratch instead (every capacity checked before the pass).
+struct PassBufs {
+  int *nnear = nullptr, *inear = nullptr, *sflag = nullptr, *xlist = nullptr, *flag = nullptr;
+  int *counts = nullptr, *lists = nullptr;
+  double *est = nullptr, *var = nullptr;
+  double* hdis = nullptr;  // exact-search heaps: hcap queries of k entries
+  int* hidx = nullptr;
+  int hcap = 0;
+  double* fb = nullptr;    // fallback matrices: fbcap blocks of fallback_words(mmax)
+  int fbcap = 0;
+};
+
+// neighbour-list stride and candidate count of a single-variable pass: with
+// sector search 2^ndim nmax rows (space-time: octants of the spatial axes)
+// from 4x as many candidates (kriging_base.F90:3005-3011, kriging.F90:623)
+void pass_dims(const Krige& K, int& mmax, int& k) {
+  const Obs& O = K.obs[0];
+  const bool sector = O.sector && O.need_search;
+  const int nsec = 1 << (K.st ? 3 : K.ndim);
+  mmax = sector ? nsec * O.nmax : O.nmax;
+  k = sector ? std::min(O.n, nsec * O.nmax * 4) : O.nmax;
+}
+
+bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   const char* W = "t_kriging_base%solve_base";
-  if (!ensure_stream(K)) return false;
+  if (!ext && !ensure_stream(K)) return false;
   hipStream_t st = K.stream;
   Obs& O = K.obs[0];
   const int nd = K.ndim, nb = K.nblock;
@@ -738,20 +763,35 @@ bool run_pipeline(Krige& K) {
   const bool sector = O.sector && O.need_search;
   // octants are over the spatial axes only: 2^3 for space-time
   // (kriging_st.F90:587, obs_nmax in kriging_base.F90:3005-3011)
-  const int nsec = 1 << (K.st ? 3 : nd);
-  const int mmax = sector ? nsec * nmax : nmax;
+  int mmax, kcand;
+  pass_dims(K, mmax, kcand);
   K.mmax = mmax;
   if (mmax > 64) {
     set_error(W, "neighbourhoods larger than 64 data rows are not supported by the GPU solver (mmax=" +
                      std::to_string(mmax) + ")");
     return false;
   }
-  if (!K.d_nnear.ensure(sizeof(int) * nb) || !K.d_inear.ensure(sizeof(int) * (size_t)nb * mmax) ||
-      !K.d_est.ensure(sizeof(double) * nb) || !K.d_var.ensure(sizeof(double) * nb) ||
-      !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 32) ||
-      !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb * 2 + 64)) {
-    set_error(W, "device allocation failed");
-    return false;
+  PassBufs B;
+  if (ext) {
+    B = *ext;
+  } else {
+    if (!K.d_nnear.ensure(sizeof(int) * nb) || !K.d_inear.ensure(sizeof(int) * (size_t)nb * mmax) ||
+        !K.d_est.ensure(sizeof(double) * nb) || !K.d_var.ensure(sizeof(double) * nb) ||
+        !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 32) ||
+        !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb * 2 + 64) ||
+        !K.d_sflag.ensure(sizeof(int) * nb) || !K.d_xlist.ensure(sizeof(int) * nb)) {
+      set_error(W, "device allocation failed");
+      return false;
+    }
+    B.nnear = K.d_nnear.as<int>();
+    B.inear = K.d_inear.as<int>();
+    B.est = K.d_est.as<double>();
+    B.var = K.d_var.as<double>();
+    B.flag = K.d_flag.as<int>();
+    B.counts = K.d_counts.as<int>();
+    B.lists = K.d_lists.as<int>();
+    B.sflag = K.d_sflag.as<int>();
+    B.xlist = K.d_xlist.as<int>();
   }
   // ---- per-node solve arguments ------------------------------------------------
   kk::SolveArgs va{};
@@ -774,13 +814,13 @@ bool run_pipeline(Krige& K) {
   va.ocoord = K.d_ocoord.as<double>();
   va.ovalue = K.d_ovalue.as<double>();
   va.ovar = K.d_ovar.as<double>();
-  va.nnear = K.d_nnear.as<int>();
-  va.inear = K.d_inear.as<int>();
+  va.nnear = B.nnear;
+  va.inear = B.inear;
   va.mmax = mmax;
   va.pivot_tol = K.pivot_tol;
-  va.est = K.d_est.as<double>();
-  va.var = K.d_var.as<double>();
-  va.flag = K.d_flag.as<int>();
+  va.est = B.est;
+  va.var = B.var;
+  va.flag = B.flag;
   if (!K.soff.empty()) {
     // uniform sub-point count per block (Gauss-Legendre blocks)
     va.nsub = K.soff[1] - K.soff[0];
@@ -801,7 +841,7 @@ bool run_pipeline(Krige& K) {
                     hipEvent_t e0, hipEvent_t e1) -> bool {
     HIPCHK(hipEventRecord(e0, s), W);
     HIPCHK(hipMemsetAsync(dcounts, 0, sizeof(int) * 16, s), W);
-    HIPCHK(kk::launch_bucket(K.d_nnear.as<int>(), sel, nsel, dcounts, lists, cap, s), W);
+    HIPCHK(kk::launch_bucket(B.nnear, sel, nsel, dcounts, lists, cap, s), W);
     HIPCHK(hipMemcpyAsync(hcounts, dcounts, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
     HIPCHK(hipEventRecord(e1, s), W);
     return true;
@@ -846,8 +886,8 @@ bool run_pipeline(Krige& K) {
   sa.cv = K.cross_validation;
   sa.maxdist2 = O.maxdist2;
   sa.mmax = mmax;
-  sa.nnear = K.d_nnear.as<int>();
-  sa.inear = K.d_inear.as<int>();
+  sa.nnear = B.nnear;
+  sa.inear = B.inear;
   if (!O.need_search) {
     // rotated_dists rotates (obs - query) of the unrotated coordinates;
     // space-time adds the scaled time difference (kriging_st.F90:609-612)
@@ -871,17 +911,13 @@ bool run_pipeline(Krige& K) {
     sa.tree.ni = K.d_ni.as<int>();
     sa.tree.nd = K.d_nd.as<double>();
     sa.sort = sector;
-    sa.k = sector ? std::min(O.n, nsec * nmax * 4) : nmax;
+    sa.k = kcand;
     const size_t lds = kk::fast_search_lds(nd, sector, sa.k, nmax);
     int nexact = nb;
     const int* exact_list = nullptr;
     // sectored space-time searches run the exact kernel for every query
     if (K.search_mode > 0 && lds <= 64 * 1024 && !(K.st && sector)) {
-      if (!K.d_sflag.ensure(sizeof(int) * nb)) {
-        set_error(W, "device allocation failed (search flags)");
-        return false;
-      }
-      sa.qflag = K.d_sflag.as<int>();
+      sa.qflag = B.sflag;
       sa.q0 = 0;
       sa.nq = nb;
       HIPCHK(hipEventRecord(K.ev[8], st), W);
@@ -899,8 +935,8 @@ bool run_pipeline(Krige& K) {
           // count) are bucketed right behind its search, into the second half
           // of the list area: queued behind the next search they would wait
           // for its waves to retire
-          if (!bucket(st, perm, split, K.d_counts.as<int>() + 16, K.h_counts + 16,
-                      K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split, K.ev[12], K.ev[13]))
+          if (!bucket(st, perm, split, B.counts + 16, K.h_counts + 16, B.lists + (size_t)(kk::NBUCKET + 1) * nb,
+                      split, K.ev[12], K.ev[13]))
             return false;
           sa.qlist = perm + split;
           sa.nq = nb - split;
@@ -918,22 +954,18 @@ bool run_pipeline(Krige& K) {
         HIPCHK(hipEventElapsedTime(&bucket_ms_first, K.ev[12], K.ev[13]), W);
         bucket_ms += bucket_ms_first;
         HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[13], 0), W);
-        if (!solve_buckets(K.stream3, K.h_counts + 16, K.d_lists.as<int>() + (size_t)(kk::NBUCKET + 1) * nb, split))
+        if (!solve_buckets(K.stream3, K.h_counts + 16, B.lists + (size_t)(kk::NBUCKET + 1) * nb, split))
           return false;
         HIPCHK(hipEventRecord(K.ev[12], K.stream3), W);
       }
-      if (!K.d_xlist.ensure(sizeof(int) * nb)) {
-        set_error(W, "device allocation failed (search flags)");
-        return false;
-      }
-      HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+      HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
       // in Morton order when the search used it: the exact kernel's lanes
       // then take neighbouring queries and follow similar tree paths
-      HIPCHK(kk::launch_collect(K.d_sflag.as<int>(), nb, 1, K.d_counts.as<int>() + 8, K.d_xlist.as<int>(), st,
+      HIPCHK(kk::launch_collect(B.sflag, nb, 1, B.counts + 8, B.xlist, st,
                                 K.search_mode == 2 && K.reorder ? K.d_perm.as<int>() : nullptr), W);
-      HIPCHK(hipMemcpyAsync(&nexact, K.d_counts.as<int>() + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
+      HIPCHK(hipMemcpyAsync(&nexact, B.counts + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
       HIPCHK(hipStreamSynchronize(st), W);
-      exact_list = K.d_xlist.as<int>();
+      exact_list = B.xlist;
     }
     K.timing[7] = exact_list ? nexact : -1;
     K.timing[8] = exact_list ? (split ? 2 : 1) : 0;  // fast-search launches between ev[8] and ev[9]
@@ -944,14 +976,19 @@ bool run_pipeline(Krige& K) {
     if (overlap) HIPCHK(hipStreamWaitEvent(xs, K.ev[9], 0), W);
     HIPCHK(hipEventRecord(K.ev[10], xs), W);
     if (nexact > 0) {
-      const int chunk = std::min(nexact, 1 << 20);
-      if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
-          !K.d_hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
-        set_error(W, "device allocation failed (search scratch)");
-        return false;
+      const int chunk = std::min(nexact, ext ? B.hcap : 1 << 20);
+      if (ext) {
+        sa.hdis = B.hdis;
+        sa.hidx = B.hidx;
+      } else {
+        if (!K.d_hdis.ensure(sizeof(double) * (size_t)sa.k * chunk) ||
+            !K.d_hidx.ensure(sizeof(int) * (size_t)sa.k * chunk)) {
+          set_error(W, "device allocation failed (search scratch)");
+          return false;
+        }
+        sa.hdis = K.d_hdis.as<double>();
+        sa.hidx = K.d_hidx.as<int>();
       }
-      sa.hdis = K.d_hdis.as<double>();
-      sa.hidx = K.d_hidx.as<int>();
       sa.hstride = chunk;
       for (int q0 = 0; q0 < nexact; q0 += chunk) {
         sa.q0 = exact_list ? 0 : q0;
@@ -971,39 +1008,43 @@ bool run_pipeline(Krige& K) {
   // decided on the second stream
   for (int round = 0; round < (nx > 0 ? 2 : 1); ++round) {
     if (round == 1) HIPCHK(hipStreamWaitEvent(st, K.ev[11], 0), W);
-    const int* sel = round ? K.d_xlist.as<int>() : (split ? K.d_perm.as<int>() + split : nullptr);
+    const int* sel = round ? B.xlist : (split ? K.d_perm.as<int>() + split : nullptr);
     const int nsel = round ? nx : nb - split;
-    if (!bucket_solve(st, sel, nsel, K.d_counts.as<int>(), K.d_lists.as<int>(), nb)) return false;
+    if (!bucket_solve(st, sel, nsel, B.counts, B.lists, nb)) return false;
   }
   if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
   HIPCHK(hipEventRecord(K.ev[3], st), W);
   // ---- FP64 re-solve of flagged FP32 systems, Bunch-Kaufman fallback -------
   int nref = 0, nfb = 0;
   if (K.precision == kk::PREC_FP32R) {
-    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-    HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 1, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
-    HIPCHK(hipMemcpyAsync(&nref, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_collect(B.flag, nb, 1, B.counts, B.lists, st), W);
+    HIPCHK(hipMemcpyAsync(&nref, B.counts, sizeof(int), hipMemcpyDeviceToHost, st), W);
     HIPCHK(hipStreamSynchronize(st), W);
     if (nref > 0) {
-      va.list = K.d_lists.as<int>();
+      va.list = B.lists;
       va.nlist = nref;
       HIPCHK(kk::launch_solve(va, kk::solve_width(mmax), kk::PREC_FP64, tb, st), W);
     }
   }
-  HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
-  HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 2, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
-  HIPCHK(hipMemcpyAsync(&nfb, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
+  HIPCHK(kk::launch_collect(B.flag, nb, 2, B.counts, B.lists, st), W);
+  HIPCHK(hipMemcpyAsync(&nfb, B.counts, sizeof(int), hipMemcpyDeviceToHost, st), W);
   HIPCHK(hipStreamSynchronize(st), W);
   if (nfb > 0) {
-    const int fchunk = 4096;
-    if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::fallback_words(mmax))) {
-      set_error(W, "device allocation failed (fallback scratch)");
-      return false;
+    const int fchunk = ext ? B.fbcap : 4096;
+    double* fb = B.fb;
+    if (!ext) {
+      if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::fallback_words(mmax))) {
+        set_error(W, "device allocation failed (fallback scratch)");
+        return false;
+      }
+      fb = K.d_fb.as<double>();
     }
     for (int f0 = 0; f0 < nfb; f0 += fchunk) {
-      va.list = K.d_lists.as<int>() + f0;
+      va.list = B.lists + f0;
       va.nlist = std::min(fchunk, nfb - f0);
-      HIPCHK(kk::launch_fallback(va, K.d_fb.as<double>(), st), W);
+      HIPCHK(kk::launch_fallback(va, fb, st), W);
     }
   }
   HIPCHK(hipEventRecord(K.ev[4], st), W);
@@ -1597,6 +1638,63 @@ bool check_solved(Krige* K, const char* caller) {
 int unsupported(const char* name, const char* what) {
   set_error(name, what);
   return ierr();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// allocation-free pass: the device memory of kk_solve_resident_noalloc is
+// this fixed array, one copy per GPU, held until the process ends.  Host code
+// addresses it only through the pointer hipGetSymbolAddress returns for the
+// current device (the array's own name is a host-side stand-in).
+// ---------------------------------------------------------------------------
+constexpr size_t KK_NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[KK_NOALLOC_BYTES];
+
+namespace {
+
+constexpr int KK_MAX_DEVICES = 64;
+void* g_scratch_dev[KK_MAX_DEVICES] = {nullptr};
+
+// the current device's copy of the scratch array (looked up once per device)
+bool scratch_ptr(void** out, const char* W) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev), W);
+  if (dev < 0 || dev >= KK_MAX_DEVICES) {
+    set_error(W, "device ordinal out of range");
+    return false;
+  }
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), W);
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return true;
+}
+
+// carves the pass buffers of nb blocks (list stride mmax, k search
+// candidates) out of base; returns the bytes used (base may be null to size)
+size_t carve_pass(char* base, int nb, int mmax, int k, PassBufs& B) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  B.nnear = reinterpret_cast<int*>(take(sizeof(int) * nb));
+  B.inear = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nb * mmax));
+  B.sflag = reinterpret_cast<int*>(take(sizeof(int) * nb));
+  B.xlist = reinterpret_cast<int*>(take(sizeof(int) * nb));
+  B.flag = reinterpret_cast<int*>(take(sizeof(int) * nb));
+  B.counts = reinterpret_cast<int*>(take(sizeof(int) * 32));
+  B.lists = reinterpret_cast<int*>(take(sizeof(int) * ((size_t)(kk::NBUCKET + 1) * nb * 2 + 64)));
+  B.hcap = std::max(1, std::min(nb, 4096));
+  B.hdis = reinterpret_cast<double*>(take(sizeof(double) * (size_t)k * B.hcap));
+  B.hidx = reinterpret_cast<int*>(take(sizeof(int) * (size_t)k * B.hcap));
+  B.fbcap = std::max(1, std::min(nb, 256));
+  B.fb = reinterpret_cast<double*>(take(sizeof(double) * B.fbcap * kk::fallback_words(mmax)));
+  return off;
 }
 
 }  // namespace
@@ -2654,6 +2752,47 @@ int kk_copy_probe(double nbytes, int reps, double* ms) {
   (void)hipFree(a);
   (void)hipFree(b);
   return 0;
+}
+
+// kk_solve_resident without any device allocation: the pass's neighbour
+// lists, flags, bucket lists, search heaps and fallback matrices live in the
+// fixed scratch array; est and var are the caller's device buffers of nblock
+// doubles.  The handle must hold a finished single-variable solve of its
+// current inputs (krige_solve or kk_solve_resident), so that its inputs,
+// streams and query order are already resident.  The handle's own results
+// are left as they were.
+int kk_solve_resident_noalloc(int64_t h, double* est, double* var) {
+  clear_error();
+  const char* W = "kk_solve_resident_noalloc";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (K->nvar != 1 || !K->solved || !K->dev_inputs_ok || !K->model_ok || !K->stream ||
+      (K->search_mode == 2 && K->reorder && !K->perm_ok)) {
+    set_error(W, "needs a finished single-variable solve of the handle's current inputs "
+                 "(krige_solve or kk_solve_resident) before the allocation-free pass");
+    return ierr();
+  }
+  if (!est || !var) {
+    set_error(W, "est and var must be device buffers of nblock doubles");
+    return ierr();
+  }
+  int mmax = 0, k = 0;
+  pass_dims(*K, mmax, k);
+  PassBufs B;
+  const size_t need = carve_pass(nullptr, K->nblock, mmax, k, B);
+  if (need > KK_NOALLOC_BYTES) {
+    set_error(W, "nblock=" + std::to_string(K->nblock) + " with " + std::to_string(mmax) +
+                     " rows needs " + std::to_string(need >> 20) + " MiB of scratch, above the " +
+                     std::to_string(KK_NOALLOC_BYTES >> 20) + " MiB of the allocation-free entry point");
+    return ierr();
+  }
+  void* base = nullptr;
+  if (!scratch_ptr(&base, W)) return ierr();
+  carve_pass(static_cast<char*>(base), K->nblock, mmax, k, B);
+  B.est = est;
+  B.var = var;
+  run_pipeline(*K, &B);
+  return ierr();
 }
 
 int kk_device_synchronize(void) {
