@@ -194,6 +194,15 @@ void krige_st_get_num_threads(int *n);
  * pivot_tol: FP32 pivot ratio below which a node is re-solved in FP64;
  * table_mode: 1 = lookup-table covariance (the reference default), 0 = analytic */
 int kk_set_options(int64_t handle, int precision, double pivot_tol, int table_mode);
+/* factor reuse over identical neighbour sets (the reference's factor cache,
+ * src/libkriging/kriging_base.F90:1746-1917,2445-2485, done as a grouping
+ * pass): blocks with the same neighbour list, range scale and local nugget
+ * share one factorisation, item_max (1..4, default 4) member blocks per
+ * factorisation.  mode -1
+ * (default): on unless cross_validation (whose sets are all distinct); 0 off;
+ * 1 on.  Results are identical either way; krige_solver_stats counts the
+ * factorisations and the reuses. */
+int kk_set_factor_reuse(int64_t handle, int mode, int item_max);
 /* mode 2 (default): wave-cooperative two-pass LDS search, mode 1: per-lane
  * two-pass LDS search (both run the exact kernel only for the queries they
  * flag), mode 0: exact-emulation search kernel for every query.

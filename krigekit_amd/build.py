@@ -36,6 +36,7 @@ UNITS = [
     ("solve.hip", ["-DKK_SOLVE_DIM=4"], "solve_d4.o"),
     ("loograd.hip", [], "loograd.o"),
     ("cokrige.hip", [], "cokrige.o"),
+    ("sets.hip", [], "sets.o"),
     ("capi.cpp", ["-x", "hip"], "capi.o"),
 ]
 

@@ -480,6 +480,9 @@ struct DBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// counters of one pipeline pass (device block and pinned host copy)
+constexpr int PASS_COUNTERS = 128;
+
 // ---------------------------------------------------------------------------
 // kriging object
 // ---------------------------------------------------------------------------
@@ -546,11 +549,18 @@ struct Krige {
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
   DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_tmodel, d_ttab;
+  // factor reuse over identical neighbour sets: -1 auto (on unless
+  // leave-one-out, whose sets are all distinct), 0 off, 1 on; members per
+  // work item
+  int reuse_mode = -1;
+  int item_max = kk::MAX_ITEM;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool chunked = true;          // with Morton order: two search chunks, the first one's solve overlapped
   bool perm_ok = false;
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
+  DBuf d_sets, d_items, d_table;  // factor reuse: per-block set data, work items, hash table
+  bool reuse_on() const { return reuse_mode > 0 || (reuse_mode < 0 && !cross_validation); }
   Model model{};
   std::vector<double> tab;
   // multi-variable path
@@ -612,7 +622,8 @@ bool ensure_stream(Krige& K) {
     HIPCHK(hipStreamCreateWithFlags(&K.stream, hipStreamNonBlocking), "solve");
     HIPCHK(hipStreamCreateWithFlags(&K.stream2, hipStreamNonBlocking), "solve");
     HIPCHK(hipStreamCreateWithFlags(&K.stream3, hipStreamNonBlocking), "solve");
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&K.h_counts), sizeof(int) * 32, hipHostMallocDefault), "solve");
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&K.h_counts), sizeof(int) * PASS_COUNTERS, hipHostMallocDefault),
+           "solve");
     for (auto& e : K.ev) HIPCHK(hipEventCreate(&e), "solve");
   }
   return true;
@@ -728,12 +739,36 @@ bool morton_perm(Krige& K) {
   return true;
 }
 
+// Counter block of a pass (device, and its pinned host copy): [0] and [8]
+// block counts of the flag collections, [16] set members placed, [17]/[18]
+// flagged work items/blocks of the grouped solves, [20]/[21] those of the
+// FP64 re-solve; per round 32 counters at ROUND_CTR[round] (bucket or
+// set-grouping counts of the first search chunk, the rest, the exact-search
+// blocks).
+constexpr int ROUND_CTR[3] = {32, 64, 96};
+// hash table slots for nsel blocks: a power of two of at least 2 nsel
+size_t set_table_slots(int nsel) {
+  size_t ts = 16;
+  while (ts < 2 * (size_t)nsel) ts <<= 1;
+  return ts;
+}
+
 // Device memory one pass of run_pipeline works in.  By default the handle's
 // buffers, grown as needed; the allocation-free entry point passes views of
 // its fixed scratch instead (every capacity checked before the pass).
 struct PassBufs {
   int *nnear = nullptr, *inear = nullptr, *sflag = nullptr, *xlist = nullptr, *flag = nullptr;
-  int *counts = nullptr, *lists = nullptr;
+  int *counts = nullptr;   // PASS_COUNTERS ints (layout at run_pipeline)
+  int *lists = nullptr;
+  // factor reuse: per-block set data, the hash table (tcap slots), work items
+  // and blocks without neighbours (nb each)
+  int *rep = nullptr, *rank = nullptr, *setcnt = nullptr, *setbase = nullptr, *setitem = nullptr,
+      *members = nullptr;
+  unsigned long long* table = nullptr;
+  size_t tcap = 0;
+  unsigned long long* hash = nullptr;  // per block
+  int2* items = nullptr;
+  int* empties = nullptr;
   double *est = nullptr, *var = nullptr;
   double* hdis = nullptr;  // exact-search heaps: hcap queries of k entries
   int* hidx = nullptr;
@@ -777,11 +812,31 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   } else {
     if (!K.d_nnear.ensure(sizeof(int) * nb) || !K.d_inear.ensure(sizeof(int) * (size_t)nb * mmax) ||
         !K.d_est.ensure(sizeof(double) * nb) || !K.d_var.ensure(sizeof(double) * nb) ||
-        !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * 32) ||
+        !K.d_flag.ensure(sizeof(int) * nb) || !K.d_counts.ensure(sizeof(int) * PASS_COUNTERS) ||
         !K.d_lists.ensure(sizeof(int) * (size_t)(kk::NBUCKET + 1) * nb * 2 + 64) ||
         !K.d_sflag.ensure(sizeof(int) * nb) || !K.d_xlist.ensure(sizeof(int) * nb)) {
       set_error(W, "device allocation failed");
       return false;
+    }
+    if (K.reuse_on()) {
+      const size_t ts = set_table_slots(nb);
+      if (!K.d_sets.ensure(sizeof(int) * (size_t)nb * 7) || !K.d_items.ensure(sizeof(int2) * (size_t)nb) ||
+          !K.d_table.ensure(sizeof(unsigned long long) * (ts + (size_t)nb))) {
+        set_error(W, "device allocation failed (neighbour sets)");
+        return false;
+      }
+      int* sb = K.d_sets.as<int>();
+      B.rep = sb;
+      B.rank = sb + nb;
+      B.setcnt = sb + 2 * (size_t)nb;
+      B.setbase = sb + 3 * (size_t)nb;
+      B.members = sb + 4 * (size_t)nb;
+      B.empties = sb + 5 * (size_t)nb;
+      B.setitem = sb + 6 * (size_t)nb;
+      B.items = K.d_items.as<int2>();
+      B.table = K.d_table.as<unsigned long long>();
+      B.tcap = ts;
+      B.hash = B.table + ts;
     }
     B.nnear = K.d_nnear.as<int>();
     B.inear = K.d_inear.as<int>();
@@ -831,52 +886,110 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   // the inline Level-B kernels need every model on one composite table: for
   // space-time both the spatial and the temporal model
   const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
-  // bucket the blocks (all, or the nsel listed in sel) by neighbour count and
-  // solve each bucket on stream s, with lists of capacity cap in their own area
+  // A round groups (factor reuse) or buckets the blocks sel[0..nsel) (all
+  // when sel is null) by neighbour count on stream s, copies its counters
+  // (round area ro of the counter block) to pinned host memory between
+  // events e0 and e1, then -- once the host has them -- launches one solve
+  // per bucket.  Work lists: plain block lists of capacity cap per bucket at
+  // lists, or work items and empty blocks at items/empties.
+  const bool reuse = B.items != nullptr;
   float bucket_ms = 0.0f, bucket_ms_first = 0.0f;  // all bucketing; the first chunk's (inside "search")
   int nempty = 0;
-  // bucketing: counts reset, bucket kernel, counts copied to pinned host
-  // memory, between events e0 and e1 on stream s
-  auto bucket = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* hcounts, int* lists, int cap,
-                    hipEvent_t e0, hipEvent_t e1) -> bool {
+  long n_items = 0, n1_items = 0, n1_blocks = 0;  // work items, those and blocks of single-neighbour sets
+  struct Round {
+    const int* sel;
+    int nsel;
+    int ro;
+    const int* skip;
+    int* lists;
+    int cap;
+    int2* items;
+    int* empties;
+  };
+  auto prepare = [&](hipStream_t s, const Round& rd, hipEvent_t e0, hipEvent_t e1) -> bool {
     HIPCHK(hipEventRecord(e0, s), W);
-    HIPCHK(hipMemsetAsync(dcounts, 0, sizeof(int) * 16, s), W);
-    HIPCHK(kk::launch_bucket(B.nnear, sel, nsel, dcounts, lists, cap, s), W);
-    HIPCHK(hipMemcpyAsync(hcounts, dcounts, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
+    if (reuse) {
+      kk::SetGroupArgs g{};
+      g.nnear = B.nnear;
+      g.inear = B.inear;
+      g.mmax = mmax;
+      g.rs = va.rangescale;
+      g.ln = va.localnugget;
+      g.sel = rd.sel;
+      g.nsel = rd.nsel;
+      g.skip = rd.skip;
+      g.table = B.table;
+      g.hash = B.hash;
+      g.tsize = std::min(B.tcap, set_table_slots(rd.nsel));
+      g.item_max = K.item_max;
+      g.rep = B.rep;
+      g.rank = B.rank;
+      g.cnt = B.setcnt;
+      g.setbase = B.setbase;
+      g.setitem = B.setitem;
+      g.members = B.members;
+      g.ctr = B.counts + rd.ro;
+      g.total = B.counts + 16;
+      g.items = rd.items;
+      g.empties = rd.empties;
+      HIPCHK(hipMemsetAsync(g.ctr, 0, sizeof(int) * kk::SET_NCTR, s), W);
+      HIPCHK(kk::launch_set_group(g, s), W);
+      HIPCHK(hipMemcpyAsync(K.h_counts + rd.ro, g.ctr, sizeof(int) * kk::SET_NCTR, hipMemcpyDeviceToHost, s), W);
+    } else {
+      HIPCHK(hipMemsetAsync(B.counts + rd.ro, 0, sizeof(int) * 16, s), W);
+      HIPCHK(kk::launch_bucket(B.nnear, rd.sel, rd.nsel, B.counts + rd.ro, rd.lists, rd.cap, s), W);
+      HIPCHK(hipMemcpyAsync(K.h_counts + rd.ro, B.counts + rd.ro, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
+    }
     HIPCHK(hipEventRecord(e1, s), W);
     return true;
   };
-  // once the counts are on the host: one solve launch per bucket on stream s
-  auto solve_buckets = [&](hipStream_t s, const int* counts, int* lists, int cap) -> bool {
-    if (counts[kk::NBUCKET] > 0) {
+  auto solve_round = [&](hipStream_t s, const Round& rd) -> bool {
+    const int* c = K.h_counts + rd.ro;
+    if (c[kk::NBUCKET] > 0) {
       set_error(W, "neighbourhood larger than 64 rows");
       return false;
     }
-    nempty += counts[0];
+    nempty += c[0];
     kk::SolveArgs v = va;
-    for (int bk = 0; bk < kk::NBUCKET; ++bk) {
-      v.list = lists + (size_t)bk * cap;
-      v.nlist = counts[bk];
-      if (bk == 0) HIPCHK(kk::launch_empty(v, s), W);
-      else HIPCHK(kk::launch_solve(v, kk::bucket_width(bk), K.precision, tb, s), W);
+    v.ctr = B.counts + 17;
+    v.list = reuse ? rd.empties : rd.lists;
+    v.nlist = c[0];
+    HIPCHK(kk::launch_empty(v, s), W);
+    if (reuse) {
+      v.setcnt = B.setcnt;
+      v.setbase = B.setbase;
+      v.members = B.members;
+      v.item_max = K.item_max;
+      for (int bk = 1; bk < kk::NBUCKET; ++bk) n_items += c[bk];
+      n1_items += c[kk::SET_N1_ITEMS];
+      n1_blocks += c[kk::SET_N1_BLOCKS];
+    }
+    for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+      if (reuse) v.items = rd.items + c[kk::SET_OFF + bk];
+      else v.list = rd.lists + (size_t)bk * rd.cap;
+      v.nlist = c[bk];
+      HIPCHK(kk::launch_solve(v, kk::bucket_width(bk), K.precision, tb, s), W);
     }
     return true;
   };
-  auto bucket_solve = [&](hipStream_t s, const int* sel, int nsel, int* dcounts, int* lists, int cap) -> bool {
-    if (!bucket(s, sel, nsel, dcounts, K.h_counts, lists, cap, K.ev[5], K.ev[6])) return false;
+  auto prepare_solve = [&](hipStream_t s, const Round& rd) -> bool {
+    if (!prepare(s, rd, K.ev[5], K.ev[6])) return false;
     HIPCHK(hipEventSynchronize(K.ev[6]), W);
     float bms = 0.0f;
     HIPCHK(hipEventElapsedTime(&bms, K.ev[5], K.ev[6]), W);
     bucket_ms += bms;
-    return solve_buckets(s, K.h_counts, lists, cap);
+    return solve_round(s, rd);
   };
+  HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * PASS_COUNTERS, st), W);
   HIPCHK(hipEventRecord(K.ev[0], st), W);
   bool overlap = false;  // exact search of flagged queries runs on stream2
+  bool exact_used = false;  // the fast search ran and flagged queries for the exact kernel
   int nx = 0;            // blocks solved in the second round
   // with a Morton order and enough blocks the search runs in two chunks, and
   // the first chunk's blocks are bucketed and solved on stream3 while the
   // second chunk is searched
   int split = 0;
+  Round first{};
   // ---- search -----------------------------------------------------------
   kk::SearchArgs sa{};
   sa.ndim = nd;
@@ -915,6 +1028,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     const size_t lds = kk::fast_search_lds(nd, sector, sa.k, nmax);
     int nexact = nb;
     const int* exact_list = nullptr;
+    exact_used = false;
     // sectored space-time searches run the exact kernel for every query
     if (K.search_mode > 0 && lds <= 64 * 1024 && !(K.st && sector)) {
       sa.qflag = B.sflag;
@@ -926,7 +1040,11 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
         const int* perm = K.reorder ? K.d_perm.as<int>() : nullptr;
         // first chunk 60% of the blocks (measured: 11.15 ms at 60/40, about
         // 11.35 ms at 50/50 and 70/30 on the 3-D workload)
-        split = (perm && K.chunked && nb >= (1 << 17)) ? (int)((size_t)nb * 3 / 5 + 63) / 64 * 64 : 0;
+        // With factor reuse the search runs in one launch: the set grouping
+        // of a first chunk, beside the second search, delayed that chunk's
+        // solves more than the overlap saved (measured at 4.2M nodes: 11.4
+        // ms per pass chunked against 11.1 ms in one launch)
+        split = (perm && K.chunked && !reuse && nb >= (1 << 17)) ? (int)((size_t)nb * 3 / 5 + 63) / 64 * 64 : 0;
         sa.qlist = perm;
         if (split) {
           sa.nq = split;
@@ -935,9 +1053,15 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
           // count) are bucketed right behind its search, into the second half
           // of the list area: queued behind the next search they would wait
           // for its waves to retire
-          if (!bucket(st, perm, split, B.counts + 16, K.h_counts + 16, B.lists + (size_t)(kk::NBUCKET + 1) * nb,
-                      split, K.ev[12], K.ev[13]))
-            return false;
+          first = Round{perm, split, ROUND_CTR[0], B.sflag, B.lists + (size_t)(kk::NBUCKET + 1) * nb, split,
+                        B.items, B.empties};
+          if (reuse) {
+            // the set grouping takes longer than plain bucketing: it runs on
+            // stream3 beside the second search instead of delaying it
+            HIPCHK(hipEventRecord(K.ev[7], st), W);
+            HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[7], 0), W);
+          }
+          if (!prepare(reuse ? K.stream3 : st, first, K.ev[12], K.ev[13])) return false;
           sa.qlist = perm + split;
           sa.nq = nb - split;
         }
@@ -954,8 +1078,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
         HIPCHK(hipEventElapsedTime(&bucket_ms_first, K.ev[12], K.ev[13]), W);
         bucket_ms += bucket_ms_first;
         HIPCHK(hipStreamWaitEvent(K.stream3, K.ev[13], 0), W);
-        if (!solve_buckets(K.stream3, K.h_counts + 16, B.lists + (size_t)(kk::NBUCKET + 1) * nb, split))
-          return false;
+        if (!solve_round(K.stream3, first)) return false;
         HIPCHK(hipEventRecord(K.ev[12], K.stream3), W);
       }
       HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
@@ -966,6 +1089,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
       HIPCHK(hipMemcpyAsync(&nexact, B.counts + 8, sizeof(int), hipMemcpyDeviceToHost, st), W);
       HIPCHK(hipStreamSynchronize(st), W);
       exact_list = B.xlist;
+      exact_used = true;
     }
     K.timing[7] = exact_list ? nexact : -1;
     K.timing[8] = exact_list ? (split ? 2 : 1) : 0;  // fast-search launches between ev[8] and ev[9]
@@ -1006,11 +1130,25 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   // round 0: every block the search has decided (the second chunk's, when
   // the first was solved on stream3); round 1: the blocks the exact search
   // decided on the second stream
+  // (in the first two, blocks left to the exact search are skipped by their
+  // search flag: their lists may still be in flight)
+  const int* skip = exact_used ? B.sflag : nullptr;
   for (int round = 0; round < (nx > 0 ? 2 : 1); ++round) {
     if (round == 1) HIPCHK(hipStreamWaitEvent(st, K.ev[11], 0), W);
-    const int* sel = round ? B.xlist : (split ? K.d_perm.as<int>() + split : nullptr);
-    const int nsel = round ? nx : nb - split;
-    if (!bucket_solve(st, sel, nsel, B.counts, B.lists, nb)) return false;
+    Round rd{};
+    rd.sel = round ? B.xlist : (split ? K.d_perm.as<int>() + split : nullptr);
+    rd.nsel = round ? nx : nb - split;
+    rd.ro = ROUND_CTR[1 + round];
+    rd.skip = round ? nullptr : skip;
+    rd.lists = B.lists;
+    rd.cap = nb;
+    // the exact-search round starts its work lists at 0: the first chunk's
+    // solves on stream3, which read that area, have finished by then
+    const size_t lo = round ? 0 : (size_t)split;
+    if (round == 1 && split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
+    rd.items = reuse ? B.items + lo : nullptr;
+    rd.empties = reuse ? B.empties + lo : nullptr;
+    if (!prepare_solve(st, rd)) return false;
   }
   if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
   HIPCHK(hipEventRecord(K.ev[3], st), W);
@@ -1024,6 +1162,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     if (nref > 0) {
       va.list = B.lists;
       va.nlist = nref;
+      va.ctr = B.counts + 20;
       HIPCHK(kk::launch_solve(va, kk::solve_width(mmax), kk::PREC_FP64, tb, st), W);
     }
   }
@@ -1048,6 +1187,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     }
   }
   HIPCHK(hipEventRecord(K.ev[4], st), W);
+  HIPCHK(hipMemcpyAsync(K.h_counts + 16, B.counts + 16, sizeof(int) * 8, hipMemcpyDeviceToHost, st), W);
   HIPCHK(hipStreamSynchronize(st), W);
   float ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[1]), W);
@@ -1062,15 +1202,29 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   K.timing[9] = K.timing[10] = 0.0;
   if (K.timing[8] > 0.0) {
     HIPCHK(hipEventElapsedTime(&ms, K.ev[8], K.ev[9]), W);
-    K.timing[9] = ms - bucket_ms_first;  // the first chunk's bucketing runs between the two launches
+    // plain bucketing of the first chunk runs between the two launches (the
+    // set grouping runs on stream3)
+    K.timing[9] = reuse ? ms : ms - bucket_ms_first;
   }
   if (O.need_search) {
     HIPCHK(hipEventElapsedTime(&ms, K.ev[10], K.ev[11]), W);
     K.timing[10] = ms;
   }
+  // counters with the reference's meaning (kriging_base.F90:3260-3330):
+  // Cholesky factorisations, blocks solved from another block's factorisation,
+  // blocks of the symmetric-indefinite fallback (failures are split off when
+  // the results are fetched)
+  const int* hc = K.h_counts;
   K.stats[0] = 0;
-  K.stats[1] = nb - nempty - nfb;
-  K.stats[2] = 0;
+  if (reuse) {
+    const long items_ok = (n_items - n1_items) - hc[17];
+    const long blocks_ok = ((long)hc[16] - n1_blocks) - hc[18];
+    K.stats[1] = (int)(items_ok + (nref - hc[20]));
+    K.stats[2] = (int)(blocks_ok - items_ok);
+  } else {
+    K.stats[1] = nb - nempty - nfb;
+    K.stats[2] = 0;
+  }
   K.stats[3] = nfb;
   K.stats[4] = 0;
   K.timing[5] = nref;
@@ -1675,7 +1829,7 @@ bool scratch_ptr(void** out, const char* W) {
 
 // carves the pass buffers of nb blocks (list stride mmax, k search
 // candidates) out of base; returns the bytes used (base may be null to size)
-size_t carve_pass(char* base, int nb, int mmax, int k, PassBufs& B) {
+size_t carve_pass(char* base, int nb, int mmax, int k, bool reuse, PassBufs& B) {
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char* p = base ? base + off : nullptr;
@@ -1687,13 +1841,27 @@ size_t carve_pass(char* base, int nb, int mmax, int k, PassBufs& B) {
   B.sflag = reinterpret_cast<int*>(take(sizeof(int) * nb));
   B.xlist = reinterpret_cast<int*>(take(sizeof(int) * nb));
   B.flag = reinterpret_cast<int*>(take(sizeof(int) * nb));
-  B.counts = reinterpret_cast<int*>(take(sizeof(int) * 32));
+  B.counts = reinterpret_cast<int*>(take(sizeof(int) * PASS_COUNTERS));
   B.lists = reinterpret_cast<int*>(take(sizeof(int) * ((size_t)(kk::NBUCKET + 1) * nb * 2 + 64)));
   B.hcap = std::max(1, std::min(nb, 4096));
   B.hdis = reinterpret_cast<double*>(take(sizeof(double) * (size_t)k * B.hcap));
   B.hidx = reinterpret_cast<int*>(take(sizeof(int) * (size_t)k * B.hcap));
   B.fbcap = std::max(1, std::min(nb, 256));
   B.fb = reinterpret_cast<double*>(take(sizeof(double) * B.fbcap * kk::fallback_words(mmax)));
+  if (reuse) {
+    int* sb = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nb * 7));
+    B.rep = sb;
+    B.rank = sb ? sb + nb : nullptr;
+    B.setcnt = sb ? sb + 2 * (size_t)nb : nullptr;
+    B.setbase = sb ? sb + 3 * (size_t)nb : nullptr;
+    B.members = sb ? sb + 4 * (size_t)nb : nullptr;
+    B.empties = sb ? sb + 5 * (size_t)nb : nullptr;
+    B.setitem = sb ? sb + 6 * (size_t)nb : nullptr;
+    B.items = reinterpret_cast<int2*>(take(sizeof(int2) * (size_t)nb));
+    B.tcap = set_table_slots(nb);
+    B.table = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * B.tcap));
+    B.hash = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * (size_t)nb));
+  }
   return off;
 }
 
@@ -2610,6 +2778,19 @@ int kk_set_options(int64_t h, int precision, double pivot_tol, int table_mode) {
 // order as given, 1 Morton order in two chunks with the first chunk's solve
 // overlapping the second chunk's search (default), 2 Morton order in one
 // launch (every phase timed on its own).
+int kk_set_factor_reuse(int64_t h, int mode, int item_max) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (mode < -1 || mode > 1 || item_max < 1 || item_max > kk::MAX_ITEM) {
+    set_error("kk_set_factor_reuse", "mode must be -1, 0 or 1 and item_max in 1.." + std::to_string(kk::MAX_ITEM));
+    return ierr();
+  }
+  K->reuse_mode = mode;
+  K->item_max = item_max;
+  return 0;
+}
+
 int kk_set_search_mode(int64_t h, int mode, int reorder) {
   clear_error();
   Krige* K = checked(h);
@@ -2779,7 +2960,7 @@ int kk_solve_resident_noalloc(int64_t h, double* est, double* var) {
   int mmax = 0, k = 0;
   pass_dims(*K, mmax, k);
   PassBufs B;
-  const size_t need = carve_pass(nullptr, K->nblock, mmax, k, B);
+  const size_t need = carve_pass(nullptr, K->nblock, mmax, k, K->reuse_on(), B);
   if (need > KK_NOALLOC_BYTES) {
     set_error(W, "nblock=" + std::to_string(K->nblock) + " with " + std::to_string(mmax) +
                      " rows needs " + std::to_string(need >> 20) + " MiB of scratch, above the " +
@@ -2788,7 +2969,7 @@ int kk_solve_resident_noalloc(int64_t h, double* est, double* var) {
   }
   void* base = nullptr;
   if (!scratch_ptr(&base, W)) return ierr();
-  carve_pass(static_cast<char*>(base), K->nblock, mmax, k, B);
+  carve_pass(static_cast<char*>(base), K->nblock, mmax, k, K->reuse_on(), B);
   B.est = est;
   B.var = var;
   run_pipeline(*K, &B);

@@ -52,6 +52,37 @@ hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* coun
 int bucket_width(int b);  // solve width (matrix rows) of bucket b (0 for the empty bucket)
 int solve_width(int n);   // smallest solve width holding n rows (BUCKET_ROWS)
 
+// neighbour-set grouping of the blocks sel[0..nsel) (factor reuse), counters
+// per round in ctr[SET_NCTR]: [0] blocks without neighbours, [1..NBUCKET]
+// work items per bucket, [SET_N1_ITEMS], [SET_N1_BLOCKS] single-neighbour
+// sets; bucket k's items start at items + ctr[SET_OFF + k]
+constexpr int SET_CURSOR = NBUCKET + 1;
+constexpr int SET_N1_ITEMS = SET_CURSOR + NBUCKET + 1;
+constexpr int SET_N1_BLOCKS = SET_N1_ITEMS + 1;
+constexpr int SET_OFF = SET_N1_BLOCKS + 1;
+constexpr int SET_NCTR = 32;
+static_assert(SET_OFF + NBUCKET + 1 <= SET_NCTR, "set counters fit");
+struct SetGroupArgs {
+  const int* nnear;
+  const int* inear;
+  int mmax;                    // list row stride, at most 64
+  const double* rs;            // per-block range scale or null
+  const double* ln;            // per-block local nugget or null
+  const int* sel;              // blocks of this round (null: 0..nsel-1)
+  int nsel;
+  const int* skip;             // per block: nonzero = leave out (null: none)
+  unsigned long long* table;   // tsize slots, a power of two >= 2 nsel
+  size_t tsize;
+  int item_max;                // members per work item
+  int *rep, *rank, *cnt, *setbase, *setitem, *members;  // per block (block ids index them)
+  unsigned long long* hash;    // per block
+  int* ctr;                    // SET_NCTR counters of this round
+  int* total;                  // members placed so far in this pass
+  int2* items;                 // nsel work items (rep, first member slot)
+  int* empties;                // blocks without neighbours
+};
+hipError_t launch_set_group(const SetGroupArgs& g, hipStream_t st);
+
 hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hipStream_t st);
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st);
 // Bunch-Kaufman fallback: scratch holds fallback_words(mmax) doubles per listed block

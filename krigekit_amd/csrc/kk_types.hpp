@@ -4,6 +4,8 @@
 // (node records + rearranged coordinates), the covariance model (structure
 // parameters + lookup tables) and the per-solve block/observation arrays.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace kk {
@@ -142,6 +144,12 @@ struct MVArgs {
   int* flag;                  // 2: K or S not positive definite (fallback); 3: fallback failed (NaN)
 };
 
+// member blocks per work item of the single-variable solve (factor reuse):
+// at most MAX_ITEM, fewer for wider systems, whose extra right-hand-side
+// columns cost registers
+constexpr int MAX_ITEM = 4;
+__host__ __device__ constexpr int item_cap(int rows) { return rows <= 16 ? 4 : rows <= 32 ? 2 : 1; }
+
 struct SolveArgs {
   const Model* model;         // device copy of the covariance model (space-time: spatial part)
   const Model* tmodel;        // space-time only: temporal structures (1-D)
@@ -164,8 +172,16 @@ struct SolveArgs {
   const int* nnear;
   const int* inear;
   int mmax;
-  const int* list;            // block ids of this bucket
-  int nlist;
+  const int* list;            // block ids of this bucket (when items is null)
+  int nlist;                  // blocks, or work items
+  // factor reuse: work items (representative block, first member slot), each
+  // with min(item_max, setbase[rep] + setcnt[rep] - slot) members in members
+  const int2* items;
+  const int* setcnt;
+  const int* setbase;
+  const int* members;
+  int item_max;
+  int* ctr;                   // [0] flagged work items, [1] their blocks (or null)
   double pivot_tol;           // FP32 pivot ratio below which a system is re-solved in FP64
   double* est;
   double* var;

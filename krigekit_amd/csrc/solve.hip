@@ -103,17 +103,28 @@ __device__ inline double pair_cov_tb(const SolveArgs& a, const TabB& Ts, const T
 // G lanes per system, R <= G matrix rows (R < G: lanes R..G-1 hold no row;
 // the 16-lane 12-row, 32-lane 24-row and 64-lane 48-row instances take the
 // systems just above the next smaller width with rows a quarter shorter:
-// fewer registers, fold offsets and trailing updates)
+// fewer registers, fold offsets and trailing updates).
+//
+// Each group of G lanes takes one work item: a neighbour set (the rows of its
+// representative block) and up to MAX_ITEM member blocks whose matrices are
+// identical to it (or, with no item list, one block on its own).  K is
+// assembled and factored once, with the unbias column F, the data values z
+// and every member's right-hand side b eliminated alongside (the forward
+// substitutions), so a block's arithmetic is the same whether it shares its
+// factorisation or not.
 template <int G, int DIM, int PREC, bool TB, int R = G>
 __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   constexpr int S = 64 / G;
+  constexpr int MB = item_cap(R);  // member blocks per work item
   static_assert(R <= G && 2 * G <= 3 * R, "lanes past R store into the unread fold slots");
   __shared__ double sx[S][DIM][G];
   // fold exchange: sp[d][r] = C(row r, row (r + d) mod R) for d = 1..R/2 as
-  // lane r computed it, sp[0][r] = diagonal; later reused for the pivot row's
-  // extra columns and (as floats) the transposed FP32 factor
+  // lane r computed it, sp[0][r] = diagonal; later the pivot row's extra
+  // columns and (as floats) the FP32 factor
   __shared__ double sp[S][R / 2 + 1][R];
   __shared__ double col[S][G];
+  // FP32+refine: the members' FP32 solutions, broadcast for the FP64 residuals
+  __shared__ double xs[PREC == PREC_FP32R ? S : 1][MB][G];
 
   const int lane = threadIdx.x, s = lane / G, r = lane % G;
   // lane r holds row r; the slots (R/2, R/2..R-1) are never read, so lanes
@@ -124,13 +135,28 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   auto wrap = [](int c) { return R == G ? (c & (G - 1)) : (c >= R ? c - R : c); };
   const int sys = blockIdx.x * S + s;
   const bool act = sys < a.nlist;
-  const int b = act ? a.list[sys] : 0;
+  // work item: representative block b, members m0 .. m0 + mc - 1 of the
+  // member array (or b alone)
+  int b = 0, m0 = 0, mc = 0;
+  if (act) {
+    if (a.items) {
+      const int2 it = a.items[sys];
+      b = it.x;
+      m0 = it.y;
+      mc = min(min(a.item_max, MB), a.setbase[b] + a.setcnt[b] - m0);
+    } else {
+      b = a.list[sys];
+      mc = 1;
+    }
+  }
   const int n = act ? a.nnear[b] : 0;
   const bool live = r < n;
   const int idx = live ? a.inear[(size_t)b * a.mmax + r] : 0;
-  // largest system in this wave; readfirstlane makes it a scalar, so the
-  // k/j loop guards below are scalar branches instead of exec-mask regions
+  // largest system and largest member count in this wave; readfirstlane makes
+  // them scalars, so the k/j loop guards below are scalar branches instead of
+  // exec-mask regions
   const int nw = __builtin_amdgcn_readfirstlane(wmax(n));
+  const int mw = __builtin_amdgcn_readfirstlane(wmax(mc));
 
   double x[4] = {0.0, 0.0, 0.0, 0.0};
   if (live) {
@@ -142,7 +168,8 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const double z = live ? a.ovalue[idx] : 0.0;
   const double ov = live ? a.ovar[idx] : 0.0;
   // lags are divided by the block's range scale only when one is set (the
-  // host passes none when every block's scale is 1)
+  // host passes none when every block's scale is 1); every member of a set
+  // has the representative's scale and local nugget
   const bool scaled = a.rangescale != nullptr;
   const double rs = (act && scaled) ? a.rangescale[b] : 1.0;
   const double ln = (act && a.localnugget) ? a.localnugget[b] : 0.0;
@@ -221,13 +248,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     }
   }
   if (row) sp[s][0][r] = live ? (cov0 + ov) + ln : 1.0;
-  // right-hand side: b_r = sum_k w_k C((x_r - g_k)/rs)
-  double rhs = 0.0;
-  if (live) {
+  // right-hand side of this lane's row for block mb:
+  // b_r = sum_k w_k C((x_r - g_k)/rs) over the block's integration points
+  auto rhs_of = [&](int mb) -> double {
+    double acc = 0.0;
+    if (!live) return acc;
     for (int q = 0; q < a.nsub; ++q) {
-      const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)b
-                                                 : a.scoord + DIM * ((size_t)b * a.nsub + q);
-      const double w = a.sweight ? a.sweight[(size_t)b * a.nsub + q] : 1.0;
+      const double* g = a.nsub == 1 && !a.scoord ? a.bcoord + DIM * (size_t)mb
+                                                 : a.scoord + DIM * ((size_t)mb * a.nsub + q);
+      const double w = a.sweight ? a.sweight[(size_t)mb * a.nsub + q] : 1.0;
       double dx, dy, dz, dt;
       lag(g, 1, dx, dy, dz, dt);
       double v;
@@ -238,8 +267,22 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       } else {
         v = pair_cov<DIM, TB>(a, M, dx, dy, dz, dt);
       }
-      rhs = rhs + v * w;
+      acc = acc + v * w;
     }
+    return acc;
+  };
+  // the item's member blocks (member j of the item; the block itself without
+  // an item list) and their right-hand sides, evaluated before the matrix
+  // rows are loaded; columns past the wave's largest member count are
+  // skipped by scalar guards
+  auto member = [&](int j) { return a.items ? a.members[m0 + j] : b; };
+  int mb[MB];
+  double bv[MB];
+#pragma unroll
+  for (int c = 0; c < MB; ++c) {
+    const bool on = c < mc;
+    mb[c] = on ? member(c) : b;
+    bv[c] = (c < mw && on) ? rhs_of(mb[c]) : 0.0;
   }
   // ones on the live rows: the unbias row F for ordinary kriging, and the
   // weight sum 1^T K^-1 b that simple kriging needs for its mean term
@@ -259,23 +302,71 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const double kdiag = sp[s][0][rr];
 
   int fail = 0;
-  double uu, uy, yy, vy, vu;
+  // outputs of member block mb from the quadratic forms uu = F'K^-1F,
+  // uy = F'K^-1b, yy = b'K^-1b, vy = z'K^-1b, vu = z'K^-1F
+  auto write_out = [&](int mb, double uu, double uy, double yy, double vy, double vu) {
+    if (n == 1) {
+      // the reference skips the solve for a single neighbour (solve_base,
+      // kriging_base.F90:2127): variance keeps its NaN fill; the estimate
+      // would reuse another block's weights and is reported as NaN
+      a.est[mb] = __builtin_nan("");
+      a.var[mb] = __builtin_nan("");
+      a.flag[mb] = 0;
+      return;
+    }
+    if (fail & 1) {
+      a.flag[mb] = 2;  // not positive definite: Bunch-Kaufman fallback
+      return;
+    }
+    if (fail & 4) {
+      a.flag[mb] = 1;  // FP32 pivot too small: FP64 re-solve
+      return;
+    }
+    double var, est;
+    const double base = a.base_cov ? a.base_cov[mb] : cov0;
+    if (a.unbias) {
+      const double lam = (uy - 1.0) / uu;
+      var = (base - yy) + lam * (uy - 1.0);
+      est = vy - lam * vu;
+    } else {
+      var = base - yy;
+      est = vy;
+      if (a.sk_mean != 0.0) est = est + (1.0 - uy) * a.sk_mean;
+    }
+    var = var > 0.0 ? var : 0.0;
+    est = est < a.bound_lo ? a.bound_lo : est;
+    est = est > a.bound_hi ? a.bound_hi : est;
+    a.est[mb] = est;
+    a.var[mb] = var;
+    a.flag[mb] = 0;
+  };
+  // flagged work items and their blocks (rare: two atomics each)
+  auto count_flagged = [&]() {
+    if (act && r == 0 && n > 1 && (fail & 5) && a.ctr) {
+      atomicAdd(&a.ctr[0], 1);
+      atomicAdd(&a.ctr[1], mc);
+    }
+  };
 
   if (PREC == PREC_FP64) {
-    // ---- L D L^T, row r in A[], with b, F, z carried as three extra
-    // columns: eliminating them alongside K is the forward substitution
-    // y = L^-1 b, u = L^-1 F, v = L^-1 z ----------------------------------
+    // ---- L D L^T, row r in A[], with F, z and the members' b carried as
+    // extra columns: eliminating them alongside K is the forward substitution
+    // u = L^-1 F, v = L^-1 z, y_c = L^-1 b_c --------------------------------
     double D = 1.0;
-    double y = rhs, u = f, v = z;
-    double* ext = &sp[s][0][0];  // pivot row's (y, u, v); sp is free after the row loads
+    double u = f, v = z;
+    double y[MB];
+#pragma unroll
+    for (int c = 0; c < MB; ++c) y[c] = bv[c];
+    double* ext = &sp[s][0][0];  // pivot row's extra columns; sp is free after the row loads
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       if (k < nw) {
         col[s][r] = A[k];
         if (r == k) {
-          ext[0] = y;
-          ext[1] = u;
-          ext[2] = v;
+          ext[0] = u;
+          ext[1] = v;
+#pragma unroll
+          for (int c = 0; c < MB; ++c) ext[2 + c] = y[c];
         }
         __syncthreads();
         const double akk = col[s][k];
@@ -286,23 +377,32 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         // harmless and keeps the unrolled loop branch-free
 #pragma unroll
         for (int j = k + 1; j < R; ++j) A[j] = fma(-t, col[s][j], A[j]);
-        y = fma(-t, ext[0], y);
-        u = fma(-t, ext[1], u);
-        v = fma(-t, ext[2], v);
+        u = fma(-t, ext[0], u);
+        v = fma(-t, ext[1], v);
+#pragma unroll
+        for (int c = 0; c < MB; ++c)
+          if (c < mw) y[c] = fma(-t, ext[2 + c], y[c]);
         __syncthreads();
       }
     }
     const double id = live ? 1.0 / D : 0.0;
-    uu = gsum<G>(u * u * id);
-    uy = gsum<G>(u * y * id);
-    yy = gsum<G>(y * y * id);
-    vy = gsum<G>(v * y * id);
-    vu = gsum<G>(v * u * id);
+    const double uu = gsum<G>(u * u * id);
+    const double vu = gsum<G>(v * u * id);
+    fail = gmax<G>(fail);
+    count_flagged();
+#pragma unroll
+    for (int c = 0; c < MB; ++c) {
+      if (c >= mw) continue;
+      const double uy = gsum<G>(u * y[c] * id);
+      const double yy = gsum<G>(y[c] * y[c] * id);
+      const double vy = gsum<G>(v * y[c] * id);
+      if (act && r == 0 && c < mc) write_out(mb[c], uu, uy, yy, vy, vu);
+    }
   } else {
-    // ---- FP32 L D L^T; b, F, z carried as extra columns (fused forward
-    // substitution), one FP32 back substitution for xb ~ K^-1 b,
-    // x1 ~ K^-1 F, xz ~ K^-1 z.  Each quadratic form c'K^-1 d is then
-    // corrected with the FP64 residual of d:
+    // ---- FP32 L D L^T; F, z and the members' b carried as extra columns
+    // (fused forward substitution), one FP32 back substitution for
+    // x1 ~ K^-1 F, xz ~ K^-1 z, xb_c ~ K^-1 b_c.  Each quadratic form
+    // c'K^-1 d is corrected with the FP64 residual of d:
     //   c'K^-1 d = c'x_d + x_c'(d - K x_d) - e_c' K e_d,
     // e = solve errors, so the dropped term is second order in the FP32
     // error (one refinement step's accuracy without a second solve) --------
@@ -313,15 +413,19 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     for (int j = 0; j < R; ++j) L[j] = (float)A[j];
     float D = 1.0f;
     float pmin = 1.0f;
-    float py = (float)rhs, pu = (float)f, pv = (float)z;
+    float pu = (float)f, pz = (float)z;
+    float py[MB];
+#pragma unroll
+    for (int c = 0; c < MB; ++c) py[c] = (float)bv[c];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       if (k < nw) {
         col32[r] = L[k];
         if (r == k) {
-          ext32[0] = py;
-          ext32[1] = pu;
-          ext32[2] = pv;
+          ext32[0] = pu;
+          ext32[1] = pz;
+#pragma unroll
+          for (int c = 0; c < MB; ++c) ext32[2 + c] = py[c];
         }
         __syncthreads();
         const float akk = col32[k];
@@ -330,9 +434,11 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         const float t = (r > k) ? L[k] * __builtin_amdgcn_rcpf(akk) : 0.0f;
 #pragma unroll
         for (int j = k + 1; j < R; ++j) L[j] = fmaf(-t, col32[j], L[j]);
-        py = fmaf(-t, ext32[0], py);
-        pu = fmaf(-t, ext32[1], pu);
-        pv = fmaf(-t, ext32[2], pv);
+        pu = fmaf(-t, ext32[0], pu);
+        pz = fmaf(-t, ext32[1], pz);
+#pragma unroll
+        for (int c = 0; c < MB; ++c)
+          if (c < mw) py[c] = fmaf(-t, ext32[2 + c], py[c]);
         L[k] = (r > k) ? t : L[k];
         __syncthreads();
       }
@@ -352,81 +458,58 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 #pragma unroll
     for (int j = 0; j < R; ++j) Lt[j] = sm32[j * (R + 1) + r];
     const float iD = 1.0f / D;
-    py *= iD;
     pu *= iD;
-    pv *= iD;
+    pz *= iD;
+#pragma unroll
+    for (int c = 0; c < MB; ++c) py[c] *= iD;
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) {
       if (k < nw) {
-        const float yk = __shfl(py, k, G), uk = __shfl(pu, k, G), vk = __shfl(pv, k, G);
         const float l = (r < k) ? Lt[k] : 0.0f;
-        py = fmaf(-l, yk, py);
+        const float uk = __shfl(pu, k, G), zk = __shfl(pz, k, G);
         pu = fmaf(-l, uk, pu);
-        pv = fmaf(-l, vk, pv);
+        pz = fmaf(-l, zk, pz);
+#pragma unroll
+        for (int c = 0; c < MB; ++c) {
+          if (c >= mw) continue;
+          const float yk = __shfl(py[c], k, G);
+          py[c] = fmaf(-l, yk, py[c]);
+        }
       }
     }
-    const double xb = py, x1 = pu, xz = pv;
-    // FP64 residuals rb = b - K xb, r1 = F - K x1 (row r of K is A[], FP64)
-    double rb = rhs, r1 = f;
-    double* xsh = &col[s][0];
+    const double x1 = pu, xz = pz;
+    // FP64 residuals r1 = F - K x1 and rb_c = b_c - K xb_c (row r of K is
+    // A[], FP64)
+    double r1 = f;
     __syncthreads();
-    xsh[r] = xb;
+    col[s][r] = x1;
+#pragma unroll
+    for (int c = 0; c < MB; ++c) xs[s][c][r] = py[c];
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < R; ++j) rb = fma(-A[j], xsh[j], rb);
-    __syncthreads();
-    xsh[r] = x1;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j) r1 = fma(-A[j], xsh[j], r1);
-    uu = gsum<G>(fma(x1, r1, f * x1));
-    uy = gsum<G>(fma(x1, rb, f * xb));
-    yy = gsum<G>(fma(xb, rb, rhs * xb));
-    vy = gsum<G>(fma(xz, rb, z * xb));
-    vu = gsum<G>(fma(xz, r1, z * x1));
+    for (int j = 0; j < R; ++j) r1 = fma(-A[j], col[s][j], r1);
+    const double uu = gsum<G>(fma(x1, r1, f * x1));
+    const double vu = gsum<G>(fma(xz, r1, z * x1));
     // pivot ratio check: re-solve in FP64 below the threshold
     float pm = live ? pmin : 1.0f;
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) pm = fminf(pm, __shfl_xor(pm, m, G));
     if (act && r == 0 && pm < (float)a.pivot_tol) fail |= 4;
+    fail = gmax<G>(fail);
+    count_flagged();
+#pragma unroll
+    for (int c = 0; c < MB; ++c) {
+      if (c >= mw) continue;
+      double rb = bv[c];
+#pragma unroll
+      for (int j = 0; j < R; ++j) rb = fma(-A[j], xs[s][c][j], rb);
+      const double xb = py[c];
+      const double uy = gsum<G>(fma(x1, rb, f * xb));
+      const double yy = gsum<G>(fma(xb, rb, bv[c] * xb));
+      const double vy = gsum<G>(fma(xz, rb, z * xb));
+      if (act && r == 0 && c < mc) write_out(mb[c], uu, uy, yy, vy, vu);
+    }
   }
-
-  fail = gmax<G>(fail);
-  if (!act || r != 0) return;
-  if (n == 1) {
-    // the reference skips the solve for a single neighbour (solve_base,
-    // kriging_base.F90:2127): variance keeps its NaN fill; the estimate would
-    // reuse another block's weights and is reported as NaN
-    a.est[b] = __builtin_nan("");
-    a.var[b] = __builtin_nan("");
-    a.flag[b] = 0;
-    return;
-  }
-  if (fail & 1) {
-    a.flag[b] = 2;  // not positive definite: Bunch-Kaufman fallback
-    return;
-  }
-  if (fail & 4) {
-    a.flag[b] = 1;  // FP32 pivot too small: FP64 re-solve
-    return;
-  }
-  double var, est;
-  const double base = a.base_cov ? a.base_cov[b] : (DIM == 4 ? a.st_cov0 : M.cov0);
-  if (a.unbias) {
-    const double lam = (uy - 1.0) / uu;
-    var = (base - yy) + lam * (uy - 1.0);
-    est = vy - lam * vu;
-  } else {
-    var = base - yy;
-    est = vy;
-    if (a.sk_mean != 0.0) est = est + (1.0 - uy) * a.sk_mean;
-  }
-  var = var > 0.0 ? var : 0.0;
-  est = est < a.bound_lo ? a.bound_lo : est;
-  est = est > a.bound_hi ? a.bound_hi : est;
-  a.est[b] = est;
-  a.var[b] = var;
-  a.flag[b] = 0;
 }
 
 #if KK_SOLVE_DIM == 0

@@ -74,6 +74,8 @@ def _load_lib():
     sig("krige_get_weight_inear", [_i64, _c_int, _c_int, _c_int, _pi])
     sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
     sig("kk_set_search_mode", [_i64, _c_int, _c_int])
+    sig("kk_set_factor_reuse", [_i64, _c_int, _c_int])
+    sig("kk_solve_resident_noalloc", [_i64, ctypes.c_void_p, ctypes.c_void_p])
     sig("kk_solve_resident", [_i64])
     sig("kk_fetch", [_i64])
     sig("kk_get_timing", [_i64, _pd])
@@ -320,6 +322,12 @@ class Kriging:
         if per_block:
             return mse.value, g, zhat, dz
         return mse.value, g
+
+    def set_factor_reuse(self, mode: int = -1, item_max: int = 4):
+        """Factor reuse over identical neighbour sets: -1 auto (on unless
+        cross-validation), 0 off, 1 on; item_max (1..4) member blocks per
+        factorisation.  Results are identical either way."""
+        _check(_load_lib().kk_set_factor_reuse(self._handle, int(mode), int(item_max)), "kk_set_factor_reuse")
 
     def set_search_mode(self, fast=True, reorder=True):
         """fast: True/2 wave-cooperative two-pass search (default), 1 per-lane

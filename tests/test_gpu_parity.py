@@ -34,13 +34,14 @@ def _ref(vec, key):
 
 def _check_counters(g, ref):
     """failures and symmetric-indefinite solves as the reference counts them;
-    Cholesky solves too when no block has a single neighbour (the reference
-    skips those solves, kriging_base.F90:2127)"""
+    blocks solved through a Cholesky factorisation (made for them or reused)
+    too when no block has a single neighbour (the reference skips those
+    solves, kriging_base.F90:2127)"""
     st = g["stats"]
     f, c, s = (int(x) for x in ref["stats"])
     assert st["fail"] == f and st["ssytrf_fact"] == s, (st, ref["stats"])
     if not np.any(ref["nnear"] == 1):
-        assert st["chol_fact"] == c, (st, ref["stats"])
+        assert st["chol_fact"] + st["chol_reuse"] == c, (st, ref["stats"])
 
 
 @pytest.mark.parametrize("spec", CASES, ids=IDS)
@@ -247,3 +248,34 @@ def test_full_size_3d_octant():
     assert np.array_equal(inn[idx][:, :w], o["inear"])
     ee, ev = max_errors(dict(estimate=e64[idx], variance=v64[idx]), o)
     assert ee < FP64_TOL and ev < FP64_TOL, (ee, ev)
+
+
+@pytest.mark.parametrize("case", ["ok3d_chunks", "gq2d", "local_params", "maxdist", "ladder", "near_duplicates"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32r"])
+def test_factor_reuse_is_bitwise_neutral(case, precision):
+    """factor reuse over identical neighbour sets (the reference's factor
+    cache, kriging_base.F90:1746-1917,2445-2485) gives bitwise the results of
+    solving every block on its own; the counters split the Cholesky solves
+    into factorisations and reuses"""
+    from krigekit_amd import synth
+    spec = {"ok3d_chunks": lambda: synth.ok3d(scale=1.0 / 16), "gq2d": lambda: problems.gq_blocks(),
+            "local_params": problems.local_params, "maxdist": problems.maxdist_problem,
+            "ladder": problems.ladder, "near_duplicates": problems.near_duplicates}[case]()
+    out = []
+    for mode in (0, 1):
+        print("case reuse", case, precision, mode, flush=True)
+        k = make_gpu(spec, precision=precision)
+        k.set_factor_reuse(mode, 4)
+        k.solve()
+        e, v = k.get_results(copy=True)
+        nn, _ = k.neighbours()
+        out.append((e, v, k.solver_stats, nn))
+        k.close()
+    (e0, v0, s0, nn), (e1, v1, s1, _) = out
+    assert np.array_equal(e0, e1, equal_nan=True) and np.array_equal(v0, v1, equal_nan=True)
+    assert s1["fail"] == s0["fail"] and s1["ssytrf_fact"] == s0["ssytrf_fact"]
+    assert s0["chol_reuse"] == 0
+    if not np.any(nn == 1):
+        assert s1["chol_fact"] + s1["chol_reuse"] == s0["chol_fact"]
+    if case == "ok3d_chunks":
+        assert s1["chol_reuse"] > s1["chol_fact"]
