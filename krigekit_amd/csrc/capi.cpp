@@ -742,7 +742,8 @@ bool morton_perm(Krige& K) {
 // Counter block of a pass (device, and its pinned host copy): [0] and [8]
 // block counts of the flag collections, [16] set members placed, [17]/[18]
 // flagged work items/blocks of the grouped solves, [20]/[21] those of the
-// FP64 re-solve; per round 32 counters at ROUND_CTR[round] (bucket or
+// FP64 re-solve, [24] blocks above 64 rows, [25] those sent to the fallback;
+// per round 32 counters at ROUND_CTR[round] (bucket or
 // set-grouping counts of the first search chunk, the rest, the exact-search
 // blocks).
 constexpr int ROUND_CTR[3] = {32, 64, 96};
@@ -801,9 +802,13 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   int mmax, kcand;
   pass_dims(K, mmax, kcand);
   K.mmax = mmax;
-  if (mmax > 64) {
-    set_error(W, "neighbourhoods larger than 64 data rows are not supported by the GPU solver (mmax=" +
-                     std::to_string(mmax) + ")");
+  // systems above 64 rows go to the one-workgroup bordered solver, which
+  // holds up to 128 rows and extra columns (not for space-time)
+  const int big_rows_max = 128 - kk::mv_extra_cols(1, 0, K.unbias);
+  const int nsub_pass = K.soff.empty() ? 1 : K.soff[1] - K.soff[0];
+  if (mmax > 64 && (K.st || mmax > big_rows_max || nsub_pass > 64)) {
+    set_error(W, "neighbourhoods larger than " + std::to_string(K.st ? 64 : big_rows_max) +
+                     " data rows are not supported by the GPU solver (mmax=" + std::to_string(mmax) + ")");
     return false;
   }
   PassBufs B;
@@ -944,11 +949,9 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     return true;
   };
   auto solve_round = [&](hipStream_t s, const Round& rd) -> bool {
+    // (blocks above 64 rows, in the overflow bucket or left out of the sets,
+    // are solved after the rounds)
     const int* c = K.h_counts + rd.ro;
-    if (c[kk::NBUCKET] > 0) {
-      set_error(W, "neighbourhood larger than 64 rows");
-      return false;
-    }
     nempty += c[0];
     kk::SolveArgs v = va;
     v.ctr = B.counts + 17;
@@ -1151,6 +1154,51 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     if (!prepare_solve(st, rd)) return false;
   }
   if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
+  // ---- systems above 64 rows: one workgroup each (FP64) --------------------
+  int nbig = 0;
+  if (mmax > 64) {
+    HIPCHK(hipMemsetAsync(B.counts + 24, 0, sizeof(int) * 2, st), W);
+    HIPCHK(kk::launch_collect_above(B.nnear, nb, 64, B.counts + 24, B.xlist, st), W);
+    HIPCHK(hipMemcpyAsync(&nbig, B.counts + 24, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    if (nbig > 0) {
+      kk::MVArgs m{};
+      m.models = K.d_model.as<Model>();
+      m.ndim = nd;
+      m.nvar = 1;
+      m.nb = nb;
+      m.std_ck = 0;
+      m.unbias = K.unbias;
+      m.nsub = va.nsub;
+      m.rows = mmax;
+      m.simple = 1;
+      for (int i = 0; i < K.model.nstruct; ++i) {
+        const int t = K.model.vtype[i];
+        m.simple &= (t == kk::NUG || t == kk::SPH || t == kk::EXP || t == kk::GAU || t == kk::BSQ || t == kk::LIN);
+      }
+      m.sk_mean[0] = O.sk_mean;
+      m.bound_lo = K.bounds[0];
+      m.bound_hi = K.bounds[1];
+      m.bcoord = va.bcoord;
+      m.scoord = va.nsub > 1 ? va.scoord : nullptr;
+      m.sweight = va.nsub > 1 ? va.sweight : nullptr;
+      m.rangescale = va.rangescale;
+      m.localnugget = va.localnugget;
+      m.ocoord[0] = va.ocoord;
+      m.ovalue[0] = va.ovalue;
+      m.ovar[0] = va.ovar;
+      m.nnear[0] = B.nnear;
+      m.inear[0] = B.inear;
+      m.mmax[0] = mmax;
+      m.list = B.xlist;
+      m.nlist = nbig;
+      m.ctr = B.counts + 25;
+      m.est = B.est;
+      m.var = B.var;
+      m.flag = B.flag;
+      HIPCHK(kk::launch_solve_mv(m, st), W);
+    }
+  }
   HIPCHK(hipEventRecord(K.ev[3], st), W);
   // ---- FP64 re-solve of flagged FP32 systems, Bunch-Kaufman fallback -------
   int nref = 0, nfb = 0;
@@ -1171,7 +1219,10 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   HIPCHK(hipMemcpyAsync(&nfb, B.counts, sizeof(int), hipMemcpyDeviceToHost, st), W);
   HIPCHK(hipStreamSynchronize(st), W);
   if (nfb > 0) {
-    const int fchunk = ext ? B.fbcap : 4096;
+    // at most 256 MiB of fallback matrices per launch
+    const int fchunk = ext ? B.fbcap
+                           : (int)std::max<size_t>(1, std::min<size_t>(4096, (size_t(256) << 20) /
+                                                                             (8 * kk::fallback_words(mmax))));
     double* fb = B.fb;
     if (!ext) {
       if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::fallback_words(mmax))) {
@@ -1187,7 +1238,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     }
   }
   HIPCHK(hipEventRecord(K.ev[4], st), W);
-  HIPCHK(hipMemcpyAsync(K.h_counts + 16, B.counts + 16, sizeof(int) * 8, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipMemcpyAsync(K.h_counts + 16, B.counts + 16, sizeof(int) * 16, hipMemcpyDeviceToHost, st), W);
   HIPCHK(hipStreamSynchronize(st), W);
   float ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[1]), W);
@@ -1217,9 +1268,10 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   const int* hc = K.h_counts;
   K.stats[0] = 0;
   if (reuse) {
+    // blocks above 64 rows are factored each on their own
     const long items_ok = (n_items - n1_items) - hc[17];
     const long blocks_ok = ((long)hc[16] - n1_blocks) - hc[18];
-    K.stats[1] = (int)(items_ok + (nref - hc[20]));
+    K.stats[1] = (int)(items_ok + (nref - hc[20]) + (nbig - hc[25]));
     K.stats[2] = (int)(blocks_ok - items_ok);
   } else {
     K.stats[1] = nb - nempty - nfb;
@@ -1846,7 +1898,8 @@ size_t carve_pass(char* base, int nb, int mmax, int k, bool reuse, PassBufs& B) 
   B.hcap = std::max(1, std::min(nb, 4096));
   B.hdis = reinterpret_cast<double*>(take(sizeof(double) * (size_t)k * B.hcap));
   B.hidx = reinterpret_cast<int*>(take(sizeof(int) * (size_t)k * B.hcap));
-  B.fbcap = std::max(1, std::min(nb, 256));
+  B.fbcap = (int)std::max<size_t>(1, std::min<size_t>(std::min(nb, 256), (size_t(4) << 20) /
+                                                                             (8 * kk::fallback_words(mmax))));
   B.fb = reinterpret_cast<double*>(take(sizeof(double) * B.fbcap * kk::fallback_words(mmax)));
   if (reuse) {
     int* sb = reinterpret_cast<int*>(take(sizeof(int) * (size_t)nb * 7));

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
 
   __shared__ Model smodels[NV * NV];
   __shared__ int sfail;
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int b = a.list ? a.list[blockIdx.x] : (int)blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
   // classes) sit in one wave, so a column's pivot and scaling never leave it
   const int tc = t >> 4, tr = t & 15;
@@ -410,7 +410,10 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   __syncthreads();
   if (sfail) {
     // K not positive definite: the Bunch-Kaufman fallback solves the block
-    if (t == 0) a.flag[b] = 2;
+    if (t == 0) {
+      a.flag[b] = 2;
+      if (a.ctr) atomicAdd(a.ctr, 1);
+    }
     return;
   }
 
@@ -474,6 +477,7 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
     }
     if (!spd) {
       a.flag[b] = 2;
+      if (a.ctr) atomicAdd(a.ctr, 1);
       return;
     }
 #pragma unroll
@@ -702,7 +706,7 @@ int mv_extra_cols(int nvar, int std_ck, int unbias) {
 
 template <int DIM, int NV>
 static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
-  dim3 g(a.nb), blk(MVT);
+  dim3 g(a.list ? a.nlist : a.nb), blk(MVT);
   if (rb == 5) {
     if (a.simple) hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, true>), g, blk, 0, st, a);
     else hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, false>), g, blk, 0, st, a);
@@ -713,13 +717,15 @@ static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
 }
 
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
-  if (a.nb <= 0) return hipSuccess;
+  if (a.nb <= 0 || (a.list && a.nlist <= 0)) return hipSuccess;
   const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
-  if (a.rows > MVR || cols > MVR || a.nsub > MAXSUB || a.nvar < 2 || a.nvar > MAXVAR) return hipErrorInvalidValue;
+  if (a.rows > MVR || cols > MVR || a.nsub > MAXSUB || a.nvar < 1 || a.nvar > MAXVAR) return hipErrorInvalidValue;
   const int rb = cols <= 80 ? 5 : 8;
+  // nvar 1: single-variable systems above the row-per-lane solver's 64 rows
 #define KK_MV(D)                                  \
   do {                                           \
-    if (a.nvar == 2) launch_nv<D, 2>(a, rb, st); \
+    if (a.nvar == 1) launch_nv<D, 1>(a, rb, st); \
+    else if (a.nvar == 2) launch_nv<D, 2>(a, rb, st); \
     else if (a.nvar == 3) launch_nv<D, 3>(a, rb, st); \
     else launch_nv<D, 4>(a, rb, st);             \
   } while (0)

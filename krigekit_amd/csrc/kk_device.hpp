@@ -98,7 +98,8 @@ hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
 int loo_reduce_words();
 hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double* val, int nb, double* work,
                              hipStream_t st);
-// multi-variable block solve; rows + mv_extra_cols(...) must not exceed 128
+// multi-variable block solve (one variable: single-variable systems above 64
+// rows); rows + mv_extra_cols(...) must not exceed 128
 int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
 // Bunch-Kaufman fallback of the blocks listed (flag 2 after launch_solve_mv):
@@ -109,6 +110,8 @@ __host__ __device__ constexpr size_t mv_fallback_words(int rows) {
   return 2 * (size_t)(rows + MAXVAR) * (rows + MAXVAR) + 2 * (size_t)(rows + MAXVAR) * MAXVAR + (rows + MAXVAR);
 }
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
+// blocks with more than limit neighbours into list (count: device counter)
+hipError_t launch_collect_above(const int* nnear, int nblock, int limit, int* count, int* list, hipStream_t st);
 // blocks whose flag equals code into list (count: device counter); with
 // perm, taken in the order perm lists them
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list, hipStream_t st,

@@ -139,6 +139,9 @@ struct MVArgs {
   const int* nnear[MAXVAR];
   const int* inear[MAXVAR];
   int mmax[MAXVAR];
+  const int* list;            // blocks to solve (null: 0..nb-1)
+  int nlist;
+  int* ctr;                   // blocks sent to the fallback (or null)
   double* est;                // est[b + nb * q]
   double* var;                // var[b + nb * (i + nvar * j)]
   int* flag;                  // 2: K or S not positive definite (fallback); 3: fallback failed (NaN)

@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void k_set_find(const int* nnear, const int* s
   if (t >= nsel) return;
   const int b = sel ? sel[t] : t;
   const int n = nnear[b];
-  // no neighbours, or left to the exact search (whose lists may still be in
+  // no neighbours, more than the row-per-lane solver's 64 rows (solved
+  // apart), or left to the exact search (whose lists may still be in
   // flight): no set
-  if (n <= 0 || (skip && skip[b])) {
+  if (n <= 0 || n > 64 || (skip && skip[b])) {
     rep[b] = -1;
     return;
   }

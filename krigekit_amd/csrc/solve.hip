@@ -663,6 +663,21 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   a.var[b] = var;
 }
 
+// blocks with more than limit neighbours into list (atomic count)
+__global__ void k_collect_above(const int* nnear, int nblock, int limit, int* count, int* list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool hit = i < nblock && nnear[i] > limit;
+  const unsigned long long m = __ballot(hit);
+  if (m == 0) return;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (hit) list[base + __popcll(m & below)] = i;
+}
+
 // plain copy of n doubles (reads 8n bytes, writes 8n bytes): calibrates HBM
 // traffic counters against a kernel of known size
 __global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, size_t n) {
@@ -684,6 +699,12 @@ hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* coun
   if (nblock <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, sel, nblock, counts,
                      lists, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_collect_above(const int* nnear, int nblock, int limit, int* count, int* list, hipStream_t st) {
+  if (nblock <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_collect_above, dim3((nblock + 255) / 256), dim3(256), 0, st, nnear, nblock, limit, count, list);
   return hipGetLastError();
 }
 

@@ -391,6 +391,15 @@ def near_duplicates(nmax=12):
     return _base("near_duplicates", 2, obs, val, grid, vgm, nmax=nmax)
 
 
+def big_systems():
+    """neighbourhoods above 64 rows (solved by the one-workgroup bordered
+    solver): plain 80 nearest, 3-D octants of 12 (96 rows), and a maxdist
+    that spreads the counts over every solve width and beyond 64"""
+    return [_named(random_problem(2, nobs=1500, nq=400, nmax=80, seed=91), "big80_2d"),
+            _named(random_problem(3, nobs=3000, nq=300, nmax=12, sector=True, seed=93), "big96_3d_octant"),
+            _named(random_problem(2, nobs=1500, nq=500, nmax=110, seed=95, maxdist=640.0), "big_mixed_2d")]
+
+
 def _named(p, name):
     p["name"] = name
     return p
@@ -405,5 +414,5 @@ def parity_cases():
              _named(random_problem(3, sector=True, nmax=3), "random3d_octant"), nested_levelA(), product_structure(),
              local_params(), obs_error(), simple_kriging(), maxdist_problem(),
              ladder(), bounds_case(), aniso_search_case(2), aniso_search_case(3),
-             aniso_search_case(3, sector=True), near_duplicates()]
+             aniso_search_case(3, sector=True), near_duplicates()] + big_systems()
     return cases
