@@ -13,8 +13,8 @@
  *   - coordinates are (ndim, n) column-major; results are block-first;
  *   - input arrays are copied; outputs go to caller-allocated buffers.
  * Entry points for features outside this backend (indicator kriging, SGSIM,
- * drift, gradient constraints, varying variograms, factor cache, weight
- * replay) exist and return 1 with a message.
+ * drift, gradient constraints, varying variograms, factor cache) exist and
+ * return 1 with a message.
  */
 #ifndef KRIGEKIT_AMD_H
 #define KRIGEKIT_AMD_H
@@ -122,17 +122,25 @@ intptr_t krige_to_str(int64_t handle);
 void krige_get_max_threads(int *n);
 void krige_get_num_threads(int *n);
 
-/* weight-store views, src/libkriging/kriging_capi_common.F90:491-577:
- * neighbour lists of the last solve when store_weight was set */
-int krige_free_weight_store(int64_t handle);
-int krige_get_weight_dims(int64_t handle, int *nmax, int *ngroups, int *nblock);
-int krige_get_weight_nnear(int64_t handle, int ngroups, int nblock, int *out);
-int krige_get_weight_inear(int64_t handle, int nmax, int ngroups, int nblock, int *out);
-/* returns 1, not provided */
-int krige_get_weight_data(int64_t handle, int nmax, int ngroups, int nvar, int nblock, double *out);
-int krige_get_weight_var(int64_t handle, int nvar, int nblock, double *out);
+/* The weight store, src/libkriging/kriging_capi_common.F90:491-619 (store
+ * layouts kriging_base.F90:284-295): filled by a solve with store_weight
+ * (and written to weight_file when one is named); krige_set_weights or a
+ * weight_file with use_old_weight make the next solve replay the stored
+ * weights against the current observation values, without search or solve.
+ * Layouts (column-major): nnear(ngroups, nblock), inear(nmax, ngroups,
+ * nblock) 1-based, weight(nmax, ngroups, nvar, nblock), var(nvar, nvar,
+ * nblock), ngroups = nvar; entries past nnear are zero.  The getters copy
+ * the leading part of the store of the requested dimensions. */
+int krige_free_weight_store(int64_t handle);                                      /* :491-500 */
+int krige_get_weight_dims(int64_t handle, int *nmax, int *ngroups, int *nblock);  /* :503-519 */
+int krige_get_weight_nnear(int64_t handle, int ngroups, int nblock, int *out);    /* :522-537 */
+int krige_get_weight_inear(int64_t handle, int nmax, int ngroups, int nblock, int *out); /* :540-555 */
+int krige_get_weight_data(int64_t handle, int nmax, int ngroups, int nvar, int nblock,
+                          double *out);                                           /* :558-573 */
+int krige_get_weight_var(int64_t handle, int nvar, int nblock, double *out);      /* :576-596 */
 int krige_set_weights(int64_t handle, int nmax, int ngroups, int nvar, int nblock, const int *nnear,
-                      const int *inear, const double *weight, const int *order, const double *var);
+                      const int *inear, const double *weight, const int *order,
+                      const double *var);                                         /* :600-619 */
 
 /* ---- space-time kriging, src/libkriging/kriging_st_capi.f90 -----------------
  * Shares the handle registry and the solve/getter entry points above
@@ -215,10 +223,11 @@ int kk_set_search_mode(int64_t handle, int mode, int reorder);
 int kk_solve_resident(int64_t handle);
 /* copy the results of the last kk_solve_resident to the host */
 int kk_fetch(int64_t handle);
-/* out[12]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
+/* out[13]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
  * nodes, Bunch-Kaufman fallback nodes, queries decided by the exact search kernel, fast
  * search launches (0: none), fast search kernel ms (all launches), exact
- * search kernel ms, LOO-gradient kernel ms.
+ * search kernel ms, LOO-gradient kernel ms, weight-store pass (store_weight)
+ * or stored-weight replay (use_old_weight) ms.
  * Times come from HIP events on the stream the kernels run on.  When the fast
  * search leaves queries to the exact kernel, that kernel runs on a second
  * stream while the other blocks are solved: it is outside "search" and its

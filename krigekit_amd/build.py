@@ -26,7 +26,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 # (source, extra flags, object name).  solve.hip is compiled five times: the
 # shared kernels and dispatch, then the k_solve instances of each coordinate
-# count (1-3 spatial, 4 space-time) in a unit of their own, in parallel.
+# count (1-3 spatial, 4 space-time) in a unit of their own, in parallel;
+# cokrige.hip likewise, its k_solve_mv instances split by coordinate count
+# and weights mode.
 UNITS = [
     ("search.hip", ["-ffp-contract=off"], "search.o"),
     ("solve.hip", [], "solve.o"),
@@ -36,7 +38,14 @@ UNITS = [
     ("solve.hip", ["-DKK_SOLVE_DIM=4"], "solve_d4.o"),
     ("loograd.hip", [], "loograd.o"),
     ("cokrige.hip", [], "cokrige.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=1"], "cokrige_d1.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=2"], "cokrige_d2.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=3"], "cokrige_d3.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=1", "-DKK_MV_W=1"], "cokrige_d1w.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=2", "-DKK_MV_W=1"], "cokrige_d2w.o"),
+    ("cokrige.hip", ["-DKK_MV_DIM=3", "-DKK_MV_W=1"], "cokrige_d3w.o"),
     ("sets.hip", [], "sets.o"),
+    ("replay.hip", [], "replay.o"),
     ("capi.cpp", ["-x", "hip"], "capi.o"),
 ]
 

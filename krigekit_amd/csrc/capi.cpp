@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -506,6 +507,36 @@ struct DevVar {
   int mmax = 0;
 };
 
+// The weight store in the reference's layouts (kriging_base.F90:284-295,
+// 2703-2746): order[nb], nnear[g + ng b], inear[k + nm (g + ng b)] (1-based,
+// zero past nnear), weight[k + nm (g + ng (q + nv b))] (zero past nnear),
+// var[i + nv (j + nv b)]; group g holds the neighbours of variable g.
+struct WStore {
+  bool alloc = false;
+  int nm = 0, ng = 0, nv = 0, nb = 0;
+  std::vector<int> order, nnear, inear;
+  std::vector<double> weight, var;
+  void init(int nm_, int ng_, int nv_, int nb_) {
+    alloc = true;
+    nm = nm_;
+    ng = ng_;
+    nv = nv_;
+    nb = nb_;
+    order.resize(nb);
+    for (int b = 0; b < nb; ++b) order[b] = b + 1;
+    nnear.assign((size_t)ng * nb, 0);
+    inear.assign((size_t)nm * ng * nb, 0);
+    weight.assign((size_t)nm * ng * nv * nb, 0.0);
+    var.assign((size_t)nv * nv * nb, 0.0);
+  }
+  // any neighbour stored (t_weight_store%stored)
+  bool stored() const {
+    for (int n : nnear)
+      if (n != 0) return true;
+    return false;
+  }
+};
+
 struct Krige {
   int ndim = 2, nvar = 1, ndrift = 0, unbias = 1, nsim = 0;
   // space-time (krige_st_*): coordinates are (x, y, z, t), ndim = 4 rows,
@@ -538,8 +569,16 @@ struct Krige {
   // results
   bool solved = false;
   std::vector<double> est, var;
-  std::vector<int> nnear, inear;          // weight-store view (0-based ids)
   int mmax = 0;
+  // weight store (store_weight / set_weights / use_old_weight): host copy,
+  // its device copy for the replay (ws_dev: up to date), the weights pass
+  // output and flags, the observation values of a replay
+  WStore ws;
+  bool ws_dev = false;
+  std::string weight_file;
+  DBuf d_wnn, d_win, d_ww, d_wvar, d_wout, d_wflag;
+  DBuf d_rz[kk::MAXVAR];
+  int mmax_w = 0;  // neighbour stride of d_wout
   int stats[5] = {0, 0, 0, 0, 0};
   double timing[16] = {0};
   std::string info;
@@ -1156,46 +1195,51 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   if (split) HIPCHK(hipStreamWaitEvent(st, K.ev[12], 0), W);
   // ---- systems above 64 rows: one workgroup each (FP64) --------------------
   int nbig = 0;
+  // the one-workgroup bordered solver on a list of blocks (nvar 1)
+  auto single_mv = [&](const int* list, int nlist) {
+    kk::MVArgs m{};
+    m.models = K.d_model.as<Model>();
+    m.ndim = nd;
+    m.nvar = 1;
+    m.nb = nb;
+    m.std_ck = 0;
+    m.unbias = K.unbias;
+    m.nsub = va.nsub;
+    m.rows = mmax;
+    m.simple = 1;
+    for (int i = 0; i < K.model.nstruct; ++i) {
+      const int t = K.model.vtype[i];
+      m.simple &= (t == kk::NUG || t == kk::SPH || t == kk::EXP || t == kk::GAU || t == kk::BSQ || t == kk::LIN);
+    }
+    m.sk_mean[0] = O.sk_mean;
+    m.bound_lo = K.bounds[0];
+    m.bound_hi = K.bounds[1];
+    m.bcoord = va.bcoord;
+    m.scoord = va.nsub > 1 ? va.scoord : nullptr;
+    m.sweight = va.nsub > 1 ? va.sweight : nullptr;
+    m.rangescale = va.rangescale;
+    m.localnugget = va.localnugget;
+    m.ocoord[0] = va.ocoord;
+    m.ovalue[0] = va.ovalue;
+    m.ovar[0] = va.ovar;
+    m.nnear[0] = B.nnear;
+    m.inear[0] = B.inear;
+    m.mmax[0] = mmax;
+    m.list = list;
+    m.nlist = nlist;
+    m.est = B.est;
+    m.var = B.var;
+    m.flag = B.flag;
+    return m;
+  };
   if (mmax > 64) {
     HIPCHK(hipMemsetAsync(B.counts + 24, 0, sizeof(int) * 2, st), W);
     HIPCHK(kk::launch_collect_above(B.nnear, nb, 64, B.counts + 24, B.xlist, st), W);
     HIPCHK(hipMemcpyAsync(&nbig, B.counts + 24, sizeof(int), hipMemcpyDeviceToHost, st), W);
     HIPCHK(hipStreamSynchronize(st), W);
     if (nbig > 0) {
-      kk::MVArgs m{};
-      m.models = K.d_model.as<Model>();
-      m.ndim = nd;
-      m.nvar = 1;
-      m.nb = nb;
-      m.std_ck = 0;
-      m.unbias = K.unbias;
-      m.nsub = va.nsub;
-      m.rows = mmax;
-      m.simple = 1;
-      for (int i = 0; i < K.model.nstruct; ++i) {
-        const int t = K.model.vtype[i];
-        m.simple &= (t == kk::NUG || t == kk::SPH || t == kk::EXP || t == kk::GAU || t == kk::BSQ || t == kk::LIN);
-      }
-      m.sk_mean[0] = O.sk_mean;
-      m.bound_lo = K.bounds[0];
-      m.bound_hi = K.bounds[1];
-      m.bcoord = va.bcoord;
-      m.scoord = va.nsub > 1 ? va.scoord : nullptr;
-      m.sweight = va.nsub > 1 ? va.sweight : nullptr;
-      m.rangescale = va.rangescale;
-      m.localnugget = va.localnugget;
-      m.ocoord[0] = va.ocoord;
-      m.ovalue[0] = va.ovalue;
-      m.ovar[0] = va.ovar;
-      m.nnear[0] = B.nnear;
-      m.inear[0] = B.inear;
-      m.mmax[0] = mmax;
-      m.list = B.xlist;
-      m.nlist = nbig;
+      kk::MVArgs m = single_mv(B.xlist, nbig);
       m.ctr = B.counts + 25;
-      m.est = B.est;
-      m.var = B.var;
-      m.flag = B.flag;
       HIPCHK(kk::launch_solve_mv(m, st), W);
     }
   }
@@ -1281,6 +1325,68 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   K.stats[4] = 0;
   K.timing[5] = nref;
   K.timing[6] = nfb;
+  K.timing[12] = 0.0;
+  if (K.store_weight && !ext) {
+    // ---- the weight store: explicit weights of every block with two or
+    // more neighbours, FP64, in the bordered solver (a block whose K is not
+    // positive definite: the Bunch-Kaufman fallback, as in the solve); for
+    // space-time and blocks of more than 64 integration nodes the
+    // Bunch-Kaufman path of each block
+    K.mmax_w = mmax;
+    if (!K.d_wout.ensure(sizeof(double) * (size_t)nb * mmax) || !K.d_wflag.ensure(sizeof(int) * nb)) {
+      set_error(W, "device allocation failed (weight store)");
+      return false;
+    }
+    HIPCHK(hipEventRecord(K.ev[5], st), W);
+    int nw = 0, nwf = 0;
+    HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_collect_above(B.nnear, nb, 1, B.counts, B.xlist, st), W);
+    HIPCHK(hipMemcpyAsync(&nw, B.counts, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    const bool mv_ok = !K.st && va.nsub <= 64;
+    if (nw > 0 && mv_ok) {
+      HIPCHK(hipMemsetAsync(K.d_wflag.p, 0, sizeof(int) * nb, st), W);
+      kk::MVArgs m = single_mv(B.xlist, nw);
+      m.weights = K.d_wout.as<double>();
+      m.mmax_w = mmax;
+      m.weights_only = 1;
+      m.flag = K.d_wflag.as<int>();
+      HIPCHK(kk::launch_solve_mv(m, st), W);
+      HIPCHK(hipMemsetAsync(B.counts, 0, sizeof(int) * 16, st), W);
+      HIPCHK(kk::launch_collect(K.d_wflag.as<int>(), nb, 2, B.counts, B.lists, st), W);
+      HIPCHK(hipMemcpyAsync(&nwf, B.counts, sizeof(int), hipMemcpyDeviceToHost, st), W);
+      HIPCHK(hipStreamSynchronize(st), W);
+      if (nwf > 0) {
+        const int fchunk = (int)std::max<size_t>(1, std::min<size_t>(4096, (size_t(256) << 20) /
+                                                                          (8 * kk::mv_fallback_words(mmax))));
+        if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::mv_fallback_words(mmax))) {
+          set_error(W, "device allocation failed (fallback scratch)");
+          return false;
+        }
+        for (int f0 = 0; f0 < nwf; f0 += fchunk)
+          HIPCHK(kk::launch_fallback_mv(m, B.lists + f0, std::min(fchunk, nwf - f0), K.d_fb.as<double>(), st), W);
+      }
+    } else if (nw > 0) {
+      const int fchunk = (int)std::max<size_t>(1, std::min<size_t>(4096, (size_t(256) << 20) /
+                                                                        (8 * kk::fallback_words(mmax))));
+      if (!K.d_fb.ensure(sizeof(double) * fchunk * kk::fallback_words(mmax))) {
+        set_error(W, "device allocation failed (fallback scratch)");
+        return false;
+      }
+      kk::SolveArgs vw = va;
+      vw.weights = K.d_wout.as<double>();
+      vw.weights_only = 1;
+      for (int f0 = 0; f0 < nw; f0 += fchunk) {
+        vw.list = B.xlist + f0;
+        vw.nlist = std::min(fchunk, nw - f0);
+        HIPCHK(kk::launch_fallback(vw, K.d_fb.as<double>(), st), W);
+      }
+    }
+    HIPCHK(hipEventRecord(K.ev[6], st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    HIPCHK(hipEventElapsedTime(&ms, K.ev[5], K.ev[6]), W);
+    K.timing[12] = ms;
+  }
   return true;
 }
 
@@ -1309,16 +1415,6 @@ bool fetch_results(Krige& K) {
       set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
       return false;
     }
-  }
-  if (K.store_weight) {
-    K.nnear.resize(nb);
-    K.inear.resize((size_t)nb * K.mmax);
-    HIPCHK(hipMemcpy(K.nnear.data(), K.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
-    HIPCHK(hipMemcpy(K.inear.data(), K.d_inear.p, sizeof(int) * (size_t)nb * K.mmax, hipMemcpyDeviceToHost), W);
-    // a failed block keeps one neighbour in the store, as the reference sets
-    // ctx%nnear = 1 for it (kriging_base.F90:3369-3383)
-    for (size_t b = 0; b < flag.size(); ++b)
-      if (flag[b] == 3) K.nnear[b] = 1;
   }
   // blocks without neighbours are an error unless neglect_error
   // (kriging_base.F90:3198-3207)
@@ -1391,8 +1487,8 @@ bool validate_for_solve(Krige& K) {
           return false;
         }
     }
-  if (K.use_old_weight || K.varying_vgm || K.weight_correction) {
-    set_error(W, "use_old_weight, varying_vgm and weight_correction are not supported by this backend");
+  if (K.varying_vgm || K.weight_correction) {
+    set_error(W, "varying_vgm and weight_correction are not supported by this backend");
     return false;
   }
   if (K.obs[0].n == 0) {
@@ -1745,6 +1841,18 @@ bool do_solve_mv(Krige& K, bool fetch) {
   a.est = K.d_mvest.as<double>();
   a.var = K.d_mvvar.as<double>();
   a.flag = K.d_flag.as<int>();
+  if (K.store_weight) {
+    // the weight store: the solver's weights instance, the fallback writes
+    // the weights of the blocks it solves
+    K.mmax_w = 0;
+    for (int v = 0; v < nvar; ++v) K.mmax_w = std::max(K.mmax_w, K.dv[v].mmax);
+    if (!K.d_wout.ensure(sizeof(double) * (size_t)K.mmax_w * nvar * nvar * nb)) {
+      set_error(W, "device allocation failed (weight store)");
+      return false;
+    }
+    a.weights = K.d_wout.as<double>();
+    a.mmax_w = K.mmax_w;
+  }
   HIPCHK(kk::launch_solve_mv(a, st), W);
   // blocks whose K or Schur complement is not positive definite: the
   // Bunch-Kaufman fallback on the full augmented system
@@ -1815,18 +1923,377 @@ bool do_solve_mv(Krige& K, bool fetch) {
 }
 
 
+// ---------------------------------------------------------------------------
+// weight store (kriging_base.F90:2703-2977, kriging_capi_common.F90:491-619)
+// ---------------------------------------------------------------------------
+// obs_nmax of variable v: 2^ndim nmax under sector search (octants over the
+// spatial axes: 2^3 for space-time), else nmax (kriging_base.F90:3003-3011)
+int obs_nmax(const Krige& K, int v) {
+  const Obs& O = K.obs[v];
+  const long long nsec = 1LL << (K.st ? 3 : K.ndim);
+  return (int)std::min<long long>(INT_MAX, O.sector ? nsec * O.nmax : O.nmax);
+}
+
+int store_nmax(const Krige& K) {
+  int nm = 0;
+  for (int v = 0; v < K.nvar; ++v) nm = std::max(nm, obs_nmax(K, v));
+  return nm;
+}
+
+// The store after a solve with store_weight (save_block_weights,
+// kriging_base.F90:2760-2782, called for every block after its solve): the
+// neighbour lists and weights of every group, the block's variance.  Blocks
+// the fallback could not solve keep one neighbour per group with a NaN
+// weight (ctx%nnear = 1 and x = NaN, kriging_base.F90:3369-3383).  A block
+// with a single neighbour is not solved by the reference, which leaves the
+// previous block's solution in its weight; it is NaN here, like its
+// estimate.  Reads the device results directly (fetched or not).
+bool store_from_device(Krige& K) {
+  const char* W = "t_kriging_base%save_block_weights";
+  const int nb = K.nblock, nvar = K.nvar;
+  const int nm = store_nmax(K);
+  const int mmw = K.mmax_w;
+  std::vector<int> flag(nb);
+  std::vector<double> w((size_t)mmw * nvar * nvar * nb), vout((size_t)nb * nvar * nvar);
+  HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+  HIPCHK(hipMemcpy(w.data(), K.d_wout.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost), W);
+  HIPCHK(hipMemcpy(vout.data(), nvar == 1 ? K.d_var.p : K.d_mvvar.p, sizeof(double) * vout.size(),
+                   hipMemcpyDeviceToHost), W);
+  std::vector<std::vector<int>> nn(nvar, std::vector<int>(nb));
+  std::vector<int> mm(nvar);
+  for (int v = 0; v < nvar; ++v) {
+    mm[v] = nvar == 1 ? K.mmax : K.dv[v].mmax;
+    HIPCHK(hipMemcpy(nn[v].data(), nvar == 1 ? K.d_nnear.p : K.dv[v].nnear.p, sizeof(int) * nb,
+                     hipMemcpyDeviceToHost), W);
+  }
+  std::vector<int> tot(nb, 0);
+  for (int v = 0; v < nvar; ++v)
+    for (int b = 0; b < nb; ++b) tot[b] += nn[v][b];
+  WStore& S = K.ws;
+  S.init(nm, nvar, nvar, nb);
+  const double qnan = std::nan("");
+  std::vector<int> in;
+  for (int g = 0; g < nvar; ++g) {
+    in.resize((size_t)nb * mm[g]);
+    HIPCHK(hipMemcpy(in.data(), nvar == 1 ? K.d_inear.p : K.dv[g].inear.p, sizeof(int) * in.size(),
+                     hipMemcpyDeviceToHost), W);
+    for (int b = 0; b < nb; ++b) {
+      const bool failed = flag[b] == 3;
+      const int n = nn[g][b];
+      const int ns = failed ? 1 : n;
+      S.nnear[g + (size_t)nvar * b] = ns;
+      int* si = &S.inear[(size_t)nm * (g + (size_t)nvar * b)];
+      for (int k = 0; k < std::min(ns, n); ++k) si[k] = in[(size_t)mm[g] * b + k] + 1;
+      for (int q = 0; q < nvar; ++q) {
+        double* sw = &S.weight[(size_t)nm * (g + (size_t)nvar * (q + (size_t)nvar * b))];
+        for (int k = 0; k < ns; ++k)
+          sw[k] = (failed || tot[b] < 2) ? qnan : w[k + (size_t)mmw * (g + (size_t)nvar * (q + (size_t)nvar * b))];
+      }
+    }
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < nvar; ++j)
+      for (int i = 0; i < nvar; ++i)
+        S.var[i + (size_t)nvar * (j + (size_t)nvar * b)] = vout[b + (size_t)nb * (i + (size_t)nvar * j)];
+  K.ws_dev = false;
+  return true;
+}
+
+// The text factor file (write_weight_store / read_weight_to_store,
+// kriging_base.F90:2838-2866, 2896-2917).  Header: nblock, nvar and
+// obs_nmax of every variable; then per block the records
+//   order nnear(1:ngroups)
+//   inear of every group, concatenated
+//   variance (nvar x nvar, column-major)
+//   weights of every group, concatenated: one record per target variable.
+// Weights carry ten decimals (F0.10) and variances ten significant digits,
+// as the reference writes them.
+bool write_weight_file(const Krige& K) {
+  const char* W = "t_kriging_base%write_weight_store";
+  FILE* f = std::fopen(K.weight_file.c_str(), "w");
+  if (!f) {
+    set_error(W, "cannot open weight_file for writing: " + K.weight_file);
+    return false;
+  }
+  const WStore& S = K.ws;
+  const int ng = S.ng, nv = S.nv, nm = S.nm;
+  auto num = [&](double x, bool fixed) {
+    if (std::isnan(x)) std::fputs("  NaN", f);
+    else if (std::isinf(x)) std::fputs(x > 0 ? "  Infinity" : "  -Infinity", f);
+    else std::fprintf(f, fixed ? "  %.10f" : "  %.9E", x);
+  };
+  std::fprintf(f, "%12d%12d", S.nb, K.nvar);
+  for (int v = 0; v < K.nvar; ++v) std::fprintf(f, "%12d", obs_nmax(K, v));
+  std::fputc('\n', f);
+  for (int b = 0; b < S.nb; ++b) {
+    const int* nn = &S.nnear[(size_t)ng * b];
+    std::fprintf(f, "%d", S.order[b]);
+    for (int g = 0; g < ng; ++g) std::fprintf(f, "  %d", nn[g]);
+    std::fputc('\n', f);
+    for (int g = 0; g < ng; ++g)
+      for (int k = 0; k < nn[g]; ++k) std::fprintf(f, "  %d", S.inear[k + (size_t)nm * (g + (size_t)ng * b)]);
+    std::fputc('\n', f);
+    for (int j = 0; j < nv; ++j)
+      for (int i = 0; i < nv; ++i) num(S.var[i + (size_t)nv * (j + (size_t)nv * b)], false);
+    std::fputc('\n', f);
+    for (int q = 0; q < nv; ++q) {
+      for (int g = 0; g < ng; ++g)
+        for (int k = 0; k < nn[g]; ++k) num(S.weight[k + (size_t)nm * (g + (size_t)ng * (q + (size_t)nv * b))], true);
+      std::fputc('\n', f);
+    }
+  }
+  const bool bad = std::ferror(f) != 0;
+  if (std::fclose(f) != 0 || bad) {
+    set_error(W, "write error on weight_file " + K.weight_file);
+    return false;
+  }
+  return true;
+}
+
+// list-directed reading: every read starts on a new record and takes items
+// from as many records as it needs; the rest of its last record is skipped,
+// and a read of no items consumes one record
+struct FactorRecords {
+  std::vector<std::string> lines;
+  size_t at = 0;
+  bool read(size_t n, std::vector<std::string>& tok) {
+    tok.clear();
+    if (n == 0) return at < lines.size() ? (++at, true) : false;
+    while (tok.size() < n) {
+      if (at >= lines.size()) return false;
+      const std::string& l = lines[at++];
+      size_t i = 0;
+      while (i < l.size()) {
+        while (i < l.size() && (l[i] == ' ' || l[i] == '\t' || l[i] == ',' || l[i] == '\r')) ++i;
+        size_t j = i;
+        while (j < l.size() && !(l[j] == ' ' || l[j] == '\t' || l[j] == ',' || l[j] == '\r')) ++j;
+        if (j > i) tok.push_back(l.substr(i, j - i));
+        i = j;
+      }
+    }
+    tok.resize(n);
+    return true;
+  }
+};
+
+bool parse_int(const std::string& t, int& v) {
+  char* e = nullptr;
+  const long x = std::strtol(t.c_str(), &e, 10);
+  if (!e || *e != 0 || x < INT_MIN || x > INT_MAX) return false;
+  v = (int)x;
+  return true;
+}
+
+bool parse_real(std::string t, double& v) {
+  for (char& c : t)
+    if (c == 'D' || c == 'd') c = 'E';  // Fortran double-precision exponents
+  char* e = nullptr;
+  v = std::strtod(t.c_str(), &e);
+  return e && *e == 0 && !t.empty();
+}
+
+// use_old_weight with a weight_file: the whole file is read into the store
+// before the block loop (prepare_common, kriging_base.F90:3074-3095)
+bool read_weight_file(Krige& K) {
+  const char* W = "t_kriging%prepare";
+  const char* WR = "t_kriging_base%read_weight_to_store";
+  FILE* f = std::fopen(K.weight_file.c_str(), "r");
+  if (!f) {
+    set_error(W, "cannot open weight_file: " + K.weight_file);
+    return false;
+  }
+  FactorRecords R;
+  {
+    std::string cur;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) {
+      if (c == '\n') {
+        R.lines.push_back(cur);
+        cur.clear();
+      } else {
+        cur.push_back((char)c);
+      }
+    }
+    if (!cur.empty()) R.lines.push_back(cur);
+    std::fclose(f);
+  }
+  const int nvar = K.nvar, nb = K.nblock;
+  std::vector<std::string> tok;
+  int hb = 0, hv = 0;
+  if (!R.read(2 + nvar, tok) || !parse_int(tok[0], hb) || !parse_int(tok[1], hv)) {
+    set_error(W, "Failed to read weight_file header.");
+    return false;
+  }
+  bool same = hb == nb && hv == nvar;
+  for (int v = 0; v < nvar && same; ++v) {
+    int x = 0;
+    same = parse_int(tok[2 + v], x) && x == obs_nmax(K, v);
+  }
+  if (!same) {
+    set_error(W, "weight_file dimensions do not match this kriging object.");
+    return false;
+  }
+  WStore& S = K.ws;
+  const int nm = store_nmax(K), ng = nvar;
+  S.init(nm, ng, nvar, nb);
+  auto fail = [&]() {
+    set_error(WR, "Unexpected end-of-file or read error in factor file.");
+    S = WStore();
+    return false;
+  };
+  for (int b = 0; b < nb; ++b) {
+    if (!R.read(1 + ng, tok) || !parse_int(tok[0], S.order[b])) return fail();
+    int tot = 0;
+    for (int g = 0; g < ng; ++g) {
+      int n = 0;
+      if (!parse_int(tok[1 + g], n) || n < 0 || n > nm) return fail();
+      S.nnear[g + (size_t)ng * b] = n;
+      tot += n;
+    }
+    if (!R.read(tot, tok)) return fail();
+    size_t t = 0;
+    for (int g = 0; g < ng; ++g)
+      for (int k = 0; k < S.nnear[g + (size_t)ng * b]; ++k)
+        if (!parse_int(tok[t++], S.inear[k + (size_t)nm * (g + (size_t)ng * b)])) return fail();
+    if (!R.read((size_t)nvar * nvar, tok)) return fail();
+    for (int j = 0; j < nvar; ++j)
+      for (int i = 0; i < nvar; ++i)
+        if (!parse_real(tok[i + (size_t)nvar * j], S.var[i + (size_t)nvar * (j + (size_t)nvar * b)])) return fail();
+    for (int q = 0; q < nvar; ++q) {
+      if (!R.read(tot, tok)) return fail();
+      t = 0;
+      for (int g = 0; g < ng; ++g)
+        for (int k = 0; k < S.nnear[g + (size_t)ng * b]; ++k)
+          if (!parse_real(tok[t++], S.weight[k + (size_t)nm * (g + (size_t)ng * (q + (size_t)nvar * b))]))
+            return fail();
+    }
+  }
+  K.ws_dev = false;
+  return true;
+}
+
+// the block loop with use_old_weight: estimates from the stored weights and
+// the current observation values, variances from the store
+// (kriging_base.F90:2125-2126, 2792-2812, 3407-3503), on the GPU
+bool do_replay(Krige& K, bool fetch) {
+  const char* W = "t_kriging%prepare";
+  if (!K.initialized) {
+    set_error(W, "Call initialize() before solve.");
+    return false;
+  }
+  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift != 0) {
+    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift=0");
+    return false;
+  }
+  for (int v = 0; v < K.nvar; ++v)
+    if (K.obs[v].n == 0) {
+      set_error(W, "set_obs() needs to be called before solve().");
+      return false;
+    }
+  if (K.nblock == 0) {
+    set_error(W, "set_grid() needs to be called before solve().");
+    return false;
+  }
+  if (!K.weight_file.empty()) {
+    if (!read_weight_file(K)) return false;
+  } else {
+    if (!K.ws.alloc) {
+      set_error(W, "use_old_weight with no weight_file requires set_weights() to be called first.");
+      return false;
+    }
+    if (!K.ws.stored()) {
+      set_error(W, "in-memory weight store is empty. Call set_weights() first.");
+      return false;
+    }
+  }
+  WStore& S = K.ws;
+  const int nb = K.nblock, nvar = K.nvar;
+  if (S.nb != nb || S.ng != nvar || S.nv != nvar) {
+    set_error(W, "the weight store does not match this kriging object (nblock, nvar)");
+    return false;
+  }
+  if (!ensure_stream(K)) return false;
+  hipStream_t st = K.stream;
+  if (!K.ws_dev) {
+    if (!upload(K.d_wnn, S.nnear, W) || !upload(K.d_win, S.inear, W) || !upload(K.d_ww, S.weight, W) ||
+        !upload(K.d_wvar, S.var, W))
+      return false;
+    K.ws_dev = true;
+  }
+  for (int v = 0; v < nvar; ++v)
+    if (!upload(K.d_rz[v], K.obs[v].value, W)) return false;
+  DBuf& de = nvar == 1 ? K.d_est : K.d_mvest;
+  DBuf& dv = nvar == 1 ? K.d_var : K.d_mvvar;
+  if (!de.ensure(sizeof(double) * (size_t)nb * nvar) || !dv.ensure(sizeof(double) * (size_t)nb * nvar * nvar)) {
+    set_error(W, "device allocation failed");
+    return false;
+  }
+  kk::ReplayArgs r{};
+  r.nb = nb;
+  r.nm = S.nm;
+  r.ng = nvar;
+  r.nvar = nvar;
+  r.nnear = K.d_wnn.as<int>();
+  r.inear = K.d_win.as<int>();
+  r.weight = K.d_ww.as<double>();
+  r.wvar = K.d_wvar.as<double>();
+  for (int v = 0; v < nvar; ++v) {
+    r.ovalue[v] = K.d_rz[v].as<double>();
+    r.nobs[v] = K.obs[v].n;
+    r.sk_mean[v] = K.obs[v].sk_mean;
+  }
+  r.std_ck = K.std_ck;
+  r.unbias = K.unbias;
+  r.bound_lo = K.bounds[0];
+  r.bound_hi = K.bounds[1];
+  r.est = de.as<double>();
+  r.var = dv.as<double>();
+  HIPCHK(hipEventRecord(K.ev[0], st), W);
+  HIPCHK(kk::launch_replay(r, st), W);
+  HIPCHK(hipEventRecord(K.ev[4], st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
+  for (double& x : K.timing) x = 0.0;
+  K.timing[4] = K.timing[12] = ms;
+  K.timing[7] = -1;
+  for (int& x : K.stats) x = 0;
+  if (!fetch) return true;
+  if (nvar == 1) {
+    K.est.resize(nb);
+    K.var.resize(nb);
+    HIPCHK(hipMemcpy(K.est.data(), K.d_est.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
+    HIPCHK(hipMemcpy(K.var.data(), K.d_var.p, sizeof(double) * nb, hipMemcpyDeviceToHost), W);
+  } else {
+    K.est_mv.resize((size_t)nb * nvar);
+    K.var_mv.resize((size_t)nb * nvar * nvar);
+    HIPCHK(hipMemcpy(K.est_mv.data(), K.d_mvest.p, sizeof(double) * K.est_mv.size(), hipMemcpyDeviceToHost), W);
+    HIPCHK(hipMemcpy(K.var_mv.data(), K.d_mvvar.p, sizeof(double) * K.var_mv.size(), hipMemcpyDeviceToHost), W);
+  }
+  return true;
+}
+
+
 bool do_solve(Krige& K, bool fetch) {
   K.solved = false;
-  if (!validate_for_solve(K)) return false;
-  if (K.nvar > 1) {
-    if (!do_solve_mv(K, fetch)) return false;
+  if (K.use_old_weight) {
+    if (!do_replay(K, fetch)) return false;
     K.solved = true;
     return true;
   }
-  if (!stage_inputs(K)) return false;
-  if (!compute_base_cov(K)) return false;
-  if (!run_pipeline(K)) return false;
-  if (fetch && !fetch_results(K)) return false;
+  if (!validate_for_solve(K)) return false;
+  if (K.nvar > 1) {
+    if (!do_solve_mv(K, fetch)) return false;
+  } else {
+    if (!stage_inputs(K)) return false;
+    if (!compute_base_cov(K)) return false;
+    if (!run_pipeline(K)) return false;
+    if (fetch && !fetch_results(K)) return false;
+  }
+  // store_weight: the store, then the factor file when one is named
+  // (kriging_base.F90:2134, 2161-2162)
+  if (K.store_weight && (!store_from_device(K) || (!K.weight_file.empty() && !write_weight_file(K))))
+    return false;
   K.solved = true;
   return true;
 }
@@ -1970,7 +2437,6 @@ int krige_initialize(int64_t h, int ndim, int nvar, int ndrift, int unbias, int 
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  (void)weight_file;
   const char* W = "t_kriging_base%initialize";
   if (store_weight && use_old_weight) {
     set_error(W, "store_weight and use_old_weight are mutually exclusive");
@@ -1991,17 +2457,16 @@ int krige_initialize(int64_t h, int ndim, int nvar, int ndrift, int unbias, int 
   const int prec = K->precision;
   const double ptol = K->pivot_tol;
   const bool tmode = K->table_mode;
-  hipStream_t st = K->stream;
-  hipEvent_t ev[12];
-  std::memcpy(ev, K->ev, sizeof ev);
-  K->stream = nullptr;
-  std::memset(K->ev, 0, sizeof K->ev);
+  // a fresh object; the streams, events and pinned counters move over
   Krige* fresh = new Krige();
+  std::swap(fresh->stream, K->stream);
+  std::swap(fresh->stream2, K->stream2);
+  std::swap(fresh->stream3, K->stream3);
+  std::swap(fresh->h_counts, K->h_counts);
+  for (size_t i = 0; i < sizeof K->ev / sizeof K->ev[0]; ++i) std::swap(fresh->ev[i], K->ev[i]);
   delete K;
   K = fresh;
   g_reg[h - 1] = K;
-  K->stream = st;
-  std::memcpy(K->ev, ev, sizeof ev);
   K->precision = prec;
   K->pivot_tol = ptol;
   K->table_mode = tmode;
@@ -2021,6 +2486,9 @@ int krige_initialize(int64_t h, int ndim, int nvar, int ndrift, int unbias, int 
   K->std_ck = std_ck;
   K->verbose = verbose;
   K->pf_cache = pf_cache;
+  // Fortran trim(): trailing blanks are not part of the name
+  K->weight_file = weight_file ? weight_file : "";
+  while (!K->weight_file.empty() && K->weight_file.back() == ' ') K->weight_file.pop_back();
   if (bounds) {
     K->bounds[0] = bounds[0];
     K->bounds[1] = bounds[1];
@@ -2106,6 +2574,17 @@ int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
   }
   K->obs[ivar - 1].value.assign(value, value + nobs);
   K->dev_inputs_ok = false;
+  // a filled weight store and a finished solve switch the next solve to
+  // the stored weights (kriging_base.F90:669-678)
+  if (K->ws.alloc && K->ws.stored() && K->solved) {
+    const std::vector<double>& e = K->nvar == 1 ? K->est : K->est_mv;
+    bool all_nan = !e.empty();
+    for (double x : e) all_nan = all_nan && std::isnan(x);
+    if (!all_nan) {
+      K->use_old_weight = true;
+      K->store_weight = false;
+    }
+  }
   return 0;
 }
 
@@ -2550,73 +3029,162 @@ int krige_free_weight_store(int64_t h) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  K->nnear.clear();
-  K->inear.clear();
+  K->ws = WStore();
+  K->ws_dev = false;
+  K->d_wnn.release();
+  K->d_win.release();
+  K->d_ww.release();
+  K->d_wvar.release();
   return 0;
+}
+
+// the getters copy out(1:n1, 1:n2, ...) of the store, as the reference does;
+// dimensions beyond the store's are an error here
+static bool store_ready(Krige* K, const char* W) {
+  if (!K->ws.alloc) {
+    set_error(W, "Weight store not allocated");
+    return false;
+  }
+  return true;
+}
+
+static bool within(const char* W, std::initializer_list<std::pair<int, int>> dims) {
+  for (const auto& d : dims)
+    if (d.first < 0 || d.first > d.second) {
+      set_error(W, "requested dimensions exceed the weight store");
+      return false;
+    }
+  return true;
 }
 
 int krige_get_weight_dims(int64_t h, int* nm, int* ng, int* nb) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (K->nnear.empty()) {
-    set_error("krige_get_weight_dims", "Weight store not allocated");
-    return ierr();
-  }
-  *nm = K->mmax;
-  *ng = K->nvar;
-  *nb = K->nblock;
+  if (!store_ready(K, "krige_get_weight_dims")) return ierr();
+  *nm = K->ws.nm;
+  *ng = K->ws.ng;
+  *nb = K->ws.nb;
   return 0;
 }
 
 int krige_get_weight_nnear(int64_t h, int ngroups_c, int nblock_c, int* out) {
   clear_error();
+  const char* W = "krige_get_weight_nnear";
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (K->nnear.empty()) {
-    set_error("krige_get_weight_nnear", "Weight store not allocated");
-    return ierr();
-  }
+  if (!store_ready(K, W)) return ierr();
+  const WStore& S = K->ws;
+  if (!within(W, {{ngroups_c, S.ng}, {nblock_c, S.nb}})) return ierr();
   for (int b = 0; b < nblock_c; ++b)
-    for (int g = 0; g < ngroups_c; ++g) out[g + (size_t)ngroups_c * b] = g == 0 ? K->nnear[b] : 0;
+    for (int g = 0; g < ngroups_c; ++g) out[g + (size_t)ngroups_c * b] = S.nnear[g + (size_t)S.ng * b];
   return 0;
 }
 
 int krige_get_weight_inear(int64_t h, int nmax_c, int ngroups_c, int nblock_c, int* out) {
   clear_error();
+  const char* W = "krige_get_weight_inear";
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (K->nnear.empty()) {
-    set_error("krige_get_weight_inear", "Weight store not allocated");
-    return ierr();
-  }
+  if (!store_ready(K, W)) return ierr();
+  const WStore& S = K->ws;
+  if (!within(W, {{nmax_c, S.nm}, {ngroups_c, S.ng}, {nblock_c, S.nb}})) return ierr();
   for (int b = 0; b < nblock_c; ++b)
     for (int g = 0; g < ngroups_c; ++g)
-      for (int i = 0; i < nmax_c; ++i) {
-        int v = 0;
-        if (g == 0 && i < K->nnear[b]) v = K->inear[(size_t)K->mmax * b + i] + 1;
-        out[i + (size_t)nmax_c * (g + (size_t)ngroups_c * b)] = v;
-      }
+      for (int k = 0; k < nmax_c; ++k)
+        out[k + (size_t)nmax_c * (g + (size_t)ngroups_c * b)] = S.inear[k + (size_t)S.nm * (g + (size_t)S.ng * b)];
   return 0;
 }
 
-int krige_get_weight_data(int64_t h, int, int, int, int, double*) {
+int krige_get_weight_data(int64_t h, int nmax_c, int ngroups_c, int nvar_c, int nblock_c, double* out) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_get_weight_data", "kriging weights are not stored by this backend");
+  const char* W = "krige_get_weight_data";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!store_ready(K, W)) return ierr();
+  const WStore& S = K->ws;
+  if (!within(W, {{nmax_c, S.nm}, {ngroups_c, S.ng}, {nvar_c, S.nv}, {nblock_c, S.nb}})) return ierr();
+  for (int b = 0; b < nblock_c; ++b)
+    for (int q = 0; q < nvar_c; ++q)
+      for (int g = 0; g < ngroups_c; ++g)
+        for (int k = 0; k < nmax_c; ++k)
+          out[k + (size_t)nmax_c * (g + (size_t)ngroups_c * (q + (size_t)nvar_c * b))] =
+              S.weight[k + (size_t)S.nm * (g + (size_t)S.ng * (q + (size_t)S.nv * b))];
+  return 0;
 }
 
-int krige_get_weight_var(int64_t h, int, int, double*) {
+int krige_get_weight_var(int64_t h, int nvar_c, int nblock_c, double* out) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_get_weight_var", "kriging weights are not stored by this backend");
+  const char* W = "krige_get_weight_var";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!store_ready(K, W)) return ierr();
+  const WStore& S = K->ws;
+  if (!within(W, {{nvar_c, S.nv}, {nblock_c, S.nb}})) return ierr();
+  for (int b = 0; b < nblock_c; ++b)
+    for (int j = 0; j < nvar_c; ++j)
+      for (int i = 0; i < nvar_c; ++i)
+        out[i + (size_t)nvar_c * (j + (size_t)nvar_c * b)] = S.var[i + (size_t)S.nv * (j + (size_t)S.nv * b)];
+  return 0;
 }
 
-int krige_set_weights(int64_t h, int, int, int, int, const int*, const int*, const double*, const int*,
-                      const double*) {
+// alloc_weight_store's preconditions and set_weights' shape checks
+// (kriging_base.F90:2703-2733, 2930-2977), then the store takes the arrays
+// as given and the next solve replays them (use_old_weight).  Neighbour
+// counts outside 0..nmax are rejected here (the replay runs on the GPU);
+// the neighbour ids are checked against the observations at the solve.
+int krige_set_weights(int64_t h, int nmax_c, int ngroups_c, int nvar_c, int nblock_c, const int* nnear,
+                      const int* inear, const double* weight, const int* order, const double* var) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_set_weights", "weight replay is not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* WA = "t_kriging_base%alloc_weight_store";
+  const char* W = "t_kriging_base%set_weights";
+  if (K->nblock == 0) {
+    set_error(WA, "call set_grid() before alloc_weight_store()");
+    return ierr();
+  }
+  if (!K->initialized) {
+    set_error(WA, "call initialize() before alloc_weight_store()");
+    return ierr();
+  }
+  const int nm = K->ws.alloc ? K->ws.nm : store_nmax(*K);
+  const int ng = K->nvar, nb = K->nblock, nv = K->nvar;
+  if (nm <= 0) {
+    set_error(WA, "call set_obs() before alloc_weight_store() so nmax is set");
+    return ierr();
+  }
+  if (ngroups_c != ng || nblock_c != nb) {
+    set_error(W, "nnear shape mismatch");
+    return ierr();
+  }
+  if (nmax_c != nm) {
+    set_error(W, "inear shape mismatch");
+    return ierr();
+  }
+  if (nvar_c != nv) {
+    set_error(W, "weight shape mismatch");
+    return ierr();
+  }
+  if (!nnear || !inear || !weight) {
+    set_error(W, "nnear, inear and weight are required");
+    return ierr();
+  }
+  for (size_t i = 0; i < (size_t)ng * nb; ++i)
+    if (nnear[i] < 0 || nnear[i] > nm) {
+      set_error(W, "nnear out of range 0.." + std::to_string(nm) + " at block " + std::to_string(i / ng + 1));
+      return ierr();
+    }
+  WStore& S = K->ws;
+  S.init(nm, ng, nv, nb);
+  std::memcpy(S.nnear.data(), nnear, sizeof(int) * S.nnear.size());
+  std::memcpy(S.inear.data(), inear, sizeof(int) * S.inear.size());
+  std::memcpy(S.weight.data(), weight, sizeof(double) * S.weight.size());
+  if (order) std::memcpy(S.order.data(), order, sizeof(int) * S.order.size());
+  if (var) std::memcpy(S.var.data(), var, sizeof(double) * S.var.size());
+  K->ws_dev = false;
+  K->use_old_weight = true;
+  return 0;
 }
 
 // ---- extensions of this backend (no counterpart in the reference) -------
@@ -2945,15 +3513,16 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
   return 0;
 }
 
-// out[0..11]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
+// out[0..12]: search ms, bucket ms, solve ms, re-solve/fallback ms, total ms,
 // FP64 re-solved nodes, symmetric-indefinite fallback nodes, queries decided by the exact
 // search kernel (-1 when the fast search was not used), fast search used
-// (1/0), fast search kernel ms, exact search kernel ms, LOO gradient kernel ms
+// (1/0), fast search kernel ms, exact search kernel ms, LOO gradient kernel ms,
+// weight store pass or stored-weight replay ms
 int kk_get_timing(int64_t h, double* out) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  for (int i = 0; i < 12; ++i) out[i] = K->timing[i];
+  for (int i = 0; i < 13; ++i) out[i] = K->timing[i];
   return 0;
 }
 
@@ -3004,6 +3573,10 @@ int kk_solve_resident_noalloc(int64_t h, double* est, double* var) {
       (K->search_mode == 2 && K->reorder && !K->perm_ok)) {
     set_error(W, "needs a finished single-variable solve of the handle's current inputs "
                  "(krige_solve or kk_solve_resident) before the allocation-free pass");
+    return ierr();
+  }
+  if (K->store_weight || K->use_old_weight) {
+    set_error(W, "the allocation-free pass neither stores nor replays kriging weights");
     return ierr();
   }
   if (!est || !var) {

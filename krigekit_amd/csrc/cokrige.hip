@@ -37,7 +37,25 @@
 #include "kk_device.hpp"
 #include "kk_types.hpp"
 
+// k_solve_mv instances are compiled in units of their own, one per
+// coordinate count (KK_MV_DIM 1-3) and weights mode (KK_MV_W 0/1), so they
+// compile in parallel; the unit without KK_MV_DIM holds the fallback and the
+// dispatch
+#ifndef KK_MV_DIM
+#define KK_MV_DIM 0
+#endif
+#ifndef KK_MV_W
+#define KK_MV_W 0
+#endif
+
 namespace kk {
+
+hipError_t launch_solve_mv_d1_w0(const MVArgs& a, int rb, hipStream_t st);
+hipError_t launch_solve_mv_d2_w0(const MVArgs& a, int rb, hipStream_t st);
+hipError_t launch_solve_mv_d3_w0(const MVArgs& a, int rb, hipStream_t st);
+hipError_t launch_solve_mv_d1_w1(const MVArgs& a, int rb, hipStream_t st);
+hipError_t launch_solve_mv_d2_w1(const MVArgs& a, int rb, hipStream_t st);
+hipError_t launch_solve_mv_d3_w1(const MVArgs& a, int rb, hipStream_t st);
 
 constexpr int MVT = 256;            // threads per workgroup (16 x 16)
 constexpr int MVR = 128;            // max rows + extra rows
@@ -92,8 +110,14 @@ __device__ inline double wg_sum(double v, double* part) {
 // registers must allow: five for the 80-column instance (its 25 KB of LDS
 // admits five workgroups per CU; measured faster than four despite a few
 // spills), unconstrained for the 128-column one.
-template <int DIM, int NV, int RB, bool SIMPLE>
-__global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
+//
+// WEIGHTS: the instance also writes the kriging weights (the weight store,
+// kriging_base.F90:2428-2439, 2760-2782).  The factor is kept in LDS
+// (packed lower triangle of the bordered matrix), and after the multipliers
+// the weights w_q = K^-1 (b_q - F mu_q) are obtained by one backward
+// substitution with the unit factor, all targets together.
+template <int DIM, int NV, int RB, bool SIMPLE, bool WEIGHTS>
+__global__ __launch_bounds__(MVT, (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv(MVArgs a) {
   // rows + extra rows fit 16 RB (the instance is chosen by the column count)
   constexpr int ROWS = 16 * RB;
   __shared__ double sx[DIM][ROWS];
@@ -117,6 +141,11 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
 
   __shared__ Model smodels[NV * NV];
   __shared__ int sfail;
+  // WEIGHTS: A(i, j), j <= i, j < n, at sl[i (i + 1) / 2 + j]; multipliers;
+  // the weights finished so far
+  __shared__ double sl[WEIGHTS ? ROWS * (ROWS + 1) / 2 : 1];
+  __shared__ double smu[NV][NV];
+  __shared__ double sw[NV][WEIGHTS ? ROWS : 1];
   const int b = a.list ? a.list[blockIdx.x] : (int)blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
   // classes) sit in one wave, so a column's pivot and scaling never leave it
@@ -134,8 +163,10 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   if (n < 2) {
     // no neighbour: NaN; one neighbour: the reference skips the solve
     // (kriging_base.F90:2127), NaN here
-    if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
-    if (t < nvar * nvar) a.var[b + (size_t)a.nb * t] = qnan;
+    if (!a.weights_only) {
+      if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
+      if (t < nvar * nvar) a.var[b + (size_t)a.nb * t] = qnan;
+    }
     if (t == 0) a.flag[b] = 0;
     return;
   }
@@ -417,6 +448,19 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
     return;
   }
 
+  if (WEIGHTS) {
+    // the factor's columns, the extra rows included, for the substitution
+#pragma unroll
+    for (int ra = 0; ra < RB; ++ra) {
+      const int i = tr + 16 * ra;
+#pragma unroll
+      for (int k = 0; k <= ra; ++k) {
+        const int j = tc + 16 * k;
+        if (i < nx && j < n && j <= i) sl[i * (i + 1) / 2 + j] = A[ra][k];
+      }
+    }
+  }
+
   // ---- Gram matrix of the eliminated extra columns: minus the trailing block ----
 #pragma unroll
   for (int ra = 0; ra < RB; ++ra) {
@@ -435,7 +479,7 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
     if (isa && nv[v] > 0) gm[v] = wg_sum((t < n && svr[t] == v) ? sz[t] : 0.0, part) / nv[v];
   }
   __syncthreads();
-  if (t != 0) return;
+  if (!WEIGHTS && t != 0) return;
 
   // ---- multipliers, estimates and variances (one thread) -----------------------
   // mu_q = S^-1 (G_q - d_q), S = F'K^-1 F, G = F'K^-1 b, through the Cholesky
@@ -443,106 +487,170 @@ __global__ __launch_bounds__(MVT, RB == 5 ? 5 : 1) void k_solve_mv(MVArgs a) {
   // (solver.f90:224-300); a pivot that is not positive (a variable without
   // neighbours leaves a zero row in S under std_ck) sends the block to the
   // Bunch-Kaufman fallback like the reference's failed SPOTRF of S
-  double mu[NV][NV];  // mu[q][c]
+  if (t == 0) {
+    double mu[NV][NV];  // mu[q][c]
+    bool ok = true;
 #pragma unroll
-  for (int q = 0; q < NV; ++q)
+    for (int q = 0; q < NV; ++q)
 #pragma unroll
-    for (int c = 0; c < NV; ++c) mu[q][c] = 0.0;
-  if (p > 0) {
-    double U[NV][NV];
+      for (int c = 0; c < NV; ++c) mu[q][c] = 0.0;
+    if (p > 0) {
+      double U[NV][NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
+      for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int j = 0; j < NV; ++j) U[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
-    bool spd = true;
+        for (int j = 0; j < NV; ++j) U[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
+      bool spd = true;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      if (j >= p) break;
-      double d = U[j][j];
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-        if (k < j) d -= U[k][j] * U[k][j];
-      spd = spd && d > 0.0;
-      d = sqrt(d);
-      U[j][j] = d;
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        if (i <= j || i >= p) continue;
-        double v = U[j][i];
+      for (int j = 0; j < NV; ++j) {
+        if (j >= p) break;
+        double d = U[j][j];
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-          if (k < j) v -= U[k][j] * U[k][i];
-        U[j][i] = v / d;
+          if (k < j) d -= U[k][j] * U[k][j];
+        spd = spd && d > 0.0;
+        d = sqrt(d);
+        U[j][j] = d;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          if (i <= j || i >= p) continue;
+          double v = U[j][i];
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            if (k < j) v -= U[k][j] * U[k][i];
+          U[j][i] = v / d;
+        }
+      }
+      if (!spd) {
+        a.flag[b] = 2;
+        if (a.ctr) atomicAdd(a.ctr, 1);
+        ok = false;
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        if (!ok) break;
+        double y[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          if (i >= p) continue;
+          const double d = a.std_ck ? (i == q ? 1.0 : 0.0) : 1.0;
+          double v = gram[ef + i][eb + q] - d;
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            if (k < i) v -= U[k][i] * y[k];
+          y[i] = v / U[i][i];
+        }
+#pragma unroll
+        for (int i = NV - 1; i >= 0; --i) {
+          if (i >= p) continue;
+          double v = y[i];
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            if (k > i && k < p) v -= U[i][k] * mu[q][k];
+          mu[q][i] = v / U[i][i];
+        }
       }
     }
-    if (!spd) {
-      a.flag[b] = 2;
-      if (a.ctr) atomicAdd(a.ctr, 1);
-      return;
+    if (ok && !a.weights_only) {
+      // c' w_q = c'K^-1 b_q - sum_f mu_q[f] c'K^-1 F_f for an extra column c
+      auto cw = [&](int e, int q) {
+        double s = gram[e][eb + q];
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+          if (f < p) s -= mu[q][f] * gram[e][ef + f];
+        return s;
+      };
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        double e = cw(ez, q);
+        if (isa) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v)
+            if (v != q && nv[v] > 0) e += (gm[q] - gm[v]) * cw(ei + v, q);
+        }
+        if (!a.unbias && a.sk_mean[q] != 0.0) e += (1.0 - cw(ez + 1, q)) * a.sk_mean[q];
+        e = e < a.bound_lo ? a.bound_lo : e;
+        e = e > a.bound_hi ? a.bound_hi : e;
+        a.est[b + (size_t)a.nb * q] = e;
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          if (j < i) continue;
+          const double base = ns > 1 ? sbase[i + NV * j] : a.models[i + nvar * j].cov0;
+          double mud = 0.0;
+#pragma unroll
+          for (int c = 0; c < NV; ++c)
+            if (c < p) mud += mu[i][c] * (a.std_ck ? (c == j ? 1.0 : 0.0) : 1.0);
+          double v = base - (cw(eb + j, i) + mud);
+          v = v > 0.0 ? v : 0.0;  // diagonal and off-diagonal clamped at 0
+          a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
+          a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
+        }
     }
+    if (ok) a.flag[b] = 0;
+    if (WEIGHTS) {
+      sfail = ok ? 0 : 1;
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) smu[q][c] = mu[q][c];
+    }
+  }
+  if (!WEIGHTS) return;
+  __syncthreads();
+  if (sfail) return;
+
+  // ---- weights: L' w_q = D^-1 (y_bq - sum_f mu_q[f] y_Ff), backward ------------
+  // Unit factor L(i, j) = A(i, j) / A(j, j), A(j, j) = d_j; an extra row
+  // holds A(n + e, j) = d_j (L^-1 x_e)_j.  Thread j < n owns row j; after
+  // w_i is final it is broadcast and every row j < i takes its share.
+  double r[NV];
+  double dj = 1.0;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) r[q] = 0.0;
+  if (t < n) {
+    dj = sl[t * (t + 1) / 2 + t];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      double y[NV];
+      const int rb_ = n + eb + q;
+      double v = sl[rb_ * (rb_ + 1) / 2 + t];
 #pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        if (i >= p) continue;
-        const double d = a.std_ck ? (i == q ? 1.0 : 0.0) : 1.0;
-        double v = gram[ef + i][eb + q] - d;
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-          if (k < i) v -= U[k][i] * y[k];
-        y[i] = v / U[i][i];
+      for (int f = 0; f < NV; ++f) {
+        if (f >= p) break;
+        const int rf = n + ef + f;
+        v -= smu[q][f] * sl[rf * (rf + 1) / 2 + t];
       }
-#pragma unroll
-      for (int i = NV - 1; i >= 0; --i) {
-        if (i >= p) continue;
-        double v = y[i];
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-          if (k > i && k < p) v -= U[i][k] * mu[q][k];
-        mu[q][i] = v / U[i][i];
-      }
+      r[q] = v / dj;
     }
   }
-  // c' w_q = c'K^-1 b_q - sum_f mu_q[f] c'K^-1 F_f for an extra column c
-  auto cw = [&](int e, int q) {
-    double s = gram[e][eb + q];
+  for (int i = n - 1; i >= 0; --i) {
+    if (t == i) {
 #pragma unroll
-    for (int f = 0; f < NV; ++f)
-      if (f < p) s -= mu[q][f] * gram[e][ef + f];
-    return s;
-  };
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double e = cw(ez, q);
-    if (isa) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        if (v != q && nv[v] > 0) e += (gm[q] - gm[v]) * cw(ei + v, q);
+      for (int q = 0; q < NV; ++q) sw[q][i] = r[q];
     }
-    if (!a.unbias && a.sk_mean[q] != 0.0) e += (1.0 - cw(ez + 1, q)) * a.sk_mean[q];
-    e = e < a.bound_lo ? a.bound_lo : e;
-    e = e > a.bound_hi ? a.bound_hi : e;
-    a.est[b + (size_t)a.nb * q] = e;
+    __syncthreads();
+    if (t < i) {
+      const double l = sl[i * (i + 1) / 2 + t] / dj;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) r[q] -= l * sw[q][i];
+    }
   }
+  if (t < n) {
+    // weight(k, group v, target q) = x(q, istart(v) + k)
+    const int v = svr[t];
+    int s0 = 0;
 #pragma unroll
-  for (int i = 0; i < NV; ++i)
+    for (int u = 1; u < NV; ++u)
+      if (u == v) s0 = st[u];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      if (j < i) continue;
-      const double base = ns > 1 ? sbase[i + NV * j] : a.models[i + nvar * j].cov0;
-      double mud = 0.0;
-#pragma unroll
-      for (int c = 0; c < NV; ++c)
-        if (c < p) mud += mu[i][c] * (a.std_ck ? (c == j ? 1.0 : 0.0) : 1.0);
-      double v = base - (cw(eb + j, i) + mud);
-      v = v > 0.0 ? v : 0.0;  // diagonal and off-diagonal clamped at 0
-      a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
-      a.var[b + (size_t)a.nb * (j + nvar * i)] = v;
-    }
-  a.flag[b] = 0;
+    for (int q = 0; q < NV; ++q)
+      a.weights[(t - s0) + (size_t)a.mmax_w * (v + nvar * (q + (size_t)nvar * b))] = r[q];
+  }
 }
 
+#if KK_MV_DIM == 0
 // Bunch-Kaufman fallback for the blocks k_solve_mv flags (K or S not positive
 // definite), one thread per block, matrices in HBM scratch: the reference's
 // SSYTRF/SSYTRS on the full augmented system with one jittered retry, then
@@ -559,6 +667,10 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
   const int b = list[t];
   const int nvar = a.nvar;
   const int ld = a.rows + MAXVAR;
+  int nsum = 0;
+#pragma unroll
+  for (int v = 0; v < MAXVAR; ++v) nsum += v < nvar ? a.nnear[v][b] : 0;
+  if (nsum < 2) return;  // nothing to solve (the weights-only pass lists every block)
   double* K = scratch + (size_t)t * mv_fallback_words(a.rows);
   double* A = K + (size_t)ld * ld;
   double* B = A + (size_t)ld * ld;       // right-hand sides, then solutions
@@ -624,6 +736,15 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
   for (int q = 0; q < nvar; ++q)
     for (int r = 0; r < m; ++r) B0[r + ld * q] = B[r + ld * q];
   const int info = bk_fallback(K, A, ipiv, n, m, ld, B, nvar);
+  if (a.weights) {
+    // the weight store: weight(k, group v, target q) = x(q, istart(v) + k)
+    for (int q = 0; q < nvar; ++q)
+      for (int v = 0; v < nvar; ++v)
+        for (int k = 0; k < nv[v]; ++k)
+          a.weights[k + (size_t)a.mmax_w * (v + nvar * (q + (size_t)nvar * b))] =
+              info ? __builtin_nan("") : B[(st[v] + k) + ld * q];
+    if (a.weights_only) return;
+  }
   if (info) {
     // x = NaN: NaN estimates, variances max(NaN, 0) = 0 (kriging.F90:1019-1025)
     for (int q = 0; q < nvar; ++q) a.est[b + (size_t)a.nb * q] = __builtin_nan("");
@@ -704,36 +825,54 @@ int mv_extra_cols(int nvar, int std_ck, int unbias) {
   return nvar + p + 1 + (isa ? nvar : 0) + (unbias ? 0 : 1);
 }
 
-template <int DIM, int NV>
-static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
-  dim3 g(a.list ? a.nlist : a.nb), blk(MVT);
-  if (rb == 5) {
-    if (a.simple) hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, true>), g, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve_mv<DIM, NV, 5, false>), g, blk, 0, st, a);
-  } else {
-    if (a.simple) hipLaunchKernelGGL((k_solve_mv<DIM, NV, 8, true>), g, blk, 0, st, a);
-    else hipLaunchKernelGGL((k_solve_mv<DIM, NV, 8, false>), g, blk, 0, st, a);
-  }
-}
-
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
   if (a.nb <= 0 || (a.list && a.nlist <= 0)) return hipSuccess;
   const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
   if (a.rows > MVR || cols > MVR || a.nsub > MAXSUB || a.nvar < 1 || a.nvar > MAXVAR) return hipErrorInvalidValue;
   const int rb = cols <= 80 ? 5 : 8;
   // nvar 1: single-variable systems above the row-per-lane solver's 64 rows
-#define KK_MV(D)                                  \
-  do {                                           \
-    if (a.nvar == 1) launch_nv<D, 1>(a, rb, st); \
-    else if (a.nvar == 2) launch_nv<D, 2>(a, rb, st); \
-    else if (a.nvar == 3) launch_nv<D, 3>(a, rb, st); \
-    else launch_nv<D, 4>(a, rb, st);             \
-  } while (0)
-  if (a.ndim == 1) KK_MV(1);
-  else if (a.ndim == 2) KK_MV(2);
-  else KK_MV(3);
-#undef KK_MV
+  if (a.weights) {
+    if (a.ndim == 1) return launch_solve_mv_d1_w1(a, rb, st);
+    if (a.ndim == 2) return launch_solve_mv_d2_w1(a, rb, st);
+    return launch_solve_mv_d3_w1(a, rb, st);
+  }
+  if (a.ndim == 1) return launch_solve_mv_d1_w0(a, rb, st);
+  if (a.ndim == 2) return launch_solve_mv_d2_w0(a, rb, st);
+  return launch_solve_mv_d3_w0(a, rb, st);
+}
+
+#endif  // KK_MV_DIM == 0
+
+#if KK_MV_DIM > 0
+template <int NV>
+static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
+  constexpr int D = KK_MV_DIM;
+  dim3 g(a.list ? a.nlist : a.nb), blk(MVT);
+#if KK_MV_W
+  // the weight store: general covariance evaluation only
+  if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, true>), g, blk, 0, st, a);
+  else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, true>), g, blk, 0, st, a);
+#else
+  if (rb == 5) {
+    if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, true, false>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false>), g, blk, 0, st, a);
+  } else {
+    if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 8, true, false>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, false>), g, blk, 0, st, a);
+  }
+#endif
+}
+
+#define KK_CAT2(x, y) x##y
+#define KK_CAT(x, y) KK_CAT2(x, y)
+hipError_t KK_CAT(KK_CAT(launch_solve_mv_d, KK_MV_DIM), KK_CAT(_w, KK_MV_W))(const MVArgs& a, int rb,
+                                                                            hipStream_t st) {
+  if (a.nvar == 1) launch_nv<1>(a, rb, st);
+  else if (a.nvar == 2) launch_nv<2>(a, rb, st);
+  else if (a.nvar == 3) launch_nv<3>(a, rb, st);
+  else launch_nv<4>(a, rb, st);
   return hipGetLastError();
 }
+#endif  // KK_MV_DIM > 0
 
 }  // namespace kk

@@ -109,6 +109,8 @@ hipError_t launch_fallback_mv(const MVArgs& a, const int* list, int nlist, doubl
 __host__ __device__ constexpr size_t mv_fallback_words(int rows) {
   return 2 * (size_t)(rows + MAXVAR) * (rows + MAXVAR) + 2 * (size_t)(rows + MAXVAR) * MAXVAR + (rows + MAXVAR);
 }
+// estimates and variances from stored weights, G lanes per block
+hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);
 hipError_t launch_copy(const double* src, double* dst, size_t n, hipStream_t st);
 // blocks with more than limit neighbours into list (count: device counter)
 hipError_t launch_collect_above(const int* nnear, int nblock, int limit, int* count, int* list, hipStream_t st);

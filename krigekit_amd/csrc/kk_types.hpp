@@ -142,6 +142,12 @@ struct MVArgs {
   const int* list;            // blocks to solve (null: 0..nb-1)
   int nlist;
   int* ctr;                   // blocks sent to the fallback (or null)
+  // kriging weights out, weights[k + mmax_w * (g + nvar * (q + nvar * b))]
+  // for neighbour k of group g and target q (or null: not written);
+  // weights_only leaves est and var untouched (k_fallback_mv also the flags)
+  double* weights;
+  int mmax_w;
+  int weights_only;
   double* est;                // est[b + nb * q]
   double* var;                // var[b + nb * (i + nvar * j)]
   int* flag;                  // 2: K or S not positive definite (fallback); 3: fallback failed (NaN)
@@ -188,8 +194,28 @@ struct SolveArgs {
   double pivot_tol;           // FP32 pivot ratio below which a system is re-solved in FP64
   double* est;
   double* var;
-  double* weights;            // nblock*mmax or nullptr
+  double* weights;            // k_fallback: kriging weights out, nblock*mmax (or nullptr)
+  int weights_only;           // k_fallback: write the weights only (est/var/flag untouched)
   int* flag;                  // per block: 0 ok, 1 needs FP64 re-solve, 2 Cholesky failed
+};
+
+// Estimates from stored weights (use_old_weight), the weight store's layouts:
+// nnear[g + ng b], inear[k + nm (g + ng b)] (1-based), weight[k + nm (g + ng
+// (q + nvar b))], var[i + nvar (j + nvar b)]; group g holds neighbours of
+// variable g
+struct ReplayArgs {
+  int nb, nm, ng, nvar;
+  const int* nnear;
+  const int* inear;
+  const double* weight;
+  const double* wvar;
+  const double* ovalue[MAXVAR];
+  int nobs[MAXVAR];           // a stored id outside 1..nobs reads as NaN
+  int std_ck, unbias;
+  double sk_mean[MAXVAR];
+  double bound_lo, bound_hi;
+  double* est;                // est[b + nb * q]
+  double* var;                // var[b + nb * (i + nvar * j)]
 };
 
 }  // namespace kk

@@ -639,6 +639,11 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   }
   for (int i = 0; i < m; ++i) xs[i] = rhs[i];
   const int info = bk_fallback(K, A, ipiv, n, m, ld, xs, 1);
+  if (a.weights) {
+    // the weight store: x(1:n), NaN when the system stays singular
+    for (int i = 0; i < n; ++i) a.weights[(size_t)b * a.mmax + i] = info ? __builtin_nan("") : xs[i];
+    if (a.weights_only) return;
+  }
   a.flag[b] = info ? 3 : 0;
   if (info) {
     // x = NaN (kriging_base.F90:3369-3383): NaN estimate; the variance is

@@ -72,6 +72,10 @@ def _load_lib():
     sig("krige_get_weight_dims", [_i64, _pi, _pi, _pi])
     sig("krige_get_weight_nnear", [_i64, _c_int, _c_int, _pi])
     sig("krige_get_weight_inear", [_i64, _c_int, _c_int, _c_int, _pi])
+    sig("krige_get_weight_data", [_i64, _c_int, _c_int, _c_int, _c_int, _pd])
+    sig("krige_get_weight_var", [_i64, _c_int, _c_int, _pd])
+    sig("krige_set_weights", [_i64, _c_int, _c_int, _c_int, _c_int, _pi, _pi, _pd, _pi, _pd])
+    sig("krige_free_weight_store", [_i64])
     sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
     sig("kk_set_search_mode", [_i64, _c_int, _c_int])
     sig("kk_set_factor_reuse", [_i64, _c_int, _c_int])
@@ -149,9 +153,11 @@ class Kriging:
         self.bounds = np.asfortranarray(bounds if bounds is not None else [-huge, huge], dtype=np.float64)
         seed = seed or random.randint(0, 2 ** 31 - 1)
         self.store_weight = store_weight
+        self.use_old_weight = use_old_weight
         self.weight_file = weight_file
-        if store_weight and weight_file:
-            raise ValueError("writing a weight file is not provided by this backend")
+        # store_weight without a weight_file keeps the weights in memory only
+        # (the reference's own check at kriging.py:572-573 compares the str
+        # with b"" and so never fires; its tests rely on that)
         if store_weight and use_old_weight:
             raise ValueError("store_weight and use_old_weight are mutually exclusive")
         if cross_validation and nsim > 0:
@@ -290,13 +296,14 @@ class Kriging:
 
     @property
     def timing(self) -> dict:
-        out = np.zeros(12)
+        out = np.zeros(13)
         _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
                     total_ms=out[4], fp64_resolved=int(out[5]), bk_fallback=int(out[6]),
                     exact_search_queries=int(out[7]), fast_search_launches=int(out[8]),
                     fast_search_kernel_ms=out[9],
-                    exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11])
+                    exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11],
+                    weights_ms=out[12])
 
     GRAD_PARAMS = ("nugget", "sill", "a_major", "a_minor1", "a_minor2", "azimuth", "dip", "plunge")
 
@@ -349,6 +356,91 @@ class Kriging:
                "kk_get_neighbours")
         mask = np.arange(mm.value)[None, :] < nnear[:, None]
         return nnear, np.where(mask, inear + 1, 0)
+
+    # ------------------------------------------------------------------
+    # weight store (reference kriging.py:1747-1960)
+    def free_weight_store(self):
+        """Release the in-memory weight store."""
+        _check(_load_lib().krige_free_weight_store(self._handle), "krige_free_weight_store")
+
+    def set_weights(self, weights: dict) -> None:
+        """Load kriging weights into the store so that solve() replays them
+        against the current observation values (no search, no solve) and
+        restores the stored variance.  ``weights`` as returned by
+        :meth:`get_weights`: ``nnear`` (nblock, ngroups), ``inear`` (nblock,
+        ngroups, nmax) 1-based, ``weight`` (nblock, ngroups, nmax) for nvar 1
+        or (nblock, nvar, ngroups, nmax), optional ``order`` (nblock,) and
+        ``variance`` (nblock,) or (nblock, nvar, nvar), zero when absent."""
+        if self.store_weight:
+            raise ValueError(
+                "set_weights() is incompatible with store_weight=True. "
+                "Construct Kriging without store_weight=True when reusing weights.")
+        nnear = np.ascontiguousarray(weights["nnear"], dtype=np.int32)
+        inear = np.ascontiguousarray(weights["inear"], dtype=np.int32)
+        weight = np.asarray(weights["weight"], dtype=np.float64)
+        if nnear.ndim != 2:
+            raise ValueError("nnear must be 2-D (nblock, ngroups)")
+        if inear.ndim != 3:
+            raise ValueError("inear must be 3-D (nblock, ngroups, nmax)")
+        nb, ng = nnear.shape
+        nm = inear.shape[2]
+        nv = self.nvar
+        order_raw = weights.get("order", None)
+        if order_raw is None or self.nsim == 0:
+            order_f = np.arange(1, nb + 1, dtype=np.int32)
+        else:
+            order_f = np.ascontiguousarray(order_raw, dtype=np.int32)
+        # the store's column-major layouts are the block-major arrays reversed
+        nnear_f = np.asfortranarray(nnear.T)
+        inear_f = np.asfortranarray(inear.T)
+        if weight.ndim == 3:
+            weight_f = np.asfortranarray(weight.T[:, :, np.newaxis, :])
+        elif weight.ndim == 4:
+            weight_f = np.asfortranarray(np.transpose(weight, (3, 2, 1, 0)))
+        else:
+            raise ValueError("weight must be 3-D (nblock, ngroups, nmax) or "
+                             "4-D (nblock, nvar, ngroups, nmax)")
+        if weights.get("variance", None) is not None:
+            var = np.asarray(weights["variance"], dtype=np.float64)
+            if var.ndim == 1:
+                var = var[:, np.newaxis, np.newaxis]
+            var_f = np.asfortranarray(np.transpose(var, (1, 2, 0)))
+        else:
+            var_f = np.zeros((nv, nv, nb), order="F")
+        if weight_f.shape != (nm, ng, nv, nb) or var_f.shape != (nv, nv, nb):
+            raise ValueError("weight/variance shapes do not match nnear and inear")
+        _check(_load_lib().krige_set_weights(
+            self._handle, nm, ng, nv, nb, _ip(nnear_f), _ip(inear_f), _dp(weight_f),
+            _ip(order_f), _dp(var_f)), "krige_set_weights")
+        self.use_old_weight = True
+
+    def get_weights(self) -> dict:
+        """The stored weights: ``nnear`` (nblock, ngroups) int32, ``inear``
+        (nblock, ngroups, nmax) int32 1-based, ``weight`` (nblock, nvar,
+        ngroups, nmax) -- (nblock, ngroups, nmax) for nvar 1 -- and
+        ``variance`` (nblock,) for nvar 1, else (nblock, nvar, nvar).
+        Entries past nnear are zero."""
+        lib = _load_lib()
+        nm, ng, nb = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.krige_get_weight_dims(self._handle, ctypes.byref(nm), ctypes.byref(ng),
+                                         ctypes.byref(nb)), "krige_get_weight_dims")
+        nm, ng, nb, nv = nm.value, ng.value, nb.value, self.nvar
+        nnear_f = np.zeros((ng, nb), dtype=np.int32, order="F")
+        inear_f = np.zeros((nm, ng, nb), dtype=np.int32, order="F")
+        weight_f = np.zeros((nm, ng, nv, nb), dtype=np.float64, order="F")
+        var_f = np.zeros((nv, nv, nb), dtype=np.float64, order="F")
+        _check(lib.krige_get_weight_nnear(self._handle, ng, nb, _ip(nnear_f)), "krige_get_weight_nnear")
+        _check(lib.krige_get_weight_inear(self._handle, nm, ng, nb, _ip(inear_f)), "krige_get_weight_inear")
+        _check(lib.krige_get_weight_data(self._handle, nm, ng, nv, nb, _dp(weight_f)),
+               "krige_get_weight_data")
+        _check(lib.krige_get_weight_var(self._handle, nv, nb, _dp(var_f)), "krige_get_weight_var")
+        weight = np.ascontiguousarray(weight_f.T)
+        var = np.ascontiguousarray(var_f.T)
+        if nv == 1:
+            weight = weight[:, 0, :, :]
+            var = var[:, 0, 0]
+        return {"nnear": np.ascontiguousarray(nnear_f.T), "inear": np.ascontiguousarray(inear_f.T),
+                "weight": weight, "variance": var}
 
     @property
     def solver_stats(self) -> dict:

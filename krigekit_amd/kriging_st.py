@@ -59,6 +59,8 @@ class SpaceTimeKriging(Kriging):
         self.ndim = None  # spatial ndim, set by the first set_obs
         self.nvar, self.ndrift, self.nsim, self.unbias = nvar, ndrift, nsim, unbias
         self.store_weight = store_weight
+        self.use_old_weight = use_old_weight
+        self.weight_file = weight_file
         self.cross_validation = cross_validation
         self.verbose = verbose
         self.seed = seed

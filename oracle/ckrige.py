@@ -88,7 +88,7 @@ def _solve_like_reference(K, R, n, p):
         return None if info else x
 
 
-def solve(spec, table_mode=True, nthread=8):
+def solve(spec, table_mode=True, nthread=8, want_weights=False):
     nvar, nd = spec["nvar"], spec["ndim"]
     std = spec.get("std_ck", True)
     unbias = spec.get("unbias", 1)
@@ -126,6 +126,9 @@ def solve(spec, table_mode=True, nthread=8):
     est = np.full((nb, nvar), np.nan)
     var = np.full((nb, nvar, nvar), np.nan)
     nnear = np.stack([l[0] for l in lists], axis=1)
+    # weight store layout: weights[b, q, g, k] for neighbour k of group g
+    wmax = max(int(nnear.max()), 1) if nb else 1
+    weights = np.zeros((nb, nvar, nvar, wmax)) if want_weights else None
     for b in range(nb):
         groups = [l[1][b, :l[0][b]] for l in lists]
         rv = np.concatenate([np.full(len(g), v) for v, g in enumerate(groups)]).astype(int)
@@ -172,6 +175,10 @@ def solve(spec, table_mode=True, nthread=8):
             var[b] = 0.0
             continue
         w = x[:n]
+        if want_weights:
+            for g in range(nvar):
+                rows = np.nonzero(rv == g)[0]
+                weights[b, :, g, :len(rows)] = w[rows, :].T
         for q in range(nvar):
             e = w[:, q] @ zr
             if not std and unbias and nvar > 1:
@@ -198,4 +205,4 @@ def solve(spec, table_mode=True, nthread=8):
             var[b, i, i] = max(var[b, i, i], 0.0)
             for j in range(i + 1, nvar):
                 var[b, j, i] = var[b, i, j] = max(var[b, j, i], 0.0)
-    return dict(estimate=est, variance=var, nnear=nnear, lists=lists)
+    return dict(estimate=est, variance=var, nnear=nnear, lists=lists, weights=weights)

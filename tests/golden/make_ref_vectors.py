@@ -6,7 +6,8 @@ by Cholesky, blocks solved by the symmetric-indefinite fallback -- the
 reference's own counters split the last two into factorisations and cache
 reuses, which depend on its thread schedule), and ck_vectors.npz: its multi-variable
 (cokriging) results on problems.ck_cases() in table and analytic mode
-(oracle/_ref/libkriging_ref_analytic.so).  Run in a container where the
+(oracle/_ref/libkriging_ref_analytic.so); for some problems also the stored
+kriging weights (krige_get_weight_data).  Run in a container where the
 reference tree and oracle/_ref exist:  python tests/golden/make_ref_vectors.py
 """
 import os
@@ -22,11 +23,25 @@ import problems  # noqa: E402
 from runners import run_ref, run_ref_ck  # noqa: E402
 
 
+# problems whose stored kriging weights are kept (every fifth block with two
+# or more neighbours; a single-neighbour block is not solved by the
+# reference and its weight slot is left over from another block)
+WEIGHT_CASES = ("ok2d", "lattice2d_s1_cv0", "random3d_octant", "simple_kriging", "rangescale_localnugget",
+                "maxdist", "ladder", "near_duplicates", "big80_2d", "loo2d", "bounds")
+
+
 def main():
     out = {}
     for spec in problems.parity_cases():
-        r = run_ref(spec)
         key = spec["name"]
+        r = run_ref(spec, want_weights=key in WEIGHT_CASES)
+        if key in WEIGHT_CASES:
+            nn = r["nnear"]
+            sel = np.arange(0, len(nn), 5)
+            sel = sel[nn[sel] >= 2].astype(np.int32)
+            w = r["weight"][sel, :int(nn.max())]
+            out[f"{key}/wblocks"] = sel
+            out[f"{key}/weight"] = np.where(np.arange(w.shape[1])[None, :] < nn[sel, None], w, 0.0)
         out[f"{key}/estimate"] = r["estimate"]
         out[f"{key}/variance"] = r["variance"]
         out[f"{key}/nnear"] = r["nnear"]
@@ -37,18 +52,29 @@ def main():
         out[f"{key}/stats"] = np.array([st["fail"], st["chol_fact"] + st["chol_reuse"],
                                         st["ssytrf_fact"] + st["ssytrf_reuse"]], dtype=np.int32)
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
-    print("wrote", len(out) // 5, "problems")
+    print("wrote", len(problems.parity_cases()), "problems")
     ck = {}
     for spec in problems.ck_cases():
         for mode in ("table", "analytic"):
-            r = run_ref_ck(spec, mode)
+            r = run_ref_ck(spec, mode, want_weights=mode == "table")
+            if mode == "table":
+                # the weight store: neighbour counts and lists per group, and
+                # the weights of every third block with two or more neighbours
+                nn = r["nnear"]
+                sel = np.arange(0, len(nn), 3)
+                sel = sel[nn[sel].sum(axis=1) >= 2].astype(np.int32)
+                wmax = max(int(nn.max()), 1)
+                ck[f"{spec['name']}/nnear"] = nn.astype(np.int16)
+                ck[f"{spec['name']}/inear"] = r["inear"][:, :, :wmax].astype(np.uint16)
+                ck[f"{spec['name']}/wblocks"] = sel
+                ck[f"{spec['name']}/weight"] = r["weight"][sel][:, :, :, :wmax]
             ck[f"{spec['name']}/{mode}/estimate"] = r["estimate"]
             ck[f"{spec['name']}/{mode}/variance"] = r["variance"]
             st = r["stats"]
             ck[f"{spec['name']}/{mode}/stats"] = np.array([st["fail"], st["chol_fact"] + st["chol_reuse"],
                                                            st["ssytrf_fact"] + st["ssytrf_reuse"]], dtype=np.int32)
     np.savez_compressed(os.path.join(HERE, "ck_vectors.npz"), **ck)
-    print("wrote", len(ck) // 3, "cokriging results")
+    print("wrote", len(problems.ck_cases()), "cokriging problems")
 
 
 if __name__ == "__main__":

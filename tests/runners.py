@@ -17,7 +17,7 @@ def run_oracle(spec, table_mode=True, nthread=8, weights=False):
     return o
 
 
-def run_ref(spec, mode="table", nthread=4):
+def run_ref(spec, mode="table", nthread=4, want_weights=False):
     from oracle.refdrv import RefKriging
     k = RefKriging(mode=mode, ndim=spec["ndim"], store_weight=True, unbias=spec.get("unbias", 1),
                    cross_validation=bool(spec.get("cross_validation", False)), bounds=spec.get("bounds"),
@@ -41,6 +41,8 @@ def run_ref(spec, mode="table", nthread=4):
     nnear = nn[:, 0].astype(np.int32)
     out = dict(estimate=e[:, 0], variance=v[:, 0, 0], nnear=nnear, inear=_mask(nnear, inn[:, 0, :]),
                stats=k.solver_stats())
+    if want_weights:
+        out["weight"] = k.weights()[:, 0, 0, :]
     k.close()
     return out
 
@@ -139,11 +141,11 @@ def max_errors(x, ref):
     return float(np.max(e, initial=0.0)), float(np.max(v, initial=0.0))
 
 
-def run_ref_ck(spec, mode="table", nthread=4):
+def run_ref_ck(spec, mode="table", nthread=4, want_weights=False):
     """the compiled reference engine on a multi-variable problem"""
     from oracle.refdrv import RefKriging
     k = RefKriging(mode=mode, ndim=spec["ndim"], nvar=spec["nvar"], std_ck=spec.get("std_ck", True),
-                   unbias=spec.get("unbias", 1))
+                   unbias=spec.get("unbias", 1), store_weight=want_weights)
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
@@ -157,9 +159,12 @@ def run_ref_ck(spec, mode="table", nthread=4):
         k.set_search(**s)
     k.solve(nthread=nthread)
     e, v = k.results()
-    stats = k.solver_stats()
+    out = dict(estimate=e, variance=v, stats=k.solver_stats())
+    if want_weights:
+        out["nnear"], out["inear"] = k.neighbours()
+        out["weight"] = k.weights()
     k.close()
-    return dict(estimate=e, variance=v, stats=stats)
+    return out
 
 
 def make_gpu_ck(spec, **kw):

@@ -116,3 +116,116 @@ def test_destroy_zeroes_handle():
     h = ctypes.c_int64(0)
     assert lib.krige_create(ctypes.byref(h)) == 0 and h.value > 0
     assert lib.krige_destroy(ctypes.byref(h)) == 0 and h.value == 0
+
+
+# ---- weight store: host-side behaviour (no device work) ---------------------
+def _store_obj(nobs=20, nb=6, nmax=4, **kw):
+    from krigekit_amd.kriging import Kriging
+    rng = np.random.default_rng(3)
+    k = Kriging(ndim=2, **kw)
+    k.set_obs(1, rng.uniform(0, 10, (nobs, 2)), rng.normal(size=nobs), nmax=nmax)
+    k.set_grid(rng.uniform(0, 10, (nb, 2)))
+    return k
+
+
+def _store(nb=6, nm=4, nobs=20, seed=4):
+    rng = np.random.default_rng(seed)
+    nnear = rng.integers(1, nm + 1, (nb, 1)).astype(np.int32)
+    inear = np.zeros((nb, 1, nm), dtype=np.int32)
+    weight = np.zeros((nb, 1, nm))
+    for b in range(nb):
+        n = nnear[b, 0]
+        inear[b, 0, :n] = np.sort(rng.choice(nobs, n, replace=False)) + 1
+        weight[b, 0, :n] = rng.dirichlet(np.ones(n))
+    return dict(nnear=nnear, inear=inear, weight=weight, variance=rng.uniform(0, 1, nb))
+
+
+def test_get_weights_without_store_raises():
+    """kriging_capi_common.F90:503-519"""
+    k = _store_obj()
+    with pytest.raises(RuntimeError, match="Weight store not allocated"):
+        k.get_weights()
+    k.close()
+
+
+def test_set_weights_round_trip_and_free():
+    """set_weights keeps the arrays as given (kriging_base.F90:2930-2977),
+    get_weights returns them; free_weight_store releases the store"""
+    k = _store_obj()
+    w = _store()
+    k.set_weights(w)
+    assert k.use_old_weight
+    g = k.get_weights()
+    for key in ("nnear", "inear", "weight", "variance"):
+        np.testing.assert_array_equal(g[key], w[key])
+    k.free_weight_store()
+    with pytest.raises(RuntimeError, match="Weight store not allocated"):
+        k.get_weights()
+    k.close()
+
+
+def test_set_weights_without_variance_gives_zero():
+    k = _store_obj()
+    w = _store()
+    w.pop("variance")
+    k.set_weights(w)
+    assert np.all(k.get_weights()["variance"] == 0.0)
+    k.close()
+
+
+def test_set_weights_shape_checks():
+    """the reference's messages (kriging_base.F90:2950-2968)"""
+    k = _store_obj()
+    w = _store(nb=5)
+    with pytest.raises(RuntimeError, match="nnear shape mismatch"):
+        k.set_weights(w)
+    w = _store(nm=3)
+    with pytest.raises(RuntimeError, match="inear shape mismatch"):
+        k.set_weights(w)
+    w = _store()
+    w["nnear"][2, 0] = 9
+    with pytest.raises(RuntimeError, match="nnear out of range"):
+        k.set_weights(w)
+    k.close()
+
+
+def test_set_weights_with_store_weight_raises():
+    k = _store_obj(store_weight=True)
+    with pytest.raises(ValueError, match="store_weight=True"):
+        k.set_weights(_store())
+    k.close()
+
+
+def test_set_weights_needs_grid():
+    from krigekit_amd.kriging import Kriging
+    k = Kriging(ndim=2)
+    k.set_obs(1, np.random.default_rng(1).uniform(0, 1, (5, 2)), np.zeros(5), nmax=4)
+    with pytest.raises(RuntimeError, match="call set_grid\\(\\) before alloc_weight_store"):
+        k.set_weights(_store())
+    k.close()
+
+
+def test_replay_checks_before_device_work(tmp_path):
+    """use_old_weight errors raised before any device work: no store, an
+    empty store, a factor file of other dimensions or cut short
+    (kriging_base.F90:3074-3106, 2896-2917)"""
+    k = _store_obj(use_old_weight=True)
+    with pytest.raises(RuntimeError, match="requires set_weights\\(\\) to be called first"):
+        k.solve()
+    w = _store()
+    w["nnear"][:] = 0
+    k.set_weights(w)
+    with pytest.raises(RuntimeError, match="in-memory weight store is empty"):
+        k.solve()
+    k.close()
+    fac = tmp_path / "w.fac"
+    fac.write_text("           6           1           5\n")
+    k = _store_obj(use_old_weight=True, weight_file=str(fac))
+    with pytest.raises(RuntimeError, match="weight_file dimensions do not match"):
+        k.solve()
+    k.close()
+    fac.write_text("           6           1           4\n1  2\n  3  4\n  0.5\n  0.25  0.75\n2  1\n")
+    k = _store_obj(use_old_weight=True, weight_file=str(fac))
+    with pytest.raises(RuntimeError, match="Unexpected end-of-file or read error in factor file"):
+        k.solve()
+    k.close()

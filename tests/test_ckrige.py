@@ -74,3 +74,27 @@ def test_point_cokriging_closed_form():
     assert o["estimate"][0, 0] == pytest.approx(2.0, rel=1e-12)
     assert o["estimate"][0, 1] == pytest.approx(-1.0, rel=1e-12)
     del rng
+
+
+@pytest.mark.parametrize("spec", CASES, ids=IDS)
+def test_oracle_weights_match_reference_store(spec, vectors):
+    """the weights of the reference's store (krige_get_weight_data after a
+    solve with store_weight) on sampled blocks, and its neighbour counts,
+    against the oracle's solution of the bordered system"""
+    key = spec["name"]
+    o = ckrige.solve(spec, want_weights=True)
+    # a block the fallback cannot solve keeps one neighbour in every group
+    # (ctx%nnear = 1, kriging_base.F90:3369-3383)
+    nn = np.array(o["nnear"])
+    failed = np.isnan(o["estimate"]).any(axis=1) & (nn.sum(axis=1) >= 2)
+    nn[failed] = 1
+    np.testing.assert_array_equal(nn, vectors[f"{key}/nnear"])
+    sel = vectors[f"{key}/wblocks"]
+    wr = vectors[f"{key}/weight"]
+    wo = o["weights"][sel]
+    k = min(wr.shape[-1], wo.shape[-1])
+    fin = np.isfinite(wr).all(axis=(1, 2, 3)) & np.isfinite(o["estimate"][sel]).all(axis=1)
+    assert fin.sum() > 0
+    d = np.abs(wo[fin][..., :k] - wr[fin][..., :k]) / np.maximum(1.0, np.abs(wr[fin][..., :k]))
+    assert d.max() < 1e-9, d.max()
+

@@ -217,6 +217,27 @@ def test_committed_reference_vectors():
         assert got == list(vec[f"{key}/stats"]), (key, got, list(vec[f"{key}/stats"]))
 
 
+def test_committed_reference_weights():
+    """the reference's stored weights (krige_get_weight_data after a solve
+    with store_weight, sampled blocks of two or more neighbours) agree with
+    the oracle's weights"""
+    vec = np.load(problems.GOLDEN + "/ref_vectors.npz")
+    n = 0
+    for spec in problems.parity_cases():
+        key = spec["name"]
+        if f"{key}/weight" not in vec.files:
+            continue
+        o = run_oracle(spec, weights=True)
+        sel = vec[f"{key}/wblocks"]
+        wr = vec[f"{key}/weight"]
+        wo = np.where(np.arange(o["weights"].shape[1])[None, :] < o["nnear"][sel, None],
+                      o["weights"][sel], 0.0)[:, :wr.shape[1]]
+        d = np.abs(wo - wr) / np.maximum(1.0, np.abs(wr))
+        assert d.max() < 1e-9, (key, d.max())
+        n += 1
+    assert n >= 10
+
+
 def test_failure_ladder_is_reached():
     """the ladder problem reaches every step of the reference's fallback
     (kriging_base.F90:3244-3398): Cholesky, the symmetric-indefinite
