@@ -204,7 +204,10 @@ bool build_model(const std::vector<VStruct>& vs, int ndim, bool table_mode, Mode
   M.ndim = ndim;
   M.nstruct = (int)vs.size();
   if (M.nstruct < 1 || M.nstruct > kk::MAXS) return false;
+  M.simple = 1;
   for (int i = 0; i < M.nstruct; ++i) {
+    const int t = vs[i].vtype;
+    M.simple &= (t == kk::NUG || t == kk::SPH || t == kk::EXP || t == kk::GAU || t == kk::BSQ || t == kk::LIN);
     M.vtype[i] = vs[i].vtype;
     M.is_product[i] = vs[i].is_product;
     M.sill[i] = vs[i].sill;

@@ -105,15 +105,60 @@ __device__ inline double table_interp_c(const double* tab, double h) {
   return t0 + fr * (t1 - t0);
 }
 
-// analytic tail beyond the table (h >= 3.5), kept out of line: it is rare
-// and its shape switch would otherwise inflate every caller's registers
+// Inline evaluation of any mode for models whose shapes are all "simple"
+// (nugget, spherical, exponential, Gaussian, bounded-square, linear): the
+// same arithmetic as cov<DIM>, without the trigonometric and power shapes
+// whose FP64 library code would inflate every caller's registers.
+__device__ inline bool simple_shape(int t) { return t == NUG || t == SPH || t == EXP || t == GAU || t == BSQ || t == LIN; }
+
+__device__ inline double corefunc_simple(int t, double r) {
+  switch (t) {
+    case SPH: return (r < 1.0) ? (1.0 - 1.5 * r) + 0.5 * (r * r * r) : 0.0;
+    case EXP: return exp(-3.0 * r);
+    case GAU: return exp(-3.0625 * (r * r));
+    case BSQ: { double a = 1.0 - r * r; return (r < 1.0) ? a * a : 0.0; }
+    case LIN: return (r < 1.0) ? 1.0 - r : 0.0;
+    default: return 0.0;
+  }
+}
+
+__device__ inline double comp_cov_h_simple(const Model& M, int c, double h) {
+  const double eps = 2.2250738585072014e-308 * 1.0e3;
+  return (h > eps) ? M.sill[c] * corefunc_simple(M.vtype[c], h) : M.sill[c] + M.nugget[c];
+}
+
+// struct_cov_h for models of simple shapes only, inline
+__device__ inline double struct_cov_h_simple(const Model& M, double h) {
+  double res = 0.0;
+  int iv = 0;
+  while (iv < M.nstruct) {
+    double g = comp_cov_h_simple(M, iv, h);
+    while (iv + 1 < M.nstruct && M.is_product[iv + 1]) {
+      ++iv;
+      g = g * comp_cov_h_simple(M, iv, h);
+    }
+    res = res + g;
+    ++iv;
+  }
+  return res;
+}
+
+// analytic tail beyond the table (h >= 3.5): inline for models of simple
+// shapes (the common exponential and Gaussian tails: a few structures, one
+// exp each); otherwise out of line, as the general shape switch would
+// inflate every caller's registers.  M.simple is kernel-uniform, so the
+// choice is a scalar branch.
 __device__ __noinline__ double struct_cov_h_tail(const Model* M, double h) { return struct_cov_h(*M, h); }
+__device__ inline double tabB_tail(const Model& M, double h) {
+  if (M.simple) return struct_cov_h_simple(M, h);
+  return struct_cov_h_tail(&M, h);
+}
 
 // Level-B table evaluation: one anisotropy transform and one lookup
 template <int DIM>
 __device__ inline double cov_tableB(const Model& M, double dx, double dy, double dz) {
   const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
-  if (h >= TAB_HB[3]) return M.b_tail ? struct_cov_h_tail(&M, h) : 0.0;
+  if (h >= TAB_HB[3]) return M.b_tail ? tabB_tail(M, h) : 0.0;
   if (h <= 0.0) return M.tab[0];
   return table_interp_c(M.tab, h);
 }
@@ -190,47 +235,11 @@ __device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy
   return cov<DIM>(*Mp, dx, dy, dz);
 }
 
-// Inline evaluation of any mode for models whose shapes are all "simple"
-// (nugget, spherical, exponential, Gaussian, bounded-square, linear): the
-// same arithmetic as cov<DIM>, without the trigonometric and power shapes
-// whose FP64 library code would inflate every caller's registers.
-__device__ inline bool simple_shape(int t) { return t == NUG || t == SPH || t == EXP || t == GAU || t == BSQ || t == LIN; }
-
-__device__ inline double corefunc_simple(int t, double r) {
-  switch (t) {
-    case SPH: return (r < 1.0) ? (1.0 - 1.5 * r) + 0.5 * (r * r * r) : 0.0;
-    case EXP: return exp(-3.0 * r);
-    case GAU: return exp(-3.0625 * (r * r));
-    case BSQ: { double a = 1.0 - r * r; return (r < 1.0) ? a * a : 0.0; }
-    case LIN: return (r < 1.0) ? 1.0 - r : 0.0;
-    default: return 0.0;
-  }
-}
-
-__device__ inline double comp_cov_h_simple(const Model& M, int c, double h) {
-  const double eps = 2.2250738585072014e-308 * 1.0e3;
-  return (h > eps) ? M.sill[c] * corefunc_simple(M.vtype[c], h) : M.sill[c] + M.nugget[c];
-}
-
 template <int DIM>
 __device__ inline double cov_simple(const Model& M, double dx, double dy, double dz) {
   if (M.mode == COV_TABLE_B) {
     const double h = aniso_h<DIM>(M.bmat, dx, dy, dz);
-    if (h >= TAB_HB[3]) {
-      if (!M.b_tail) return 0.0;
-      double res = 0.0;
-      int iv = 0;
-      while (iv < M.nstruct) {
-        double g = comp_cov_h_simple(M, iv, h);
-        while (iv + 1 < M.nstruct && M.is_product[iv + 1]) {
-          ++iv;
-          g = g * comp_cov_h_simple(M, iv, h);
-        }
-        res = res + g;
-        ++iv;
-      }
-      return res;
-    }
+    if (h >= TAB_HB[3]) return M.b_tail ? struct_cov_h_simple(M, h) : 0.0;
     if (h <= 0.0) return M.tab[0];
     return table_interp_c(M.tab, h);
   }

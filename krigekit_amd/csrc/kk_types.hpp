@@ -44,6 +44,9 @@ struct Model {
   int mode;
   int nstruct;
   int b_tail;                 // Level-B table has an analytic tail
+  int simple;                 // every structure has a simple shape (nugget, spherical,
+                              // exponential, Gaussian, bounded-square, linear)
+  int pad0;
   int vtype[MAXS];
   int is_product[MAXS];
   double sill[MAXS];
