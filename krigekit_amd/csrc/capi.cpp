@@ -1725,6 +1725,49 @@ bool search_var(Krige& K, int v, hipStream_t st) {
   return true;
 }
 
+// estimates, variances and failure counts of a multi-variable pass, from
+// the device (the reference's n_fail / Singular matrix rules)
+bool fetch_results_mv(Krige& K) {
+  const char* W = "t_kriging_base%solve_base";
+  const int nb = K.nblock, nvar = K.nvar;
+  // blocks sent to the fallback by the last pass
+  const int nfb = (int)K.timing[6];
+  K.est_mv.resize((size_t)nb * nvar);
+  K.var_mv.resize((size_t)nb * nvar * nvar);
+  HIPCHK(hipMemcpy(K.est_mv.data(), K.d_mvest.p, sizeof(double) * K.est_mv.size(), hipMemcpyDeviceToHost), W);
+  HIPCHK(hipMemcpy(K.var_mv.data(), K.d_mvvar.p, sizeof(double) * K.var_mv.size(), hipMemcpyDeviceToHost), W);
+  std::vector<int> flag(nb), tot(nb, 0), nn(nb);
+  HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+  for (int v = 0; v < nvar; ++v) {
+    HIPCHK(hipMemcpy(nn.data(), K.dv[v].nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
+    for (int b = 0; b < nb; ++b) tot[b] += nn[b];
+  }
+  int nfail = 0, nsolved = 0;
+  long first = -1;
+  for (int b = 0; b < nb; ++b) {
+    if (flag[b] == 3) {
+      ++nfail;
+      if (first < 0) first = b + 1;
+    } else if (tot[b] > 1) {
+      ++nsolved;  // a single neighbour is not solved (kriging_base.F90:2127)
+    }
+    if (tot[b] == 0 && !K.neglect_error) {
+      set_error("t_kriging_base%assemble_linear_system", "not enough neighbors for kriging at block", b + 1);
+      return false;
+    }
+  }
+  K.stats[0] = nfail;
+  K.stats[1] = nsolved - (nfb - nfail);
+  K.stats[2] = K.stats[4] = 0;
+  K.stats[3] = nfb - nfail;
+  if (nfail > 0 && !K.neglect_error) {
+    set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
+    return false;
+  }
+  return true;
+}
+
+
 bool do_solve_mv(Krige& K, bool fetch) {
   const char* W = "t_kriging_base%solve_base";
   if (!ensure_stream(K)) return false;
@@ -1856,14 +1899,31 @@ bool do_solve_mv(Krige& K, bool fetch) {
     a.weights = K.d_wout.as<double>();
     a.mmax_w = K.mmax_w;
   }
+  a.prec = K.precision;
+  a.pivot_tol = K.pivot_tol;
   HIPCHK(kk::launch_solve_mv(a, st), W);
-  // blocks whose K or Schur complement is not positive definite: the
-  // Bunch-Kaufman fallback on the full augmented system
-  int nfb = 0;
+  int nfb = 0, nres = 0;
   if (!K.d_lists.ensure(sizeof(int) * (size_t)nb)) {
     set_error(W, "device allocation failed");
     return false;
   }
+  // FP32 + refinement: blocks whose FP32 pivot ratio fell below pivot_tol
+  // (or whose FP32 pivot was not positive) are solved again in FP64
+  if (kk::mv_fp32r_instance(a)) {
+    HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
+    HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 1, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
+    HIPCHK(hipMemcpyAsync(&nres, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
+    HIPCHK(hipStreamSynchronize(st), W);
+    if (nres > 0) {
+      kk::MVArgs r = a;
+      r.prec = kk::PREC_FP64;
+      r.list = K.d_lists.as<int>();
+      r.nlist = nres;
+      HIPCHK(kk::launch_solve_mv(r, st), W);
+    }
+  }
+  // blocks whose K or Schur complement is not positive definite: the
+  // Bunch-Kaufman fallback on the full augmented system
   HIPCHK(hipMemsetAsync(K.d_counts.p, 0, sizeof(int) * 16, st), W);
   HIPCHK(kk::launch_collect(K.d_flag.as<int>(), nb, 2, K.d_counts.as<int>(), K.d_lists.as<int>(), st), W);
   HIPCHK(hipMemcpyAsync(&nfb, K.d_counts.p, sizeof(int), hipMemcpyDeviceToHost, st), W);
@@ -1888,43 +1948,12 @@ bool do_solve_mv(Krige& K, bool fetch) {
   K.timing[2] = ms;
   HIPCHK(hipEventElapsedTime(&ms, K.ev[0], K.ev[4]), W);
   K.timing[4] = ms;
+  K.timing[5] = nres;
+  K.timing[6] = nfb;
   K.timing[7] = -1;
   if (!fetch) return true;
-  K.est_mv.resize((size_t)nb * nvar);
-  K.var_mv.resize((size_t)nb * nvar * nvar);
-  HIPCHK(hipMemcpy(K.est_mv.data(), K.d_mvest.p, sizeof(double) * K.est_mv.size(), hipMemcpyDeviceToHost), W);
-  HIPCHK(hipMemcpy(K.var_mv.data(), K.d_mvvar.p, sizeof(double) * K.var_mv.size(), hipMemcpyDeviceToHost), W);
-  std::vector<int> flag(nb), tot(nb, 0), nn(nb);
-  HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
-  for (int v = 0; v < nvar; ++v) {
-    HIPCHK(hipMemcpy(nn.data(), K.dv[v].nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
-    for (int b = 0; b < nb; ++b) tot[b] += nn[b];
-  }
-  int nfail = 0, nsolved = 0;
-  long first = -1;
-  for (int b = 0; b < nb; ++b) {
-    if (flag[b] == 3) {
-      ++nfail;
-      if (first < 0) first = b + 1;
-    } else if (tot[b] > 1) {
-      ++nsolved;  // a single neighbour is not solved (kriging_base.F90:2127)
-    }
-    if (tot[b] == 0 && !K.neglect_error) {
-      set_error("t_kriging_base%assemble_linear_system", "not enough neighbors for kriging at block", b + 1);
-      return false;
-    }
-  }
-  K.stats[0] = nfail;
-  K.stats[1] = nsolved - (nfb - nfail);
-  K.stats[2] = K.stats[4] = 0;
-  K.stats[3] = nfb - nfail;
-  if (nfail > 0 && !K.neglect_error) {
-    set_error("t_kriging_base%solve_linear_system", "Singular matrix", first);
-    return false;
-  }
-  return true;
+  return fetch_results_mv(K);
 }
-
 
 // ---------------------------------------------------------------------------
 // weight store (kriging_base.F90:2703-2977, kriging_capi_common.F90:491-619)
@@ -3444,7 +3473,7 @@ int kk_fetch(int64_t h) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (!fetch_results(*K)) K->solved = false;
+  if (!(K->nvar > 1 ? fetch_results_mv(*K) : fetch_results(*K))) K->solved = false;
   return ierr();
 }
 

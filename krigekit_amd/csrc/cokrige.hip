@@ -29,7 +29,21 @@
 // needed: it gives S, F'K^-1 b, z'K^-1 b, ..., and one thread finishes the
 // small algebra.  The factorisation takes four pivots per barrier (a panel
 // factored by the one wave that holds its columns).  The variable pairs'
-// models are staged in LDS.  FP64 throughout.
+// models are staged in LDS.
+//
+// MF (FP32 factorisation, MFMA trailing update, FP64 refinement; up to 80
+// columns): the FP64 bordered matrix stays in registers for the residuals;
+// an FP32 copy in LDS (column-major) is factored by right-looking blocked
+// Cholesky over 16-column panels: one wave factors the diagonal 16 x 16 tile,
+// the rows below are solved against it (one row per thread), and the
+// trailing tiles take the rank-16 update A -= L L' on the matrix cores
+// (v_mfma_f32_16x16x4f32, four per tile, tiles spread over the waves).  The
+// extra rows are eliminated alongside, as in the FP64 path.  Each extra
+// column x_f = K^-1 X_f' is then obtained in FP32 by one backward
+// substitution, and the Gram matrix is refined with FP64 residuals:
+// X_e K^-1 X_f' = X_e x_f + x_e' (X_f' - K x_f) + O(e^2), e the FP32 solve
+// error.  A block whose FP32 pivot is not positive or whose pivot ratio
+// falls below pivot_tol is flagged (1) and solved again by the FP64 path.
 #include <hip/hip_runtime.h>
 
 #include "kk_bk.hpp"
@@ -116,8 +130,12 @@ __device__ inline double wg_sum(double v, double* part) {
 // (packed lower triangle of the bordered matrix), and after the multipliers
 // the weights w_q = K^-1 (b_q - F mu_q) are obtained by one backward
 // substitution with the unit factor, all targets together.
-template <int DIM, int NV, int RB, bool SIMPLE, bool WEIGHTS>
-__global__ __launch_bounds__(MVT, (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv(MVArgs a) {
+constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+template <int DIM, int NV, int RB, bool SIMPLE, bool WEIGHTS, bool MF = false>
+__global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv(MVArgs a) {
+  static_assert(!MF || (RB == 5 && !WEIGHTS), "the FP32 MFMA instance covers up to 80 columns, no weight store");
   // rows + extra rows fit 16 RB (the instance is chosen by the column count)
   constexpr int ROWS = 16 * RB;
   __shared__ double sx[DIM][ROWS];
@@ -128,12 +146,25 @@ __global__ __launch_bounds__(MVT, (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv
   __shared__ double sgw[MAXSUB];
   // right-hand sides as NPART partial sums over the integration nodes
   constexpr int NPART = NV == 2 ? 8 : 2;
-  __shared__ double sb[NPART][NV][ROWS];
   // assembly: each thread's RB values of one row slot; then, for the
-  // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered
+  // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered.
+  // MF: after the assembly the same area holds the packed lower triangle
+  // (mirror exchange), then the FP32 matrix (column stride LD32 floats).
   static_assert(RB * MVT >= 2 * ROWS * 4, "panel buffer fits the assembly slots");
-  __shared__ __align__(16) double sbuf[RB * MVT];
+  constexpr int LD32 = ROWS + 4;
+  constexpr int SBW = NPART * NV * ROWS;
+  constexpr int UW = MF ? cmax3(SBW + RB * MVT, ROWS * LD32 / 2, ROWS * (ROWS + 1) / 2) : SBW + RB * MVT;
+  __shared__ __align__(16) double uarea[UW];
+  double(*const sb)[NV][ROWS] = reinterpret_cast<double(*)[NV][ROWS]>(uarea);
+  double* const sbuf = uarea + SBW;
   double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
+  // MF: the refined solutions x_f (columns of K^-1 X', FP64), the matrix-vector
+  // products M [x_f; -e_f], 1 / L(k, k), K's diagonal
+  constexpr int MXE = 2 * NV + 2;  // extra columns at most (non-standard cokriging)
+  __shared__ double sxd[MF ? MXE : 1][MF ? ROWS : 1];
+  __shared__ double red[MF ? ROWS : 1][MF ? MXE : 1];
+  __shared__ float sli[MF ? ROWS : 1];
+  __shared__ float sdg[MF ? ROWS : 1];
   __shared__ double gram[MAXE][MAXE];
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
@@ -350,6 +381,250 @@ __global__ __launch_bounds__(MVT, (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv
 #pragma unroll
     for (int cb = 0; cb < RB; ++cb) A[ra][cb] = sbuf[cb * MVT + t];
   }
+  if constexpr (MF) {
+    // ---- FP32 blocked Cholesky with an MFMA trailing update, FP64 refinement -----
+    float* const M32 = reinterpret_cast<float*>(uarea);  // M32[j * LD32 + i] = M(i, j)
+    double* const L64 = uarea;                             // packed lower triangle
+    const int NT = (nx + 15) >> 4;                         // 16 x 16 tiles per side
+    const int lane = t & 63;
+    // the full symmetric matrix in registers (the residuals need both
+    // triangles): the upper entries from the lower ones, through LDS
+    __syncthreads();  // every thread has read its assembly slots
+#pragma unroll
+    for (int ra = 0; ra < RB; ++ra)
+#pragma unroll
+      for (int cb = 0; cb <= ra; ++cb) {
+        const int i = tr + 16 * ra, j = tc + 16 * cb;
+        if (i < nx && j <= i) L64[i * (i + 1) / 2 + j] = A[ra][cb];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int ra = 0; ra < RB; ++ra)
+#pragma unroll
+      for (int cb = ra; cb < RB; ++cb) {
+        const int i = tr + 16 * ra, j = tc + 16 * cb;
+        if (j > i && j < nx) A[ra][cb] = L64[j * (j + 1) / 2 + i];
+      }
+    __syncthreads();
+    // FP32 copy, zero past nx; K's diagonal for the pivot ratios
+#pragma unroll
+    for (int ra = 0; ra < RB; ++ra)
+#pragma unroll
+      for (int cb = 0; cb < RB; ++cb) {
+        const int i = tr + 16 * ra, j = tc + 16 * cb;
+        if (i < 16 * NT && j < 16 * NT) M32[j * LD32 + i] = (i < nx && j < nx) ? (float)A[ra][cb] : 0.0f;
+      }
+    if (tr == tc) {
+#pragma unroll
+      for (int ra = 0; ra < RB; ++ra)
+        if (tr + 16 * ra < n) sdg[tr + 16 * ra] = (float)A[ra][ra];
+    }
+    if (t == 0) sfail = 0;
+    __syncthreads();
+    const int npan = (n + 15) >> 4;
+    for (int P = 0; P < npan; ++P) {
+      const int c0 = 16 * P, npv = n - c0 < 16 ? n - c0 : 16;
+      // diagonal tile: lane t holds row c0 + t; pivot k from lane k, the
+      // scaled column broadcast by readlane, rank-1 updates of the later
+      // columns (extra columns inside the tile are updated, not pivoted)
+      if (t < 16) {
+        float r[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) r[j] = M32[(c0 + j) * LD32 + c0 + t];
+        float pm = 1.0f;
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          if (k >= npv) break;
+          const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r[k]), k));
+          bad = bad || !(piv > 0.0f);
+          pm = fminf(pm, piv / sdg[c0 + k]);
+          float d = __builtin_amdgcn_rsqf(piv);
+          d = d * fmaf(-0.5f * piv * d, d, 1.5f);
+          r[k] = t == k ? piv * d : (t > k ? r[k] * d : r[k]);
+#pragma unroll
+          for (int j = k + 1; j < 16; ++j) {
+            const float ljk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r[k]), j));
+            if (t >= j) r[j] = fmaf(-r[k], ljk, r[j]);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) M32[(c0 + j) * LD32 + c0 + t] = r[j];
+        if (t < npv) sli[c0 + t] = 1.0f / r[t];
+        if (t == 0 && (bad || pm < (float)a.pivot_tol)) sfail = 1;
+      }
+      __syncthreads();
+      // rows below the tile (extra rows included): solved against it, one
+      // row per thread
+      const int nrow = nx - (c0 + 16);
+      if (t < nrow) {
+        const int i = c0 + 16 + t;
+        float r[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) r[j] = M32[(c0 + j) * LD32 + i];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          if (k >= npv) break;
+          r[k] *= sli[c0 + k];
+#pragma unroll
+          for (int j = k + 1; j < 16; ++j) r[j] = fmaf(-r[k], M32[(c0 + k) * LD32 + c0 + j], r[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) M32[(c0 + j) * LD32 + i] = r[j];
+      }
+      __syncthreads();
+      // trailing tiles (I, J), P < J <= I < NT: C -= L_I L_J' over this
+      // panel's pivots, four 16x16x4 MFMAs per tile, tiles round-robin over
+      // the waves.  A(m, k) = -L(16 I + m, c0 + k) at lane m + 16 k,
+      // B(k, n) = L(16 J + n, c0 + k) at lane n + 16 k, C(m, n) in register
+      // m % 4 of lane n + 16 (m / 4).
+      const int T0 = P + 1, ntr = NT - T0;
+      const int npair = ntr > 0 ? ntr * (ntr + 1) / 2 : 0;
+      const int col = lane & 15, kq = lane >> 4;
+      for (int q = wave; q < npair; q += 4) {
+        int J = 0, rem = q;
+        while (rem >= ntr - J) {
+          rem -= ntr - J;
+          ++J;
+        }
+        const int Jt = T0 + J, It = Jt + rem;
+        mf_f4* cp = reinterpret_cast<mf_f4*>(&M32[(16 * Jt + col) * LD32 + 16 * It + 4 * kq]);
+        mf_f4 acc = *cp;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (4 * c >= npv) break;
+          const int kc = c0 + 4 * c + kq;
+          const bool on = 4 * c + kq < npv;
+          const float av = on ? -M32[kc * LD32 + 16 * It + col] : 0.0f;
+          const float bv = on ? M32[kc * LD32 + 16 * Jt + col] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        }
+        *cp = acc;
+      }
+      __syncthreads();
+    }
+    if (sfail) {
+      // FP32 pivot not positive or below pivot_tol x K(k, k): FP64 re-solve
+      if (t == 0) a.flag[b] = 1;
+      return;
+    }
+    // x_e = K~^-1 X_e' for every extra column, refined twice.  First the
+    // backward substitution with L' from y_e = L^-1 X_e' (row n + e of the
+    // factor); then the residual r_e = X_e' - K x_e in FP64 and the FP32
+    // correction K~ d_e = r_e (forward and backward substitution), x_e += d_e;
+    // then the residual of the corrected x_e.  32 threads per extra column,
+    // rows l32 + 32 m.
+    constexpr int NM = (ROWS + 31) / 32;
+    const int g = t >> 5, l32 = t & 31;
+    // lane src of this thread's half wave (readlane: no LDS round trip)
+    auto bcast = [&](float v, int src) {
+      const int lo = __builtin_amdgcn_readlane(__float_as_int(v), src);
+      const int hi = __builtin_amdgcn_readlane(__float_as_int(v), src + 32);
+      return __int_as_float(lane < 32 ? lo : hi);
+    };
+    // y <- L'^-1 y (rows i < n)
+    auto backward = [&](float (&y)[NM]) {
+      for (int k = n - 1; k >= 0; --k) {
+        const int mk = k >> 5;
+        float yk = y[0];
+#pragma unroll
+        for (int m = 1; m < NM; ++m) yk = m == mk ? y[m] : yk;
+        const float wk = bcast(yk, k & 31) * sli[k];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+          const int i = l32 + 32 * m;
+          if (i < k) y[m] = fmaf(-M32[i * LD32 + k], wk, y[m]);
+          else if (i == k) y[m] = wk;
+        }
+      }
+    };
+    for (int e = g; e < ne; e += MVT / 32) {
+      float y[NM];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const int i = l32 + 32 * m;
+        y[m] = i < n ? M32[i * LD32 + n + e] : 0.0f;
+      }
+      backward(y);
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        const int i = l32 + 32 * m;
+        if (i < n) sxd[e][i] = (double)y[m];
+      }
+    }
+    __syncthreads();
+    for (int step = 0; step < 2; ++step) {
+      // red[j][f] = (M v_f)_j with v_f = [x_f; -e_f]: -(X_f' - K x_f)_j for
+      // j < n, X_e x_f at j = n + e.  FP64, from the registers: M is
+      // symmetric, so thread (tr, tc) forms sum_a M(tr + 16 a, tc + 16 b)
+      // v(tr + 16 a) for output row tc + 16 b, and the 16 lanes of one column
+      // class (one wave) complete the sum
+      for (int f = 0; f < ne; ++f) {
+        double v[RB], pr[RB];
+#pragma unroll
+        for (int ra = 0; ra < RB; ++ra) {
+          const int i = tr + 16 * ra;
+          v[ra] = i < n ? sxd[f][i] : (i == n + f ? -1.0 : 0.0);
+        }
+#pragma unroll
+        for (int cb = 0; cb < RB; ++cb) {
+          double acc = 0.0;
+#pragma unroll
+          for (int ra = 0; ra < RB; ++ra) acc = fma(A[ra][cb], v[ra], acc);
+#pragma unroll
+          for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+          pr[cb] = acc;
+        }
+        if (tr == 0) {
+#pragma unroll
+          for (int cb = 0; cb < RB; ++cb)
+            if (tc + 16 * cb < nx) red[tc + 16 * cb][f] = pr[cb];
+        }
+      }
+      __syncthreads();
+      if (step == 0) {
+        // the correction d_e: L u = r_e, L' d_e = u, in FP32
+        for (int e = g; e < ne; e += MVT / 32) {
+          float y[NM];
+#pragma unroll
+          for (int m = 0; m < NM; ++m) {
+            const int i = l32 + 32 * m;
+            y[m] = i < n ? (float)-red[i][e] : 0.0f;
+          }
+          for (int k = 0; k < n; ++k) {
+            const int mk = k >> 5;
+            float yk = y[0];
+#pragma unroll
+            for (int m = 1; m < NM; ++m) yk = m == mk ? y[m] : yk;
+            const float uk = bcast(yk, k & 31) * sli[k];
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+              const int i = l32 + 32 * m;
+              if (i > k && i < n) y[m] = fmaf(-M32[k * LD32 + i], uk, y[m]);
+              else if (i == k) y[m] = uk;
+            }
+          }
+          backward(y);
+#pragma unroll
+          for (int m = 0; m < NM; ++m) {
+            const int i = l32 + 32 * m;
+            if (i < n) sxd[e][i] += (double)y[m];
+          }
+        }
+      } else {
+        // H_ef = X_e x_f + x_e' (X_f' - K x_f), e <= f, mirrored
+        for (int w = t; w < ne * ne; w += MVT) {
+          const int e = w / ne, f = w - e * ne;
+          if (e > f) continue;
+          double h = red[n + e][f];
+          for (int i = 0; i < n; ++i) h = fma(-sxd[e][i], red[i][f], h);
+          gram[e][f] = h;
+          gram[f][e] = h;
+        }
+      }
+      __syncthreads();
+    }
+  } else {
   // the panel buffer takes the assembly slots: cleared once every thread has
   // read its own (rows past nx keep zero factors throughout)
   __syncthreads();
@@ -470,6 +745,7 @@ __global__ __launch_bounds__(MVT, (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv
       const int j = tc + 16 * k;
       if (i >= n && i < nx && j >= n && j <= i) gram[i - n][j - n] = gram[j - n][i - n] = -A[ra][k];
     }
+  }
   }
   // group means of the neighbour values (Isaaks-Srivastava)
   double gm[NV];
@@ -825,6 +1101,12 @@ int mv_extra_cols(int nvar, int std_ck, int unbias) {
   return nvar + p + 1 + (isa ? nvar : 0) + (unbias ? 0 : 1);
 }
 
+// FP32 + MFMA + refinement runs for systems up to 80 columns (weights: FP64)
+bool mv_fp32r_instance(const MVArgs& a) {
+  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
+  return a.prec == PREC_FP32R && !a.weights && cols <= 80;
+}
+
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
   if (a.nb <= 0 || (a.list && a.nlist <= 0)) return hipSuccess;
   const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
@@ -853,7 +1135,10 @@ static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
   if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, true>), g, blk, 0, st, a);
   else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, true>), g, blk, 0, st, a);
 #else
-  if (rb == 5) {
+  if (rb == 5 && a.prec == PREC_FP32R) {
+    if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, true, false, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false, true>), g, blk, 0, st, a);
+  } else if (rb == 5) {
     if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, true, false>), g, blk, 0, st, a);
     else hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false>), g, blk, 0, st, a);
   } else {

@@ -102,6 +102,9 @@ hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double*
 // rows); rows + mv_extra_cols(...) must not exceed 128
 int mv_extra_cols(int nvar, int std_ck, int unbias);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
+// true when launch_solve_mv runs the FP32 + MFMA + refinement instance, which
+// flags blocks for an FP64 re-solve (flag 1)
+bool mv_fp32r_instance(const MVArgs& a);
 // Bunch-Kaufman fallback of the blocks listed (flag 2 after launch_solve_mv):
 // scratch holds mv_fallback_words(a.rows) doubles per listed block; flag 0
 // when solved, 3 when the system stays singular (NaN estimates)

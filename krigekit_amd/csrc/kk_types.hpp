@@ -153,7 +153,10 @@ struct MVArgs {
   int weights_only;
   double* est;                // est[b + nb * q]
   double* var;                // var[b + nb * (i + nvar * j)]
-  int* flag;                  // 2: K or S not positive definite (fallback); 3: fallback failed (NaN)
+  int* flag;                  // 1: FP32 pivot too small (FP64 re-solve); 2: K or S not positive
+                              // definite (fallback); 3: fallback failed (NaN)
+  int prec;                   // PREC_FP64 or PREC_FP32R (FP32 + MFMA factorisation, FP64 refinement)
+  double pivot_tol;           // FP32 pivot ratio below which a block is re-solved in FP64
 };
 
 // member blocks per work item of the single-variable solve (factor reuse):

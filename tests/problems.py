@@ -293,6 +293,25 @@ def ck4_small(nb=100, seed=19):
     return s
 
 
+def ck_near_duplicates(nb=120, seed=21):
+    """two-variable point cokriging with 40 primary observations duplicated
+    0.01 m away and no nugget: the FP32 pivot ratio of such a pair (about
+    3e-5) falls below 1e-3, so the FP32+refine path re-solves those blocks in
+    FP64"""
+    s = ck_small(std=True, gq=False, nb=nb, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    o = s["obs"][0]
+    pick = rng.choice(len(o["coord"]), 40, replace=False)
+    ang = rng.uniform(0, 2 * np.pi, len(pick))
+    dup = o["coord"][pick] + 0.01 * np.stack([np.cos(ang), np.sin(ang)], 1)
+    o["coord"] = np.concatenate([o["coord"], dup])
+    o["value"] = np.concatenate([o["value"], o["value"][pick] + rng.normal(0, 0.01, len(pick))])
+    # blocks centred on the duplicated points, so their pairs enter the systems
+    s["grid"] = np.concatenate([s["grid"], o["coord"][pick[:60]] + 3.0])
+    s["name"] = "ck_near_duplicates"
+    return s
+
+
 def ck_cases():
     return [ck_small(True, True), ck_small(True, False), ck_small(False, True), ck_small(False, False),
             ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small(), ck_empty_var(True), ck_empty_var(False),
