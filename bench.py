@@ -50,7 +50,9 @@ def time_path(spec, precision, steps, warmup, reorder=1):
     from krigekit_amd.kriging import device_synchronize, from_spec
     k = from_spec(spec, precision=precision)
     k.set_search_mode(True, reorder)
-    k.solve()  # uploads inputs, first full pass
+    t0 = time.perf_counter()
+    k.solve()  # uploads inputs, first full pass, results copied back
+    e2e = time.perf_counter() - t0
     for _ in range(warmup):
         k.solve_resident()
     device_synchronize()
@@ -66,6 +68,7 @@ def time_path(spec, precision, steps, warmup, reorder=1):
     nnear, _ = k.neighbours()
     k.close()
     agg = {key: float(np.mean([p[key] for p in phases])) for key in phases[0]}
+    agg["end_to_end_ms"] = e2e * 1e3
     return wall, agg, est, var, nnear
 
 
@@ -75,8 +78,24 @@ def rel_stats(x, ref, nnear, floor=1e-4):
     return dict(max=float(np.max(e)), p99=float(np.percentile(e, 99)))
 
 
+def host_cpu():
+    """host CPU model and the number of online CPUs"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
 def cpu_baseline(spec, stride=4):
-    """the compiled reference engine on every `stride`-th node, all host cores"""
+    """the compiled reference engine on every `stride`-th node, with as many
+    threads as the host has cores up to 16 (the CPU share of one GPU on the
+    GPU hosts); returns the baseline and the reference's results on the sample"""
     nthread = min(os.cpu_count() or 1, 16)
     sub = dict(spec)
     sub["grid"] = spec["grid"][::stride]
@@ -97,17 +116,88 @@ def cpu_baseline(spec, stride=4):
         t = time.perf_counter()
         k.solve(nthread=nthread)
         t = time.perf_counter() - t
+        e, v = k.results()
+        ref = dict(estimate=e[:, 0], variance=v[:, 0, 0])
         k.close()
         kind = "reference"
     except (FileNotFoundError, OSError):
         from oracle import okrige
         t = time.perf_counter()
-        okrige.solve(sub, nthread=nthread)
+        ref = okrige.solve(sub, nthread=nthread)
         t = time.perf_counter() - t
         kind = "port"
-    return dict(value=n / t, unit="nodes/s", cores=nthread, kind=kind,
+    model, ncpu = host_cpu()
+    return dict(value=n / t, unit="nodes/s", cores=nthread, kind=kind, host_cpu=model, host_nproc=ncpu,
                 sample=f"every {stride}th node of the 256x256x64 grid ({n} nodes), solve() only, "
-                       f"kd-tree built beforehand; {t:.2f} s wall")
+                       f"kd-tree built beforehand; {t:.2f} s wall"), ref
+
+
+def err_stats(x, ref, floor=1e-4):
+    """max and p99 of |x - ref| / max(|ref|, floor) over the values the
+    reference defines (blocks with a single neighbour, whose reference estimate
+    depends on thread scheduling, and failed blocks are NaN on both sides)"""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    ref = np.asarray(ref, dtype=np.float64).ravel()
+    ok = np.isfinite(ref) & np.isfinite(x)
+    if not ok.any():
+        return dict(max=None, p99=None, n=0)
+    e = np.abs(x[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), floor)
+    return dict(max=float(np.max(e)), p99=float(np.percentile(e, 99)), n=int(ok.sum()))
+
+
+def errors_vs_reference(runs, c2_ref, c2_nnear, stride=4):
+    """the checker leg: estimate and variance errors of every workload's GPU
+    results (FP64 and FP32+refine) against the compiled reference where it
+    runs, else the oracle, on a bounded sample of each workload"""
+    from krigekit_amd import synth
+    out = {}
+    keep = c2_nnear[::stride] != 1
+    c2 = {}
+    for prec, (e, v) in runs["ok3d"].items():
+        c2[prec] = dict(estimate=err_stats(np.where(keep, e[::stride], np.nan), c2_ref["estimate"]),
+                        variance=err_stats(np.where(keep, v[::stride], np.nan), c2_ref["variance"]))
+    out["3d_octant_4.2M"] = dict(against="compiled reference, table mode", sample=f"every {stride}th node", **c2)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from runners import run_ref
+    from oracle import ckrige, okrige
+    for name, spec_fn, label in (("2d_ok_40k", synth.ok2d, "all 40,000 nodes"),
+                                 ("loo_100k", synth.loo2d, "all 100,000 leave-one-out estimates")):
+        spec = spec_fn()
+        r = run_ref(spec, mode="table" if spec_fn is synth.ok2d else "analytic", nthread=16)
+        keep = r["nnear"] != 1
+        res = {}
+        for prec, (e, v) in runs[name].items():
+            res[prec] = dict(estimate=err_stats(np.where(keep, e, np.nan), r["estimate"]),
+                             variance=err_stats(np.where(keep, v, np.nan), r["variance"]))
+        out[name] = dict(against="compiled reference, " + ("table" if spec_fn is synth.ok2d else "analytic")
+                         + " mode", sample=label, **res)
+    # cokriging: the numpy oracle (Cholesky and Schur complement) on every
+    # 4999th block
+    spec = synth.ck2d()
+    idx = np.arange(0, len(spec["grid"]), 4999)
+    sub = dict(spec)
+    sub["grid"] = spec["grid"][idx]
+    sub["sub_coord"] = spec["sub_coord"].reshape(-1, 16, 2)[idx].reshape(-1, 2)
+    sub["sub_weight"] = spec["sub_weight"].reshape(-1, 16)[idx].ravel()
+    o = ckrige.solve(sub)
+    res = {}
+    for prec, (e, v) in runs["cokriging"].items():
+        res[prec] = dict(estimate=err_stats(e[idx], o["estimate"]), variance=err_stats(v[idx], o["variance"]))
+    out["cokriging_1M_blocks"] = dict(against="numpy oracle", sample=f"{len(idx)} blocks (every 4999th)", **res)
+    # space-time: the C oracle on every 7499th node (the reference's
+    # space-time entry points cannot be run here)
+    spec = synth.st()
+    idx = np.arange(0, len(spec["grid"]), 7499)
+    sub = dict(spec)
+    sub["grid"] = spec["grid"][idx]
+    o = okrige.solve(sub, nthread=16)
+    keep = o["nnear"] != 1
+    res = {}
+    for prec, (e, v) in runs["space_time"].items():
+        res[prec] = dict(estimate=err_stats(np.where(keep, e[idx], np.nan), o["estimate"]),
+                         variance=err_stats(np.where(keep, v[idx], np.nan), o["variance"]))
+    out["space_time_7.5M"] = dict(against="C oracle", sample=f"{len(idx)} nodes (every 7499th)", **res)
+    return out
 
 
 def loo_gradient(steps=3):
@@ -156,9 +246,10 @@ def loo_gradient(steps=3):
 
 def resident_rate(k, nodes, steps=3):
     """nodes/s of repeated device passes over resident inputs (one warm pass
-    first), with the last pass's per-phase device timings"""
+    first), with the last pass's per-phase device timings and the results"""
     from krigekit_amd.kriging import device_synchronize
     k.solve()
+    res = k.get_results(copy=True)
     k.solve_resident()
     device_synchronize()
     t0 = time.perf_counter()
@@ -168,34 +259,55 @@ def resident_rate(k, nodes, steps=3):
     wall = (time.perf_counter() - t0) / steps
     tm = k.timing
     return dict(nodes=nodes, nodes_per_s=nodes / wall, ms_per_pass=wall * 1e3,
-                phase_ms={key: round(float(v), 4) for key, v in tm.items()})
+                phase_ms={key: round(float(v), 4) for key, v in tm.items()}), res
 
 
-def space_time():
+def space_time(precision="fp64"):
     """space-time product-sum kriging: 500 stations x 365 days (182,500
     observations), 500 x 500 km grid at 1 km x 30 times (7.5 million nodes),
     Cs exponential (nugget 0.05, sill 1, 50 km), Ct exponential (sill 0.5,
-    5 days), k 0.5, time_at 10 km/day, nmax 48, maxdist 40, FP64"""
+    5 days), k 0.5, time_at 10 km/day, nmax 48, maxdist 40"""
     from krigekit_amd import synth
     from krigekit_amd.kriging_st import from_st_spec
     spec = synth.st()
-    k = from_st_spec(spec)
-    out = resident_rate(k, len(spec["grid"]))
+    k = from_st_spec(spec, precision=precision)
+    out, res = resident_rate(k, len(spec["grid"]))
     k.close()
-    return out
+    return dict(precision=precision, **out), res
 
 
-def cokriging():
+def cokriging(precision="fp64"):
     """2-variable standard cokriging (linear model of coregionalisation):
     20,000 + 200,000 observations, 10^6 blocks of 4 x 4 Gauss-Legendre nodes,
-    24 + 48 neighbours, FP64"""
+    24 + 48 neighbours (74 unknowns); fp32r = FP32 factorisation with the
+    MFMA trailing update and two FP64 refinement steps"""
     from krigekit_amd import synth
     from krigekit_amd.kriging import from_spec
     spec = synth.ck2d()
-    k = from_spec(spec)
-    out = resident_rate(k, len(spec["grid"]))
+    k = from_spec(spec, precision=precision)
+    out, res = resident_rate(k, len(spec["grid"]))
     k.close()
-    return out
+    return dict(precision=precision, **out), res
+
+
+def small_run(spec_fn, precision):
+    """one full solve of a small workload (results only)"""
+    from krigekit_amd.kriging import from_spec
+    k = from_spec(spec_fn(), precision=precision, table_mode=spec_fn.__name__ != "loo2d")
+    k.solve()
+    res = k.get_results(copy=True)
+    k.close()
+    return res
+
+
+def load_issue():
+    """VALU and SALU wave-instructions per search launch from the committed
+    SQ counter summary (profiles/search_issue.json), or None"""
+    path = os.path.join(ROOT, "profiles", "search_issue.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
 
 
 def load_traffic():
@@ -232,10 +344,16 @@ def main():
         sys.exit("bench.py runs on one GPU only: this build has no multi-GPU path "
                  "(nodes are independent: each GPU would take a contiguous range of the Morton-ordered nodes).")
     if args.only:
-        out = space_time() if args.only == "space-time" else cokriging()
+        out, _ = space_time() if args.only == "space-time" else cokriging()
         print(json.dumps({"workload": args.only, **out}), flush=True)
         return
     from krigekit_amd import synth
+    from krigekit_amd.kriging import from_spec
+    # one small solve first: the process's one-time device set-up stays out
+    # of the end-to-end figures
+    kw = from_spec(synth.ok3d(scale=1.0 / 1024))
+    kw.solve()
+    kw.close()
     spec = synth.ok3d()
     nb = len(spec["grid"])
     w64, t64, e64, v64, nn = time_path(spec, "fp64", args.steps, args.warmup)
@@ -293,6 +411,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes / nlaunch,
             "launch_ms": search_ms / nlaunch,
         },
+        "search_issue": None,
         "solve_roofline": {
             "kernel": "k_solve (assembly + LDL^T + forward solves, FP64)",
             "bound": "valu_fp64",
@@ -305,13 +424,46 @@ def main():
                     else "solve of the default pipeline: part of it overlaps the search and is not counted",
         },
     }
+    issue = load_issue()
+    if issue:
+        # wave-instructions issue at one per 4 cycles per SIMD: 1,024 SIMDs at
+        # 2.4 GHz; the search is bound by this issue rate, not by HBM
+        cap = 1024 * 2.4e9 / 4 * (search_ms / nlaunch) * 1e-3
+        line["search_issue"] = {"bound": "valu+salu instruction issue",
+                                "valu_per_launch": issue["valu_per_launch"],
+                                "salu_per_launch": issue["salu_per_launch"],
+                                "valu_issue_frac": issue["valu_per_launch"] / cap,
+                                "salu_issue_frac": issue["salu_per_launch"] / cap,
+                                "source": issue.get("source")}
+    line["end_to_end"] = {
+        "note": "Kriging.solve() on host arrays: inputs copied to HBM, one pass, results copied back (PCIe "
+                "included; never the value)",
+        "fp32r_ms": t32["end_to_end_ms"], "fp64_ms": t64["end_to_end_ms"],
+        "fp32r_nodes_per_s": nb / (t32["end_to_end_ms"] * 1e-3),
+        "fp64_nodes_per_s": nb / (t64["end_to_end_ms"] * 1e-3)}
     if not args.no_loo_gradient:
         line["loo_gradient"] = loo_gradient()
+    runs = {"ok3d": {"fp64": (e64, v64), "fp32r": (e32, v32)}}
     if not args.no_secondary:
-        line["space_time_product_sum"] = space_time()
-        line["cokriging_2var_blocks"] = cokriging()
+        st64, r = space_time("fp64")
+        runs["space_time"] = {"fp64": r}
+        st32, r = space_time("fp32r")
+        runs["space_time"]["fp32r"] = r
+        line["space_time_product_sum"] = st64
+        line["space_time_product_sum_fp32r"] = st32
+        ck64, r = cokriging("fp64")
+        runs["cokriging"] = {"fp64": r}
+        ck32, r = cokriging("fp32r")
+        runs["cokriging"]["fp32r"] = r
+        line["cokriging_2var_blocks"] = ck64
+        line["cokriging_2var_blocks_fp32r_mfma"] = ck32
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(spec)
+        line["cpu_baseline"], c2_ref = cpu_baseline(spec)
+        if not args.no_secondary:
+            from krigekit_amd import synth as _s
+            runs["2d_ok_40k"] = {p: small_run(_s.ok2d, p) for p in ("fp64", "fp32r")}
+            runs["loo_100k"] = {p: small_run(_s.loo2d, p) for p in ("fp64", "fp32r")}
+            line["cpu_baseline"]["errors_vs_reference"] = errors_vs_reference(runs, c2_ref, nn)
     print(json.dumps(line), flush=True)
 
 
