@@ -602,7 +602,11 @@ struct Krige {
   bool perm_ok = false;
   DBuf d_nnear, d_inear, d_est, d_var, d_flag, d_hdis, d_hidx, d_counts, d_lists, d_fb;
   DBuf d_sets, d_items, d_table;  // factor reuse: per-block set data, work items, hash table
-  bool reuse_on() const { return reuse_mode > 0 || (reuse_mode < 0 && !cross_validation); }
+  // auto mode: a pass in which fewer than a quarter of the solved blocks
+  // shared another's factorisation turns the grouping off for the next passes
+  // over the same inputs (results are the same either way)
+  bool reuse_low = false;
+  bool reuse_on() const { return reuse_mode > 0 || (reuse_mode < 0 && !cross_validation && !reuse_low); }
   Model model{};
   std::vector<double> tab;
   // multi-variable path
@@ -738,6 +742,7 @@ bool stage_inputs(Krige& K) {
       return false;
   }
   K.dev_inputs_ok = true;
+  K.reuse_low = false;
   return true;
 }
 
@@ -1320,6 +1325,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     const long blocks_ok = ((long)hc[16] - n1_blocks) - hc[18];
     K.stats[1] = (int)(items_ok + (nref - hc[20]) + (nbig - hc[25]));
     K.stats[2] = (int)(blocks_ok - items_ok);
+    if (K.reuse_mode < 0 && 4L * K.stats[2] < (long)K.stats[1] + K.stats[2]) K.reuse_low = true;
   } else {
     K.stats[1] = nb - nempty - nfb;
     K.stats[2] = 0;

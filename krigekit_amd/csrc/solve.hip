@@ -380,8 +380,11 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         u = fma(-t, ext[0], u);
         v = fma(-t, ext[1], v);
 #pragma unroll
+        // (columns past the wave's member count hold zeros: updating them
+        // is harmless, and a guard here costs the wide instances a wave of
+        // occupancy in registers)
         for (int c = 0; c < MB; ++c)
-          if (c < mw) y[c] = fma(-t, ext[2 + c], y[c]);
+          y[c] = fma(-t, ext[2 + c], y[c]);
         __syncthreads();
       }
     }
@@ -437,8 +440,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         pu = fmaf(-t, ext32[0], pu);
         pz = fmaf(-t, ext32[1], pz);
 #pragma unroll
-        for (int c = 0; c < MB; ++c)
-          if (c < mw) py[c] = fmaf(-t, ext32[2 + c], py[c]);
+        for (int c = 0; c < MB; ++c) py[c] = fmaf(-t, ext32[2 + c], py[c]);
         L[k] = (r > k) ? t : L[k];
         __syncthreads();
       }

@@ -279,3 +279,25 @@ def test_factor_reuse_is_bitwise_neutral(case, precision):
         assert s1["chol_fact"] + s1["chol_reuse"] == s0["chol_fact"]
     if case == "ok3d_chunks":
         assert s1["chol_reuse"] > s1["chol_fact"]
+
+
+def test_factor_reuse_auto_mode_turns_off_for_distinct_sets():
+    """auto mode groups the first pass; when fewer than a quarter of the solved
+    blocks shared a factorisation, the next pass over the same inputs skips the
+    grouping (chol_reuse 0), with bitwise the same results; new inputs group
+    again"""
+    spec = problems.random_problem(ndim=2, nobs=1500, nq=3000, nmax=12, seed=31)
+    k = make_gpu(spec)
+    k.solve()
+    s0 = k.solver_stats
+    e0, v0 = k.get_results(copy=True)
+    assert 0 < 4 * s0["chol_reuse"] < s0["chol_fact"] + s0["chol_reuse"], s0
+    k.solve()
+    s1 = k.solver_stats
+    e1, v1 = k.get_results(copy=True)
+    assert s1["chol_reuse"] == 0 and s1["chol_fact"] == s0["chol_fact"] + s0["chol_reuse"], (s0, s1)
+    assert np.array_equal(e0, e1, equal_nan=True) and np.array_equal(v0, v1, equal_nan=True)
+    k.set_grid(spec["grid"][::-1].copy())
+    k.solve()
+    assert k.solver_stats["chol_reuse"] > 0
+    k.close()
