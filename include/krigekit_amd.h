@@ -223,6 +223,26 @@ int kk_set_search_mode(int64_t handle, int mode, int reorder);
 int kk_solve_resident(int64_t handle);
 /* copy the results of the last kk_solve_resident to the host */
 int kk_fetch(int64_t handle);
+/* capture the device pass of a solved single-variable handle into a HIP
+ * graph: search, set grouping or bucketing, one solve per bucket, exact
+ * search of flagged queries (second stream), FP64 re-solve and fallback, with
+ * every count consumed on the device (no host synchronisation inside).
+ * Needs the wave-cooperative search, at most 64 rows per system and an
+ * exact-search heap that fits LDS; any setter invalidates the graph. */
+int kk_graph_capture(int64_t handle);
+/* replay the captured pass over the resident inputs; results stay in HBM
+ * (kk_fetch copies them back), timing total = the graph's device time */
+int kk_graph_launch(int64_t handle);
+/* per-bucket solve schedules (k_solve instance width and looping grid),
+ * timed on a solved single-variable handle's own work lists (reps launches
+ * per candidate); the fastest go into the library's table, used by later
+ * passes and captures, and into the text file at path when path is not
+ * NULL or empty.  The library reads such a file when it is loaded
+ * (KRIGEKIT_AMD_TUNING, else tuning.txt beside the library). */
+int kk_autotune(int64_t handle, const char *path, int reps);
+/* replace the library's schedule table by the file at path (NULL or empty:
+ * every bucket on its own width with a full grid); drops captured graphs */
+int kk_set_tuning(const char *path);
 /* out[13]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
  * nodes, Bunch-Kaufman fallback nodes, queries decided by the exact search kernel, fast
  * search launches (0: none), fast search kernel ms (all launches), exact

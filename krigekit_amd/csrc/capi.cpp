@@ -8,6 +8,7 @@
 // kd-tree (at set_search, as the reference does) and covariance tables on the
 // host, keeps inputs resident in HBM across solves, runs search -> bucket ->
 // per-node solve on the GPU and copies estimates and variances back.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
@@ -618,7 +620,22 @@ struct Krige {
   hipStream_t stream3 = nullptr;  // solve of the first search chunk, overlapped with the second
   int* h_counts = nullptr;        // pinned: bucket counts read while the GPU works on
   hipEvent_t ev[14] = {nullptr};
+  // graph-captured pass (kk_graph_capture / kk_graph_launch): valid while the
+  // inputs of the pass it was captured from are unchanged (solved stays true)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t gev[4] = {nullptr};
+  const void* graph_key[3] = {nullptr, nullptr, nullptr};  // buffers the graph reads
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
   ~Krige() {
+    drop_graph();
+    for (auto& e : gev)
+      if (e) (void)hipEventDestroy(e);
     if (h_counts) (void)hipHostFree(h_counts);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -836,29 +853,100 @@ void pass_dims(const Krige& K, int& mmax, int& k) {
   k = sector ? std::min(O.n, nsec * O.nmax * 4) : O.nmax;
 }
 
-bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
+// ---------------------------------------------------------------------------
+// Per-bucket solve schedules.  A bucket's systems (at most BUCKET_ROWS[bk]
+// rows) run on a k_solve instance of some width W >= that (G lanes per
+// system, 64 / G systems per wave) with a grid of at most `grid` workgroups
+// that loop over the work list (0: one workgroup per 64 / G systems).  The
+// table is indexed by coordinate count, precision and covariance mode; an
+// entry with width 0 keeps the bucket's own width and a full grid.
+// kk_autotune times candidate schedules on a solved handle's work lists and
+// writes the winners to a text file; the library reads such a file at load
+// (KRIGEKIT_AMD_TUNING, else tuning.txt beside the library).
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+  int width = 0;
+  int grid = 0;
+  float ms = 0.0f;
+};
+TuneEntry g_tune[5][2][2][kk::NBUCKET];
+constexpr int GRAPH_GRID = 8192;  // default persistent grid of a graph-captured solve
+
+TuneEntry& tune_entry(int dim, int prec, bool tb, int bk) {
+  return g_tune[dim < 1 || dim > 4 ? 0 : dim][prec ? 1 : 0][tb ? 1 : 0][bk];
+}
+// instance width for bucket bk: the tuned one when it holds the bucket's rows
+int tuned_width(int dim, int prec, bool tb, int bk) {
+  const int w = tune_entry(dim, prec, tb, bk).width;
+  return w >= kk::bucket_width(bk) && w <= 64 ? w : kk::bucket_width(bk);
+}
+int tuned_grid(int dim, int prec, bool tb, int bk) { return std::max(0, tune_entry(dim, prec, tb, bk).grid); }
+
+// lines "dim prec tb bucket width grid ms"; '#' starts a comment
+int read_tuning(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return 0;
+  int n = 0;
+  char line[256];
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int d, p, t, bk, w, g;
+    float ms = 0.0f;
+    if (std::sscanf(line, "%d %d %d %d %d %d %f", &d, &p, &t, &bk, &w, &g, &ms) >= 6 && d >= 1 && d <= 4 &&
+        (p == 0 || p == 1) && (t == 0 || t == 1) && bk >= 1 && bk < kk::NBUCKET && w >= kk::bucket_width(bk) &&
+        w <= 64 && kk::solve_width(w) == w && g >= 0) {
+      TuneEntry& e = tune_entry(d, p, t, bk);
+      e.width = w;
+      e.grid = g;
+      e.ms = ms;
+      ++n;
+    }
+  }
+  std::fclose(f);
+  return n;
+}
+
+int write_tuning(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return 1;
+  std::fprintf(f, "# krigekit_amd per-bucket solve schedules (kk_autotune): dim prec tb bucket width grid ms\n"
+                  "# bucket b holds systems of at most 8/12/16/24/32/48/64 rows (b = 1..7); width = k_solve\n"
+                  "# rows per system; grid = workgroups looping over the list (0 = one per 64/G systems)\n");
+  for (int d = 1; d <= 4; ++d)
+    for (int p = 0; p < 2; ++p)
+      for (int t = 0; t < 2; ++t)
+        for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+          const TuneEntry& e = tune_entry(d, p, t, bk);
+          if (e.width > 0) std::fprintf(f, "%d %d %d %d %d %d %.4f\n", d, p, t, bk, e.width, e.grid, e.ms);
+        }
+  std::fclose(f);
+  return 0;
+}
+
+// the tuning file read when the library is loaded
+struct TuneLoader {
+  TuneLoader() {
+    std::string path;
+    if (const char* env = std::getenv("KRIGEKIT_AMD_TUNING")) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr(reinterpret_cast<void*>(&read_tuning), &info) && info.dli_fname) {
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.txt";
+      }
+    }
+    if (!path.empty()) read_tuning(path);
+  }
+} g_tune_loader;
+
+// The device buffers and per-node solve arguments of one single-variable
+// pass (the handle's buffers, grown as needed, or the views in ext).
+bool setup_pass(Krige& K, const PassBufs* ext, int mmax, PassBufs& B, kk::SolveArgs& va, bool& tb) {
   const char* W = "t_kriging_base%solve_base";
-  if (!ext && !ensure_stream(K)) return false;
-  hipStream_t st = K.stream;
   Obs& O = K.obs[0];
   const int nd = K.ndim, nb = K.nblock;
-  int nmax = O.nmax;  // already normalised at set_search
-  const bool sector = O.sector && O.need_search;
-  // octants are over the spatial axes only: 2^3 for space-time
-  // (kriging_st.F90:587, obs_nmax in kriging_base.F90:3005-3011)
-  int mmax, kcand;
-  pass_dims(K, mmax, kcand);
-  K.mmax = mmax;
-  // systems above 64 rows go to the one-workgroup bordered solver, which
-  // holds up to 128 rows and extra columns (not for space-time)
-  const int big_rows_max = 128 - kk::mv_extra_cols(1, 0, K.unbias);
-  const int nsub_pass = K.soff.empty() ? 1 : K.soff[1] - K.soff[0];
-  if (mmax > 64 && (K.st || mmax > big_rows_max || nsub_pass > 64)) {
-    set_error(W, "neighbourhoods larger than " + std::to_string(K.st ? 64 : big_rows_max) +
-                     " data rows are not supported by the GPU solver (mmax=" + std::to_string(mmax) + ")");
-    return false;
-  }
-  PassBufs B;
   if (ext) {
     B = *ext;
   } else {
@@ -901,7 +989,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     B.xlist = K.d_xlist.as<int>();
   }
   // ---- per-node solve arguments ------------------------------------------------
-  kk::SolveArgs va{};
+  va = kk::SolveArgs{};
   va.model = K.d_model.as<Model>();
   va.ndim = nd;
   va.unbias = K.unbias;
@@ -937,7 +1025,36 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   }
   // the inline Level-B kernels need every model on one composite table: for
   // space-time both the spatial and the temporal model
-  const bool tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
+  tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
+  return true;
+}
+
+bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
+  const char* W = "t_kriging_base%solve_base";
+  if (!ext && !ensure_stream(K)) return false;
+  hipStream_t st = K.stream;
+  Obs& O = K.obs[0];
+  const int nd = K.ndim, nb = K.nblock;
+  int nmax = O.nmax;  // already normalised at set_search
+  const bool sector = O.sector && O.need_search;
+  // octants are over the spatial axes only: 2^3 for space-time
+  // (kriging_st.F90:587, obs_nmax in kriging_base.F90:3005-3011)
+  int mmax, kcand;
+  pass_dims(K, mmax, kcand);
+  K.mmax = mmax;
+  // systems above 64 rows go to the one-workgroup bordered solver, which
+  // holds up to 128 rows and extra columns (not for space-time)
+  const int big_rows_max = 128 - kk::mv_extra_cols(1, 0, K.unbias);
+  const int nsub_pass = K.soff.empty() ? 1 : K.soff[1] - K.soff[0];
+  if (mmax > 64 && (K.st || mmax > big_rows_max || nsub_pass > 64)) {
+    set_error(W, "neighbourhoods larger than " + std::to_string(K.st ? 64 : big_rows_max) +
+                     " data rows are not supported by the GPU solver (mmax=" + std::to_string(mmax) + ")");
+    return false;
+  }
+  PassBufs B;
+  kk::SolveArgs va;
+  bool tb = false;
+  if (!setup_pass(K, ext, mmax, B, va, tb)) return false;
   // A round groups (factor reuse) or buckets the blocks sel[0..nsel) (all
   // when sel is null) by neighbour count on stream s, copies its counters
   // (round area ro of the counter block) to pinned host memory between
@@ -1018,7 +1135,8 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
       if (reuse) v.items = rd.items + c[kk::SET_OFF + bk];
       else v.list = rd.lists + (size_t)bk * rd.cap;
       v.nlist = c[bk];
-      HIPCHK(kk::launch_solve(v, kk::bucket_width(bk), K.precision, tb, s), W);
+      v.grid_cap = tuned_grid(nd, K.precision, tb, bk);
+      HIPCHK(kk::launch_solve(v, tuned_width(nd, K.precision, tb, bk), K.precision, tb, s), W);
     }
     return true;
   };
@@ -1396,6 +1514,199 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     HIPCHK(hipEventElapsedTime(&ms, K.ev[5], K.ev[6]), W);
     K.timing[12] = ms;
   }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Graph-captured pass.  The same search -> (group | bucket) -> solve per
+// bucket -> exact search of flagged queries -> their solves -> FP64 re-solve
+// -> fallback sequence as run_pipeline, with every count consumed on the
+// device: the solve, empty-block, exact-search and fallback kernels read
+// their list lengths (and work-item offsets) from the counter block and loop
+// over them with a fixed grid, and counters are cleared by a fill kernel.
+// So the whole pass has no host synchronisation and is captured once into a
+// HIP graph, then replayed.  The exact search runs on the second stream,
+// forked after the collection of flagged queries and joined before their
+// solves.  Supported: single-variable passes with the wave-cooperative
+// search, at most 64 rows per system, and the exact search's heap in LDS.
+// Counter slots: [0] FP64 re-solves, [1] fallbacks, [8] flagged queries,
+// [16..] set members, rounds at ROUND_CTR[1] (decided blocks) and
+// ROUND_CTR[2] (flagged blocks).
+// ---------------------------------------------------------------------------
+bool capture_pass(Krige& K) {
+  const char* W = "kk_graph_capture";
+  Obs& O = K.obs[0];
+  const int nd = K.ndim, nb = K.nblock;
+  int mmax, kcand;
+  pass_dims(K, mmax, kcand);
+  const bool sector = O.sector && O.need_search;
+  if (K.nvar != 1 || !O.need_search || mmax > 64 || K.search_mode != 2 || (K.st && sector) ||
+      kk::fast_search_lds(nd, sector, kcand, O.nmax) > 64 * 1024 || !kk::exact_search_fits_lds(kcand) ||
+      K.store_weight || K.use_old_weight) {
+    set_error(W, "graph capture needs a single-variable pass with the wave-cooperative search, at most 64 "
+                 "rows per system and an exact-search heap in LDS");
+    return false;
+  }
+  if (!ensure_stream(K) || !morton_perm(K)) return false;
+  for (auto& e : K.gev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming), W);
+  PassBufs B;
+  kk::SolveArgs va;
+  bool tb = false;
+  if (!setup_pass(K, nullptr, mmax, B, va, tb)) return false;
+  const bool reuse = B.items != nullptr;
+  // fallback scratch for a fixed grid of threads
+  constexpr int FB_THREADS = 1024;
+  if (!K.d_fb.ensure(sizeof(double) * FB_THREADS * kk::fallback_words(mmax))) {
+    set_error(W, "device allocation failed (fallback scratch)");
+    return false;
+  }
+  hipStream_t st = K.stream, s2 = K.stream2;
+  const int* perm = K.d_perm.as<int>();
+  int* C = B.counts;
+  K.drop_graph();
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), W);
+  bool ok = false;
+  do {
+    if (kk::launch_fill(C, PASS_COUNTERS, 0, st) != hipSuccess) break;
+    // search, one launch over the Morton order
+    kk::SearchArgs sa{};
+    sa.ndim = nd;
+    sa.qcoord = (O.aniso || K.st) ? K.d_qcoord.as<double>() : K.d_bcoord.as<double>();
+    sa.ocoord = K.d_socoord.as<double>();
+    sa.nmax = O.nmax;
+    sa.cv = K.cross_validation;
+    sa.maxdist2 = O.maxdist2;
+    sa.mmax = mmax;
+    sa.nnear = B.nnear;
+    sa.inear = B.inear;
+    sa.tree.dim = nd;
+    sa.tree.n = O.n;
+    sa.tree.nnodes = O.tree.nnodes();
+    sa.tree.root = O.tree.root;
+    sa.tree.rdata = K.d_rdata.as<double>();
+    sa.tree.ind = K.d_ind.as<int>();
+    sa.tree.ni = K.d_ni.as<int>();
+    sa.tree.nd = K.d_nd.as<double>();
+    sa.sort = sector;
+    sa.k = kcand;
+    sa.qflag = B.sflag;
+    sa.q0 = 0;
+    sa.nq = nb;
+    sa.qlist = perm;
+    if (kk::launch_search_wave(sa, st) != hipSuccess) break;
+    // flagged queries in Morton order; the exact search on the second stream
+    if (kk::launch_collect(B.sflag, nb, 1, C + 8, B.xlist, st, perm) != hipSuccess) break;
+    if (hipEventRecord(K.gev[0], st) != hipSuccess || hipStreamWaitEvent(s2, K.gev[0], 0) != hipSuccess) break;
+    kk::SearchArgs sx = sa;
+    sx.qlist = B.xlist;
+    sx.dnq = C + 8;
+    sx.qflag = nullptr;
+    if (kk::launch_search_dn(sx, s2) != hipSuccess) break;
+    if (hipEventRecord(K.gev[1], s2) != hipSuccess) break;
+    // one solve per bucket, lengths and offsets read on the device
+    auto solve_buckets = [&](const int* ctr, const int* lists, const int2* items, const int* empties) -> bool {
+      kk::SolveArgs v = va;
+      v.nlist = 256 * 64;
+      v.grid_cap = 64;
+      v.list = items ? empties : lists;
+      v.dn = ctr;
+      v.doff = nullptr;
+      if (kk::launch_empty(v, st) != hipSuccess) return false;
+      if (items) {
+        v.setcnt = B.setcnt;
+        v.setbase = B.setbase;
+        v.members = B.members;
+        v.item_max = K.item_max;
+      }
+      for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+        v.dn = ctr + bk;
+        if (items) {
+          v.items = items;
+          v.doff = ctr + kk::SET_OFF + bk;
+          v.list = nullptr;
+        } else {
+          v.items = nullptr;
+          v.doff = nullptr;
+          v.list = lists + (size_t)bk * nb;
+        }
+        const int g = tuned_grid(nd, K.precision, tb, bk);
+        v.grid_cap = g > 0 ? g : GRAPH_GRID;
+        v.nlist = nb;
+        if (kk::launch_solve(v, tuned_width(nd, K.precision, tb, bk), K.precision, tb, st) != hipSuccess)
+          return false;
+      }
+      return true;
+    };
+    // round 0: the blocks the fast search decided (flagged ones are skipped)
+    int* r0 = C + ROUND_CTR[1];
+    if (reuse) {
+      kk::SetGroupArgs g{};
+      g.nnear = B.nnear;
+      g.inear = B.inear;
+      g.mmax = mmax;
+      g.rs = va.rangescale;
+      g.ln = va.localnugget;
+      g.sel = nullptr;
+      g.nsel = nb;
+      g.skip = B.sflag;
+      g.table = B.table;
+      g.hash = B.hash;
+      g.tsize = std::min(B.tcap, set_table_slots(nb));
+      g.item_max = K.item_max;
+      g.rep = B.rep;
+      g.rank = B.rank;
+      g.cnt = B.setcnt;
+      g.setbase = B.setbase;
+      g.setitem = B.setitem;
+      g.members = B.members;
+      g.ctr = r0;
+      g.total = C + 16;
+      g.items = B.items;
+      g.empties = B.empties;
+      if (kk::launch_set_group(g, st) != hipSuccess) break;
+      if (!solve_buckets(r0, nullptr, B.items, B.empties)) break;
+    } else {
+      if (kk::launch_bucket(B.nnear, nullptr, nb, r0, B.lists, nb, st) != hipSuccess) break;
+      if (!solve_buckets(r0, B.lists, nullptr, nullptr)) break;
+    }
+    // round 1: the exact search's blocks, bucketed on the device count
+    if (hipStreamWaitEvent(st, K.gev[1], 0) != hipSuccess) break;
+    int* r1 = C + ROUND_CTR[2];
+    int* l1 = B.lists + (size_t)(kk::NBUCKET + 1) * nb;
+    if (kk::launch_bucket(B.nnear, B.xlist, nb, r1, l1, nb, st, C + 8) != hipSuccess) break;
+    if (!solve_buckets(r1, l1, nullptr, nullptr)) break;
+    // FP64 re-solve of the FP32 systems whose pivot ratio fell below pivot_tol
+    kk::SolveArgs v = va;
+    v.nlist = nb;
+    if (K.precision == kk::PREC_FP32R) {
+      if (kk::launch_collect(B.flag, nb, 1, C + 0, B.lists, st) != hipSuccess) break;
+      v.list = B.lists;
+      v.dn = C + 0;
+      v.grid_cap = GRAPH_GRID;
+      if (kk::launch_solve(v, kk::solve_width(mmax), kk::PREC_FP64, tb, st) != hipSuccess) break;
+    }
+    // Bunch-Kaufman fallback
+    if (kk::launch_collect(B.flag, nb, 2, C + 1, B.lists + nb, st) != hipSuccess) break;
+    v.list = B.lists + nb;
+    v.dn = C + 1;
+    v.nlist = FB_THREADS;
+    v.grid_cap = 0;
+    if (kk::launch_fallback(v, K.d_fb.as<double>(), st) != hipSuccess) break;
+    ok = true;
+  } while (false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (!ok || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error(W, "stream capture of the pass failed");
+    return false;
+  }
+  K.graph = g;
+  HIPCHK(hipGraphInstantiate(&K.gexec, K.graph, nullptr, nullptr, 0), W);
+  K.graph_key[0] = B.nnear;
+  K.graph_key[1] = K.d_fb.p;
+  K.graph_key[2] = reuse ? (const void*)B.items : (const void*)B.lists;
   return true;
 }
 
@@ -3471,6 +3782,164 @@ int kk_solve_resident(int64_t h) {
   Krige* K = checked(h);
   if (!K) return ierr();
   do_solve(*K, false);
+  return ierr();
+}
+
+// Captures the device pass of a solved single-variable handle into a HIP
+// graph (no host synchronisation inside: every count is consumed on the
+// device); kk_graph_launch replays it over the same resident inputs.
+int kk_graph_capture(int64_t h) {
+  clear_error();
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->solved) {
+    set_error("kk_graph_capture", "solve the handle first: the graph replays a solved pass's inputs");
+    return ierr();
+  }
+  capture_pass(*K);
+  return ierr();
+}
+
+// Replays the captured pass: results stay in HBM (kk_fetch copies them back).
+// The solver statistics are those of the solve the graph was captured after
+// (same inputs, same outcome).
+int kk_graph_launch(int64_t h) {
+  clear_error();
+  const char* W = "kk_graph_launch";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  if (!K->gexec || !K->solved || K->graph_key[0] != K->d_nnear.p || K->graph_key[1] != K->d_fb.p ||
+      (K->graph_key[2] != K->d_items.p && K->graph_key[2] != K->d_lists.p)) {
+    K->drop_graph();
+    set_error(W, "no valid captured pass: inputs or options changed since kk_graph_capture");
+    return ierr();
+  }
+  HIPCHK(hipEventRecord(K->ev[0], K->stream), W);
+  HIPCHK(hipGraphLaunch(K->gexec, K->stream), W);
+  HIPCHK(hipEventRecord(K->ev[4], K->stream), W);
+  HIPCHK(hipStreamSynchronize(K->stream), W);
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, K->ev[0], K->ev[4]), W);
+  for (double& x : K->timing) x = 0.0;
+  K->timing[4] = ms;
+  K->timing[7] = -1;
+  return ierr();
+}
+
+// Times candidate solve schedules (instance width and looping grid) for every
+// non-empty bucket on the solved handle's own work lists, keeps the fastest
+// in the library's table (used by later passes and captures) and, with a
+// path, writes the table there.  reps timed launches per candidate.
+int kk_autotune(int64_t h, const char* path, int reps) {
+  clear_error();
+  const char* W = "kk_autotune";
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  int mmax, kcand;
+  pass_dims(*K, mmax, kcand);
+  if (!K->solved || K->nvar != 1 || mmax > 64) {
+    set_error(W, "autotuning needs a solved single-variable handle with at most 64 rows per system");
+    return ierr();
+  }
+  reps = std::max(1, std::min(reps, 50));
+  PassBufs B;
+  kk::SolveArgs va;
+  bool tb = false;
+  if (!setup_pass(*K, nullptr, mmax, B, va, tb)) return ierr();
+  hipStream_t st = K->stream;
+  const int nb = K->nblock, nd = K->ndim, prec = K->precision;
+  const bool reuse = B.items != nullptr;
+  int* ctr = B.counts + ROUND_CTR[0];
+  // the whole grid's work lists, as one round
+  if (reuse) {
+    kk::SetGroupArgs g{};
+    g.nnear = B.nnear;
+    g.inear = B.inear;
+    g.mmax = mmax;
+    g.rs = va.rangescale;
+    g.ln = va.localnugget;
+    g.nsel = nb;
+    g.table = B.table;
+    g.hash = B.hash;
+    g.tsize = std::min(B.tcap, set_table_slots(nb));
+    g.item_max = K->item_max;
+    g.rep = B.rep;
+    g.rank = B.rank;
+    g.cnt = B.setcnt;
+    g.setbase = B.setbase;
+    g.setitem = B.setitem;
+    g.members = B.members;
+    g.ctr = ctr;
+    g.total = B.counts + 16;
+    g.items = B.items;
+    g.empties = B.empties;
+    HIPCHK(kk::launch_fill(B.counts + 16, 1, 0, st), W);
+    HIPCHK(kk::launch_set_group(g, st), W);
+  } else {
+    HIPCHK(kk::launch_fill(ctr, 32, 0, st), W);
+    HIPCHK(kk::launch_bucket(B.nnear, nullptr, nb, ctr, B.lists, nb, st), W);
+  }
+  int hc[32];
+  HIPCHK(hipMemcpyAsync(hc, ctr, sizeof hc, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  static const int grids[] = {0, 2048, 4096, 8192, 16384};
+  for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+    if (hc[bk] <= 0) continue;
+    kk::SolveArgs v = va;
+    v.nlist = hc[bk];
+    if (reuse) {
+      v.items = B.items + hc[kk::SET_OFF + bk];
+      v.setcnt = B.setcnt;
+      v.setbase = B.setbase;
+      v.members = B.members;
+      v.item_max = K->item_max;
+    } else {
+      v.list = B.lists + (size_t)bk * nb;
+    }
+    TuneEntry best;
+    best.ms = FLT_MAX;
+    int nw = 0;
+    for (int w : kk::BUCKET_ROWS) {
+      if (w < kk::bucket_width(bk) || nw == 3) continue;
+      ++nw;
+      for (int g : grids) {
+        v.grid_cap = g;
+        HIPCHK(kk::launch_solve(v, w, prec, tb, st), W);  // warm-up
+        std::vector<float> t(reps);
+        for (int r = 0; r < reps; ++r) {
+          HIPCHK(hipEventRecord(K->ev[0], st), W);
+          HIPCHK(kk::launch_solve(v, w, prec, tb, st), W);
+          HIPCHK(hipEventRecord(K->ev[1], st), W);
+          HIPCHK(hipEventSynchronize(K->ev[1]), W);
+          HIPCHK(hipEventElapsedTime(&t[r], K->ev[0], K->ev[1]), W);
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[reps / 2];
+        if (med < best.ms) {
+          best.ms = med;
+          best.width = w;
+          best.grid = g;
+        }
+      }
+    }
+    tune_entry(nd, prec, tb, bk) = best;
+  }
+  K->drop_graph();
+  if (path && path[0] && write_tuning(path)) set_error(W, std::string("cannot write ") + path);
+  return ierr();
+}
+
+// Replaces the library's schedule table by the file at path (NULL or empty:
+// every bucket back to its own width and a full grid).
+int kk_set_tuning(const char* path) {
+  clear_error();
+  for (auto& a : g_tune)
+    for (auto& b : a)
+      for (auto& c : b)
+        for (auto& e : c) e = TuneEntry{};
+  if (path && path[0] && read_tuning(path) == 0) set_error("kk_set_tuning", std::string("no schedules read from ") + path);
+  for (Krige* K : g_reg)
+    if (K) K->drop_graph();
   return ierr();
 }
 

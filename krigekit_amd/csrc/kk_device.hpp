@@ -25,6 +25,10 @@ struct Rot3 {
 };
 
 hipError_t launch_search(const SearchArgs& a, hipStream_t st);
+// exact search of *a.dnq listed queries (device-resident count, heap in LDS):
+// a fixed grid that loops over them; a.nq must bound the count
+hipError_t launch_search_dn(const SearchArgs& a, hipStream_t st);
+bool exact_search_fits_lds(int k);
 // two-pass LDS search; a.qflag marks queries the exact kernel must decide
 hipError_t launch_search_fast(const SearchArgs& a, hipStream_t st);
 // wave-cooperative traversal, same results and flags as launch_search_fast
@@ -48,7 +52,9 @@ __host__ __device__ constexpr int bucket_of(int n) {  // 1 <= n <= 64
 // blocks 0..nblock-1, or the nblock blocks listed in sel; blocks with a
 // negative neighbour count are left out
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
-                         hipStream_t st);
+                         hipStream_t st, const int* dnsel = nullptr);
+// p[0..n) = v by a kernel (no memset node in captured graphs)
+hipError_t launch_fill(int* p, int n, int v, hipStream_t st);
 int bucket_width(int b);  // solve width (matrix rows) of bucket b (0 for the empty bucket)
 int solve_width(int n);   // smallest solve width holding n rows (BUCKET_ROWS)
 

@@ -95,6 +95,7 @@ struct SearchArgs {
   int* nnear;                 // [nblock]
   int* inear;                 // [nblock * mmax], 0-based obs ids, ascending
   const int* qlist;           // exact kernel: query ids to process (nullptr: q0..q0+nq)
+  const int* dnq;             // exact kernel: device-resident query count (graph-captured passes)
   int* qflag;                 // fast kernel: 1 where the exact kernel must decide
 };
 
@@ -189,6 +190,12 @@ struct SolveArgs {
   int mmax;
   const int* list;            // block ids of this bucket (when items is null)
   int nlist;                  // blocks, or work items
+  // device-resident count and offset (graph-captured passes): when dn is set
+  // the list holds *dn entries and starts *doff entries in (doff may be
+  // null), and the grid loops over them (the launch is sized without them)
+  const int* dn;
+  const int* doff;
+  int grid_cap;               // at most this many workgroups, looping over the list (0: one pass)
   // factor reuse: work items (representative block, first member slot), each
   // with min(item_max, setbase[rep] + setcnt[rep] - slot) members in members
   const int2* items;

@@ -223,8 +223,14 @@ __device__ inline int octant(const double* o, const double* q) {
 template <int DIM, bool LDSHEAP>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ double lds_heap[];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nq) return;
+  // with a device-resident count (graph-captured passes) the grid loops over
+  // the queries: one per workgroup while there are no more of them than
+  // workgroups, else one per lane; otherwise the grid covers them once
+  const int nq = a.dnq ? *a.dnq : a.nq;
+  const bool one = a.dnq && nq <= (int)gridDim.x;
+  if (one && threadIdx.x != 0) return;
+  const int step = one ? (int)gridDim.x : (int)(gridDim.x * blockDim.x);
+  for (int t = one ? (int)blockIdx.x : (int)(blockIdx.x * blockDim.x + threadIdx.x); t < nq; t += step) {
   const int qid = a.qlist ? a.qlist[t] : a.q0 + t;
   double q[DIM];
 #pragma unroll
@@ -335,6 +341,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     isort(out, cnt);
   }
   a.nnear[qid] = cnt;
+  }
 }
 
 __device__ inline float f_next(float f) { return __int_as_float(__float_as_int(f) + 1); }
@@ -1417,6 +1424,20 @@ hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
   else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, false>), grd, blk, 0, st, a);
   else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, false>), grd, blk, 0, st, a);
   else hipLaunchKernelGGL((k_search<4, false>), grd, blk, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search_dn(const SearchArgs& a, hipStream_t st) {
+  if (!a.qlist || !a.dnq || !exact_search_fits_lds(a.k)) return hipErrorInvalidValue;
+  // 16 lanes per workgroup; up to 4,096 queries one per workgroup (lane 0),
+  // more: one per lane over the whole grid
+  constexpr int lanes = 16;
+  const size_t lds = (size_t)a.k * 12 * lanes;
+  dim3 blk(lanes), grd(4096);
+  if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, true>), grd, blk, lds, st, a);
+  else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, true>), grd, blk, lds, st, a);
+  else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);
+  else hipLaunchKernelGGL((k_search<4, true>), grd, blk, lds, st, a);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 // kriging_base.F90:3299-3366).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "kk_bk.hpp"
@@ -127,25 +128,31 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   __shared__ double xs[PREC == PREC_FP32R ? S : 1][MB][G];
 
   const int lane = threadIdx.x, s = lane / G, r = lane % G;
+  // a launch with a device-resident count loops over its work (a persistent
+  // grid); otherwise the grid covers the list and the loop runs once
+  const int nlist = a.dn ? *a.dn : a.nlist;
+  const int2* items0 = a.items ? a.items + (a.doff ? *a.doff : 0) : nullptr;
+  const int* list0 = a.list + (a.doff && !a.items ? *a.doff : 0);
+  for (int base = blockIdx.x * S; base < nlist; base += gridDim.x * S) {
   // lane r holds row r; the slots (R/2, R/2..R-1) are never read, so lanes
   // past R store there instead of branching around their stores
   const bool row = R == G || r < R;
   const int rr = row ? r : 0;
   auto slot = [&](int d) -> double* { return row ? &sp[s][d][r] : &sp[s][R / 2][r - R / 2]; };
   auto wrap = [](int c) { return R == G ? (c & (G - 1)) : (c >= R ? c - R : c); };
-  const int sys = blockIdx.x * S + s;
-  const bool act = sys < a.nlist;
+  const int sys = base + s;
+  const bool act = sys < nlist;
   // work item: representative block b, members m0 .. m0 + mc - 1 of the
   // member array (or b alone)
   int b = 0, m0 = 0, mc = 0;
   if (act) {
-    if (a.items) {
-      const int2 it = a.items[sys];
+    if (items0) {
+      const int2 it = items0[sys];
       b = it.x;
       m0 = it.y;
       mc = min(min(a.item_max, MB), a.setbase[b] + a.setcnt[b] - m0);
     } else {
-      b = a.list[sys];
+      b = list0[sys];
       mc = 1;
     }
   }
@@ -512,17 +519,22 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       if (act && r == 0 && c < mc) write_out(mb[c], uu, uy, yy, vy, vu);
     }
   }
+  // the next item rewrites the LDS exchange areas
+  __syncthreads();
+  }
 }
 
 #if KK_SOLVE_DIM == 0
 // blocks without neighbours: NaN outputs (kriging_base.F90:3421-3425)
 __global__ void k_empty(SolveArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nlist) return;
-  const int b = a.list[t];
-  a.est[b] = __builtin_nan("");
-  a.var[b] = __builtin_nan("");
-  a.flag[b] = 0;
+  const int nl = a.dn ? *a.dn : a.nlist;
+  const int* list = a.list + (a.doff ? *a.doff : 0);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += gridDim.x * blockDim.x) {
+    const int b = list[t];
+    a.est[b] = __builtin_nan("");
+    a.var[b] = __builtin_nan("");
+    a.flag[b] = 0;
+  }
 }
 
 // Bucket counts are aggregated in LDS per 1024-thread workgroup, so each
@@ -530,7 +542,10 @@ __global__ void k_empty(SolveArgs a) {
 // Blocks 0..nblock-1 (or the blocks listed in `sel`) into per-size lists.
 // A negative count marks a block whose neighbours are still being searched.
 __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* sel, int nblock, int* counts,
-                                                 int* lists, int cap) {
+                                                 int* lists, int cap, const int* dnsel) {
+  // a device-resident count of the listed blocks (graph-captured passes;
+  // the grid covers nblock)
+  if (dnsel) nblock = *dnsel;
   // order-preserving within the workgroup: ballot ranks within a wave, an
   // exclusive scan over the 16 waves, one global atomic per bucket, so a
   // workgroup's blocks keep their order (and their run of the list is the
@@ -596,15 +611,13 @@ __device__ inline double fb_cov(const SolveArgs& a, const Model& M, const double
   return a.k_ps * cs * ct + cs + ct;
 }
 
+// block b, with fallback_words(mmax) doubles of scratch at `scratch`
 template <int DIM>
-__global__ void k_fallback(SolveArgs a, double* scratch) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nlist) return;
-  const int b = a.list[t];
+__device__ void fallback_one(const SolveArgs& a, int b, double* scratch) {
   const int n = a.nnear[b];
   const int p = a.unbias ? 1 : 0, m = n + p;
   const int ld = a.mmax + 1;
-  double* K = scratch + (size_t)t * fallback_words(a.mmax);
+  double* K = scratch;
   double* A = K + ld * ld;
   double* rhs = A + ld * ld;
   double* xs = rhs + ld;
@@ -670,6 +683,17 @@ __global__ void k_fallback(SolveArgs a, double* scratch) {
   a.var[b] = var;
 }
 
+// one thread per listed block; with a device-resident count the grid loops,
+// each thread reusing its own scratch slot
+template <int DIM>
+__global__ void k_fallback(SolveArgs a, double* scratch) {
+  const int nl = a.dn ? *a.dn : a.nlist;
+  const int* list = a.list + (a.doff ? *a.doff : 0);
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int t = t0; t < nl; t += gridDim.x * blockDim.x)
+    fallback_one<DIM>(a, list[t], scratch + (size_t)t0 * fallback_words(a.mmax));
+}
+
 // blocks with more than limit neighbours into list (atomic count)
 __global__ void k_collect_above(const int* nnear, int nblock, int limit, int* count, int* list) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -702,10 +726,20 @@ int bucket_width(int bk) { return bk >= 1 && bk < NBUCKET ? BUCKET_ROWS[bk - 1] 
 int solve_width(int n) { return BUCKET_ROWS[bucket_of(n) - 1]; }
 
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
-                         hipStream_t st) {
+                         hipStream_t st, const int* dnsel) {
   if (nblock <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, sel, nblock, counts,
-                     lists, cap);
+                     lists, cap, dnsel);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill(int* p, int n, int v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 
@@ -736,7 +770,9 @@ hipError_t launch_solve_d4(const SolveArgs& a, int width, int prec, bool tableB,
 template <int G, int PREC, int R = G>
 static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
   constexpr int S = 64 / G;
-  dim3 grd((a.nlist + S - 1) / S), blk(64);
+  int ng = (a.nlist + S - 1) / S;
+  if (a.grid_cap > 0 && ng > a.grid_cap) ng = a.grid_cap;
+  dim3 grd(ng), blk(64);
   if (tb) hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, true, R>), grd, blk, 0, st, a);
   else hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, false, R>), grd, blk, 0, st, a);
 }
@@ -783,7 +819,9 @@ hipError_t launch_solve(const SolveArgs& a, int width, int prec, bool tableB, hi
 
 hipError_t launch_empty(const SolveArgs& a, hipStream_t st) {
   if (a.nlist <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_empty, dim3((a.nlist + 255) / 256), dim3(256), 0, st, a);
+  int ng = (a.nlist + 255) / 256;
+  if (a.grid_cap > 0 && ng > a.grid_cap) ng = a.grid_cap;
+  hipLaunchKernelGGL(k_empty, dim3(ng), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -79,6 +79,10 @@ def _load_lib():
     sig("kk_set_options", [_i64, _c_int, _c_double, _c_int])
     sig("kk_set_search_mode", [_i64, _c_int, _c_int])
     sig("kk_set_factor_reuse", [_i64, _c_int, _c_int])
+    sig("kk_graph_capture", [_i64])
+    sig("kk_graph_launch", [_i64])
+    sig("kk_autotune", [_i64, ctypes.c_char_p, _c_int])
+    sig("kk_set_tuning", [ctypes.c_char_p])
     sig("kk_solve_resident_noalloc", [_i64, ctypes.c_void_p, ctypes.c_void_p])
     sig("kk_solve_resident", [_i64])
     sig("kk_fetch", [_i64])
@@ -330,6 +334,19 @@ class Kriging:
             return mse.value, g, zhat, dz
         return mse.value, g
 
+    def graph_capture(self):
+        """capture the solved handle's device pass into a HIP graph"""
+        _check(_load_lib().kk_graph_capture(self._handle), "kk_graph_capture")
+
+    def graph_launch(self):
+        """replay the captured pass; results stay on the device (``fetch``)"""
+        _check(_load_lib().kk_graph_launch(self._handle), "kk_graph_launch")
+
+    def autotune(self, path: Optional[str] = None, reps: int = 5):
+        """time candidate per-bucket solve schedules on this solved handle's
+        work lists, keep the fastest (and write them to ``path``)"""
+        _check(_load_lib().kk_autotune(self._handle, (path or "").encode("utf-8"), int(reps)), "kk_autotune")
+
     def set_factor_reuse(self, mode: int = -1, item_max: int = 4):
         """Factor reuse over identical neighbour sets: -1 auto (on unless
         cross-validation), 0 off, 1 on; item_max (1..4) member blocks per
@@ -520,6 +537,11 @@ def copy_probe(nbytes, reps=10):
     ms = ctypes.c_double(0.0)
     _check(_load_lib().kk_copy_probe(float(nbytes), int(reps), ctypes.byref(ms)), "kk_copy_probe")
     return ms.value
+
+
+def set_tuning(path: Optional[str] = None):
+    """load a schedule table written by ``Kriging.autotune`` (None: defaults)"""
+    _check(_load_lib().kk_set_tuning((path or "").encode("utf-8")), "kk_set_tuning")
 
 
 def device_synchronize():
