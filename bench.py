@@ -43,10 +43,13 @@ def solve_flops(nnear, c=30, p=1, q=1):
     return float(np.sum(f))
 
 
-def time_path(spec, precision, steps, warmup, reorder=1):
+def time_path(spec, precision, steps, warmup, reorder=1, graph=False):
     """reorder 1: the default pipeline (two search chunks, the first chunk's
-    solve overlapping the second search); 2: one search launch, so the solve
-    phase is timed on its own"""
+    solve overlapping the second search, unless factor reuse runs the search
+    in one launch); 2: one search launch, so the solve phase is timed on its
+    own.  graph: after the host-orchestrated passes, the pass captured into a
+    HIP graph is replayed `steps` times, and the wall time returned is the
+    replays'."""
     from krigekit_amd.kriging import device_synchronize, from_spec
     k = from_spec(spec, precision=precision)
     k.set_search_mode(True, reorder)
@@ -66,9 +69,30 @@ def time_path(spec, precision, steps, warmup, reorder=1):
     k.fetch()
     est, var = k.get_results(copy=True)
     nnear, _ = k.neighbours()
-    k.close()
     agg = {key: float(np.mean([p[key] for p in phases])) for key in phases[0]}
     agg["end_to_end_ms"] = e2e * 1e3
+    agg["pipeline_ms_per_step"] = wall * 1e3
+    if graph:
+        # the same pass captured into a HIP graph (every count consumed on the
+        # device) and replayed: the production path, timed the same way
+        k.graph_capture()
+        for _ in range(warmup):
+            k.graph_launch()
+        device_synchronize()
+        gms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            k.graph_launch()
+            gms.append(k.timing["total_ms"])
+        device_synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        agg["graph_ms_per_step"] = wall * 1e3
+        agg["graph_device_ms"] = float(np.mean(gms))
+        k.fetch()
+        eg, vg = k.get_results(copy=True)
+        agg["graph_bitwise_equal"] = bool(np.array_equal(eg, est, equal_nan=True) and
+                                          np.array_equal(vg, var, equal_nan=True))
+    k.close()
     return wall, agg, est, var, nnear
 
 
@@ -300,6 +324,18 @@ def small_run(spec_fn, precision):
     return res
 
 
+def load_schedules():
+    """the per-bucket solve schedules the library loaded (tuning.txt beside
+    it): width and looping grid per (coordinates, precision, bucket)"""
+    path = os.path.join(ROOT, "krigekit_amd", "tuning.txt")
+    if not os.path.exists(path):
+        return None
+    rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    return [dict(dim=int(r[0]), precision=("fp64", "fp32r")[int(r[1])], table=bool(int(r[2])),
+                 bucket_rows=(8, 12, 16, 24, 32, 48, 64)[int(r[3]) - 1], width=int(r[4]), grid=int(r[5]),
+                 tuned_ms=float(r[6])) for r in rows if int(r[0]) == 3]
+
+
 def load_issue():
     """VALU and SALU wave-instructions per search launch from the committed
     SQ counter summary (profiles/search_issue.json), or None"""
@@ -356,8 +392,8 @@ def main():
     kw.close()
     spec = synth.ok3d()
     nb = len(spec["grid"])
-    w64, t64, e64, v64, nn = time_path(spec, "fp64", args.steps, args.warmup)
-    w32, t32, e32, v32, _ = time_path(spec, "fp32r", args.steps, args.warmup)
+    w64, t64, e64, v64, nn = time_path(spec, "fp64", args.steps, args.warmup, graph=True)
+    w32, t32, e32, v32, _ = time_path(spec, "fp32r", args.steps, args.warmup, graph=True)
     # the solves timed apart from the search (no overlapped chunk)
     if args.no_solve_alone:
         t64s, t32s = t64, t32
@@ -397,6 +433,10 @@ def main():
         "solve_kernel_speedup_fp32r_vs_fp64": t64s["solve_ms"] / t32s["solve_ms"],
         "err_fp32r_vs_fp64": {"estimate": rel_stats(e32, e64, nn), "variance": rel_stats(v32, v64, nn),
                               "floor": 1e-4},
+        "step": "one graph replay of the captured pass (search, set grouping, solve per bucket, exact search of "
+                "flagged queries, re-solves, fallback) over all nodes; the host-orchestrated pass is "
+                "pipeline_ms_per_step in the phase objects",
+        "tuned_schedules": load_schedules(),
         "phase_ms_fp32r": {k: round(v, 4) for k, v in t32.items()},
         "phase_ms_fp64": {k: round(v, 4) for k, v in t64.items()},
         "roofline": {

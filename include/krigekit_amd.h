@@ -233,7 +233,7 @@ int kk_graph_capture(int64_t handle);
 /* replay the captured pass over the resident inputs; results stay in HBM
  * (kk_fetch copies them back), timing total = the graph's device time */
 int kk_graph_launch(int64_t handle);
-/* per-bucket solve schedules (k_solve instance width and looping grid),
+/* per-bucket solve schedules (k_solve instance width: lanes and rows per system),
  * timed on a solved single-variable handle's own work lists (reps launches
  * per candidate); the fastest go into the library's table, used by later
  * passes and captures, and into the text file at path when path is not

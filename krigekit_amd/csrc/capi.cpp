@@ -626,6 +626,7 @@ struct Krige {
   hipGraphExec_t gexec = nullptr;
   hipEvent_t gev[4] = {nullptr};
   const void* graph_key[3] = {nullptr, nullptr, nullptr};  // buffers the graph reads
+  int* graph_overflow = nullptr;  // device flag: a count exceeded its captured bound
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -855,11 +856,14 @@ void pass_dims(const Krige& K, int& mmax, int& k) {
 
 // ---------------------------------------------------------------------------
 // Per-bucket solve schedules.  A bucket's systems (at most BUCKET_ROWS[bk]
-// rows) run on a k_solve instance of some width W >= that (G lanes per
-// system, 64 / G systems per wave) with a grid of at most `grid` workgroups
-// that loop over the work list (0: one workgroup per 64 / G systems).  The
-// table is indexed by coordinate count, precision and covariance mode; an
-// entry with width 0 keeps the bucket's own width and a full grid.
+// rows) run on a k_solve instance of some width W >= that: G lanes (a
+// quarter, half or whole wave) per system, 64 / G systems per wave, rows a
+// quarter shorter for the 12/24/48-row instances.  The table is indexed by
+// coordinate count, precision and covariance mode; an entry with width 0
+// keeps the bucket's own width.  (The file format keeps a grid column for
+// looping grids; looping k_solve over its list cost the instances a wave of
+// occupancy in registers, so every solve runs one workgroup per 64 / G
+// systems and the column is written as 0.)
 // kk_autotune times candidate schedules on a solved handle's work lists and
 // writes the winners to a text file; the library reads such a file at load
 // (KRIGEKIT_AMD_TUNING, else tuning.txt beside the library).
@@ -870,7 +874,6 @@ struct TuneEntry {
   float ms = 0.0f;
 };
 TuneEntry g_tune[5][2][2][kk::NBUCKET];
-constexpr int GRAPH_GRID = 8192;  // default persistent grid of a graph-captured solve
 
 TuneEntry& tune_entry(int dim, int prec, bool tb, int bk) {
   return g_tune[dim < 1 || dim > 4 ? 0 : dim][prec ? 1 : 0][tb ? 1 : 0][bk];
@@ -880,7 +883,6 @@ int tuned_width(int dim, int prec, bool tb, int bk) {
   const int w = tune_entry(dim, prec, tb, bk).width;
   return w >= kk::bucket_width(bk) && w <= 64 ? w : kk::bucket_width(bk);
 }
-int tuned_grid(int dim, int prec, bool tb, int bk) { return std::max(0, tune_entry(dim, prec, tb, bk).grid); }
 
 // lines "dim prec tb bucket width grid ms"; '#' starts a comment
 int read_tuning(const std::string& path) {
@@ -1106,7 +1108,8 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
       HIPCHK(hipMemcpyAsync(K.h_counts + rd.ro, g.ctr, sizeof(int) * kk::SET_NCTR, hipMemcpyDeviceToHost, s), W);
     } else {
       HIPCHK(hipMemsetAsync(B.counts + rd.ro, 0, sizeof(int) * 16, s), W);
-      HIPCHK(kk::launch_bucket(B.nnear, rd.sel, rd.nsel, B.counts + rd.ro, rd.lists, rd.cap, s), W);
+      HIPCHK(kk::launch_bucket(B.nnear, rd.sel, rd.nsel, B.counts + rd.ro, rd.lists, rd.cap, s, nullptr, rd.skip),
+             W);
       HIPCHK(hipMemcpyAsync(K.h_counts + rd.ro, B.counts + rd.ro, sizeof(int) * 16, hipMemcpyDeviceToHost, s), W);
     }
     HIPCHK(hipEventRecord(e1, s), W);
@@ -1135,7 +1138,6 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
       if (reuse) v.items = rd.items + c[kk::SET_OFF + bk];
       else v.list = rd.lists + (size_t)bk * rd.cap;
       v.nlist = c[bk];
-      v.grid_cap = tuned_grid(nd, K.precision, tb, bk);
       HIPCHK(kk::launch_solve(v, tuned_width(nd, K.precision, tb, bk), K.precision, tb, s), W);
     }
     return true;
@@ -1564,11 +1566,17 @@ bool capture_pass(Krige& K) {
   hipStream_t st = K.stream, s2 = K.stream2;
   const int* perm = K.d_perm.as<int>();
   int* C = B.counts;
+  int* overflow = C + PASS_COUNTERS - 1;
   K.drop_graph();
-  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), W);
-  bool ok = false;
-  do {
-    if (kk::launch_fill(C, PASS_COUNTERS, 0, st) != hipSuccess) break;
+  // The pass, enqueued on st (and s2).  bnd: launch bounds per count (the
+  // list lengths of an earlier run of the same pass), or null for bounds of
+  // nb everywhere.  Kernels read the actual counts on the device and set the
+  // overflow slot if one exceeds its bound.
+  struct Bounds {
+    int r0[kk::NBUCKET + 1], r1[kk::NBUCKET + 1], nexact, nres;
+  };
+  auto enqueue = [&](const Bounds* bnd) -> bool {
+    if (kk::launch_fill(C, PASS_COUNTERS, 0, st) != hipSuccess) return false;
     // search, one launch over the Morton order
     kk::SearchArgs sa{};
     sa.ndim = nd;
@@ -1594,18 +1602,21 @@ bool capture_pass(Krige& K) {
     sa.q0 = 0;
     sa.nq = nb;
     sa.qlist = perm;
-    if (kk::launch_search_wave(sa, st) != hipSuccess) break;
+    if (kk::launch_search_wave(sa, st) != hipSuccess) return false;
     // flagged queries in Morton order; the exact search on the second stream
-    if (kk::launch_collect(B.sflag, nb, 1, C + 8, B.xlist, st, perm) != hipSuccess) break;
-    if (hipEventRecord(K.gev[0], st) != hipSuccess || hipStreamWaitEvent(s2, K.gev[0], 0) != hipSuccess) break;
+    if (kk::launch_collect(B.sflag, nb, 1, C + 8, B.xlist, st, perm) != hipSuccess) return false;
+    if (hipEventRecord(K.gev[0], st) != hipSuccess || hipStreamWaitEvent(s2, K.gev[0], 0) != hipSuccess)
+      return false;
     kk::SearchArgs sx = sa;
     sx.qlist = B.xlist;
     sx.dnq = C + 8;
     sx.qflag = nullptr;
-    if (kk::launch_search_dn(sx, s2) != hipSuccess) break;
-    if (hipEventRecord(K.gev[1], s2) != hipSuccess) break;
+    sx.nq = bnd ? std::max(bnd->nexact, 1) : nb;
+    if (kk::launch_search_dn(sx, s2) != hipSuccess) return false;
+    if (hipEventRecord(K.gev[1], s2) != hipSuccess) return false;
     // one solve per bucket, lengths and offsets read on the device
-    auto solve_buckets = [&](const int* ctr, const int* lists, const int2* items, const int* empties) -> bool {
+    auto solve_buckets = [&](const int* ctr, const int* bound, const int* lists, const int2* items,
+                             const int* empties) -> bool {
       kk::SolveArgs v = va;
       v.nlist = 256 * 64;
       v.grid_cap = 64;
@@ -1613,6 +1624,7 @@ bool capture_pass(Krige& K) {
       v.dn = ctr;
       v.doff = nullptr;
       if (kk::launch_empty(v, st) != hipSuccess) return false;
+      v.grid_cap = 0;
       if (items) {
         v.setcnt = B.setcnt;
         v.setbase = B.setbase;
@@ -1620,6 +1632,9 @@ bool capture_pass(Krige& K) {
         v.item_max = K.item_max;
       }
       for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+        // bounded by the captured length (at least 1, so that a count that
+        // grew is still seen); workgroups past the device count exit
+        v.nlist = bound ? std::max(bound[bk], 1) : nb;
         v.dn = ctr + bk;
         if (items) {
           v.items = items;
@@ -1630,9 +1645,6 @@ bool capture_pass(Krige& K) {
           v.doff = nullptr;
           v.list = lists + (size_t)bk * nb;
         }
-        const int g = tuned_grid(nd, K.precision, tb, bk);
-        v.grid_cap = g > 0 ? g : GRAPH_GRID;
-        v.nlist = nb;
         if (kk::launch_solve(v, tuned_width(nd, K.precision, tb, bk), K.precision, tb, st) != hipSuccess)
           return false;
       }
@@ -1664,37 +1676,70 @@ bool capture_pass(Krige& K) {
       g.total = C + 16;
       g.items = B.items;
       g.empties = B.empties;
-      if (kk::launch_set_group(g, st) != hipSuccess) break;
-      if (!solve_buckets(r0, nullptr, B.items, B.empties)) break;
+      if (kk::launch_set_group(g, st) != hipSuccess) return false;
+      if (!solve_buckets(r0, bnd ? bnd->r0 : nullptr, nullptr, B.items, B.empties)) return false;
     } else {
-      if (kk::launch_bucket(B.nnear, nullptr, nb, r0, B.lists, nb, st) != hipSuccess) break;
-      if (!solve_buckets(r0, B.lists, nullptr, nullptr)) break;
+      if (kk::launch_bucket(B.nnear, nullptr, nb, r0, B.lists, nb, st, nullptr, B.sflag) != hipSuccess) return false;
+      if (!solve_buckets(r0, bnd ? bnd->r0 : nullptr, B.lists, nullptr, nullptr)) return false;
     }
     // round 1: the exact search's blocks, bucketed on the device count
-    if (hipStreamWaitEvent(st, K.gev[1], 0) != hipSuccess) break;
+    if (hipStreamWaitEvent(st, K.gev[1], 0) != hipSuccess) return false;
     int* r1 = C + ROUND_CTR[2];
     int* l1 = B.lists + (size_t)(kk::NBUCKET + 1) * nb;
-    if (kk::launch_bucket(B.nnear, B.xlist, nb, r1, l1, nb, st, C + 8) != hipSuccess) break;
-    if (!solve_buckets(r1, l1, nullptr, nullptr)) break;
+    const int nx = bnd ? std::max(bnd->nexact, 1) : nb;
+    if (kk::launch_bucket(B.nnear, B.xlist, nx, r1, l1, nb, st, C + 8) != hipSuccess) return false;
+    if (!solve_buckets(r1, bnd ? bnd->r1 : nullptr, l1, nullptr, nullptr)) return false;
     // FP64 re-solve of the FP32 systems whose pivot ratio fell below pivot_tol
     kk::SolveArgs v = va;
-    v.nlist = nb;
     if (K.precision == kk::PREC_FP32R) {
-      if (kk::launch_collect(B.flag, nb, 1, C + 0, B.lists, st) != hipSuccess) break;
+      if (kk::launch_collect(B.flag, nb, 1, C + 0, B.lists, st) != hipSuccess) return false;
       v.list = B.lists;
       v.dn = C + 0;
-      v.grid_cap = GRAPH_GRID;
-      if (kk::launch_solve(v, kk::solve_width(mmax), kk::PREC_FP64, tb, st) != hipSuccess) break;
+      v.nlist = bnd ? std::max(bnd->nres, 1) : nb;
+      if (kk::launch_solve(v, kk::solve_width(mmax), kk::PREC_FP64, tb, st) != hipSuccess) return false;
     }
-    // Bunch-Kaufman fallback
-    if (kk::launch_collect(B.flag, nb, 2, C + 1, B.lists + nb, st) != hipSuccess) break;
+    // Bunch-Kaufman fallback (a fixed grid looping over the count)
+    if (kk::launch_collect(B.flag, nb, 2, C + 1, B.lists + nb, st) != hipSuccess) return false;
     v.list = B.lists + nb;
     v.dn = C + 1;
     v.nlist = FB_THREADS;
     v.grid_cap = 0;
-    if (kk::launch_fallback(v, K.d_fb.as<double>(), st) != hipSuccess) break;
-    ok = true;
-  } while (false);
+    if (kk::launch_fallback(v, K.d_fb.as<double>(), st) != hipSuccess) return false;
+    if (bnd) {
+      // every count whose launches were sized by a bound
+      kk::BoundCheck c{};
+      auto add = [&](int idx, int bound) {
+        c.idx[c.n] = idx;
+        c.bound[c.n] = bound;
+        ++c.n;
+      };
+      for (int bk = 1; bk < kk::NBUCKET; ++bk) {
+        add(ROUND_CTR[1] + bk, std::max(bnd->r0[bk], 1));
+        add(ROUND_CTR[2] + bk, std::max(bnd->r1[bk], 1));
+      }
+      add(8, std::max(bnd->nexact, 1));
+      if (K.precision == kk::PREC_FP32R) add(0, std::max(bnd->nres, 1));
+      if (kk::launch_check_bounds(C, c, overflow, st) != hipSuccess) return false;
+    }
+    return true;
+  };
+  // one run with bounds of nb, for the pass's list lengths
+  if (!enqueue(nullptr)) {
+    set_error(W, "HIP error in the pass to be captured");
+    return false;
+  }
+  std::vector<int> hc(PASS_COUNTERS);
+  HIPCHK(hipMemcpyAsync(hc.data(), C, sizeof(int) * PASS_COUNTERS, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  Bounds bd{};
+  for (int bk = 0; bk <= kk::NBUCKET; ++bk) {
+    bd.r0[bk] = hc[ROUND_CTR[1] + bk];
+    bd.r1[bk] = hc[ROUND_CTR[2] + bk];
+  }
+  bd.nexact = hc[8];
+  bd.nres = hc[0];
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), W);
+  const bool ok = enqueue(&bd);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(st, &g);
   if (!ok || e != hipSuccess || !g) {
@@ -1704,6 +1749,7 @@ bool capture_pass(Krige& K) {
   }
   K.graph = g;
   HIPCHK(hipGraphInstantiate(&K.gexec, K.graph, nullptr, nullptr, 0), W);
+  K.graph_overflow = overflow;
   K.graph_key[0] = B.nnear;
   K.graph_key[1] = K.d_fb.p;
   K.graph_key[2] = reuse ? (const void*)B.items : (const void*)B.lists;
@@ -3817,7 +3863,15 @@ int kk_graph_launch(int64_t h) {
   HIPCHK(hipEventRecord(K->ev[0], K->stream), W);
   HIPCHK(hipGraphLaunch(K->gexec, K->stream), W);
   HIPCHK(hipEventRecord(K->ev[4], K->stream), W);
+  int over = 0;
+  HIPCHK(hipMemcpyAsync(&over, K->graph_overflow, sizeof(int), hipMemcpyDeviceToHost, K->stream), W);
   HIPCHK(hipStreamSynchronize(K->stream), W);
+  if (over) {
+    K->drop_graph();
+    K->solved = false;
+    set_error(W, "a list of the replayed pass outgrew the captured launch bounds (inputs changed); capture again");
+    return ierr();
+  }
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, K->ev[0], K->ev[4]), W);
   for (double& x : K->timing) x = 0.0;
@@ -3882,7 +3936,7 @@ int kk_autotune(int64_t h, const char* path, int reps) {
   int hc[32];
   HIPCHK(hipMemcpyAsync(hc, ctr, sizeof hc, hipMemcpyDeviceToHost, st), W);
   HIPCHK(hipStreamSynchronize(st), W);
-  static const int grids[] = {0, 2048, 4096, 8192, 16384};
+  static const int grids[] = {0};
   for (int bk = 1; bk < kk::NBUCKET; ++bk) {
     if (hc[bk] <= 0) continue;
     kk::SolveArgs v = va;
@@ -3903,7 +3957,7 @@ int kk_autotune(int64_t h, const char* path, int reps) {
       if (w < kk::bucket_width(bk) || nw == 3) continue;
       ++nw;
       for (int g : grids) {
-        v.grid_cap = g;
+        (void)g;
         HIPCHK(kk::launch_solve(v, w, prec, tb, st), W);  // warm-up
         std::vector<float> t(reps);
         for (int r = 0; r < reps; ++r) {
@@ -3915,6 +3969,9 @@ int kk_autotune(int64_t h, const char* path, int reps) {
         }
         std::sort(t.begin(), t.end());
         const float med = t[reps / 2];
+        if (std::getenv("KRIGEKIT_AMD_TUNE_VERBOSE"))
+          std::fprintf(stderr, "kk_autotune dim %d prec %d bucket %d items %d width %d grid %d: %.4f ms\n", nd, prec,
+                       bk, hc[bk], w, g, med);
         if (med < best.ms) {
           best.ms = med;
           best.width = w;

@@ -51,8 +51,12 @@ __host__ __device__ constexpr int bucket_of(int n) {  // 1 <= n <= 64
 // bucket blocks by neighbour count: counts[NB] and lists (device), returns via counts
 // blocks 0..nblock-1, or the nblock blocks listed in sel; blocks with a
 // negative neighbour count are left out
+// (dnsel: device-resident count of listed blocks; skip: per block, nonzero =
+// leave out)
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
-                         hipStream_t st, const int* dnsel = nullptr);
+                         hipStream_t st, const int* dnsel = nullptr, const int* skip = nullptr);
+// *overflow = 1 when counts[c.idx[i]] > c.bound[i] for some i < c.n (<= 32)
+hipError_t launch_check_bounds(const int* counts, const BoundCheck& c, int* overflow, hipStream_t st);
 // p[0..n) = v by a kernel (no memset node in captured graphs)
 hipError_t launch_fill(int* p, int n, int v, hipStream_t st);
 int bucket_width(int b);  // solve width (matrix rows) of bucket b (0 for the empty bucket)

@@ -166,6 +166,13 @@ struct MVArgs {
 constexpr int MAX_ITEM = 4;
 __host__ __device__ constexpr int item_cap(int rows) { return rows <= 16 ? 4 : rows <= 32 ? 2 : 1; }
 
+// counters and the bounds a graph-captured pass's launches were sized for
+struct BoundCheck {
+  int n;
+  int idx[32];
+  int bound[32];
+};
+
 struct SolveArgs {
   const Model* model;         // device copy of the covariance model (space-time: spatial part)
   const Model* tmodel;        // space-time only: temporal structures (1-D)
@@ -192,10 +199,10 @@ struct SolveArgs {
   int nlist;                  // blocks, or work items
   // device-resident count and offset (graph-captured passes): when dn is set
   // the list holds *dn entries and starts *doff entries in (doff may be
-  // null), and the grid loops over them (the launch is sized without them)
+  // null); nlist then bounds the count (the launch is sized for it)
   const int* dn;
   const int* doff;
-  int grid_cap;               // at most this many workgroups, looping over the list (0: one pass)
+  int grid_cap;               // k_empty / k_fallback: at most this many workgroups, looping (0: one pass)
   // factor reuse: work items (representative block, first member slot), each
   // with min(item_max, setbase[rep] + setcnt[rep] - slot) members in members
   const int2* items;

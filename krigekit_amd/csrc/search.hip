@@ -220,17 +220,17 @@ __device__ inline int octant(const double* o, const double* q) {
 // could not decide (a.qlist).
 // LDSHEAP: the heap lives in LDS (lane-interleaved, 32-lane workgroups),
 // used when only the queries the fast kernel flagged are left.
+constexpr int EXACT_ONE = 4096;
 template <int DIM, bool LDSHEAP>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ double lds_heap[];
-  // with a device-resident count (graph-captured passes) the grid loops over
-  // the queries: one per workgroup while there are no more of them than
-  // workgroups, else one per lane; otherwise the grid covers them once
+  // with a device-resident count (graph-captured passes, a grid sized for
+  // the largest count): up to EXACT_ONE queries one per workgroup (lane 0),
+  // more one per lane; workgroups and lanes past the count exit
   const int nq = a.dnq ? *a.dnq : a.nq;
-  const bool one = a.dnq && nq <= (int)gridDim.x;
-  if (one && threadIdx.x != 0) return;
-  const int step = one ? (int)gridDim.x : (int)(gridDim.x * blockDim.x);
-  for (int t = one ? (int)blockIdx.x : (int)(blockIdx.x * blockDim.x + threadIdx.x); t < nq; t += step) {
+  const bool one = a.dnq && nq <= EXACT_ONE;
+  const int t = one ? (int)blockIdx.x : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if ((one && threadIdx.x != 0) || t >= nq) return;
   const int qid = a.qlist ? a.qlist[t] : a.q0 + t;
   double q[DIM];
 #pragma unroll
@@ -341,7 +341,6 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     isort(out, cnt);
   }
   a.nnear[qid] = cnt;
-  }
 }
 
 __device__ inline float f_next(float f) { return __int_as_float(__float_as_int(f) + 1); }
@@ -1429,11 +1428,12 @@ hipError_t launch_search(const SearchArgs& a, hipStream_t st) {
 
 hipError_t launch_search_dn(const SearchArgs& a, hipStream_t st) {
   if (!a.qlist || !a.dnq || !exact_search_fits_lds(a.k)) return hipErrorInvalidValue;
-  // 16 lanes per workgroup; up to 4,096 queries one per workgroup (lane 0),
-  // more: one per lane over the whole grid
+  // 16 lanes per workgroup; up to EXACT_ONE queries one per workgroup
+  // (lane 0), more one per lane: a grid for a.nq queries either way
+  if (a.nq <= 0) return hipSuccess;
   constexpr int lanes = 16;
   const size_t lds = (size_t)a.k * 12 * lanes;
-  dim3 blk(lanes), grd(4096);
+  dim3 blk(lanes), grd(std::max((a.nq + lanes - 1) / lanes, std::min(a.nq, EXACT_ONE)));
   if (a.ndim == 1) hipLaunchKernelGGL((k_search<1, true>), grd, blk, lds, st, a);
   else if (a.ndim == 2) hipLaunchKernelGGL((k_search<2, true>), grd, blk, lds, st, a);
   else if (a.ndim == 3) hipLaunchKernelGGL((k_search<3, true>), grd, blk, lds, st, a);

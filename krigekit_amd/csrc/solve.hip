@@ -128,12 +128,16 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   __shared__ double xs[PREC == PREC_FP32R ? S : 1][MB][G];
 
   const int lane = threadIdx.x, s = lane / G, r = lane % G;
-  // a launch with a device-resident count loops over its work (a persistent
-  // grid); otherwise the grid covers the list and the loop runs once
+  // with a device-resident count (graph-captured passes) the grid is sized
+  // for the largest possible list and workgroups past the count exit (a
+  // loop over the list here would cost the instances a wave of occupancy in
+  // registers)
   const int nlist = a.dn ? *a.dn : a.nlist;
   const int2* items0 = a.items ? a.items + (a.doff ? *a.doff : 0) : nullptr;
   const int* list0 = a.list + (a.doff && !a.items ? *a.doff : 0);
-  for (int base = blockIdx.x * S; base < nlist; base += gridDim.x * S) {
+  const int base = blockIdx.x * S;
+  if (base >= nlist) return;
+  {
   // lane r holds row r; the slots (R/2, R/2..R-1) are never read, so lanes
   // past R store there instead of branching around their stores
   const bool row = R == G || r < R;
@@ -519,8 +523,6 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
       if (act && r == 0 && c < mc) write_out(mb[c], uu, uy, yy, vy, vu);
     }
   }
-  // the next item rewrites the LDS exchange areas
-  __syncthreads();
   }
 }
 
@@ -542,7 +544,7 @@ __global__ void k_empty(SolveArgs a) {
 // Blocks 0..nblock-1 (or the blocks listed in `sel`) into per-size lists.
 // A negative count marks a block whose neighbours are still being searched.
 __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* sel, int nblock, int* counts,
-                                                 int* lists, int cap, const int* dnsel) {
+                                                 int* lists, int cap, const int* dnsel, const int* skip) {
   // a device-resident count of the listed blocks (graph-captured passes;
   // the grid covers nblock)
   if (dnsel) nblock = *dnsel;
@@ -555,7 +557,9 @@ __global__ __launch_bounds__(1024) void k_bucket(const int* nnear, const int* se
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int b = t < nblock ? (sel ? sel[t] : t) : -1;
-  const int n = b >= 0 ? nnear[b] : -1;
+  // blocks flagged for the exact search are left out whatever their count
+  // reads: that search may be writing it on another stream
+  const int n = (b >= 0 && !(skip && skip[b])) ? nnear[b] : -1;
   const int bk = n < 0 ? -1 : n == 0 ? 0 : n > 64 ? NBUCKET : bucket_of(n);
   int local = 0;
 #pragma unroll
@@ -726,10 +730,22 @@ int bucket_width(int bk) { return bk >= 1 && bk < NBUCKET ? BUCKET_ROWS[bk - 1] 
 int solve_width(int n) { return BUCKET_ROWS[bucket_of(n) - 1]; }
 
 hipError_t launch_bucket(const int* nnear, const int* sel, int nblock, int* counts, int* lists, int cap,
-                         hipStream_t st, const int* dnsel) {
+                         hipStream_t st, const int* dnsel, const int* skip) {
   if (nblock <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bucket, dim3((nblock + 1023) / 1024), dim3(1024), 0, st, nnear, sel, nblock, counts,
-                     lists, cap, dnsel);
+                     lists, cap, dnsel, skip);
+  return hipGetLastError();
+}
+
+// graph-captured passes: any counter above the bound its launches were
+// sized for sets *overflow
+__global__ void k_check_bounds(const int* counts, BoundCheck c, int* overflow) {
+  const int i = threadIdx.x;
+  if (i < c.n && counts[c.idx[i]] > c.bound[i]) atomicOr(overflow, 1);
+}
+
+hipError_t launch_check_bounds(const int* counts, const BoundCheck& c, int* overflow, hipStream_t st) {
+  hipLaunchKernelGGL(k_check_bounds, dim3(1), dim3(64), 0, st, counts, c, overflow);
   return hipGetLastError();
 }
 
@@ -770,9 +786,7 @@ hipError_t launch_solve_d4(const SolveArgs& a, int width, int prec, bool tableB,
 template <int G, int PREC, int R = G>
 static void launch_g(const SolveArgs& a, bool tb, hipStream_t st) {
   constexpr int S = 64 / G;
-  int ng = (a.nlist + S - 1) / S;
-  if (a.grid_cap > 0 && ng > a.grid_cap) ng = a.grid_cap;
-  dim3 grd(ng), blk(64);
+  dim3 grd((a.nlist + S - 1) / S), blk(64);
   if (tb) hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, true, R>), grd, blk, 0, st, a);
   else hipLaunchKernelGGL((k_solve<G, KK_SOLVE_DIM, PREC, false, R>), grd, blk, 0, st, a);
 }
