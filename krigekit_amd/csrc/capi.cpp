@@ -502,13 +502,14 @@ struct Obs {
   double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   std::vector<double> rcoord;  // search coordinates: rotated (aniso search), space-time t * time_at
   double time_at = 1.0;        // space-time search scaling of the time axis
+  std::vector<double> drift;   // (ndrift, n): external drift values (zeros until set_obs_drift)
   HostTree tree;
 };
 
 // device copies of one variable's observations, tree and neighbour lists
 // (multi-variable solves)
 struct DevVar {
-  DBuf ocoord, socoord, ovalue, ovar, rdata, ind, ni, nd, qcoord, nnear, inear, sflag, xlist, hdis, hidx;
+  DBuf ocoord, socoord, ovalue, ovar, rdata, ind, ni, nd, qcoord, nnear, inear, sflag, xlist, hdis, hidx, odrift;
   int mmax = 0;
 };
 
@@ -615,6 +616,10 @@ struct Krige {
   DevVar dv[kk::MAXVAR];
   DBuf d_models, d_mvtab, d_mvest, d_mvvar;
   std::vector<double> est_mv, var_mv;  // [nb * nvar], [nb * nvar * nvar]
+  // external drift at the blocks, (ndrift, nvar, nb): target variable q's
+  // right-hand side (zeros until set_grid_drift)
+  std::vector<double> bdrift;
+  DBuf d_bdrift;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // exact search of flagged queries, overlapped with the solve
   hipStream_t stream3 = nullptr;  // solve of the first search chunk, overlapped with the second
@@ -645,6 +650,14 @@ struct Krige {
     if (stream3) (void)hipStreamDestroy(stream3);
   }
 };
+
+// The results of the last pass live in the multi-variable buffers: cokriging,
+// and single-variable kriging with external drift or weight correction
+// (both solved by the bordered solver); stored-weight replays keep the
+// single-variable ones.
+bool mv_results(const Krige& K) {
+  return K.nvar > 1 || (!K.use_old_weight && (K.ndrift > 0 || K.weight_correction));
+}
 
 std::vector<Krige*> g_reg;
 
@@ -1542,7 +1555,7 @@ bool capture_pass(Krige& K) {
   int mmax, kcand;
   pass_dims(K, mmax, kcand);
   const bool sector = O.sector && O.need_search;
-  if (K.nvar != 1 || !O.need_search || mmax > 64 || K.search_mode != 2 || (K.st && sector) ||
+  if (mv_results(K) || !O.need_search || mmax > 64 || K.search_mode != 2 || (K.st && sector) ||
       kk::fast_search_lds(nd, sector, kcand, O.nmax) > 64 * 1024 || !kk::exact_search_fits_lds(kcand) ||
       K.store_weight || K.use_old_weight) {
     set_error(W, "graph capture needs a single-variable pass with the wave-cooperative search, at most 64 "
@@ -1802,8 +1815,8 @@ bool validate_for_solve(Krige& K) {
     set_error(W, "Call initialize() before solve.");
     return false;
   }
-  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift != 0) {
-    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift=0");
+  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift < 0 || K.ndrift > kk::MAXDRIFT) {
+    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift <= 4");
     return false;
   }
   if (K.st) {
@@ -1824,8 +1837,13 @@ bool validate_for_solve(Krige& K) {
       return false;
     }
   }
-  if (K.nvar > 1 && K.cross_validation) {
-    set_error(W, "cross validation with nvar > 1 is not supported by this backend");
+  if (mv_results(K) && K.cross_validation) {
+    set_error(W, "cross validation with nvar > 1, external drift or weight correction is not supported by this "
+                 "backend");
+    return false;
+  }
+  if (K.st && (K.ndrift > 0 || K.weight_correction)) {
+    set_error(W, "space-time kriging with external drift or weight correction is not supported by this backend");
     return false;
   }
   for (int v = 1; v < K.nvar; ++v) {
@@ -1853,8 +1871,8 @@ bool validate_for_solve(Krige& K) {
           return false;
         }
     }
-  if (K.varying_vgm || K.weight_correction) {
-    set_error(W, "varying_vgm and weight_correction are not supported by this backend");
+  if (K.varying_vgm) {
+    set_error(W, "varying_vgm is not supported by this backend");
     return false;
   }
   if (K.obs[0].n == 0) {
@@ -2174,6 +2192,7 @@ bool do_solve_mv(Krige& K, bool fetch) {
   if (!K.dev_inputs_ok) {
     if (!upload(K.d_bcoord, K.bcoord, W) || !upload(K.d_rs, K.rangescale, W) || !upload(K.d_ln, K.localnugget, W))
       return false;
+    if (K.ndrift > 0 && !upload(K.d_bdrift, K.bdrift, W)) return false;
     if (!K.soff.empty() && (!upload(K.d_scoord, K.scoord, W) || !upload(K.d_sweight, K.sweight, W))) return false;
     for (int v = 0; v < nvar; ++v) {
       Obs& O = K.obs[v];
@@ -2181,6 +2200,7 @@ bool do_solve_mv(Krige& K, bool fetch) {
       if (!upload(D.ocoord, O.coord, W) || !upload(D.ovalue, O.value, W) || !upload(D.ovar, O.var, W) ||
           !upload(D.socoord, O.aniso ? O.rcoord : O.coord, W))
         return false;
+      if (K.ndrift > 0 && !upload(D.odrift, O.drift, W)) return false;
       if (O.need_search && (!upload(D.rdata, O.tree.rdata, W) || !upload(D.ind, O.tree.ind, W) ||
                             !upload(D.ni, O.tree.ni, W) || !upload(D.nd, O.tree.nd, W)))
         return false;
@@ -2210,7 +2230,7 @@ bool do_solve_mv(Krige& K, bool fetch) {
     rows += K.dv[v].mmax;
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
-  const int cols = rows + kk::mv_extra_cols(nvar, K.std_ck, K.unbias);
+  const int cols = rows + kk::mv_extra_cols(nvar, K.std_ck, K.unbias, K.ndrift);
   if (cols > 128) {
     set_error(W, "cokriging systems larger than 128 rows + right-hand sides are not supported by the GPU solver (" +
                      std::to_string(cols) + ")");
@@ -2250,7 +2270,14 @@ bool do_solve_mv(Krige& K, bool fetch) {
   a.est = K.d_mvest.as<double>();
   a.var = K.d_mvvar.as<double>();
   a.flag = K.d_flag.as<int>();
-  if (K.store_weight) {
+  a.ndrift = K.ndrift;
+  if (K.ndrift > 0) {
+    for (int v = 0; v < nvar; ++v) a.odrift[v] = K.dv[v].odrift.as<double>();
+    a.bdrift = K.d_bdrift.as<double>();
+  }
+  // weight correction needs the weights: the weights instance
+  a.wcorr = K.weight_correction && nvar == 1;
+  if (K.store_weight || a.wcorr) {
     // the weight store: the solver's weights instance, the fallback writes
     // the weights of the blocks it solves
     K.mmax_w = 0;
@@ -2346,19 +2373,20 @@ int store_nmax(const Krige& K) {
 bool store_from_device(Krige& K) {
   const char* W = "t_kriging_base%save_block_weights";
   const int nb = K.nblock, nvar = K.nvar;
+  const bool mv = mv_results(K);
   const int nm = store_nmax(K);
   const int mmw = K.mmax_w;
   std::vector<int> flag(nb);
   std::vector<double> w((size_t)mmw * nvar * nvar * nb), vout((size_t)nb * nvar * nvar);
   HIPCHK(hipMemcpy(flag.data(), K.d_flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
   HIPCHK(hipMemcpy(w.data(), K.d_wout.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost), W);
-  HIPCHK(hipMemcpy(vout.data(), nvar == 1 ? K.d_var.p : K.d_mvvar.p, sizeof(double) * vout.size(),
+  HIPCHK(hipMemcpy(vout.data(), !mv ? K.d_var.p : K.d_mvvar.p, sizeof(double) * vout.size(),
                    hipMemcpyDeviceToHost), W);
   std::vector<std::vector<int>> nn(nvar, std::vector<int>(nb));
   std::vector<int> mm(nvar);
   for (int v = 0; v < nvar; ++v) {
-    mm[v] = nvar == 1 ? K.mmax : K.dv[v].mmax;
-    HIPCHK(hipMemcpy(nn[v].data(), nvar == 1 ? K.d_nnear.p : K.dv[v].nnear.p, sizeof(int) * nb,
+    mm[v] = !mv ? K.mmax : K.dv[v].mmax;
+    HIPCHK(hipMemcpy(nn[v].data(), !mv ? K.d_nnear.p : K.dv[v].nnear.p, sizeof(int) * nb,
                      hipMemcpyDeviceToHost), W);
   }
   std::vector<int> tot(nb, 0);
@@ -2370,7 +2398,7 @@ bool store_from_device(Krige& K) {
   std::vector<int> in;
   for (int g = 0; g < nvar; ++g) {
     in.resize((size_t)nb * mm[g]);
-    HIPCHK(hipMemcpy(in.data(), nvar == 1 ? K.d_inear.p : K.dv[g].inear.p, sizeof(int) * in.size(),
+    HIPCHK(hipMemcpy(in.data(), !mv ? K.d_inear.p : K.dv[g].inear.p, sizeof(int) * in.size(),
                      hipMemcpyDeviceToHost), W);
     for (int b = 0; b < nb; ++b) {
       const bool failed = flag[b] == 3;
@@ -2576,8 +2604,8 @@ bool do_replay(Krige& K, bool fetch) {
     set_error(W, "Call initialize() before solve.");
     return false;
   }
-  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift != 0) {
-    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift=0");
+  if (K.nvar < 1 || K.nvar > kk::MAXVAR || K.nsim != 0 || K.ndrift < 0 || K.ndrift > kk::MAXDRIFT) {
+    set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift <= 4");
     return false;
   }
   for (int v = 0; v < K.nvar; ++v)
@@ -2677,7 +2705,7 @@ bool do_solve(Krige& K, bool fetch) {
     return true;
   }
   if (!validate_for_solve(K)) return false;
-  if (K.nvar > 1) {
+  if (mv_results(K)) {
     if (!do_solve_mv(K, fetch)) return false;
   } else {
     if (!stage_inputs(K)) return false;
@@ -2948,15 +2976,48 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
   O.nmax = nmax;
   O.maxdist2 = maxdist * maxdist;
   O.sk_mean = sk_mean;
+  if (K->ndrift > 0) O.drift.assign((size_t)K->ndrift * nobs, 0.0);
   K->dev_inputs_ok = false;
   K->solved = false;
   return 0;
 }
 
-int krige_set_obs_drift(int64_t h, int, int, int, const double*) {
+// external drift values at variable ivar's observations, (ndrift, nobs)
+// column-major (set_obs_drift_base, kriging_base.F90:756-791)
+int krige_set_obs_drift(int64_t h, int ivar, int ndrift_c, int nobs, const double* drift) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_set_obs_drift", "external drift is not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_obs_drift";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_obs_drift.");
+    return ierr();
+  }
+  if (ivar < 1 || ivar > K->nvar) {
+    set_error(W, "ivar out of range: " + std::to_string(ivar) + " not in 1.." + std::to_string(K->nvar));
+    return ierr();
+  }
+  Obs& O = K->obs[ivar - 1];
+  if (O.n == 0) {
+    set_error(W, "Observation needs to be set before adding drift.");
+    return ierr();
+  }
+  if (K->ndrift == 0) {
+    set_error(W, "Observation drift is specified but ndrift==0");
+    return ierr();
+  }
+  if (ndrift_c != K->ndrift) {
+    set_error(W, "size(drift, 1) /= ndrift");
+    return ierr();
+  }
+  if (nobs != O.n) {
+    set_error(W, "size(drift, 2) /= nobs");
+    return ierr();
+  }
+  O.drift.assign(drift, drift + (size_t)ndrift_c * nobs);
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  return 0;
 }
 
 int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
@@ -2972,7 +3033,7 @@ int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
   // a filled weight store and a finished solve switch the next solve to
   // the stored weights (kriging_base.F90:669-678)
   if (K->ws.alloc && K->ws.stored() && K->solved) {
-    const std::vector<double>& e = K->nvar == 1 ? K->est : K->est_mv;
+    const std::vector<double>& e = !mv_results(*K) ? K->est : K->est_mv;
     bool all_nan = !e.empty();
     for (double x : e) all_nan = all_nan && std::isnan(x);
     if (!all_nan) {
@@ -3061,6 +3122,7 @@ static int set_grid_common(Krige* K, int nblock, const double* rangescale, const
   K->dev_inputs_ok = false;
   K->perm_ok = false;
   K->solved = false;
+  K->bdrift.assign((size_t)K->ndrift * K->nvar * nblock, 0.0);
   return 0;
 }
 
@@ -3195,13 +3257,55 @@ int krige_set_grid_cv(int64_t h) {
   // kriging_base.F90:1167: one extra neighbour so self can be excluded
   Obs& O = K->obs[0];
   if (O.nmax > 0) O.nmax = (int)((unsigned)O.nmax + 1u);
+  // the blocks are the observations: their drift is variable 1's, for every
+  // target (finish_grid_setup_common with copy_cv_drift, kriging_base.F90:1358-1366)
+  for (int b = 0; b < O.n; ++b)
+    for (int q = 0; q < K->nvar; ++q)
+      for (int k = 0; k < K->ndrift; ++k)
+        K->bdrift[k + (size_t)K->ndrift * (q + (size_t)K->nvar * b)] = O.drift[k + (size_t)K->ndrift * b];
   return 0;
 }
 
-int krige_set_grid_drift(int64_t h, int, int, int, const double*) {
+// external drift values at the blocks, (ndrift, nblocks) column-major, for
+// target variable ivar, or for every target when ivar < 0
+// (set_grid_drift_base, kriging_base.F90:1173-1220)
+int krige_set_grid_drift(int64_t h, int ivar, int ndrift_c, int nblocks, const double* drift) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_set_grid_drift", "external drift is not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  const char* W = "t_kriging_base%set_grid_drift";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_grid_drift.");
+    return ierr();
+  }
+  if (K->nblock == 0) {
+    set_error(W, "Grid needs to be set before adding drift.");
+    return ierr();
+  }
+  if (K->ndrift == 0) {
+    set_error(W, "grid/block drift is specified but ndrift==0");
+    return ierr();
+  }
+  if (ndrift_c != K->ndrift) {
+    set_error(W, "size(drift, 1) /= ndrift");
+    return ierr();
+  }
+  if (nblocks != K->nblock) {
+    set_error(W, "size(drift, 2) /= block%n; one drift value per block, not per grid node");
+    return ierr();
+  }
+  if (ivar == 0 || ivar > K->nvar) {
+    set_error(W, "ivar must be 1..nvar or < 0 (broadcast)");
+    return ierr();
+  }
+  const int nd = K->ndrift, nv = K->nvar;
+  for (int b = 0; b < nblocks; ++b)
+    for (int q = 0; q < nv; ++q)
+      if (ivar < 0 || q == ivar - 1)
+        for (int k = 0; k < nd; ++k) K->bdrift[k + (size_t)nd * (q + (size_t)nv * b)] = drift[k + (size_t)nd * b];
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  return 0;
 }
 
 int krige_set_sim(int64_t h, int, const void*, int, int, const void*) {
@@ -3330,7 +3434,7 @@ int krige_get_estimate(int64_t h, int nsim_c, int nblocks, double* out) {
   if (!check_solved(K, "krige_get_estimate")) return ierr();
   for (int s = 0; s < nsim_c; ++s)
     for (int b = 0; b < nblocks; ++b)
-      out[b + (size_t)nblocks * s] = K->nvar > 1 ? K->est_mv[b] : K->est[b];
+      out[b + (size_t)nblocks * s] = mv_results(*K) ? K->est_mv[b] : K->est[b];
   return 0;
 }
 
@@ -3343,7 +3447,7 @@ int krige_get_estimate_all(int64_t h, int nblocks, int nvar_c, int nsim_c, doubl
     for (int v = 0; v < nvar_c; ++v)
       for (int b = 0; b < nblocks; ++b)
         out[b + (size_t)nblocks * (v + (size_t)nvar_c * s)] =
-            K->nvar > 1 ? K->est_mv[b + (size_t)nblocks * v] : K->est[b];
+            mv_results(*K) ? K->est_mv[b + (size_t)nblocks * v] : K->est[b];
   return 0;
 }
 
@@ -3352,7 +3456,7 @@ int krige_get_variance(int64_t h, int nblocks, double* out) {
   Krige* K = checked(h);
   if (!K) return ierr();
   if (!check_solved(K, "krige_get_variance")) return ierr();
-  for (int b = 0; b < nblocks; ++b) out[b] = K->nvar > 1 ? K->var_mv[b] : K->var[b];
+  for (int b = 0; b < nblocks; ++b) out[b] = mv_results(*K) ? K->var_mv[b] : K->var[b];
   return 0;
 }
 
@@ -3365,7 +3469,7 @@ int krige_get_variance_all(int64_t h, int nblocks, int nvar_c, double* out) {
     for (int j = 0; j < nvar_c; ++j)
       for (int b = 0; b < nblocks; ++b)
         out[b + (size_t)nblocks * (i + (size_t)nvar_c * j)] =
-            K->nvar > 1 ? K->var_mv[b + (size_t)nblocks * (i + (size_t)K->nvar * j)] : K->var[b];
+            mv_results(*K) ? K->var_mv[b + (size_t)nblocks * (i + (size_t)K->nvar * j)] : K->var[b];
   return 0;
 }
 
@@ -3891,7 +3995,7 @@ int kk_autotune(int64_t h, const char* path, int reps) {
   if (!K) return ierr();
   int mmax, kcand;
   pass_dims(*K, mmax, kcand);
-  if (!K->solved || K->nvar != 1 || mmax > 64) {
+  if (!K->solved || mv_results(*K) || mmax > 64) {
     set_error(W, "autotuning needs a solved single-variable handle with at most 64 rows per system");
     return ierr();
   }
@@ -4005,7 +4109,7 @@ int kk_fetch(int64_t h) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (!(K->nvar > 1 ? fetch_results_mv(*K) : fetch_results(*K))) K->solved = false;
+  if (!(mv_results(*K) ? fetch_results_mv(*K) : fetch_results(*K))) K->solved = false;
   return ierr();
 }
 
@@ -4133,7 +4237,7 @@ int kk_solve_resident_noalloc(int64_t h, double* est, double* var) {
   const char* W = "kk_solve_resident_noalloc";
   Krige* K = checked(h);
   if (!K) return ierr();
-  if (K->nvar != 1 || !K->solved || !K->dev_inputs_ok || !K->model_ok || !K->stream ||
+  if (mv_results(*K) || !K->solved || !K->dev_inputs_ok || !K->model_ok || !K->stream ||
       (K->search_mode == 2 && K->reorder && !K->perm_ok)) {
     set_error(W, "needs a finished single-variable solve of the handle's current inputs "
                  "(krige_solve or kk_solve_resident) before the allocation-free pass");

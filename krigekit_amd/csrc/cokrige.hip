@@ -73,7 +73,7 @@ hipError_t launch_solve_mv_d3_w1(const MVArgs& a, int rb, hipStream_t st);
 
 constexpr int MVT = 256;            // threads per workgroup (16 x 16)
 constexpr int MVR = 128;            // max rows + extra rows
-constexpr int MAXE = 3 * MAXVAR + 1;  // extra columns: b, F, z, indicators or ones
+constexpr int MAXE = 3 * MAXVAR + 1 + MAXDRIFT;  // extra columns: b, F (drift + unbiasedness), z, indicators or ones
 constexpr int MAXSUB = 64;
 
 template <int DIM, bool SIMPLE>
@@ -160,12 +160,13 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
   // MF: the refined solutions x_f (columns of K^-1 X', FP64), the matrix-vector
   // products M [x_f; -e_f], 1 / L(k, k), K's diagonal
-  constexpr int MXE = 2 * NV + 2;  // extra columns at most (non-standard cokriging)
+  constexpr int MXE = 2 * NV + 2 + MAXDRIFT;  // extra columns at most (non-standard cokriging, drift)
   __shared__ double sxd[MF ? MXE : 1][MF ? ROWS : 1];
   __shared__ double red[MF ? ROWS : 1][MF ? MXE : 1];
   __shared__ float sli[MF ? ROWS : 1];
   __shared__ float sdg[MF ? ROWS : 1];
   __shared__ double gram[MAXE][MAXE];
+  __shared__ double sdr[MAXDRIFT][ROWS];  // external drift values of the rows
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
   __shared__ double partb[4][NV * NV];
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // WEIGHTS: A(i, j), j <= i, j < n, at sl[i (i + 1) / 2 + j]; multipliers;
   // the weights finished so far
   __shared__ double sl[WEIGHTS ? ROWS * (ROWS + 1) / 2 : 1];
-  __shared__ double smu[NV][NV];
+  __shared__ double smu[NV][NV + MAXDRIFT];
   __shared__ double sw[NV][WEIGHTS ? ROWS : 1];
   const int b = a.list ? a.list[blockIdx.x] : (int)blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
@@ -201,8 +202,16 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
     if (t == 0) a.flag[b] = 0;
     return;
   }
-  const int p = a.unbias ? (a.std_ck ? nvar : 1) : 0;
+  // constraint rows: the external drifts, then unbiasedness (one per variable
+  // under std_ck, else one row of ones)
+  const int nd = a.ndrift;
+  const int p = nd + (a.unbias ? (a.std_ck ? nvar : 1) : 0);
   const bool isa = !a.std_ck && a.unbias && nvar > 1;  // Isaaks-Srivastava shift
+  // right-hand side of constraint row c for target q
+  auto dval = [&](int c, int q) -> double {
+    if (c < nd) return a.bdrift[c + nd * ((size_t)q + nvar * (size_t)b)];
+    return a.std_ck ? (c - nd == q ? 1.0 : 0.0) : 1.0;
+  };
   // extra column layout (offsets from n)
   const int eb = 0, ef = nvar, ez = nvar + p, ei = ez + 1;
   const int ne = ez + 1 + (isa ? nvar : 0) + (a.unbias ? 0 : 1);
@@ -238,6 +247,13 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
     sz[t] = ovl[idx];
     sov[t] = ovr[idx];
     svr[t] = vr;
+    if (nd > 0) {
+      const double* odr = a.odrift[0];
+#pragma unroll
+      for (int v = 1; v < NV; ++v)
+        if (v == vr) odr = a.odrift[v];
+      for (int c = 0; c < nd; ++c) sdr[c][t] = odr[c + nd * (size_t)idx];
+    }
   }
   const int ns = a.nsub;
   if (t < ns) {
@@ -371,7 +387,10 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
             for (int h = 0; h + step < NPART; h += 2 * step) w[h] += w[h + step];
           v = w[0];
         }
-        else if (e < ez) v = (a.std_ck ? (vj == e - ef) : true) ? 1.0 : 0.0;
+        else if (e < ez) {
+          const int c = e - ef;
+          v = c < nd ? sdr[c][j] : ((a.std_ck ? (vj == c - nd) : true) ? 1.0 : 0.0);
+        }
         else if (e == ez) v = sz[j];
         else if (isa) v = (vj == e - ei) ? 1.0 : 0.0;
         else v = 1.0;  // simple kriging: the ones column for the mean term
@@ -764,35 +783,36 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // neighbours leaves a zero row in S under std_ck) sends the block to the
   // Bunch-Kaufman fallback like the reference's failed SPOTRF of S
   if (t == 0) {
-    double mu[NV][NV];  // mu[q][c]
+    constexpr int PM = NV + MAXDRIFT;  // constraint rows at most
+    double mu[NV][PM];  // mu[q][c]
     bool ok = true;
 #pragma unroll
     for (int q = 0; q < NV; ++q)
 #pragma unroll
-      for (int c = 0; c < NV; ++c) mu[q][c] = 0.0;
+      for (int c = 0; c < PM; ++c) mu[q][c] = 0.0;
     if (p > 0) {
-      double U[NV][NV];
+      double U[PM][PM];
 #pragma unroll
-      for (int i = 0; i < NV; ++i)
+      for (int i = 0; i < PM; ++i)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) U[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
+        for (int j = 0; j < PM; ++j) U[i][j] = (i < p && j < p) ? gram[ef + i][ef + j] : 0.0;
       bool spd = true;
 #pragma unroll
-      for (int j = 0; j < NV; ++j) {
+      for (int j = 0; j < PM; ++j) {
         if (j >= p) break;
         double d = U[j][j];
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
+        for (int k = 0; k < PM; ++k)
           if (k < j) d -= U[k][j] * U[k][j];
         spd = spd && d > 0.0;
         d = sqrt(d);
         U[j][j] = d;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
+        for (int i = 0; i < PM; ++i) {
           if (i <= j || i >= p) continue;
           double v = U[j][i];
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
+          for (int k = 0; k < PM; ++k)
             if (k < j) v -= U[k][j] * U[k][i];
           U[j][i] = v / d;
         }
@@ -805,23 +825,22 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         if (!ok) break;
-        double y[NV];
+        double y[PM];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
+        for (int i = 0; i < PM; ++i) {
           if (i >= p) continue;
-          const double d = a.std_ck ? (i == q ? 1.0 : 0.0) : 1.0;
-          double v = gram[ef + i][eb + q] - d;
+          double v = gram[ef + i][eb + q] - dval(i, q);
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
+          for (int k = 0; k < PM; ++k)
             if (k < i) v -= U[k][i] * y[k];
           y[i] = v / U[i][i];
         }
 #pragma unroll
-        for (int i = NV - 1; i >= 0; --i) {
+        for (int i = PM - 1; i >= 0; --i) {
           if (i >= p) continue;
           double v = y[i];
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
+          for (int k = 0; k < PM; ++k)
             if (k > i && k < p) v -= U[i][k] * mu[q][k];
           mu[q][i] = v / U[i][i];
         }
@@ -832,7 +851,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       auto cw = [&](int e, int q) {
         double s = gram[e][eb + q];
 #pragma unroll
-        for (int f = 0; f < NV; ++f)
+        for (int f = 0; f < PM; ++f)
           if (f < p) s -= mu[q][f] * gram[e][ef + f];
         return s;
       };
@@ -857,8 +876,8 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
           const double base = ns > 1 ? sbase[i + NV * j] : a.models[i + nvar * j].cov0;
           double mud = 0.0;
 #pragma unroll
-          for (int c = 0; c < NV; ++c)
-            if (c < p) mud += mu[i][c] * (a.std_ck ? (c == j ? 1.0 : 0.0) : 1.0);
+          for (int c = 0; c < PM; ++c)
+            if (c < p) mud += mu[i][c] * dval(c, j);
           double v = base - (cw(eb + j, i) + mud);
           v = v > 0.0 ? v : 0.0;  // diagonal and off-diagonal clamped at 0
           a.var[b + (size_t)a.nb * (i + nvar * j)] = v;
@@ -871,7 +890,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
 #pragma unroll
       for (int q = 0; q < NV; ++q)
 #pragma unroll
-        for (int c = 0; c < NV; ++c) smu[q][c] = mu[q][c];
+        for (int c = 0; c < PM; ++c) smu[q][c] = mu[q][c];
     }
   }
   if (!WEIGHTS) return;
@@ -893,7 +912,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       const int rb_ = n + eb + q;
       double v = sl[rb_ * (rb_ + 1) / 2 + t];
 #pragma unroll
-      for (int f = 0; f < NV; ++f) {
+      for (int f = 0; f < NV + MAXDRIFT; ++f) {
         if (f >= p) break;
         const int rf = n + ef + f;
         v -= smu[q][f] * sl[rf * (rf + 1) / 2 + t];
@@ -911,6 +930,44 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       const double l = sl[i * (i + 1) / 2 + t] / dj;
 #pragma unroll
       for (int q = 0; q < NV; ++q) r[q] -= l * sw[q][i];
+    }
+  }
+  if (NV == 1 && a.wcorr) {
+    // weight correction (nvar 1, kriging_base.F90:3388-3391,
+    // kriging_clip_positive_normalize at :1995-2001): negative weights to 0,
+    // the rest divided by their sum when it is positive; then the estimate
+    // (estimate_block_base) and the variance base - (w'b + mu'd)
+    // (calc_variance, kriging.F90:968-1028) with the corrected weights
+    const double wp = (t < n && r[0] > 0.0) ? r[0] : 0.0;
+    const double tot = wg_sum(wp, part);
+    if (t < n) r[0] = tot > 0.0 ? wp / tot : wp;
+    double bj = 0.0;
+    if (t < n) {
+      // the row's right-hand side, its partial sums added as in the assembly
+      double w2[NPART];
+#pragma unroll
+      for (int h = 0; h < NPART; ++h) w2[h] = sb[h][0][t];
+#pragma unroll
+      for (int step = 1; step < NPART; step *= 2)
+#pragma unroll
+        for (int h = 0; h + step < NPART; h += 2 * step) w2[h] += w2[h + step];
+      bj = w2[0];
+    }
+    const double est = wg_sum(t < n ? r[0] * sz[t] : 0.0, part);
+    const double wb = wg_sum(t < n ? r[0] * bj : 0.0, part);
+    const double tw = wg_sum(t < n ? r[0] : 0.0, part);
+    if (t == 0 && !a.weights_only) {
+      double mud = 0.0;
+      for (int c = 0; c < p; ++c) mud += smu[0][c] * dval(c, 0);
+      const double base = ns > 1 ? sbase[0] : a.models[0].cov0;
+      double v = base - (wb + mud);
+      v = v > 0.0 ? v : 0.0;
+      double e = est;
+      if (!a.unbias && a.sk_mean[0] != 0.0) e += (1.0 - tw) * a.sk_mean[0];
+      e = e < a.bound_lo ? a.bound_lo : e;
+      e = e > a.bound_hi ? a.bound_hi : e;
+      a.est[b] = e;
+      a.var[b] = v;
     }
   }
   if (t < n) {
@@ -942,7 +999,7 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
   if (t >= nlist) return;
   const int b = list[t];
   const int nvar = a.nvar;
-  const int ld = a.rows + MAXVAR;
+  const int ld = a.rows + MAXP;
   int nsum = 0;
 #pragma unroll
   for (int v = 0; v < MAXVAR; ++v) nsum += v < nvar ? a.nnear[v][b] : 0;
@@ -958,11 +1015,22 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
     st[v] = n;
     n += nv[v];
   }
-  const int p = a.unbias ? (a.std_ck ? nvar : 1) : 0, m = n + p;
+  const int nd = a.ndrift;
+  const int p = nd + (a.unbias ? (a.std_ck ? nvar : 1) : 0), m = n + p;
   const double rs = a.rangescale ? a.rangescale[b] : 1.0;
   const double ln = a.localnugget ? a.localnugget[b] : 0.0;
   const int ns = a.nsub;
   const double* x0 = a.bcoord + DIM * (size_t)b;
+  // constraint row f of data row c (drift values, then unbiasedness), and its
+  // right-hand side for target q
+  auto frow = [&](int f, int vc, int ic) -> double {
+    if (f < nd) return a.odrift[vc][f + nd * (size_t)ic];
+    return a.std_ck ? (vc == f - nd ? 1.0 : 0.0) : 1.0;
+  };
+  auto dval = [&](int f, int q) -> double {
+    if (f < nd) return a.bdrift[f + nd * ((size_t)q + nvar * (size_t)b)];
+    return a.std_ck ? (f - nd == q ? 1.0 : 0.0) : 1.0;
+  };
   // row r: variable, observation index
   auto rowvar = [&](int r) {
     int v = 0;
@@ -987,7 +1055,7 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
       K[c + ld * r] = v;
     }
     for (int f = 0; f < p; ++f) {
-      const double e = a.std_ck ? (vc == f ? 1.0 : 0.0) : 1.0;
+      const double e = frow(f, vc, ic);
       K[(n + f) + ld * c] = e;
       K[c + ld * (n + f)] = e;
     }
@@ -1007,11 +1075,21 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
   }
   for (int f = 0; f < p; ++f) {
     for (int g = 0; g < p; ++g) K[(n + f) + ld * (n + g)] = 0.0;
-    for (int q = 0; q < nvar; ++q) B[(n + f) + ld * q] = a.std_ck ? (f == q ? 1.0 : 0.0) : 1.0;
+    for (int q = 0; q < nvar; ++q) B[(n + f) + ld * q] = dval(f, q);
   }
   for (int q = 0; q < nvar; ++q)
     for (int r = 0; r < m; ++r) B0[r + ld * q] = B[r + ld * q];
   const int info = bk_fallback(K, A, ipiv, n, m, ld, B, nvar);
+  if (!info && a.wcorr && nvar == 1) {
+    // weight correction (kriging_base.F90:3388-3391, 1995-2001)
+    double tot = 0.0;
+    for (int k = 0; k < n; ++k) {
+      B[k] = B[k] > 0.0 ? B[k] : 0.0;
+      tot = tot + B[k];
+    }
+    if (tot > 0.0)
+      for (int k = 0; k < n; ++k) B[k] = B[k] / tot;
+  }
   if (a.weights) {
     // the weight store: weight(k, group v, target q) = x(q, istart(v) + k)
     for (int q = 0; q < nvar; ++q)
@@ -1095,21 +1173,22 @@ hipError_t launch_fallback_mv(const MVArgs& a, const int* list, int nlist, doubl
   return hipGetLastError();
 }
 
-int mv_extra_cols(int nvar, int std_ck, int unbias) {
-  const int p = unbias ? (std_ck ? nvar : 1) : 0;
+int mv_extra_cols(int nvar, int std_ck, int unbias, int ndrift) {
+  const int p = ndrift + (unbias ? (std_ck ? nvar : 1) : 0);
   const bool isa = !std_ck && unbias && nvar > 1;
   return nvar + p + 1 + (isa ? nvar : 0) + (unbias ? 0 : 1);
 }
 
 // FP32 + MFMA + refinement runs for systems up to 80 columns (weights: FP64)
 bool mv_fp32r_instance(const MVArgs& a) {
-  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
+  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias, a.ndrift);
   return a.prec == PREC_FP32R && !a.weights && cols <= 80;
 }
 
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
   if (a.nb <= 0 || (a.list && a.nlist <= 0)) return hipSuccess;
-  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias);
+  const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias, a.ndrift);
+  if (a.ndrift < 0 || a.ndrift > MAXDRIFT || (a.wcorr && (a.nvar != 1 || !a.weights))) return hipErrorInvalidValue;
   if (a.rows > MVR || cols > MVR || a.nsub > MAXSUB || a.nvar < 1 || a.nvar > MAXVAR) return hipErrorInvalidValue;
   const int rb = cols <= 80 ? 5 : 8;
   // nvar 1: single-variable systems above the row-per-lane solver's 64 rows

@@ -110,7 +110,7 @@ hipError_t launch_loo_reduce(const double* zhat, const double* dz, const double*
                              hipStream_t st);
 // multi-variable block solve (one variable: single-variable systems above 64
 // rows); rows + mv_extra_cols(...) must not exceed 128
-int mv_extra_cols(int nvar, int std_ck, int unbias);
+int mv_extra_cols(int nvar, int std_ck, int unbias, int ndrift = 0);
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st);
 // true when launch_solve_mv runs the FP32 + MFMA + refinement instance, which
 // flags blocks for an FP64 re-solve (flag 1)
@@ -120,7 +120,7 @@ bool mv_fp32r_instance(const MVArgs& a);
 // when solved, 3 when the system stays singular (NaN estimates)
 hipError_t launch_fallback_mv(const MVArgs& a, const int* list, int nlist, double* scratch, hipStream_t st);
 __host__ __device__ constexpr size_t mv_fallback_words(int rows) {
-  return 2 * (size_t)(rows + MAXVAR) * (rows + MAXVAR) + 2 * (size_t)(rows + MAXVAR) * MAXVAR + (rows + MAXVAR);
+  return 2 * (size_t)(rows + MAXP) * (rows + MAXP) + 2 * (size_t)(rows + MAXP) * MAXVAR + (rows + MAXP);
 }
 // estimates and variances from stored weights, G lanes per block
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);

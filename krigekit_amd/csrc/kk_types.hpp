@@ -120,6 +120,8 @@ struct GradArgs {
 
 // Multi-variable (co)kriging, one workgroup per block.
 constexpr int MAXVAR = 4;
+constexpr int MAXDRIFT = 4;                // external drift functions
+constexpr int MAXP = MAXVAR + MAXDRIFT;    // constraint rows: drift + unbiasedness
 struct MVArgs {
   const Model* models;        // vgm(a+1, b+1) at a + nvar * b
   int ndim;
@@ -158,6 +160,17 @@ struct MVArgs {
                               // definite (fallback); 3: fallback failed (NaN)
   int prec;                   // PREC_FP64 or PREC_FP32R (FP32 + MFMA factorisation, FP64 refinement)
   double pivot_tol;           // FP32 pivot ratio below which a block is re-solved in FP64
+  // external drift (kriging.F90:675-849, 864-939): ndrift constraint rows
+  // before the unbiasedness rows, with the drift values at the observations,
+  // odrift[v][k + ndrift * i], and at the blocks, bdrift[k + ndrift * (q +
+  // nvar * b)] for target variable q, as right-hand sides
+  int ndrift;
+  const double* odrift[MAXVAR];
+  const double* bdrift;
+  // weight correction (nvar 1, kriging_base.F90:3388-3391): negative weights
+  // clipped to 0 and the rest rescaled to sum 1 before the estimate and
+  // variance (needs weights, the weights instance)
+  int wcorr;
 };
 
 // member blocks per work item of the single-variable solve (factor reuse):

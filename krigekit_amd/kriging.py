@@ -54,6 +54,8 @@ def _load_lib():
     sig("krige_initialize", [_i64] + [_c_int] * 16 + [ctypes.c_char_p, _pd, _c_int])
     sig("krige_set_obs", [_i64, _c_int, _c_int, _c_int, _pd, _pd, _pd, _c_int, _c_double, _c_double])
     sig("krige_update_obs_value", [_i64, _c_int, _c_int, _pd])
+    sig("krige_set_obs_drift", [_i64, _c_int, _c_int, _c_int, _pd])
+    sig("krige_set_grid_drift", [_i64, _c_int, _c_int, _c_int, _pd])
     sig("krige_reset_vgm", [_i64, _c_int, _c_int])
     sig("krige_set_vgm", [_i64, _c_int, _c_int, ctypes.c_char_p] + [_c_double] * 8 + [_c_int])
     sig("krige_set_grid", [_i64, _c_int, _c_int, _pd, _pd, _pd])
@@ -207,6 +209,26 @@ class Kriging:
         _check(_load_lib().krige_set_obs(self._handle, ivar, nobs, cf.shape[0], _dp(cf), _dp(v),
                                          _dp(var), nm, md, sk_mean), "krige_set_obs")
         self._nobs[ivar - 1] = nobs
+
+    def set_obs_drift(self, ivar: int, drift):
+        """external drift values at variable ``ivar``'s observations, shape
+        (nobs, ndrift), passed to the engine as (ndrift, nobs)
+        (src/krigekit/kriging.py:665-687)"""
+        d = np.asfortranarray(np.asarray(drift, dtype=np.float64).T)
+        if d.ndim == 1:
+            d = d.reshape(1, -1)
+        _check(_load_lib().krige_set_obs_drift(self._handle, ivar, d.shape[0], d.shape[1], _dp(d)),
+               "krige_set_obs_drift")
+
+    def set_grid_drift(self, drift, ivar: Optional[int] = None):
+        """external drift values at the blocks, shape (nblocks, ndrift), for
+        target variable ``ivar`` (None: every target)
+        (src/krigekit/kriging.py:1096-1132)"""
+        d = np.asfortranarray(np.asarray(drift, dtype=np.float64).T)
+        if d.ndim == 1:
+            d = d.reshape(1, -1)
+        _check(_load_lib().krige_set_grid_drift(self._handle, ivar if ivar is not None else -1, d.shape[0],
+                                                d.shape[1], _dp(d)), "krige_set_grid_drift")
 
     def update_obs_value(self, ivar: int, value):
         v = np.asfortranarray(np.asarray(value, dtype=np.float64).ravel())

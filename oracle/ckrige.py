@@ -14,12 +14,16 @@ Follows the reference's per-block computation for nvar > 1:
     (variogram.f90:365); K's cross blocks use vgm(col var, row var) with the
     column group after the row group, so they are analytic, and the
     right-hand side uses vgm(group var, target var);
-  * constraint rows (kriging_base.F90:614-624, 940-950): std_ck -> one per
-    variable (row v(r) of F is 1), otherwise one row of ones; none for
-    simple kriging;
+  * constraint rows (kriging_base.F90:614-624, 940-950): first the ndrift
+    external drifts (the drift values at the row's observation,
+    kriging.F90:675-849), then unbiasedness: std_ck -> one per variable (row
+    v(r) of F is 1), otherwise one row of ones; none for simple kriging;
   * right-hand side per target variable q: sum over the block's integration
-    nodes of w_k C_{v(r) q}((x_r - g_k) / rs), constraint part e_q (std_ck) or
-    1 (kriging.F90:864-939);
+    nodes of w_k C_{v(r) q}((x_r - g_k) / rs), constraint part the block's
+    drift values, then e_q (std_ck) or 1 (kriging.F90:864-939);
+  * weight correction (nvar 1, kriging_base.F90:3388-3391, 1995-2001): the
+    data weights clipped at 0 and divided by their sum when it is positive,
+    before the estimate and variance;
   * estimate (kriging_base.F90:3407-3503): sum of weights x values, the
     Isaaks-Srivastava mean shift for non-standard cokriging, the simple
     kriging mean term, bounds;
@@ -122,7 +126,14 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
         return okrige.cov(s, np.asarray(lags, dtype=np.float64).reshape(-1, nd),
                           table_mode=table_mode and a <= b)
 
-    p = (nvar if std else 1) if unbias else 0
+    ndr = int(spec.get("ndrift", 0))
+    D = [np.zeros((len(z), ndr)) if o.get("drift") is None else np.asarray(o["drift"], dtype=np.float64).reshape(len(z), ndr)
+         for o, z in zip(spec["obs"], Z)]
+    gd = np.zeros((nb, ndr)) if spec.get("grid_drift") is None else np.asarray(spec["grid_drift"],
+                                                                                dtype=np.float64).reshape(nb, ndr)
+    wcorr = bool(spec.get("weight_correction", False)) and nvar == 1
+    pu = (nvar if std else 1) if unbias else 0
+    p = ndr + pu
     est = np.full((nb, nvar), np.nan)
     var = np.full((nb, nvar, nvar), np.nan)
     nnear = np.stack([l[0] for l in lists], axis=1)
@@ -138,6 +149,7 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
         xr = np.concatenate([X[v][g] for v, g in enumerate(groups)])
         zr = np.concatenate([Z[v][g] for v, g in enumerate(groups)])
         vr = np.concatenate([V[v][g] for v, g in enumerate(groups)])
+        dr = np.concatenate([D[v][g] for v, g in enumerate(groups)])
         K = np.zeros((n + p, n + p))
         for i in range(nvar):
             for j in range(nvar):
@@ -149,10 +161,12 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
         K[np.arange(n), np.arange(n)] = np.array([cov0[(v, v)] for v in rv]) + vr + ln[b]
         if p:
             F = np.zeros((n, p))
-            if std:
-                F[np.arange(n), rv] = 1.0
-            else:
-                F[:, 0] = 1.0
+            F[:, :ndr] = dr
+            if pu:
+                if std:
+                    F[np.arange(n), ndr + rv] = 1.0
+                else:
+                    F[:, ndr] = 1.0
             K[:n, n:] = F
             K[n:, :n] = F.T
         R = np.zeros((n + p, nvar))
@@ -164,7 +178,9 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
                 lag = (xr[ri][:, None, :] - sub[b][None, :, :]) / rs[b]
                 R[ri, q] = cov(i, q, lag).reshape(len(ri), -1) @ sw[b]
             if p:
-                R[n + (q if std else 0), q] = 1.0
+                R[n:n + ndr, q] = gd[b]
+            if pu:
+                R[n + ndr + (q if std else 0), q] = 1.0
         if n == 1:
             # the reference skips the solve of a one-neighbour block
             # (kriging_base.F90:2127): NaN, as the library reports
@@ -174,6 +190,15 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
             est[b] = np.nan
             var[b] = 0.0
             continue
+        if wcorr:
+            wc = np.where(x[:n, 0] > 0.0, x[:n, 0], 0.0)
+            tot = 0.0
+            for v in wc:
+                tot += v
+            if tot > 0.0:
+                wc = wc / tot
+            x = x.copy()
+            x[:n, 0] = wc
         w = x[:n]
         if want_weights:
             for g in range(nvar):

@@ -111,6 +111,23 @@ class RefKriging:
                   "krige_set_obs")
         self.nobs[ivar] = v.size
 
+    def set_obs_drift(self, ivar, drift):
+        """drift (nobs, ndrift), passed as (ndrift, nobs) column-major (kriging.py:665-685)"""
+        d = np.asfortranarray(np.asarray(drift, dtype=np.float64).T)
+        f = self.lib.krige_set_obs_drift
+        f.restype = _c_int
+        f.argtypes = [_i64, _c_int, _c_int, _c_int, _pd]
+        self._chk(f(self.h, ivar, d.shape[0], d.shape[1], _dp(d)), "krige_set_obs_drift")
+
+    def set_grid_drift(self, drift, ivar=None):
+        """drift (nblocks, ndrift); ivar None broadcasts to every target (kriging.py:1096-1132)"""
+        d = np.asfortranarray(np.asarray(drift, dtype=np.float64).T)
+        f = self.lib.krige_set_grid_drift
+        f.restype = _c_int
+        f.argtypes = [_i64, _c_int, _c_int, _c_int, _pd]
+        self._chk(f(self.h, -1 if ivar is None else ivar, d.shape[0], d.shape[1], _dp(d)),
+                  "krige_set_grid_drift")
+
     def set_vgm(self, ivar, jvar, vtype, nugget=0.0, sill=1.0, a_major=1.0, a_minor1=None,
                 a_minor2=None, azimuth=0.0, dip=0.0, plunge=0.0, product=False):
         if a_minor1 is None:

@@ -7,7 +7,9 @@ reference's own counters split the last two into factorisations and cache
 reuses, which depend on its thread schedule), and ck_vectors.npz: its multi-variable
 (cokriging) results on problems.ck_cases() in table and analytic mode
 (oracle/_ref/libkriging_ref_analytic.so); for some problems also the stored
-kriging weights (krige_get_weight_data).  Run in a container where the
+kriging weights (krige_get_weight_data); and drift_vectors.npz: external
+drift and weight-correction problems (problems.drift_cases()).  Run in a
+container where the
 reference tree and oracle/_ref exist:  python tests/golden/make_ref_vectors.py
 """
 import os
@@ -75,6 +77,32 @@ def main():
                                                            st["ssytrf_fact"] + st["ssytrf_reuse"]], dtype=np.int32)
     np.savez_compressed(os.path.join(HERE, "ck_vectors.npz"), **ck)
     print("wrote", len(problems.ck_cases()), "cokriging problems")
+    write_drift()
+
+
+def write_drift():
+    """drift_vectors.npz: external drift (KED, UK on blocks, without
+    unbiasedness, cokriging) and weight-correction problems, table and
+    analytic mode, with the stored weights in table mode"""
+    dv = {}
+    for spec in problems.drift_cases():
+        for mode in ("table", "analytic"):
+            r = run_ref_ck(spec, mode, want_weights=mode == "table")
+            key = spec["name"]
+            if mode == "table":
+                nn = r["nnear"]
+                sel = np.arange(0, len(nn), 3)
+                sel = sel[nn[sel].sum(axis=1) >= 2].astype(np.int32)
+                wmax = max(int(nn.max()), 1)
+                dv[f"{key}/wblocks"] = sel
+                dv[f"{key}/weight"] = r["weight"][sel][:, :, :, :wmax]
+            dv[f"{key}/{mode}/estimate"] = r["estimate"]
+            dv[f"{key}/{mode}/variance"] = r["variance"]
+            st = r["stats"]
+            dv[f"{key}/{mode}/stats"] = np.array([st["fail"], st["chol_fact"] + st["chol_reuse"],
+                                                  st["ssytrf_fact"] + st["ssytrf_reuse"]], dtype=np.int32)
+    np.savez_compressed(os.path.join(HERE, "drift_vectors.npz"), **dv)
+    print("wrote", len(problems.drift_cases()), "drift problems")
 
 
 if __name__ == "__main__":

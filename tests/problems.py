@@ -312,6 +312,92 @@ def ck_near_duplicates(nb=120, seed=21):
     return s
 
 
+def _one_var(s):
+    """variable 1 of a cokriging problem on its own"""
+    s["nvar"] = 1
+    s["obs"] = s["obs"][:1]
+    s["search"] = s["search"][:1]
+    s["vgm"] = [v for v in s["vgm"] if v["ivar"] == 1 and v["jvar"] == 1]
+    return s
+
+
+def drift_ked(nb=150, seed=23):
+    """external-drift kriging: one drift function (a smooth trend surface),
+    ordinary unbiasedness (p = 2 constraint rows)"""
+    s = _one_var(ck_small(std=True, gq=False, nb=nb, seed=seed))
+    trend = lambda x: 0.8 * np.sin(x[:, 0] / 450.0) + 0.5 * np.cos(x[:, 1] / 620.0)  # noqa: E731
+    o = s["obs"][0]
+    o["drift"] = trend(o["coord"])[:, None]
+    o["value"] = o["value"] + 2.0 * o["drift"][:, 0]
+    s["grid_drift"] = trend(s["grid"])[:, None]
+    s["ndrift"] = 1
+    s["name"] = "drift_ked"
+    return s
+
+
+def drift_uk(nb=120, seed=25):
+    """universal kriging: the coordinates (km) as two drift functions with
+    ordinary unbiasedness, on 4 x 4 Gauss-Legendre blocks (block drift at
+    the centre)"""
+    s = _one_var(ck_small(std=True, gq=True, nb=nb, seed=seed))
+    o = s["obs"][0]
+    o["drift"] = o["coord"] / 1000.0
+    o["value"] = o["value"] + 0.5 * o["drift"][:, 0] - 0.3 * o["drift"][:, 1]
+    s["grid_drift"] = s["grid"] / 1000.0
+    s["ndrift"] = 2
+    s["name"] = "drift_uk_blocks"
+    return s
+
+
+def drift_sk(nb=100, seed=27):
+    """two drift functions without the unbiasedness row (unbias=0), as the
+    reference's own drift test (tests/test_api.py:290-306)"""
+    s = drift_uk(nb=nb, seed=seed)
+    for k in ("sub_coord", "sub_weight", "blocksize"):
+        s.pop(k, None)
+    s["nsub"] = 1
+    s["unbias"] = 0
+    s["name"] = "drift_sk"
+    return s
+
+
+def drift_ck(nb=100, seed=29):
+    """two-variable standard cokriging with one drift function on both
+    variables (p = 1 + 2 constraint rows), the block drift broadcast to both
+    targets"""
+    s = ck_small(std=True, gq=False, nb=nb, seed=seed)
+    trend = lambda x: x[:, 0] / 1000.0 - 0.5 * x[:, 1] / 1000.0  # noqa: E731
+    for o in s["obs"]:
+        o["drift"] = trend(o["coord"])[:, None]
+        o["value"] = o["value"] + o["drift"][:, 0]
+    s["grid_drift"] = trend(s["grid"])[:, None]
+    s["ndrift"] = 1
+    s["name"] = "drift_ck"
+    return s
+
+
+def wcorr_ok(nb=150, seed=31):
+    """weight correction (negative weights clipped, the rest rescaled to sum 1)
+    on ordinary kriging with clustered data, where screening gives negative
+    weights"""
+    s = _one_var(ck_small(std=True, gq=False, nb=nb, seed=seed))
+    s["weight_correction"] = True
+    s["name"] = "wcorr_ok"
+    return s
+
+
+def wcorr_drift(nb=120, seed=33):
+    """weight correction with one external drift function"""
+    s = drift_ked(nb=nb, seed=seed)
+    s["weight_correction"] = True
+    s["name"] = "wcorr_drift"
+    return s
+
+
+def drift_cases():
+    return [drift_ked(), drift_uk(), drift_sk(), drift_ck(), wcorr_ok(), wcorr_drift()]
+
+
 def ck_cases():
     return [ck_small(True, True), ck_small(True, False), ck_small(False, True), ck_small(False, False),
             ck_small(True, False, unbias=0, sk=(0.2, -0.1)), ck3_small(), ck_empty_var(True), ck_empty_var(False),

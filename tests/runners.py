@@ -145,14 +145,19 @@ def run_ref_ck(spec, mode="table", nthread=4, want_weights=False):
     """the compiled reference engine on a multi-variable problem"""
     from oracle.refdrv import RefKriging
     k = RefKriging(mode=mode, ndim=spec["ndim"], nvar=spec["nvar"], std_ck=spec.get("std_ck", True),
-                   unbias=spec.get("unbias", 1), store_weight=want_weights)
+                   unbias=spec.get("unbias", 1), store_weight=want_weights, ndrift=spec.get("ndrift", 0),
+                   weight_correction=spec.get("weight_correction", False))
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
+        if o.get("drift") is not None:
+            k.set_obs_drift(v, o["drift"])
     if spec.get("sub_coord") is not None:
         k.set_grid_block_gq(spec["grid"], spec["blocksize"])
     else:
         k.set_grid(spec["grid"])
+    if spec.get("grid_drift") is not None:
+        k.set_grid_drift(spec["grid_drift"])
     for v in spec["vgm"]:
         k.set_vgm(**v)
     for s in spec["search"]:
@@ -169,16 +174,22 @@ def run_ref_ck(spec, mode="table", nthread=4, want_weights=False):
 
 def make_gpu_ck(spec, **kw):
     from krigekit_amd.kriging import Kriging
+    kw.setdefault("ndrift", spec.get("ndrift", 0))
+    kw.setdefault("weight_correction", spec.get("weight_correction", False))
     k = Kriging(ndim=spec["ndim"], nvar=spec["nvar"], std_ck=spec.get("std_ck", True),
                 unbias=spec.get("unbias", 1), **kw)
     for v, o in enumerate(spec["obs"], 1):
         k.set_obs(v, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
                   maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
+        if o.get("drift") is not None:
+            k.set_obs_drift(v, o["drift"])
     rl = dict(rangescale=spec.get("rangescale"), localnugget=spec.get("localnugget"))
     if spec.get("sub_coord") is not None:
         k.set_grid_block(spec["grid"], -4, blocksize=np.asarray(spec["blocksize"]), **rl)
     else:
         k.set_grid(spec["grid"], **rl)
+    if spec.get("grid_drift") is not None:
+        k.set_grid_drift(spec["grid_drift"])
     for v in spec["vgm"]:
         k.set_vgm(**v)
     for s in spec["search"]:

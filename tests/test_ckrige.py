@@ -98,3 +98,43 @@ def test_oracle_weights_match_reference_store(spec, vectors):
     d = np.abs(wo[fin][..., :k] - wr[fin][..., :k]) / np.maximum(1.0, np.abs(wr[fin][..., :k]))
     assert d.max() < 1e-9, d.max()
 
+
+
+DRIFT = problems.drift_cases()
+
+
+@pytest.mark.parametrize("mode", ["table", "analytic"])
+@pytest.mark.parametrize("spec", DRIFT, ids=[c["name"] for c in DRIFT])
+def test_drift_oracle_matches_compiled_reference(spec, mode):
+    """external drift (KED, UK on blocks, drift without unbiasedness, drift
+    in cokriging) and weight correction: the numpy oracle against the
+    compiled reference's committed vectors"""
+    v = np.load(problems.GOLDEN + "/drift_vectors.npz")
+    o = ckrige.solve(spec, table_mode=(mode == "table"))
+    ref = dict(estimate=v[f"{spec['name']}/{mode}/estimate"], variance=v[f"{spec['name']}/{mode}/variance"])
+    got = dict(estimate=o["estimate"].reshape(ref["estimate"].shape),
+               variance=o["variance"].reshape(ref["variance"].shape))
+    ee, ev = ck_errors(got, ref)
+    assert ee < 1e-10 and ev < 1e-10, (ee, ev)
+
+
+@pytest.mark.parametrize("name", ["wcorr_ok", "drift_ked"])
+def test_drift_and_correction_terms_matter(name):
+    """the committed vectors exercise what they pin: without the weight
+    correction (or without the drift rows) the oracle misses them by far more
+    than the bar"""
+    spec = {c["name"]: c for c in DRIFT}[name]
+    v = np.load(problems.GOLDEN + "/drift_vectors.npz")
+    off = dict(spec)
+    if name == "wcorr_ok":
+        off["weight_correction"] = False
+    else:
+        off["ndrift"] = 0
+        off["grid_drift"] = None
+        off["obs"] = [dict(o, drift=None) for o in spec["obs"]]
+    o = ckrige.solve(off)
+    ref = dict(estimate=v[f"{name}/table/estimate"], variance=v[f"{name}/table/variance"])
+    got = dict(estimate=o["estimate"].reshape(ref["estimate"].shape),
+               variance=o["variance"].reshape(ref["variance"].shape))
+    ee, ev = ck_errors(got, ref)
+    assert max(ee, ev) > 1e-3, (ee, ev)
