@@ -546,9 +546,15 @@ struct WStore {
 struct Krige {
   int ndim = 2, nvar = 1, ndrift = 0, unbias = 1, nsim = 0;
   // space-time (krige_st_*): coordinates are (x, y, z, t), ndim = 4 rows,
-  // spatial structures in vgm, temporal ones in vgm_t, product-sum k_ps
+  // spatial structures in vgm, temporal ones in vgm_t; the product-sum
+  // coefficient k_ps, or for the sum-metric model (the reference's default,
+  // kriging_st.F90:52-56) the joint sills (one per spatial structure) and the
+  // temporal transform f(dt)
   bool st = false;
-  int st_model = 0;  // 1: product_sum
+  int st_model = 0;  // 0: sum_metric, 1: product_sum
+  int t_vtype = 8;   // transform shape (linear)
+  double t_nug = 0.0, t_sill = 1.0, t_at = 1.0;
+  std::vector<double> sill_st;
   int mv_simple = 0;  // cokriging models use only inline-evaluated shapes
   double k_ps = 0.0, st_cov0 = 0.0;
   std::vector<VStruct> vgm_t;
@@ -738,7 +744,15 @@ bool stage_inputs(Krige& K) {
         return false;
       }
       HIPCHK(hipMemcpy(K.d_tmodel.p, &K.tmodel, sizeof(Model), hipMemcpyHostToDevice), W);
-      K.st_cov0 = K.k_ps * K.model.cov0 * K.tmodel.cov0 + K.model.cov0 + K.tmodel.cov0;
+      if (K.st_model == 1) {
+        K.st_cov0 = K.k_ps * K.model.cov0 * K.tmodel.cov0 + K.model.cov0 + K.tmodel.cov0;
+      } else {
+        // sum-metric: Cs(0) + Ct(0) + the joint sills of the non-nugget
+        // structures (a nugget shape is 0 at every lag)
+        K.st_cov0 = K.model.cov0 + K.tmodel.cov0;
+        for (size_t k = 0; k < K.sill_st.size(); ++k)
+          if (K.vgm[0][k].vtype != kk::NUG) K.st_cov0 = K.st_cov0 + K.sill_st[k];
+      }
     }
     K.model_ok = true;
   }
@@ -1020,6 +1034,12 @@ bool setup_pass(Krige& K, const PassBufs* ext, int mmax, PassBufs& B, kk::SolveA
     va.tmodel = K.d_tmodel.as<Model>();
     va.k_ps = K.k_ps;
     va.st_cov0 = K.st_cov0;
+    va.st_sum = K.st_model == 0;
+    va.t_vtype = K.t_vtype;
+    va.t_nug = K.t_nug;
+    va.t_sill = K.t_sill;
+    va.t_at = K.t_at;
+    for (size_t k = 0; k < K.sill_st.size() && k < (size_t)kk::MAXS; ++k) va.sill_st[k] = K.sill_st[k];
   }
   va.ocoord = K.d_ocoord.as<double>();
   va.ovalue = K.d_ovalue.as<double>();
@@ -1039,8 +1059,9 @@ bool setup_pass(Krige& K, const PassBufs* ext, int mmax, PassBufs& B, kk::SolveA
     va.base_cov = K.d_basecov.as<double>();
   }
   // the inline Level-B kernels need every model on one composite table: for
-  // space-time both the spatial and the temporal model
-  tb = K.model.mode == kk::COV_TABLE_B && (!K.st || K.tmodel.mode == kk::COV_TABLE_B);
+  // space-time both the spatial and the temporal model, and the product-sum
+  // (the sum-metric joint term is evaluated out of line)
+  tb = K.model.mode == kk::COV_TABLE_B && (!K.st || (K.tmodel.mode == kk::COV_TABLE_B && K.st_model == 1));
   return true;
 }
 
@@ -1824,9 +1845,22 @@ bool validate_for_solve(Krige& K) {
       set_error(W, "space-time kriging with nvar > 1 is not supported by this backend");
       return false;
     }
-    if (K.st_model != 1) {
-      set_error("t_kriging_st%prepare", "call set_st_model('product_sum', ...) before solve()");
-      return false;
+    if (K.st_model == 0) {
+      // is_valid_vgm_st, variogram_st.f90:388-402
+      if (K.sill_st.empty()) {
+        set_error("is_valid_vgm_st", "WARNING vgm_struct_st(1,1): sill_st not set for sum-metric model");
+        return false;
+      }
+      if (K.sill_st.size() != K.vgm[0].size()) {
+        set_error("is_valid_vgm_st", "WARNING vgm_struct_st(1,1): one joint sill per spatial structure is needed");
+        return false;
+      }
+      for (size_t k = 0; k < K.sill_st.size(); ++k)
+        if (K.sill_st[k] < 0.0) {
+          set_error("is_valid_vgm_st",
+                    "WARNING vgm_struct_st(1,1): negative joint sill at structure " + std::to_string(k + 1));
+          return false;
+        }
     }
     if (K.vgm_t.empty()) {
       set_error("t_kriging_st%prepare", "Invalid ST variogram: no temporal structure (set_vgm_temporal)");
@@ -3726,22 +3760,38 @@ int krige_st_set_st_model(int64_t h, const char* model, const char* transform, d
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  (void)transform;
-  (void)at;
-  (void)time_nugget;
-  (void)time_sill;
   const std::string m = model ? model : "";
-  if (m == "product_sum") {
-    K->st_model = 1;
-    K->k_ps = k_ps;
-    K->model_ok = false;
-    K->solved = false;
-    return 0;
+  if (m != "product_sum" && m != "sum_metric") {
+    set_error("krige_st_set_st_model", "model must be \"sum_metric\" or \"product_sum\", got: " + m);
+    return ierr();
   }
-  if (m == "sum_metric")
-    return unsupported("krige_st_set_st_model", "the sum-metric space-time model is not provided by this backend");
-  set_error("krige_st_set_st_model", "model must be \"sum_metric\" or \"product_sum\", got: " + m);
-  return ierr();
+  // transform id and checks: kriging_st_capi.f90:139-150, kriging_st.F90:143-165
+  const int tid = vtype_from_str(transform ? transform : "");
+  if (tid < 0) {
+    set_error("vtype_from_str", std::string("unknown vtype: ") + (transform ? transform : ""));
+    return ierr();
+  }
+  if (tid < kk::NUG || tid > kk::LIN || tid == kk::HOL) {
+    set_error("set_st_model", "transform must be a monotone vgmfunc id: nug, sph, exp, gau, pow, bsq, cir, or lin.");
+    return ierr();
+  }
+  if (!(at > 1.e-10)) {
+    set_error("set_st_model", "at must be positive.");
+    return ierr();
+  }
+  if (time_nugget < 0.0 || time_sill < 0.0) {
+    set_error("set_st_model", "time_nugget and time_sill must be non-negative.");
+    return ierr();
+  }
+  K->st_model = m == "product_sum" ? 1 : 0;
+  K->t_vtype = tid;
+  K->t_nug = time_nugget;
+  K->t_sill = time_sill;
+  K->t_at = at;
+  K->k_ps = k_ps;
+  K->model_ok = false;
+  K->solved = false;
+  return 0;
 }
 
 int krige_st_set_obs(int64_t h, int ivar, int nobs, int ndim, const double* coord, const double* value,
@@ -3774,6 +3824,7 @@ int krige_st_reset_vgm(int64_t h, int ivar, int jvar) {
   if (!K) return ierr();
   if (krige_reset_vgm(h, ivar, jvar)) return ierr();
   K->vgm_t.clear();
+  K->sill_st.clear();
   K->model_ok = false;
   return 0;
 }
@@ -3809,10 +3860,24 @@ int krige_st_set_vgm_temporal(int64_t h, int ivar, int jvar, const char* vtype, 
   return 0;
 }
 
-int krige_st_set_vgm_joint_sills(int64_t h, int, int, int, const double*) {
+int krige_st_set_vgm_joint_sills(int64_t h, int ivar, int jvar, int n, const double* sills) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_st_set_vgm_joint_sills", "the sum-metric space-time model is not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  // kriging_st.F90:267-288 -> variogram_st.f90:325-337: one joint sill per
+  // spatial structure added so far
+  if (!K->st || ivar != 1 || jvar != 1) {
+    set_error("set_vgm_joint_sills", "joint sills need a krige_st handle with nvar = 1");
+    return ierr();
+  }
+  if (n != (int)K->vgm[0].size() || (n > 0 && !sills)) {
+    set_error("set_joint_sills_vgm_st", "vgm_struct_st%set_joint_sills: length of sills must equal cs%nstruct");
+    return ierr();
+  }
+  K->sill_st.assign(sills, sills + n);
+  K->model_ok = false;
+  K->solved = false;
+  return 0;
 }
 
 int krige_st_set_grid(int64_t h, int ngrid, const double* coord, const double* time, const double* rangescale,

@@ -229,6 +229,15 @@ __device__ inline double cov(const Model& M, double dx, double dy, double dz) {
   return res;
 }
 
+// one structure's own table at a dimensionless lag h (Level A tables), with
+// its analytic tail beyond the table: comp_cov_tab_h, variogram.f90:353-382
+__device__ inline double comp_tab_h(const Model& M, int c, double h) {
+  const double* tab = M.tab + (size_t)c * (TAB_N + 1);
+  if (h >= TAB_HB[3]) return has_tail(M.vtype[c]) ? comp_cov_h(M, c, h) : 0.0;
+  if (h <= 0.0) return tab[0];
+  return table_interp_c(tab, h);
+}
+
 // every mode (Level A tables, analytic), out of line
 template <int DIM>
 __device__ __noinline__ double cov_general(const Model* Mp, double dx, double dy, double dz) {

@@ -190,7 +190,15 @@ struct SolveArgs {
   const Model* model;         // device copy of the covariance model (space-time: spatial part)
   const Model* tmodel;        // space-time only: temporal structures (1-D)
   double k_ps;                // space-time product-sum coefficient
-  double st_cov0;             // space-time C(0, 0) = k Cs0 Ct0 + Cs0 + Ct0
+  double st_cov0;             // space-time C(0, 0): k Cs0 Ct0 + Cs0 + Ct0 (product-sum),
+                              // Cs0 + Ct0 + joint sills of non-nugget structures (sum-metric)
+  // space-time sum-metric instead of product-sum: Cs + Ct + sum_k sill_st[k]
+  // shape_k(sqrt(h_k^2 + f(dt)^2)), f(dt) = max(t_nug + t_sill (1 -
+  // corefunc(t_vtype, |dt| / t_at)), 0), f(0) = 0
+  int st_sum;
+  int t_vtype;
+  double t_nug, t_sill, t_at;
+  double sill_st[MAXS];
   int ndim;
   int unbias;                 // 1 ordinary, 0 simple
   double sk_mean;

@@ -64,16 +64,46 @@ __device__ inline int wmax(int v) {
   return v;
 }
 
+// The sum-metric joint term added to base = Cs + Ct (variogram_st.f90:178-194
+// analytic, 241-261 tables): per spatial structure k, sill_st[k] times its
+// shape at h_st = sqrt(h_k^2 + f(dt)^2), h_k the structure's anisotropic lag
+// and f the temporal transform (f_time_vgm_st, :126-141).  With per-structure
+// tables (Level A) and a positive sill the shape is the structure's table
+// value divided by its sill, otherwise the analytic shape.
+__device__ __noinline__ double st_sum_cov(const SolveArgs* ap, const Model* Mp, double base, double dx, double dy,
+                                          double dz, double dt) {
+  const SolveArgs& a = *ap;
+  const Model& M = *Mp;
+  const double adt = fabs(dt);
+  double dw = 0.0;
+  if (!(adt <= 2.220446049250313e-16)) {
+    dw = a.t_nug + a.t_sill * (1.0 - corefunc(a.t_vtype, adt / a.t_at));
+    dw = dw > 0.0 ? dw : 0.0;
+  }
+  double res = base;
+  for (int k = 0; k < M.nstruct; ++k) {
+    const double hs = aniso_h<3>(M.mat[k], dx, dy, dz);
+    const double hst = sqrt(hs * hs + dw * dw);
+    if (M.mode == COV_TABLE_A && M.sill[k] > 0.0)
+      res = res + a.sill_st[k] * comp_tab_h(M, k, hst) / M.sill[k];
+    else
+      res = res + a.sill_st[k] * corefunc(M.vtype[k], hst);
+  }
+  return res;
+}
+
 // Covariance of one pair of rows.  DIM 4 is space-time (x, y, z, t): the
 // product-sum k Cs(dx, dy, dz) Ct(|dt|) + Cs + Ct of the spatial model and
-// the 1-D temporal model (variogram_st.f90:222-282).
-// With TB both models are single Level-B tables and both lookups are inline.
+// the 1-D temporal model (variogram_st.f90:222-282), or the sum-metric.
+// With TB both models are single Level-B tables and both lookups are inline
+// (product-sum only: the host runs sum-metric models without TB).
 template <int DIM, bool TB>
 __device__ inline double pair_cov(const SolveArgs& a, const Model& M, double dx, double dy, double dz,
                                   double dt) {
   if (DIM <= 3) return cov_eval<DIM, TB>(M, dx, dy, dz);
   const double cs = cov_eval<3, TB>(M, dx, dy, dz);
   const double ct = cov_eval<1, TB>(*a.tmodel, fabs(dt), 0.0, 0.0);
+  if (!TB && a.st_sum) return st_sum_cov(&a, &M, cs + ct, dx, dy, dz, dt);
   return a.k_ps * cs * ct + cs + ct;
 }
 
@@ -611,7 +641,9 @@ __device__ inline double fb_cov(const SolveArgs& a, const Model& M, const double
   const double dz = DIM >= 3 ? (xi[2] - xj[2]) / rs : 0.0;
   if (DIM <= 3) return cov<DIM>(M, dx, dy, dz);
   const double cs = cov<3>(M, dx, dy, dz);
-  const double ct = cov<1>(*a.tmodel, fabs(xi[DIM - 1] - xj[DIM - 1]), 0.0, 0.0);
+  const double dt = xi[DIM - 1] - xj[DIM - 1];
+  const double ct = cov<1>(*a.tmodel, fabs(dt), 0.0, 0.0);
+  if (a.st_sum) return st_sum_cov(&a, &M, cs + ct, dx, dy, dz, dt);
   return a.k_ps * cs * ct + cs + ct;
 }
 

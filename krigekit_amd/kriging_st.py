@@ -4,9 +4,11 @@ Same call sequence as the reference class -- ``set_st_model``, ``set_obs``
 (combined ``(nobs, ndim+1)`` coordinates or split ``coord`` + ``time``),
 ``set_vgm`` (spatial structures), ``set_vgm_temporal``, ``set_grid``,
 ``set_search(ivar, time_at, ...)``, ``solve``, ``get_results`` -- over the
-``krige_st_*`` entry points of ``libkrigekit_amd.so``.  The product-sum model
-``C = k Cs Ct + Cs + Ct`` is provided; the sum-metric model, gradients,
-blocks, drift and simulation raise ``RuntimeError`` from the library.
+``krige_st_*`` entry points of ``libkrigekit_amd.so``.  Both space-time
+models are provided: the sum-metric (the default; joint sills through
+``set_vgm_joint_sills``) and the product-sum ``C = k Cs Ct + Cs + Ct``;
+gradients, blocks, drift and simulation raise ``RuntimeError`` from the
+library.
 """
 import ctypes
 import random
@@ -16,6 +18,9 @@ from typing import Optional
 import numpy as np
 
 from .kriging import _PRECISIONS, Kriging, _check, _dp, _i64, _load_lib
+
+
+_TRANSFORM_ALIASES = {"linear": "lin", "bounded": "exp", "power": "pow"}
 
 
 def _split(coord, time):
@@ -72,10 +77,16 @@ class SpaceTimeKriging(Kriging):
 
     def set_st_model(self, model: str = "sum_metric", transform: str = "linear", at: float = 1.0,
                      time_nugget: float = 0.0, time_sill: float = 1.0, k_ps: float = 0.0):
+        """Global space-time model (reference kriging_st.py:356-391): the
+        sum-metric Cs + Ct + sum_k sill_st_k shape_k(sqrt(h_k^2 + f(dt)^2)),
+        f(dt) = time_nugget + time_sill (1 - corefunc(transform, |dt| / at)),
+        or the product-sum k_ps Cs Ct + Cs + Ct.  transform aliases: linear
+        -> lin, bounded -> exp, power -> pow."""
         model = model.lower()
         if model not in ("sum_metric", "product_sum"):
             raise ValueError("model must be 'sum_metric' or 'product_sum'")
-        _check(_load_lib().krige_st_set_st_model(self._handle, model.encode(), transform.lower()[:3].encode(),
+        t = _TRANSFORM_ALIASES.get(str(transform).lower(), str(transform).lower())
+        _check(_load_lib().krige_st_set_st_model(self._handle, model.encode(), t.encode(),
                                                  at, time_nugget, time_sill, k_ps), "krige_st_set_st_model")
 
     def set_obs(self, ivar: int, coord, value, time=None, variance=None, nmax: Optional[int] = None,
@@ -143,7 +154,11 @@ def from_st_spec(spec, **kw):
     description (``krigekit_amd.synth.st``)."""
     k = SpaceTimeKriging(unbias=spec.get("unbias", 1), cross_validation=bool(spec.get("cross_validation", False)),
                          **kw)
-    k.set_st_model("product_sum", k_ps=spec["k_ps"])
+    if spec.get("st_model", "product_sum") == "sum_metric":
+        k.set_st_model("sum_metric", spec.get("transform", "lin"), at=spec.get("at", 1.0),
+                       time_nugget=spec.get("time_nugget", 0.0), time_sill=spec.get("time_sill", 1.0))
+    else:
+        k.set_st_model("product_sum", k_ps=spec["k_ps"])
     o = spec["obs"][0]
     k.set_obs(1, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"),
               maxdist=o.get("maxdist"), sk_mean=o.get("sk_mean", 0.0))
@@ -155,6 +170,8 @@ def from_st_spec(spec, **kw):
         k.set_vgm(**{key: val for key, val in v.items() if key != "product"})
     for v in spec["vgm_t"]:
         k.set_vgm_temporal(1, 1, v["vtype"], nugget=v.get("nugget", 0.0), sill=v.get("sill", 1.0), at_k=v["at_k"])
+    if spec.get("sill_st") is not None:
+        k.set_vgm_joint_sills(1, 1, *spec["sill_st"])
     for s in spec["search"]:
         k.set_search(**s)
     return k

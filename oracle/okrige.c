@@ -922,17 +922,65 @@ typedef struct {
   const model_t *Mt;    /* temporal structures */
   double k_ps, cov0;    /* cov0 of the (ST) model */
   double time_at;
+  /* sum-metric: joint sills and the temporal transform */
+  int st_sum, t_vtype;
+  double t_nug, t_sill, t_at;
+  const double *sill_st;
 } ctx_t;
+
+/* f_time_vgm_st, variogram_st.f90:126-141: f(0) = 0 (|dt| <= epsilon),
+ * otherwise max(nugget + sill (1 - corefunc(|dt| / at)), 0) */
+static double f_time(const ctx_t *C, double dt) {
+  if (fabs(dt) <= DBL_EPSILON) return 0.0;
+  double adt = fabs(dt) / C->t_at;
+  double dw = C->t_nug + C->t_sill * (1.0 - corefunc(C->t_vtype, adt));
+  return dw > 0.0 ? dw : 0.0;
+}
 
 /* COV / COV_ST: cov_lag or, for space-time, the product-sum
  * k_ps Cs(lag_s) Ct(|dt|) + Cs(lag_s) + Ct(|dt|) (variogram_st.f90:222-282;
- * the temporal lag is the 1-D vector [|dt|, 0, 0]) */
+ * the temporal lag is the 1-D vector [|dt|, 0, 0]) or the sum-metric
+ * Cs + Ct + sum_k sill_st[k] shape_k(h_st), h_st = sqrt(h_k^2 + f(dt)^2) with
+ * h_k structure k's anisotropic lag (variogram_st.f90:178-194 analytic,
+ * 241-261 tables: the shape from the structure's own table divided by its
+ * sill when the spatial model has per-structure tables and the sill is
+ * positive, analytic otherwise) */
 static double ctx_cov(const ctx_t *C, const double *lag) {
   if (!C->st) return cov_lag(C->M, lag);
   double ls[3] = {lag[0], lag[1], lag[2]};
   double lt[3] = {fabs(lag[3]), 0.0, 0.0};
   double cs = cov_lag(C->M, ls), ct = cov_lag(C->Mt, lt);
-  return C->k_ps * cs * ct + cs + ct;
+  if (!C->st_sum) return C->k_ps * cs * ct + cs + ct;
+  const model_t *M = C->M;
+  double res = cs + ct;
+  double dw = f_time(C, lag[3]);
+  for (int k = 0; k < M->nstruct; ++k) {
+    double hs = aniso_h(M->mat[k], 3, ls);
+    double hst = sqrt(hs * hs + dw * dw);
+    if (M->table_mode && !M->levelB && M->tabA[k] && M->s[k].sill > 0.0)
+      res = res + C->sill_st[k] * tab_lookup(M, M->tabA[k], hst, has_tail(M->s[k].vtype), tail_comp, k) / M->s[k].sill;
+    else
+      res = res + C->sill_st[k] * corefunc(M->s[k].vtype, hst);
+  }
+  return res;
+}
+
+/* compute_cov0_vgm_st, variogram_st.f90:344-375 */
+static double st_cov0(const ctx_t *C) {
+  if (!C->st_sum) return C->k_ps * C->M->cov0 * C->Mt->cov0 + C->M->cov0 + C->Mt->cov0;
+  double c0 = C->M->cov0 + C->Mt->cov0;
+  for (int k = 0; k < C->M->nstruct; ++k)
+    if (C->M->s[k].vtype != 0) c0 = c0 + C->sill_st[k];
+  return c0;
+}
+
+static void ctx_set_st(ctx_t *C, const okr_problem *P) {
+  C->st_sum = P->st_sum;
+  C->t_vtype = P->t_vtype;
+  C->t_nug = P->t_nugget;
+  C->t_sill = P->t_sill;
+  C->t_at = P->t_at;
+  C->sill_st = P->sill_st;
 }
 
 static void rot_pt(const double *R, int ndim, const double *x, double *y) {
@@ -1183,6 +1231,7 @@ int okr_cov(const okr_problem *P, int n, const double *lags, double *out) {
     C.st = 1;
     C.Mt = &Mt;
     C.k_ps = P->k_ps;
+    ctx_set_st(&C, P);
     nl = 4;
   }
   for (int i = 0; i < n; ++i) {
@@ -1227,7 +1276,8 @@ int okr_solve(const okr_problem *P, okr_result *R) {
     C.Mt = &Mt;
     C.k_ps = P->k_ps;
     C.time_at = P->time_at;
-    C.cov0 = P->k_ps * M.cov0 * Mt.cov0 + M.cov0 + Mt.cov0;
+    ctx_set_st(&C, P);
+    C.cov0 = st_cov0(&C);
   }
   C.nmax = eff_nmax(P);
   C.need_search = P->nobs > C.nmax;

@@ -51,12 +51,19 @@ typedef struct {
    * (x, y, z, t) with ndim = 3; the search runs on (x, y, z, t * time_at) and
    * the covariance is the product-sum k_ps Cs Ct + Cs + Ct
    * (variogram_st.f90:222-282) with the spatial structures above and the
-   * isotropic 1-D temporal structures below */
+   * isotropic 1-D temporal structures below; with st_sum the sum-metric
+   * Cs + Ct + sum_k sill_st[k] shape_k(sqrt(h_k^2 + f(dt)^2)) instead
+   * (variogram_st.f90:178-194, 241-261), f(dt) = max(t_nugget + t_sill
+   * (1 - corefunc(t_vtype, |dt| / t_at)), 0) and f(0) = 0 (:126-141) */
   int st;
   double time_at;
   double k_ps;
   int nstruct_t;
   const okr_struct *structs_t;
+  int st_sum;
+  int t_vtype;
+  double t_nugget, t_sill, t_at;
+  const double *sill_st;    /* nstruct joint sills */
 } okr_problem;
 
 typedef struct {

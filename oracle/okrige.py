@@ -42,7 +42,9 @@ class Problem(ctypes.Structure):
         ("table_mode", ctypes.c_int), ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double),
         ("neglect_error", ctypes.c_int), ("nthread", ctypes.c_int),
         ("st", ctypes.c_int), ("time_at", ctypes.c_double), ("k_ps", ctypes.c_double),
-        ("nstruct_t", ctypes.c_int), ("structs_t", ctypes.POINTER(Struct))]
+        ("nstruct_t", ctypes.c_int), ("structs_t", ctypes.POINTER(Struct)),
+        ("st_sum", ctypes.c_int), ("t_vtype", ctypes.c_int), ("t_nugget", ctypes.c_double),
+        ("t_sill", ctypes.c_double), ("t_at", ctypes.c_double), ("sill_st", _pd)]
 
 
 class Result(ctypes.Structure):
@@ -156,6 +158,19 @@ def make_problem(spec, table_mode=True, nthread=8, bounds=None, neglect_error=Tr
                           for v in spec["vgm_t"]])
         P.nstruct_t = len(spec["vgm_t"])
         P.structs_t = ctypes.cast(h.stt, ctypes.POINTER(Struct))
+        if spec.get("st_model", "product_sum") == "sum_metric":
+            # sum-metric: joint sills per spatial structure and the temporal
+            # transform f(dt) (transform id as vgmfunc numbers it)
+            P.st_sum = 1
+            t = str(spec.get("transform", "lin")).lower()
+            P.t_vtype = VTYPES[{"linear": "lin", "bounded": "exp", "power": "pow"}.get(t, t)[:3]]
+            P.t_nugget = spec.get("time_nugget", 0.0)
+            P.t_sill = spec.get("time_sill", 1.0)
+            P.t_at = spec.get("at", 1.0)
+            h.sst = np.ascontiguousarray(spec["sill_st"], dtype=np.float64)
+            if len(h.sst) != P.nstruct:
+                raise ValueError("sill_st needs one joint sill per spatial structure")
+            P.sill_st = _d(h.sst)
     return P, h
 
 

@@ -425,6 +425,36 @@ def st_cases():
             st_small(nstation=30, ndays=20, cv=True, nmax=12, name="st_loo")]
 
 
+def st_sum(base, sill_st, transform="lin", at=20.0, time_nugget=0.0, time_sill=1.0, vgm=None, name="st_sum"):
+    """a space-time problem switched to the sum-metric model
+    (variogram_st.f90:178-194): joint sills per spatial structure and the
+    temporal transform f(dt)"""
+    p = dict(base)
+    p.update(st_model="sum_metric", transform=transform, at=at, time_nugget=time_nugget, time_sill=time_sill,
+             sill_st=list(sill_st), name=name)
+    if vgm is not None:
+        p["vgm"] = vgm
+    return p
+
+
+def st_sum_cases():
+    """sum-metric variants: linear transform that saturates within the data's
+    time span (one spatial structure: a composite table, analytic joint term);
+    bounded transform with a temporal nugget over two spatial structures of
+    different anisotropy (per-structure tables, joint term from them); a
+    nugget structure with joint sill 0 beside a spherical one (the nested
+    case of tests/test_spacetime_kriging.py:474-492); octant search"""
+    two = [dict(ivar=1, jvar=1, vtype="exp", nugget=0.05, sill=0.6, a_major=50.0, a_minor1=30.0, azimuth=30.0),
+           dict(ivar=1, jvar=1, vtype="sph", nugget=0.0, sill=0.3, a_major=90.0)]
+    nug = [dict(ivar=1, jvar=1, vtype="nug", nugget=0.1, sill=0.0, a_major=1.0),
+           dict(ivar=1, jvar=1, vtype="sph", nugget=0.0, sill=0.7, a_major=80.0, a_minor1=60.0)]
+    return [st_sum(st_small(), [0.3], name="stsm_linear"),
+            st_sum(st_small(nmax=20), [0.25, 0.15], transform="bounded", at=10.0, time_nugget=0.1, time_sill=0.9,
+                   vgm=two, name="stsm_bounded_tabA"),
+            st_sum(st_small(nmax=16), [0.0, 0.3], transform="sph", at=30.0, vgm=nug, name="stsm_nested_nug"),
+            st_sum(st_small(sector=True), [0.4], transform="gau", at=15.0, time_sill=2.0, name="stsm_octant")]
+
+
 def ladder():
     """the reference's failure ladder (kriging_base.F90:3244-3398) on exact
     zeros: observations on a 10 m lattice, a spherical model whose range

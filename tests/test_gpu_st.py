@@ -105,3 +105,71 @@ def test_st_full_size():
     assert same_lists(sel, o)
     ee, ev = max_errors(sel, o)
     assert ee < 1e-10 and ev < 1e-10, (ee, ev)
+
+
+SM_CASES = problems.st_sum_cases()
+SM_IDS = [c["name"] for c in SM_CASES]
+
+
+@pytest.mark.parametrize("mode", ["table", "analytic"])
+@pytest.mark.parametrize("spec", SM_CASES, ids=SM_IDS)
+def test_st_sum_metric_matches_oracle(spec, mode):
+    """the sum-metric model (oracle pinned in test_st_sum_metric.py)"""
+    print("case", spec["name"], mode, flush=True)
+    o = run_oracle(spec, table_mode=(mode == "table"))
+    g = run_gpu_st(spec, table_mode=(mode == "table"))
+    assert same_lists(g, o)
+    ee, ev = max_errors(g, o)
+    assert ee < 1e-10 and ev < 1e-10, (ee, ev)
+
+
+@pytest.mark.parametrize("spec", SM_CASES[:2], ids=SM_IDS[:2])
+def test_st_sum_metric_fp32r(spec):
+    print("case", spec["name"], "fp32r", flush=True)
+    o = run_oracle(spec)
+    g = run_gpu_st(spec, precision="fp32r")
+    assert same_lists(g, o)
+    ee, ev = max_errors(g, o)
+    assert ee < 1e-6 and ev < 1e-6, (ee, ev)
+
+
+def test_st_sum_metric_setup_errors():
+    """set_st_model / set_vgm_joint_sills / solve checks of the reference
+    (kriging_st.F90:143-165, variogram_st.f90:325-337, 388-402)"""
+    from krigekit_amd.kriging_st import SpaceTimeKriging
+    spec = problems.st_small(nstation=10, ndays=10, grid_side=2, ntimes=1, name="st_err")
+    o = spec["obs"][0]
+    k = SpaceTimeKriging()
+    for bad in (dict(transform="hol"), dict(transform="cyc"), dict(at=0.0), dict(time_sill=-1.0),
+                dict(transform="xyz")):
+        args = dict(model="sum_metric", transform="linear", at=5.0)
+        args.update(bad)
+        with pytest.raises(RuntimeError):
+            k.set_st_model(**args)
+    k.set_st_model("sum_metric", "bounded", at=5.0)
+    k.set_obs(1, o["coord"], o["value"], nmax=o["nmax"])
+    k.set_vgm(**spec["vgm"][0])
+    k.set_vgm_temporal(1, 1, "exp", sill=0.5, at_k=5.0)
+    with pytest.raises(RuntimeError, match="cs%nstruct"):
+        k.set_vgm_joint_sills(1, 1, 0.3, 0.2)
+    k.set_grid(spec["grid"])
+    k.set_search(1, time_at=10.0)
+    with pytest.raises(RuntimeError, match="sill_st not set"):
+        k.solve()
+    k.set_vgm_joint_sills(1, 1, 0.3)
+    k.solve()
+    e, v = k.get_results()
+    assert np.all(np.isfinite(e)) and np.all(v >= 0)
+    k.close()
+
+
+@pytest.mark.parametrize("time_at,expect", [(0.1, 10.0), (100.0, 20.0)])
+def test_st_sum_metric_time_at_selects_nearest_pair(time_at, expect):
+    """tests/test_spacetime_kriging.py:66-84, 243-255 of the reference (its
+    probe is sum-metric, linear transform, at 100, time_sill 10)"""
+    obs = [[0, 0, 0, 10], [0, 0, 0, 11], [3, 0, 0, 0], [3.1, 0, 0, 0], [100, 0, 0, 0], [0, 100, 0, 0],
+           [0, 0, 100, 0]]
+    spec = problems.st_sum(_kat(obs, [10, 10, 20, 20, -1, -2, -3], [[0, 0, 0, 0]], nmax=2, time_at=time_at), [0.3],
+                           transform="linear", at=100.0, time_sill=10.0, name="kat_sm")
+    g = run_gpu_st(spec)
+    assert g["estimate"][0] == pytest.approx(expect)
