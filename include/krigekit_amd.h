@@ -45,7 +45,8 @@ int krige_initialize(int64_t handle, int ndim, int nvar, int ndrift, int unbias,
 int krige_set_obs(int64_t handle, int ivar, int nobs, int ndim, const double *coord,
                   const double *value, const double *variance, int nmax, double maxdist,
                   double sk_mean);
-/* src/libkriging/kriging_capi_common.F90:178: returns 1, not provided */
+/* src/libkriging/kriging_capi_common.F90:178: external drift values at the
+ * observations, drift (ndrift, nobs) column-major */
 int krige_set_obs_drift(int64_t handle, int ivar, int ndrift, int nobs, const double *drift);
 /* src/libkriging/kriging_capi_common.F90:136 */
 int krige_update_obs_value(int64_t handle, int ivar, int nobs, const double *value);
@@ -76,9 +77,14 @@ int krige_set_grid_block(int64_t handle, int block_type, int ngrid, int ndim, co
                          const double *localnugget);
 /* src/libkriging/kriging_capi_common.F90:195 */
 int krige_set_grid_cv(int64_t handle);
-/* src/libkriging/kriging_capi.F90 (krige_set_grid_drift): returns 1, not provided */
+/* src/libkriging/kriging_capi.F90 (krige_set_grid_drift): drift values at
+ * the targets, (ndrift, nblocks) column-major; ivar < 1 for every variable */
 int krige_set_grid_drift(int64_t handle, int ivar, int ndrift, int nblocks, const double *drift);
-/* src/libkriging/kriging_capi.F90 (krige_set_sim): returns 1, not provided */
+/* src/libkriging/kriging_capi.F90:397-443: sequential Gaussian simulation
+ * path (int, 1-based permutation of the blocks) and standard normal draws
+ * (double, (nsim, nvar, nblocks) column-major); NULL: drawn by the library's
+ * own generator from the seed (the reference's Fortran generator is not
+ * reproduced).  Simulation is provided for nvar = 1, point targets. */
 int krige_set_sim(int64_t handle, int nblocks, const void *randpath, int nsim, int nvar,
                   const void *sample);
 /* src/libkriging/kriging_capi.F90:448-467 (builds the kd-tree) */
@@ -176,7 +182,8 @@ int krige_st_set_vgm(int64_t handle, int ivar, int jvar, const char *vtype, doub
 /* kriging_st_capi.f90:288-307: temporal structure, isotropic range at_k */
 int krige_st_set_vgm_temporal(int64_t handle, int ivar, int jvar, const char *vtype, double nugget,
                               double sill, double at_k);
-/* kriging_st_capi.f90:309-327: returns 1 (sum-metric not provided) */
+/* kriging_st_capi.f90:309-327: sum-metric joint sills, one per spatial
+ * structure of the pair added so far (nvar = 1) */
 int krige_st_set_vgm_joint_sills(int64_t handle, int ivar, int jvar, int n, const double *sills);
 /* kriging_st_capi.f90:330-353: coord (3, ngrid), time (ngrid) */
 int krige_st_set_grid(int64_t handle, int ngrid, const double *coord, const double *time,
@@ -243,11 +250,13 @@ int kk_autotune(int64_t handle, const char *path, int reps);
 /* replace the library's schedule table by the file at path (NULL or empty:
  * every bucket on its own width with a full grid); drops captured graphs */
 int kk_set_tuning(const char *path);
-/* out[13]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
+/* out[15]: search, bucket, solve, re-solve/fallback, total (ms), FP64 re-solved
  * nodes, Bunch-Kaufman fallback nodes, queries decided by the exact search kernel, fast
  * search launches (0: none), fast search kernel ms (all launches), exact
  * search kernel ms, LOO-gradient kernel ms, weight-store pass (store_weight)
- * or stored-weight replay (use_old_weight) ms.
+ * or stored-weight replay (use_old_weight) ms, sequential-simulation
+ * recurrence ms and its dependency levels (nsim > 0; the other entries then
+ * describe the pass that solved every block's system).
  * Times come from HIP events on the stream the kernels run on.  When the fast
  * search leaves queries to the exact kernel, that kernel runs on a second
  * stream while the other blocks are solved: it is outside "search" and its

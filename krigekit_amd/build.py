@@ -46,6 +46,7 @@ UNITS = [
     ("cokrige.hip", ["-DKK_MV_DIM=3", "-DKK_MV_W=1"], "cokrige_d3w.o"),
     ("sets.hip", [], "sets.o"),
     ("replay.hip", [], "replay.o"),
+    ("sgsim.hip", [], "sgsim.o"),
     ("capi.cpp", ["-x", "hip"], "capi.o"),
 ]
 

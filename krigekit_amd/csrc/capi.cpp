@@ -496,6 +496,7 @@ struct Obs {
   int n = 0;
   std::vector<double> coord, value, var;
   int nmax = INT_MAX;
+  int nmax_in = INT_MAX;  // as given to set_obs (nmax is normalised at set_search)
   double maxdist2 = HUGE_VAL;
   double sk_mean = 0.0;
   bool sector = false, search_set = false, need_search = false, aniso = false;
@@ -606,6 +607,16 @@ struct Krige {
   int reuse_mode = -1;
   int item_max = kk::MAX_ITEM;
   int search_mode = 2;          // 0 exact, 1 per-lane fast, 2 wave-cooperative fast
+  // sequential simulation (nsim > 0): the path (0-based block of each path
+  // position), the standard normal draws (draw s of path position p at
+  // s + nsim p) and the realisations (block b, realisation s at b + nblock s);
+  // simlim >= 0 marks the inner pass of a simulation (its search sees the
+  // points before each block on the path, simlim = the observation count)
+  std::vector<int> sgs_path;
+  std::vector<double> sgs_sample, est_sim;
+  bool sim_set = false;
+  int simlim = -1;
+  DBuf d_sgs_val, d_sgs_sample, d_sgs_order;
   bool reorder = true;          // Morton order of queries for the wave-cooperative search
   bool chunked = true;          // with Morton order: two search chunks, the first one's solve overlapped
   bool perm_ok = false;
@@ -1205,6 +1216,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
   sa.mmax = mmax;
   sa.nnear = B.nnear;
   sa.inear = B.inear;
+  sa.simlim = K.simlim;
   if (!O.need_search) {
     // rotated_dists rotates (obs - query) of the unrotated coordinates;
     // space-time adds the scaled time difference (kriging_st.F90:609-612)
@@ -1327,6 +1339,10 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     }
     HIPCHK(hipEventRecord(K.ev[11], xs), W);
     nx = overlap ? nexact : 0;
+    // simulation pass: drop simulated neighbours that duplicate an
+    // observation of the same row (the exact search ran on st)
+    if (K.simlim >= 0)
+      HIPCHK(kk::launch_sgsim_lists(B.nnear, B.inear, mmax, nb, K.simlim, nd, K.d_ocoord.as<double>(), st), W);
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
   HIPCHK(hipEventRecord(K.ev[2], st), W);
@@ -2731,8 +2747,175 @@ bool do_replay(Krige& K, bool fetch) {
 }
 
 
+// Sequential Gaussian simulation, one variable (kriging_base.F90:1226-1316,
+// 2055-2181, 3405-3505; kriging.F90:535-612).  The reference's search pool
+// is the observations followed by the blocks in path order, each block
+// searching the points before it (kdtree2_n_nearest_maxidx); its kriging
+// weights depend only on that geometry.  So one inner pass of the ordinary
+// pipeline -- a handle of its own over that pool, the blocks in path order
+// as targets, the exact search limited per block, co-located simulated
+// neighbours dropped, FP64 weights stored -- gives every block's weights and
+// variance, and the realisations follow by the recurrence over dependency
+// levels (k_sgsim_level).
+bool build_search(Krige* K, Obs& O, bool force_tree);
+
+bool do_solve_sgsim(Krige& K, bool fetch) {
+  const char* W = "t_kriging_base%solve_base";
+  if (K.nvar != 1 || K.st || !K.soff.empty() || K.ndrift != 0 || K.weight_correction || K.store_weight ||
+      K.varying_vgm) {
+    set_error(W, "sequential simulation is provided for one variable and point targets, without drift, weight "
+                 "correction or a weight store");
+    return false;
+  }
+  if (!K.sim_set) {
+    set_error("t_kriging%set_search", "set_sim() needs to be called before set_search().");
+    return false;
+  }
+  Obs& O = K.obs[0];
+  if (O.n == 0 || !O.search_set || K.nblock == 0 || K.vgm[0].empty()) {
+    set_error(W, "set_obs, set_grid, set_vgm and set_search are needed before solve()");
+    return false;
+  }
+  const int nd = K.ndim, nb = K.nblock, nobs = O.n, ns = K.nsim;
+  // the pool: observations, then the blocks in path order (no observation
+  // error, value 0: their part of each estimate is added per realisation)
+  const int np = nobs + nb;
+  std::vector<double> pc((size_t)nd * np), pv(np, 0.0), pe(np, 0.0), bc((size_t)nd * nb), rs(nb), ln(nb);
+  std::copy(O.coord.begin(), O.coord.end(), pc.begin());
+  std::copy(O.value.begin(), O.value.end(), pv.begin());
+  std::copy(O.var.begin(), O.var.end(), pe.begin());
+  for (int p = 0; p < nb; ++p) {
+    const int b = K.sgs_path[p];
+    for (int k = 0; k < nd; ++k) {
+      bc[k + (size_t)nd * p] = K.bcoord[k + (size_t)nd * b];
+      pc[k + (size_t)nd * (nobs + p)] = K.bcoord[k + (size_t)nd * b];
+    }
+    rs[p] = K.rangescale.empty() ? 1.0 : K.rangescale[b];
+    ln[p] = K.localnugget.empty() ? 0.0 : K.localnugget[b];
+  }
+  int64_t hi = 0;
+  if (krige_create(&hi)) return false;
+  // the inner handle goes when this returns, keeping any error set here
+  struct Drop {
+    int64_t h;
+    ~Drop() {
+      const int e = g_ierr;
+      const std::string m = g_msg;
+      krige_destroy(&h);
+      g_ierr = e;
+      g_msg = m;
+    }
+  } drop{hi};
+  const double huge_b[2] = {-DBL_MAX, DBL_MAX};
+  if (krige_initialize(hi, nd, 1, 0, K.unbias, 0, K.aniso_search, 0, 0, 1, 0, 0, K.neglect_error, 0, 1, 0, 0, "",
+                       huge_b, K.seed))
+    return false;
+  Krige& I = *get_obj(hi);
+  I.precision = kk::PREC_FP64;
+  I.pivot_tol = K.pivot_tol;
+  I.table_mode = K.table_mode;
+  I.search_mode = 0;
+  I.reorder = false;
+  I.reuse_mode = 0;
+  // the pool is set directly: a block may lie on an observation (set_sim
+  // extends the coordinates without set_obs's duplicate check)
+  Obs& IO = I.obs[0];
+  IO = Obs();
+  IO.n = np;
+  IO.coord = std::move(pc);
+  IO.value = std::move(pv);
+  IO.var = std::move(pe);
+  IO.nmax = IO.nmax_in = O.nmax_in;
+  IO.maxdist2 = O.maxdist2;
+  IO.sk_mean = O.sk_mean;
+  if (krige_set_grid(hi, nb, nd, bc.data(), rs.data(), ln.data())) return false;
+  I.vgm = K.vgm;
+  IO.sector = O.sector;
+  IO.aniso = O.aniso;
+  std::memcpy(IO.rot, O.rot, sizeof IO.rot);
+  if (!build_search(&I, IO, true)) return false;
+  I.simlim = nobs;
+  if (!validate_for_solve(I) || !stage_inputs(I) || !run_pipeline(I)) return false;
+  // dependency levels: a block's level is one above its simulated neighbours'
+  const int mmax = I.mmax;
+  std::vector<int> nn(nb), in((size_t)nb * mmax), lev(nb), order(nb);
+  hipStream_t st = I.stream;
+  HIPCHK(hipMemcpyAsync(nn.data(), I.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipMemcpyAsync(in.data(), I.d_inear.p, sizeof(int) * in.size(), hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  int nlev = 0;
+  for (int p = 0; p < nb; ++p) {
+    int l = 0;
+    for (int k = 0; k < nn[p]; ++k) {
+      const int c = in[(size_t)p * mmax + k];
+      if (c >= nobs) l = std::max(l, lev[c - nobs] + 1);
+    }
+    lev[p] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<int> loff(nlev + 1, 0);
+  for (int p = 0; p < nb; ++p) ++loff[lev[p] + 1];
+  for (int l = 0; l < nlev; ++l) loff[l + 1] += loff[l];
+  {
+    std::vector<int> fill(loff.begin(), loff.end() - 1);
+    for (int p = 0; p < nb; ++p) order[fill[lev[p]]++] = p;
+  }
+  if (!upload(K.d_sgs_order, order, W) || !upload(K.d_sgs_sample, K.sgs_sample, W) ||
+      !K.d_sgs_val.ensure(sizeof(double) * (size_t)ns * nb)) {
+    if (!failed()) set_error(W, "device allocation failed (simulation)");
+    return false;
+  }
+  kk::SgsimArgs a{};
+  a.nb = nb;
+  a.nobs = nobs;
+  a.nsim = ns;
+  a.mmax = mmax;
+  a.unbias = K.unbias;
+  a.sk_mean = O.sk_mean;
+  a.bound_lo = K.bounds[0];
+  a.bound_hi = K.bounds[1];
+  a.order = K.d_sgs_order.as<int>();
+  a.nnear = I.d_nnear.as<int>();
+  a.inear = I.d_inear.as<int>();
+  a.weights = I.d_wout.as<double>();
+  a.var = I.d_var.as<double>();
+  a.ovalue = I.d_ovalue.as<double>();
+  a.sample = K.d_sgs_sample.as<double>();
+  a.val = K.d_sgs_val.as<double>();
+  HIPCHK(hipEventRecord(I.ev[3], st), W);
+  for (int l = 0; l < nlev; ++l) HIPCHK(kk::launch_sgsim_level(a, loff[l], loff[l + 1], st), W);
+  HIPCHK(hipEventRecord(I.ev[4], st), W);
+  std::vector<double> val((size_t)ns * nb), var(nb);
+  HIPCHK(hipMemcpyAsync(val.data(), a.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipMemcpyAsync(var.data(), a.var, sizeof(double) * nb, hipMemcpyDeviceToHost, st), W);
+  HIPCHK(hipStreamSynchronize(st), W);
+  float sim_ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&sim_ms, I.ev[3], I.ev[4]), W);
+  // back to block order (reorder_sgsim_base, kriging_base.F90:2268-2282)
+  K.est_sim.assign((size_t)ns * nb, 0.0);
+  K.var.assign(nb, 0.0);
+  K.est.assign(nb, 0.0);
+  for (int p = 0; p < nb; ++p) {
+    const int b = K.sgs_path[p];
+    for (int s2 = 0; s2 < ns; ++s2) K.est_sim[b + (size_t)nb * s2] = val[s2 + (size_t)ns * p];
+    K.var[b] = var[p];
+    K.est[b] = val[(size_t)ns * p];
+  }
+  for (int i = 0; i < (int)(sizeof K.timing / sizeof K.timing[0]); ++i) K.timing[i] = I.timing[i];
+  K.timing[13] = sim_ms;
+  K.timing[14] = nlev;
+  for (int i = 0; i < 5; ++i) K.stats[i] = I.stats[i];
+  (void)fetch;
+  return true;
+}
+
 bool do_solve(Krige& K, bool fetch) {
   K.solved = false;
+  if (K.nsim > 0) {
+    if (!do_solve_sgsim(K, fetch)) return false;
+    K.solved = true;
+    return true;
+  }
   if (K.use_old_weight) {
     if (!do_replay(K, fetch)) return false;
     K.solved = true;
@@ -2768,6 +2951,40 @@ bool check_solved(Krige* K, const char* caller) {
 int unsupported(const char* name, const char* what) {
   set_error(name, what);
   return ierr();
+}
+
+// The search structures of one variable (its sector, anisotropy and rotation
+// already set): nmax against the point count, search coordinates and the
+// kd-tree; force_tree builds the tree even when every point fits in nmax.
+bool build_search(Krige* K, Obs& O, bool force_tree) {
+  O.nmax = normalize_nmax(O.nmax, O.n);
+  O.need_search = O.n > O.nmax || force_tree;
+  O.tree = HostTree();
+  O.rcoord.clear();
+  if (O.aniso || K->st) {
+    // search coordinates: spatial part rotated/scaled, space-time time axis
+    // scaled by time_at (kriging_st.F90:387-395)
+    O.rcoord.resize(O.coord.size());
+    kk::Rot3 R;
+    std::memcpy(R.m, O.rot, sizeof R.m);
+    const int nd = K->ndim;
+    for (int i = 0; i < O.n; ++i) {
+      const double* c = &O.coord[(size_t)nd * i];
+      double* r = &O.rcoord[(size_t)nd * i];
+      if (nd == 1) R.apply<1>(c, r);
+      else if (nd == 2) R.apply<2>(c, r);
+      else R.apply<3>(c, r);
+      if (K->st) r[3] = c[3] * O.time_at;
+    }
+  }
+  if (O.need_search) {
+    if (!build_tree(O.tree, K->ndim, O.n, (O.aniso || K->st) ? O.rcoord.data() : O.coord.data())) {
+      set_error("kdtree2_create", "kd-tree construction did not terminate (degenerate coordinates)");
+      return false;
+    }
+  }
+  O.search_set = true;
+  return true;
 }
 
 }  // namespace
@@ -3008,6 +3225,7 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
   O.var.assign(nobs, 0.0);
   if (variance) O.var.assign(variance, variance + nobs);
   O.nmax = nmax;
+  O.nmax_in = nmax;
   O.maxdist2 = maxdist * maxdist;
   O.sk_mean = sk_mean;
   if (K->ndrift > 0) O.drift.assign((size_t)K->ndrift * nobs, 0.0);
@@ -3342,10 +3560,104 @@ int krige_set_grid_drift(int64_t h, int ivar, int ndrift_c, int nblocks, const d
   return 0;
 }
 
-int krige_set_sim(int64_t h, int, const void*, int, int, const void*) {
+int krige_set_sim(int64_t h, int nblocks, const void* randpath, int nsim_c, int nvar_c, const void* sample) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_set_sim", "sequential simulation is not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  // set_sim_base, kriging_base.F90:1226-1316
+  const char* W = "t_kriging_base%set_sim";
+  if (K->nsim == 0) return 0;
+  if (!K->initialized) {
+    set_error(W, "call initialize() before set_sim().");
+    return ierr();
+  }
+  if (K->nblock == 0) {
+    set_error(W, "Grid needs to be set first.");
+    return ierr();
+  }
+  for (const Obs& O : K->obs) {
+    if (O.n == 0) {
+      set_error(W, "Observations need to be set first.");
+      return ierr();
+    }
+    if (O.nmax_in > (int)(INT_MAX * 0.001)) {
+      set_error(W, "`nmax` must be set for all observations for SGSIM.");
+      return ierr();
+    }
+  }
+  const int nb = K->nblock, ns = K->nsim;
+  // the library's own generator when the caller gives no path or draws
+  // (xoshiro256** seeded by splitmix64 from seed; normals by Box-Muller): the
+  // reference's Fortran generator is not reproduced
+  uint64_t st[4], z = (uint64_t)(int64_t)K->seed;
+  for (auto& w : st) {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    w = x ^ (x >> 31);
+  }
+  auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+  auto next = [&]() {
+    const uint64_t r = rotl(st[1] * 5, 7) * 9, t = st[1] << 17;
+    st[2] ^= st[0];
+    st[3] ^= st[1];
+    st[1] ^= st[2];
+    st[0] ^= st[3];
+    st[2] ^= t;
+    st[3] = rotl(st[3], 45);
+    return r;
+  };
+  auto unif = [&]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+  std::vector<int> path(nb);
+  if (randpath) {
+    if (nblocks != nb) {
+      set_error(W, "size(randpath) /= nblock");
+      return ierr();
+    }
+    const int* rp = static_cast<const int*>(randpath);
+    std::vector<char> seen(nb, 0);
+    for (int p = 0; p < nb; ++p) {
+      if (rp[p] < 1 || rp[p] > nb || seen[rp[p] - 1]) {
+        set_error(W, "randpath must be a 1-based permutation of 1..nblocks");
+        return ierr();
+      }
+      seen[rp[p] - 1] = 1;
+      path[p] = rp[p] - 1;
+    }
+  } else {
+    for (int p = 0; p < nb; ++p) path[p] = p;
+    for (int p = nb - 1; p > 0; --p) std::swap(path[p], path[(int)(unif() * (p + 1)) % (p + 1)]);
+  }
+  std::vector<double> draws((size_t)ns * nb);
+  if (sample) {
+    if (nsim_c != ns) {
+      set_error(W, "size(sample, 1) /= nsim");
+      return ierr();
+    }
+    if (nvar_c != K->nvar) {
+      set_error(W, "size(sample, 2) /= nvar");
+      return ierr();
+    }
+    if (nblocks != nb) {
+      set_error(W, "size(sample, 3) /= nblock");
+      return ierr();
+    }
+    // (nsim, nvar, nblock) column-major: the first variable's draws
+    const double* sp = static_cast<const double*>(sample);
+    for (int p = 0; p < nb; ++p)
+      for (int s2 = 0; s2 < ns; ++s2) draws[s2 + (size_t)ns * p] = sp[s2 + (size_t)ns * K->nvar * p];
+  } else {
+    for (size_t i = 0; i < draws.size(); ++i) {
+      const double u1 = unif(), u2 = unif();
+      draws[i] = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPI * u2);
+    }
+  }
+  K->sgs_path = std::move(path);
+  K->sgs_sample = std::move(draws);
+  K->sim_set = true;
+  K->solved = false;
+  return 0;
 }
 
 int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azimuth, double dip, double plunge,
@@ -3373,40 +3685,15 @@ int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azi
   }
   O.sector = sector_search != 0;
   O.aniso = (std::fabs(anis1 - 1.0) > 1e-10 || std::fabs(anis2 - 1.0) > 1e-10) && K->aniso_search;
-  O.nmax = normalize_nmax(O.nmax, O.n);
-  O.need_search = O.n > O.nmax;
-  O.tree = HostTree();
-  O.rcoord.clear();
   std::fill(O.rot, O.rot + 9, 0.0);
   O.rot[0] = O.rot[4] = O.rot[8] = 1.0;
   if (O.aniso) calc_rotmat(azimuth, dip, plunge, anis1, anis2, O.rot);
-  if (O.aniso || K->st) {
-    // search coordinates: spatial part rotated/scaled, space-time time axis
-    // scaled by time_at (kriging_st.F90:387-395)
-    O.rcoord.resize(O.coord.size());
-    kk::Rot3 R;
-    std::memcpy(R.m, O.rot, sizeof R.m);
-    const int nd = K->ndim;
-    for (int i = 0; i < O.n; ++i) {
-      const double* c = &O.coord[(size_t)nd * i];
-      double* r = &O.rcoord[(size_t)nd * i];
-      if (nd == 1) R.apply<1>(c, r);
-      else if (nd == 2) R.apply<2>(c, r);
-      else R.apply<3>(c, r);
-      if (K->st) r[3] = c[3] * O.time_at;
-    }
-  }
-  if (O.need_search) {
-    if (!build_tree(O.tree, K->ndim, O.n, (O.aniso || K->st) ? O.rcoord.data() : O.coord.data())) {
-      set_error("kdtree2_create", "kd-tree construction did not terminate (degenerate coordinates)");
-      return ierr();
-    }
-  }
-  O.search_set = true;
+  if (!build_search(K, O, false)) return ierr();
   K->dev_inputs_ok = false;
   K->solved = false;
   return 0;
 }
+
 
 int krige_set_grad(int64_t h, int, int, int, const double*, const double*, const double*, const double*, int,
                    const double*) {
@@ -3468,7 +3755,8 @@ int krige_get_estimate(int64_t h, int nsim_c, int nblocks, double* out) {
   if (!check_solved(K, "krige_get_estimate")) return ierr();
   for (int s = 0; s < nsim_c; ++s)
     for (int b = 0; b < nblocks; ++b)
-      out[b + (size_t)nblocks * s] = mv_results(*K) ? K->est_mv[b] : K->est[b];
+      out[b + (size_t)nblocks * s] = K->nsim > 0 ? K->est_sim[b + (size_t)K->nblock * std::min(s, K->nsim - 1)]
+                                    : mv_results(*K) ? K->est_mv[b] : K->est[b];
   return 0;
 }
 
@@ -3481,7 +3769,8 @@ int krige_get_estimate_all(int64_t h, int nblocks, int nvar_c, int nsim_c, doubl
     for (int v = 0; v < nvar_c; ++v)
       for (int b = 0; b < nblocks; ++b)
         out[b + (size_t)nblocks * (v + (size_t)nvar_c * s)] =
-            mv_results(*K) ? K->est_mv[b + (size_t)nblocks * v] : K->est[b];
+            K->nsim > 0 ? K->est_sim[b + (size_t)K->nblock * std::min(s, K->nsim - 1)]
+            : mv_results(*K) ? K->est_mv[b + (size_t)nblocks * v] : K->est[b];
   return 0;
 }
 
@@ -4255,7 +4544,7 @@ int kk_get_timing(int64_t h, double* out) {
   clear_error();
   Krige* K = checked(h);
   if (!K) return ierr();
-  for (int i = 0; i < 13; ++i) out[i] = K->timing[i];
+  for (int i = 0; i < 15; ++i) out[i] = K->timing[i];
   return 0;
 }
 

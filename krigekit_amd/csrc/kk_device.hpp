@@ -132,4 +132,11 @@ hipError_t launch_collect_above(const int* nnear, int nblock, int limit, int* co
 hipError_t launch_collect(const int* flag, int nblock, int code, int* count, int* list, hipStream_t st,
                           const int* perm = nullptr);
 
+// sequential simulation: drop simulated neighbours that lie on one of the
+// row's observations (combined-index rows, in place)
+hipError_t launch_sgsim_lists(int* nnear, int* inear, int mmax, int nb, int nobs, int ndim, const double* coord,
+                              hipStream_t st);
+// one dependency level (order[l0..l1)) of every realisation
+hipError_t launch_sgsim_level(const SgsimArgs& a, int l0, int l1, hipStream_t st);
+
 }  // namespace kk

@@ -97,6 +97,28 @@ struct SearchArgs {
   const int* qlist;           // exact kernel: query ids to process (nullptr: q0..q0+nq)
   const int* dnq;             // exact kernel: device-resident query count (graph-captured passes)
   int* qflag;                 // fast kernel: 1 where the exact kernel must decide
+  // sequential simulation (exact kernel): query qid sees only the points with
+  // index < simlim + qid (the observations, then the blocks earlier on the
+  // path) and takes min(k, simlim + qid) candidates; -1 = every point
+  int simlim = -1;
+};
+
+// Sequential simulation, one dependency level at a time (blocks in path
+// order p; combined neighbour indices: observations, then blocks by path
+// position + nobs)
+struct SgsimArgs {
+  int nb, nobs, nsim, mmax;
+  int unbias;
+  double sk_mean;
+  double bound_lo, bound_hi;
+  const int* order;           // blocks grouped by level
+  const int* nnear;           // [nb]
+  const int* inear;           // [nb * mmax]
+  const double* weights;      // [nb * mmax], matching inear
+  const double* var;          // [nb] kriging variance
+  const double* ovalue;       // [nobs]
+  const double* sample;       // [nsim * nb], standard normal draw s of path position p at s + nsim p
+  double* val;                // [nsim * nb], realisation s of path position p at s + nsim p
 };
 
 // Leave-one-out gradient: per block, the estimate and its derivatives with

@@ -244,7 +244,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
 
   double ballsize = 1.7976931348623157e308;  // huge(1.0)
   int nfound = 0;
-  const int nn = a.k;
+  // sequential simulation: kdtree2_n_nearest_maxidx (kdtree2_maxidx.f90:
+  // 1928-1964, 2062-2108) over the points before this block on the path,
+  // asking for no more candidates than there are (kriging.F90:535-557)
+  const int lim = a.simlim >= 0 ? a.simlim + qid : 0x7fffffff;
+  const int nn = a.k < lim ? a.k : lim;
   traverse<DIM>(T, q, ballsize, [&](int l, int u) {
     // process_terminal_node, kdtree2_maxidx.f90:1510-1613.  The bucket's
     // points (at most 13, contiguous in tree order) are loaded up front so
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
           sd = p == r ? sds[r] : sd;
           idx = p == r ? ids[r] : idx;
         }
-        if (sd > ballsize) continue;
+        if (idx >= lim || sd > ballsize) continue;
         if (nfound < nn) {
           double newpri = pq_insert(h, nfound, sd, idx);
           if (nfound == nn) ballsize = newpri;

@@ -61,6 +61,7 @@ def _load_lib():
     sig("krige_set_grid", [_i64, _c_int, _c_int, _pd, _pd, _pd])
     sig("krige_set_grid_block", [_i64, _c_int, _c_int, _c_int, _pd, _c_int, _pi, _pd, _pd, _pd, _pd])
     sig("krige_set_grid_cv", [_i64])
+    sig("krige_set_sim", [_i64, _c_int, ctypes.c_void_p, _c_int, _c_int, ctypes.c_void_p])
     sig("krige_set_search", [_i64, _c_int] + [_c_double] * 5 + [_c_int])
     sig("krige_solve", [_i64, _c_int, _c_int])
     sig("krige_get_nblocks", [_i64, _pi])
@@ -304,6 +305,42 @@ class Kriging:
         _check(_load_lib().krige_set_grid_cv(self._handle), "krige_set_grid_cv")
         self._nblock = int(self._nobs[0])
 
+    def set_sim(self, randpath: Optional[np.ndarray] = None, sample: Optional[np.ndarray] = None):
+        """Sequential Gaussian simulation path and draws (reference
+        kriging.py:1134-1205): randpath a 1-based permutation of the blocks,
+        sample standard normal draws (nblocks,), (nblocks, nsim) or (nblocks,
+        nvar, nsim), the one of path position p at row p.  Either left out is
+        drawn by the library from the seed."""
+        if self.nsim <= 0:
+            raise ValueError("nsim must be > 0 when setting SGSIM parameters.")
+        nb = self._nblock
+        rp_ptr = s_ptr = None
+        keep = []
+        if randpath is not None:
+            rp = np.ascontiguousarray(np.asarray(randpath, dtype=np.int32).ravel())
+            if rp.size != nb:
+                raise ValueError(f"randpath length ({rp.size}) must match nblocks ({nb})")
+            if not np.array_equal(np.sort(rp), np.arange(1, nb + 1, dtype=np.int32)):
+                raise ValueError("randpath must be a 1-based permutation of 1..nblocks")
+            keep.append(rp)
+            rp_ptr = rp.ctypes.data
+        nsim_c, nvar_c = 0, 0
+        if sample is not None:
+            sa = np.asarray(sample, dtype=np.float64)
+            if sa.ndim == 1:
+                sa = sa[:, None, None]
+            elif sa.ndim == 2:
+                sa = sa[:, None, :]
+            elif sa.ndim != 3:
+                raise ValueError(f"sample must be 2-D (nblocks, nsim) or 3-D (nblocks, nvar, nsim), got {sa.shape}")
+            sf = np.asfortranarray(sa.transpose(2, 1, 0))  # (nsim, nvar, nblocks)
+            nsim_c, nvar_c = sf.shape[0], sf.shape[1]
+            if nsim_c != self.nsim or nvar_c != self.nvar or sf.shape[2] != nb:
+                raise ValueError(f"sample shape {sf.shape} must be ({self.nsim}, {self.nvar}, {nb})")
+            keep.append(sf)
+            s_ptr = sf.ctypes.data
+        _check(_load_lib().krige_set_sim(self._handle, nb, rp_ptr, nsim_c, nvar_c, s_ptr), "krige_set_sim")
+
     def set_search(self, ivar: int = 1, anis1: float = 1.0, anis2: float = 1.0, azimuth: float = 0.0,
                    dip: float = 0.0, plunge: float = 0.0, sector_search: bool = False):
         _check(_load_lib().krige_set_search(self._handle, ivar, anis1, anis2, azimuth, dip, plunge,
@@ -322,14 +359,14 @@ class Kriging:
 
     @property
     def timing(self) -> dict:
-        out = np.zeros(13)
+        out = np.zeros(15)
         _check(_load_lib().kk_get_timing(self._handle, _dp(out)), "kk_get_timing")
         return dict(search_ms=out[0], bucket_ms=out[1], solve_ms=out[2], fallback_ms=out[3],
                     total_ms=out[4], fp64_resolved=int(out[5]), bk_fallback=int(out[6]),
                     exact_search_queries=int(out[7]), fast_search_launches=int(out[8]),
                     fast_search_kernel_ms=out[9],
                     exact_search_kernel_ms=out[10], loo_grad_kernel_ms=out[11],
-                    weights_ms=out[12])
+                    weights_ms=out[12], sim_ms=out[13], sim_levels=int(out[14]))
 
     GRAD_PARAMS = ("nugget", "sill", "a_major", "a_minor1", "a_minor2", "azimuth", "dip", "plunge")
 

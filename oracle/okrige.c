@@ -493,6 +493,7 @@ typedef struct {
   double ballsize;
   kres_t *heap;  /* 1-based heap positions stored at heap[pos-1] */
   int heap_size;
+  int max_idx;   /* only points with 1-based index <= max_idx (kdtree2_n_nearest_maxidx) */
 } srec_t;
 
 /* pq_insert :307-346 */
@@ -546,6 +547,8 @@ static void process_terminal(srec_t *s, const knode_t *nd) {
   const ktree_t *t = s->t;
   double ballsize = s->ballsize;
   for (int i = nd->l; i <= nd->u; ++i) {
+    /* index filter first: process_terminal_node_maxidx :2062-2108 */
+    if (t->ind[i] + 1 > s->max_idx) continue;
     double sd = 0.0;
     int skip = 0;
     for (int k = 0; k < t->dim; ++k) {
@@ -645,8 +648,39 @@ static void knn(const ktree_t *t, const double *q, int nn, int sort, kres_t *res
   s.ballsize = DBL_MAX;
   s.heap = res;
   s.heap_size = 0;
+  s.max_idx = INT_MAX;
   search(&s, t->root);
   if (sort && nn > 1) heapsort_struct(res, nn);
+}
+
+/* kdtree2_n_nearest_maxidx :1928-1964 (the caller guarantees at least nn
+ * points below the limit, so the heap fills) */
+static void knn_maxidx(const ktree_t *t, const double *q, int nn, int sort, int max_idx, kres_t *res) {
+  srec_t s;
+  s.t = t;
+  s.qv = q;
+  s.nn = nn;
+  s.nfound = 0;
+  s.ballsize = DBL_MAX;
+  s.heap = res;
+  s.heap_size = 0;
+  s.max_idx = max_idx;
+  search(&s, t->root);
+  if (sort && nn > 1) heapsort_struct(res, nn);
+}
+
+int okr_knn_maxidx(int ndim, int n, const double *coord, int nq, const double *q, const int *k, int sort,
+                   const int *max_idx, int kmax, int *idx, double *dis) {
+  ktree_t t;
+  if (tree_build(&t, ndim, n, coord)) { tree_free(&t); return 1; }
+  kres_t *r = (kres_t *)malloc(sizeof(kres_t) * (kmax > 0 ? kmax : 1));
+  for (int j = 0; j < nq; ++j) {
+    knn_maxidx(&t, q + (size_t)ndim * j, k[j], sort, max_idx[j], r);
+    for (int i = 0; i < k[j]; ++i) { idx[(size_t)kmax * j + i] = r[i].idx; dis[(size_t)kmax * j + i] = r[i].dis; }
+  }
+  free(r);
+  tree_free(&t);
+  return 0;
 }
 
 int okr_knn(int ndim, int n, const double *coord, const double *q, int k, int sort, int *idx,

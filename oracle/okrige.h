@@ -91,6 +91,12 @@ int okr_cov(const okr_problem *P, int n, const double *lags, double *out);
 int okr_knn(int ndim, int n, const double *coord, const double *q, int k, int sort,
             int *idx, double *dis);
 
+/* kdtree2_n_nearest_maxidx over one tree of n points, nq queries: query j
+ * takes its k[j] nearest among the points with 1-based index <= max_idx[j]
+ * (idx/dis rows of stride kmax, sorted ascending by distance when sort) */
+int okr_knn_maxidx(int ndim, int n, const double *coord, int nq, const double *q, const int *k, int sort,
+                   const int *max_idx, int kmax, int *idx, double *dis);
+
 /* the reference's symmetric-indefinite fallback (SSYTRF/SSYTRS with one
  * jittered retry) on one column-major m x m augmented system K of n data rows
  * (K is modified by the jitter); B holds nrhs right-hand sides (ld m) and

@@ -230,3 +230,25 @@ def knn(coord, q, k, sort=False):
                      idx.ctypes.data_as(_pi), _d(dis)):
         raise RuntimeError("okr_knn failed")
     return idx, dis
+
+
+def knn_maxidx(coord, q, k, max_idx, sort):
+    """kdtree2_n_nearest_maxidx over one tree of `coord` (n, ndim): query j
+    takes its k[j] nearest among the points with 1-based index <= max_idx[j]
+    (1-based idx and squared distances, rows padded with 0 past k[j]; sorted
+    ascending by distance when sort)"""
+    c = np.ascontiguousarray(np.asarray(coord, dtype=np.float64))
+    qq = np.ascontiguousarray(np.asarray(q, dtype=np.float64))
+    kk = np.ascontiguousarray(np.asarray(k, dtype=np.int32))
+    mi = np.ascontiguousarray(np.asarray(max_idx, dtype=np.int32))
+    nq, nd = qq.shape
+    kmax = max(int(kk.max()), 1) if nq else 1
+    idx = np.zeros((nq, kmax), dtype=np.int32)
+    dis = np.zeros((nq, kmax))
+    L = lib()
+    L.okr_knn_maxidx.argtypes = [ctypes.c_int, ctypes.c_int, _pd, ctypes.c_int, _pd, _pi, ctypes.c_int, _pi,
+                                 ctypes.c_int, _pi, _pd]
+    if L.okr_knn_maxidx(nd, len(c), _d(c), nq, _d(qq), kk.ctypes.data_as(_pi), int(sort),
+                        mi.ctypes.data_as(_pi), kmax, idx.ctypes.data_as(_pi), _d(dis)):
+        raise RuntimeError("okr_knn_maxidx failed")
+    return idx, dis

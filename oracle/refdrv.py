@@ -187,6 +187,20 @@ class RefKriging:
         self._chk(f(self.h, ivar, anis1, anis2, azimuth, dip, plunge, int(sector_search)),
                   "krige_set_search")
 
+    def set_sim(self, randpath, sample):
+        """SGSIM path (1-based permutation of the blocks) and standard normal
+        draws (nblock, nsim) for nvar 1, passed as (nsim, nvar, nblock)
+        column-major (kriging_capi.F90:397-443)"""
+        rp = np.ascontiguousarray(randpath, dtype=np.int32).ravel()
+        smp = np.asarray(sample, dtype=np.float64)
+        nb, ns = smp.shape
+        sf = np.asfortranarray(smp.T.reshape(ns, 1, nb))
+        f = self.lib.krige_set_sim
+        f.restype = _c_int
+        f.argtypes = [_i64, _c_int, ctypes.c_void_p, _c_int, _c_int, ctypes.c_void_p]
+        self._chk(f(self.h, nb, rp.ctypes.data, ns, 1, sf.ctypes.data), "krige_set_sim")
+        self.nsim = ns
+
     def solve(self, nthread=1, ncache=-1):
         f = self.lib.krige_solve
         f.restype = _c_int
@@ -194,7 +208,22 @@ class RefKriging:
         self._chk(f(self.h, nthread, ncache), "krige_solve")
 
     def results(self):
+        """estimate (nblock, nvar), or (nblock, nvar, nsim) after set_sim, and
+        variance (nblock, nvar, nvar)"""
         nb, nv = self.nblock, self.nvar
+        ns = getattr(self, "nsim", 0)
+        if ns:
+            est = np.empty((nb, nv, ns), order="F")
+            f = self.lib.krige_get_estimate_all
+            f.restype = _c_int
+            f.argtypes = [_i64, _c_int, _c_int, _c_int, _pd]
+            self._chk(f(self.h, nb, nv, ns, _dp(est)), "krige_get_estimate_all")
+            var = np.empty((nb, nv, nv), order="F")
+            g = self.lib.krige_get_variance_all
+            g.restype = _c_int
+            g.argtypes = [_i64, _c_int, _c_int, _pd]
+            self._chk(g(self.h, nb, nv, _dp(var)), "krige_get_variance_all")
+            return np.ascontiguousarray(est), np.ascontiguousarray(var)
         est = np.empty((nb, nv, 1), order="F")
         var = np.empty((nb, nv, nv), order="F")
         f = self.lib.krige_get_estimate_all

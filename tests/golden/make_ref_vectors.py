@@ -7,10 +7,11 @@ reference's own counters split the last two into factorisations and cache
 reuses, which depend on its thread schedule), and ck_vectors.npz: its multi-variable
 (cokriging) results on problems.ck_cases() in table and analytic mode
 (oracle/_ref/libkriging_ref_analytic.so); for some problems also the stored
-kriging weights (krige_get_weight_data); and drift_vectors.npz: external
-drift and weight-correction problems (problems.drift_cases()).  Run in a
-container where the
-reference tree and oracle/_ref exist:  python tests/golden/make_ref_vectors.py
+kriging weights (krige_get_weight_data); drift_vectors.npz: external
+drift and weight-correction problems (problems.drift_cases()); and
+sgsim_vectors.npz: sequential simulation with given paths and draws
+(problems.sgsim_cases()).  Run in a container where the reference tree and
+oracle/_ref exist:  python tests/golden/make_ref_vectors.py [drift|sgsim]
 """
 import os
 import sys
@@ -22,7 +23,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 
 import problems  # noqa: E402
-from runners import run_ref, run_ref_ck  # noqa: E402
+from runners import run_ref, run_ref_ck, run_ref_sgsim  # noqa: E402
 
 
 # problems whose stored kriging weights are kept (every fifth block with two
@@ -78,6 +79,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ck_vectors.npz"), **ck)
     print("wrote", len(problems.ck_cases()), "cokriging problems")
     write_drift()
+    write_sgsim()
 
 
 def write_drift():
@@ -105,5 +107,23 @@ def write_drift():
     print("wrote", len(problems.drift_cases()), "drift problems")
 
 
+def write_sgsim():
+    """sgsim_vectors.npz: sequential simulation with given paths and draws
+    (tests/problems.py sgsim_cases), realisations and variances in table and
+    analytic mode"""
+    sv = {}
+    for spec in problems.sgsim_cases():
+        for mode in ("table", "analytic"):
+            e, v = run_ref_sgsim(spec, mode)
+            sv[f"{spec['name']}/{mode}/sim"] = e
+            sv[f"{spec['name']}/{mode}/variance"] = v
+    np.savez_compressed(os.path.join(HERE, "sgsim_vectors.npz"), **sv)
+    print("wrote", len(problems.sgsim_cases()), "simulation problems")
+
+
 if __name__ == "__main__":
-    main()
+    # an argument names one writer (drift, sgsim); none runs them all
+    if len(sys.argv) > 1:
+        globals()["write_" + sys.argv[1]]()
+    else:
+        main()

@@ -455,6 +455,53 @@ def st_sum_cases():
             st_sum(st_small(sector=True), [0.4], transform="gau", at=15.0, time_sill=2.0, name="stsm_octant")]
 
 
+def sgsim_case(name, ndim=2, nobs=250, side=1000.0, ngrid=12, nmax=12, nsim=3, sector=False, maxdist=None,
+               unbias=1, sk_mean=0.0, bounds=None, on_obs=0, obs_var=None, vgm=None, seed=2024):
+    """sequential simulation with a given path and draws: observations
+    uniform in a square (cube 10x flatter in z), values a sinusoid field plus
+    noise; targets an ngrid^ndim lattice of cell centres, the first on_obs of
+    them moved onto observations (the co-location rule).  Targets on error-
+    free observations have a variance that is zero up to rounding, whose
+    square root (about 1e-8) scales the draw: such cases give the
+    observations a measurement variance."""
+    rng = np.random.default_rng(seed)
+    lo = np.zeros(ndim)
+    hi = np.full(ndim, side)
+    if ndim == 3:
+        hi[2] = side / 10.0
+    x = rng.uniform(lo, hi, (nobs, ndim))
+    v = np.sin(x[:, 0] / 150.0) + np.cos(x[:, 1] / 230.0) + rng.normal(0.0, 0.1, nobs)
+    axes = [(np.arange(ngrid) + 0.5) * (hi[k] / ngrid) for k in range(ndim)]
+    g = np.stack([m.ravel() for m in np.meshgrid(*axes, indexing="ij")], 1)
+    if on_obs:
+        g[:on_obs] = x[:on_obs]
+    nb = len(g)
+    vg = vgm or [dict(ivar=1, jvar=1, vtype="sph", nugget=0.05, sill=1.0, a_major=400.0, a_minor1=250.0,
+                      azimuth=30.0)]
+    spec = _base(name, ndim, x, v, g, vg, nmax=nmax, maxdist=maxdist)
+    spec["obs"][0]["sk_mean"] = sk_mean
+    if obs_var is not None:
+        spec["obs"][0]["variance"] = np.full(nobs, obs_var)
+    spec["unbias"] = unbias
+    spec["search"][0]["sector_search"] = sector
+    if bounds is not None:
+        spec["bounds"] = bounds
+    spec["nsim"] = nsim
+    spec["path"] = rng.permutation(nb) + 1
+    spec["sample"] = rng.normal(size=(nb, nsim))
+    return spec
+
+
+def sgsim_cases():
+    return [sgsim_case("sgs_2d"),
+            sgsim_case("sgs_octant", nmax=3, sector=True, seed=2025),
+            sgsim_case("sgs_3d_maxdist", ndim=3, ngrid=6, nobs=400, nmax=16, maxdist=300.0, seed=2026,
+                       vgm=[dict(ivar=1, jvar=1, vtype="exp", nugget=0.02, sill=0.8, a_major=500.0, a_minor1=300.0,
+                                 a_minor2=60.0, azimuth=20.0)]),
+            sgsim_case("sgs_sk_bounds", unbias=0, sk_mean=0.4, bounds=(-0.8, 1.2), nsim=4, seed=2027),
+            sgsim_case("sgs_on_obs", on_obs=20, nmax=10, obs_var=0.02, seed=2028)]
+
+
 def ladder():
     """the reference's failure ladder (kriging_base.F90:3244-3398) on exact
     zeros: observations on a 10 m lattice, a spherical model whose range

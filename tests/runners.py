@@ -227,3 +227,45 @@ def run_gpu_st(spec, fast=True, **kw):
     out = dict(estimate=np.asarray(e), variance=np.asarray(v), nnear=nn, inear=inn, timing=k.timing)
     k.close()
     return out
+
+
+def run_ref_sgsim(spec, mode="table"):
+    """the compiled reference's sequential simulation with the spec's path
+    and draws: realisations (nblock, nsim) and variances (nblock,)"""
+    from oracle.refdrv import RefKriging
+    k = RefKriging(mode=mode, ndim=spec["ndim"], nsim=spec["nsim"], unbias=spec.get("unbias", 1),
+                   bounds=spec.get("bounds"))
+    o = spec["obs"][0]
+    k.set_obs(1, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"), maxdist=o.get("maxdist"),
+              sk_mean=o.get("sk_mean", 0.0))
+    k.set_grid(spec["grid"])
+    k.set_sim(spec["path"], spec["sample"])
+    for v in spec["vgm"]:
+        k.set_vgm(**v)
+    for s in spec["search"]:
+        k.set_search(**s)
+    k.solve()
+    e, v = k.results()
+    k.close()
+    return e[:, 0, :], v[:, 0, 0]
+
+
+def run_gpu_sgsim(spec, **kw):
+    """the library's sequential simulation with the spec's path and draws"""
+    from krigekit_amd.kriging import Kriging
+    k = Kriging(ndim=spec["ndim"], nsim=spec["nsim"], unbias=spec.get("unbias", 1), bounds=spec.get("bounds"),
+                **kw)
+    o = spec["obs"][0]
+    k.set_obs(1, o["coord"], o["value"], variance=o.get("variance"), nmax=o.get("nmax"), maxdist=o.get("maxdist"),
+              sk_mean=o.get("sk_mean", 0.0))
+    k.set_grid(spec["grid"])
+    k.set_sim(spec["path"], spec["sample"])
+    for v in spec["vgm"]:
+        k.set_vgm(**v)
+    for s in spec["search"]:
+        k.set_search(**s)
+    k.solve()
+    e, v = k.get_results(copy=True, squeeze=False)
+    t = k.timing
+    k.close()
+    return e[:, 0, :], v[:, 0, 0], t

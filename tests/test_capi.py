@@ -88,8 +88,17 @@ def test_bad_handle():
 def test_out_of_scope_entry_points_report(K):
     from krigekit_amd import kriging as kk
     lib = kk._load_lib()
-    assert lib.krige_set_sim(ctypes.c_int64(K._handle), 0, None, 0, 0, None) == 1
+    lib.krige_set_grad.argtypes = [ctypes.c_int64] + [ctypes.c_int] * 3 + [ctypes.c_void_p] * 4 + \
+        [ctypes.c_int, ctypes.c_void_p]
+    assert lib.krige_set_grad(ctypes.c_int64(K._handle), 1, 0, 2, None, None, None, None, 0, None) == 1
     assert "not provided by this backend" in kk._last_error()
+
+
+def test_set_sim_without_simulations_is_a_no_op(K):
+    """set_sim_base returns at once when nsim = 0 (kriging_base.F90:1235)"""
+    from krigekit_amd import kriging as kk
+    lib = kk._load_lib()
+    assert lib.krige_set_sim(ctypes.c_int64(K._handle), 0, None, 0, 0, None) == 0
 
 
 def test_error_slot_cleared_by_next_call(K):
