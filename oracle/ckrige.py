@@ -21,6 +21,16 @@ Follows the reference's per-block computation for nvar > 1:
   * right-hand side per target variable q: sum over the block's integration
     nodes of w_k C_{v(r) q}((x_r - g_k) / rs), constraint part the block's
     drift values, then e_q (std_ck) or 1 (kriging.F90:864-939);
+  * gradient (increment) rows (kriging.F90:796-840, 904-920; kriging_base.F90:
+    2591-2660): every increment observation z(x1) - z(x2) of a variable joins
+    every block's system after the neighbour groups, with covariances
+    C(x1 - x) - C(x2 - x) against data rows (vgm(increment variable, data
+    variable)), c11 - c12 - c21 + c22 between increments of one variable
+    (zero across variables), diagonal 2 C(0) - 2 C(x1 - x2) + variance (no
+    local nugget), its drift values in the drift rows and 0 in the
+    unbiasedness rows; right-hand side sum_k w_k C(x1 - g_k) - sum_k w_k
+    C(x2 - g_k); the estimate adds the increments' weights times their
+    values; a block without data neighbours is NaN however many increments;
   * weight correction (nvar 1, kriging_base.F90:3388-3391, 1995-2001): the
     data weights clipped at 0 and divided by their sum when it is positive,
     before the estimate and variance;
@@ -132,6 +142,28 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
     gd = np.zeros((nb, ndr)) if spec.get("grid_drift") is None else np.asarray(spec["grid_drift"],
                                                                                 dtype=np.float64).reshape(nb, ndr)
     wcorr = bool(spec.get("weight_correction", False)) and nvar == 1
+    # increments per variable: (x1, x2, value, variance, drift)
+    G = []
+    for v in range(nvar):
+        gs = [g for g in spec.get("grad", []) if g["ivar"] == v + 1]
+        if gs:
+            g = gs[-1]
+            c1 = np.asarray(g["coord1"], dtype=np.float64).reshape(-1, nd)
+            ng = len(c1)
+            G.append((c1, np.asarray(g["coord2"], dtype=np.float64).reshape(-1, nd),
+                      np.asarray(g["value"], dtype=np.float64).ravel(),
+                      np.zeros(ng) if g.get("variance") is None else np.asarray(g["variance"], dtype=np.float64),
+                      np.zeros((ng, ndr)) if g.get("drift") is None else np.asarray(g["drift"], dtype=np.float64).reshape(ng, ndr)))
+        else:
+            G.append(None)
+    gv = np.concatenate([np.full(len(g[0]), v) for v, g in enumerate(G) if g is not None]).astype(int) \
+        if any(g is not None for g in G) else np.zeros(0, dtype=int)
+    gx1 = np.concatenate([g[0] for g in G if g is not None]) if len(gv) else np.zeros((0, nd))
+    gx2 = np.concatenate([g[1] for g in G if g is not None]) if len(gv) else np.zeros((0, nd))
+    gz = np.concatenate([g[2] for g in G if g is not None]) if len(gv) else np.zeros(0)
+    gvr = np.concatenate([g[3] for g in G if g is not None]) if len(gv) else np.zeros(0)
+    gdr = np.concatenate([g[4] for g in G if g is not None]) if len(gv) else np.zeros((0, ndr))
+    nge = len(gv)
     pu = (nvar if std else 1) if unbias else 0
     p = ndr + pu
     est = np.full((nb, nvar), np.nan)
@@ -143,9 +175,10 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
     for b in range(nb):
         groups = [l[1][b, :l[0][b]] for l in lists]
         rv = np.concatenate([np.full(len(g), v) for v, g in enumerate(groups)]).astype(int)
-        n = len(rv)
-        if n == 0:
+        no = len(rv)
+        if no == 0:
             continue
+        n = no + nge
         xr = np.concatenate([X[v][g] for v, g in enumerate(groups)])
         zr = np.concatenate([Z[v][g] for v, g in enumerate(groups)])
         vr = np.concatenate([V[v][g] for v, g in enumerate(groups)])
@@ -158,15 +191,46 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
                     continue
                 lag = (xr[ri][:, None, :] - xr[rj][None, :, :]) / rs[b]
                 K[np.ix_(ri, rj)] = cov(max(i, j), min(i, j), lag).reshape(len(ri), len(rj))
-        K[np.arange(n), np.arange(n)] = np.array([cov0[(v, v)] for v in rv]) + vr + ln[b]
+        K[np.arange(no), np.arange(no)] = np.array([cov0[(v, v)] for v in rv]) + vr + ln[b]
+        for g in range(nvar):
+            ga = np.nonzero(gv == g)[0]
+            if len(ga) == 0:
+                continue
+            # increment - data: C(x1 - x) - C(x2 - x), vgm(increment var, data var)
+            for i in range(nvar):
+                ri = np.nonzero(rv == i)[0]
+                if len(ri) == 0:
+                    continue
+                l1 = (gx1[ga][:, None, :] - xr[ri][None, :, :]) / rs[b]
+                l2 = (gx2[ga][:, None, :] - xr[ri][None, :, :]) / rs[b]
+                c = (cov(g, i, l1) - cov(g, i, l2)).reshape(len(ga), len(ri))
+                K[np.ix_(no + ga, ri)] = c
+                K[np.ix_(ri, no + ga)] = c.T
+            # increments of one variable: c11 - c12 - c21 + c22 (pair i before j),
+            # diagonal 2 C(0) - 2 C(x1 - x2) + variance
+            ll = lambda u, w: (u[:, None, :] - w[None, :, :]) / rs[b]  # noqa: E731
+            m = len(ga)
+            c11 = cov(g, g, ll(gx1[ga], gx1[ga])).reshape(m, m)
+            c12 = cov(g, g, ll(gx1[ga], gx2[ga])).reshape(m, m)
+            c21 = cov(g, g, ll(gx2[ga], gx1[ga])).reshape(m, m)
+            c22 = cov(g, g, ll(gx2[ga], gx2[ga])).reshape(m, m)
+            gg = c11 - c12 - c21 + c22
+            iu = np.triu_indices(m, 1)
+            blk = np.zeros((m, m))
+            blk[iu] = gg[iu]
+            blk = blk + blk.T
+            dg = 2.0 * cov0[(g, g)] - 2.0 * cov(g, g, (gx1[ga] - gx2[ga]) / rs[b]) + gvr[ga]
+            blk[np.arange(m), np.arange(m)] = dg
+            K[np.ix_(no + ga, no + ga)] = blk
         if p:
             F = np.zeros((n, p))
-            F[:, :ndr] = dr
+            F[:no, :ndr] = dr
+            F[no:, :ndr] = gdr
             if pu:
                 if std:
-                    F[np.arange(n), ndr + rv] = 1.0
+                    F[np.arange(no), ndr + rv] = 1.0
                 else:
-                    F[:, ndr] = 1.0
+                    F[:no, ndr] = 1.0
             K[:n, n:] = F
             K[n:, :n] = F.T
         R = np.zeros((n + p, nvar))
@@ -177,6 +241,13 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
                     continue
                 lag = (xr[ri][:, None, :] - sub[b][None, :, :]) / rs[b]
                 R[ri, q] = cov(i, q, lag).reshape(len(ri), -1) @ sw[b]
+            for g in range(nvar):
+                ga = np.nonzero(gv == g)[0]
+                if len(ga) == 0:
+                    continue
+                t1 = cov(g, q, (gx1[ga][:, None, :] - sub[b][None, :, :]) / rs[b]).reshape(len(ga), -1) @ sw[b]
+                t2 = cov(g, q, (gx2[ga][:, None, :] - sub[b][None, :, :]) / rs[b]).reshape(len(ga), -1) @ sw[b]
+                R[no + ga, q] = t1 - t2
             if p:
                 R[n:n + ndr, q] = gd[b]
             if pu:
@@ -199,13 +270,15 @@ def solve(spec, table_mode=True, nthread=8, want_weights=False):
                 wc = wc / tot
             x = x.copy()
             x[:n, 0] = wc
-        w = x[:n]
+        w = x[:no]
         if want_weights:
             for g in range(nvar):
                 rows = np.nonzero(rv == g)[0]
                 weights[b, :, g, :len(rows)] = w[rows, :].T
         for q in range(nvar):
             e = w[:, q] @ zr
+            if nge:
+                e += x[no:n, q] @ gz
             if not std and unbias and nvar > 1:
                 means = [zr[rv == i].mean() if np.any(rv == i) else 0.0 for i in range(nvar)]
                 for i in range(nvar):

@@ -187,6 +187,23 @@ class RefKriging:
         self._chk(f(self.h, ivar, anis1, anis2, azimuth, dip, plunge, int(sector_search)),
                   "krige_set_search")
 
+    def set_grad(self, ivar, coord1, coord2, value, variance=None, drift=None):
+        """increment observations z(coord1) - z(coord2) of variable ivar
+        (kriging_capi.F90:472-506); drift (n, ndrift) when ndrift > 0"""
+        c1, c2 = _fcoord(coord1), _fcoord(coord2)
+        v = np.ascontiguousarray(value, dtype=np.float64).ravel()
+        var = np.zeros(v.size) if variance is None else np.ascontiguousarray(variance, dtype=np.float64).ravel()
+        if drift is None:
+            d, nd = np.zeros((1, max(v.size, 1)), order="F"), 0
+        else:
+            d = np.asfortranarray(np.asarray(drift, dtype=np.float64).reshape(v.size, -1).T)
+            nd = d.shape[0]
+        f = self.lib.krige_set_grad
+        f.restype = _c_int
+        f.argtypes = [_i64, _c_int, _c_int, _c_int, _pd, _pd, _pd, _pd, _c_int, _pd]
+        self._chk(f(self.h, ivar, v.size, c1.shape[0], _dp(c1), _dp(c2), _dp(v), _dp(var), nd, _dp(d)),
+                  "krige_set_grad")
+
     def set_sim(self, randpath, sample):
         """SGSIM path (1-based permutation of the blocks) and standard normal
         draws (nblock, nsim) for nvar 1, passed as (nsim, nvar, nblock)

@@ -10,8 +10,9 @@ reuses, which depend on its thread schedule), and ck_vectors.npz: its multi-vari
 kriging weights (krige_get_weight_data); drift_vectors.npz: external
 drift and weight-correction problems (problems.drift_cases()); and
 sgsim_vectors.npz: sequential simulation with given paths and draws
-(problems.sgsim_cases()).  Run in a container where the reference tree and
-oracle/_ref exist:  python tests/golden/make_ref_vectors.py [drift|sgsim]
+(problems.sgsim_cases()); grad_vectors.npz: increment constraints
+(problems.grad_cases()).  Run in a container where the reference tree and
+oracle/_ref exist:  python tests/golden/make_ref_vectors.py [drift|grad|sgsim]
 """
 import os
 import sys
@@ -79,6 +80,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ck_vectors.npz"), **ck)
     print("wrote", len(problems.ck_cases()), "cokriging problems")
     write_drift()
+    write_grad()
     write_sgsim()
 
 
@@ -107,6 +109,20 @@ def write_drift():
     print("wrote", len(problems.drift_cases()), "drift problems")
 
 
+def write_grad():
+    """grad_vectors.npz: increment (gradient-pair) constraints
+    (problems.grad_cases()), estimates and variances in table and analytic
+    mode"""
+    gv = {}
+    for spec in problems.grad_cases():
+        for mode in ("table", "analytic"):
+            r = run_ref_ck(spec, mode)
+            gv[f"{spec['name']}/{mode}/estimate"] = r["estimate"]
+            gv[f"{spec['name']}/{mode}/variance"] = r["variance"]
+    np.savez_compressed(os.path.join(HERE, "grad_vectors.npz"), **gv)
+    print("wrote", len(problems.grad_cases()), "gradient problems")
+
+
 def write_sgsim():
     """sgsim_vectors.npz: sequential simulation with given paths and draws
     (tests/problems.py sgsim_cases), realisations and variances in table and
@@ -122,7 +138,7 @@ def write_sgsim():
 
 
 if __name__ == "__main__":
-    # an argument names one writer (drift, sgsim); none runs them all
+    # an argument names one writer (drift, grad, sgsim); none runs them all
     if len(sys.argv) > 1:
         globals()["write_" + sys.argv[1]]()
     else:

@@ -394,6 +394,43 @@ def wcorr_drift(nb=120, seed=33):
     return s
 
 
+def _add_grads(s, ivar, n, seed, sep=40.0, var=0.01, trend=None):
+    """n increment observations z(x1) - z(x2) of variable ivar: pairs sep m
+    apart inside the domain, values from a smooth field plus noise; with
+    trend, their drift values trend(x1) - trend(x2)"""
+    rng = np.random.default_rng(seed)
+    x1 = rng.uniform(100.0, 1500.0, (n, 2))
+    ang = rng.uniform(0.0, 2.0 * np.pi, n)
+    x2 = x1 + sep * np.stack([np.cos(ang), np.sin(ang)], 1)
+    f = lambda x: np.sin(x[:, 0] / 300.0) + 0.5 * np.cos(x[:, 1] / 410.0)  # noqa: E731
+    g = dict(ivar=ivar, coord1=x1, coord2=x2, value=f(x1) - f(x2) + rng.normal(0.0, 0.02, n),
+             variance=np.full(n, var))
+    if trend is not None:
+        g["drift"] = (trend(x1) - trend(x2))[:, None]
+    s.setdefault("grad", []).append(g)
+    return s
+
+
+def grad_cases():
+    """increment (gradient-pair) constraints (kriging.F90:796-840, 904-920):
+    ordinary kriging; with an external drift; standard cokriging with
+    increments of both variables; non-standard cokriging on Gauss-Legendre
+    blocks; simple kriging with a mean"""
+    a = _add_grads(_one_var(ck_small(std=True, gq=False, nb=120, seed=41)), 1, 8, 42)
+    a["name"] = "grad_ok"
+    b = drift_ked(nb=100, seed=43)
+    trend = lambda x: 0.8 * np.sin(x[:, 0] / 450.0) + 0.5 * np.cos(x[:, 1] / 620.0)  # noqa: E731
+    b = _add_grads(b, 1, 6, 44, trend=trend)
+    b["name"] = "grad_ked"
+    c = _add_grads(_add_grads(ck_small(std=True, gq=False, nb=100, seed=45), 2, 6, 46), 1, 4, 47, var=0.0)
+    c["name"] = "grad_ck"
+    d = _add_grads(ck_small(std=False, gq=True, nb=80, seed=48), 1, 5, 49)
+    d["name"] = "grad_ck_nonstd_blocks"
+    e = _add_grads(_one_var(ck_small(True, False, nb=100, seed=50, unbias=0, sk=(0.2, -0.1))), 1, 6, 51)
+    e["name"] = "grad_sk"
+    return [a, b, c, d, e]
+
+
 def drift_cases():
     return [drift_ked(), drift_uk(), drift_sk(), drift_ck(), wcorr_ok(), wcorr_drift()]
 

@@ -162,6 +162,8 @@ def run_ref_ck(spec, mode="table", nthread=4, want_weights=False):
         k.set_vgm(**v)
     for s in spec["search"]:
         k.set_search(**s)
+    for g in spec.get("grad", []):
+        k.set_grad(g["ivar"], g["coord1"], g["coord2"], g["value"], variance=g.get("variance"), drift=g.get("drift"))
     k.solve(nthread=nthread)
     e, v = k.results()
     out = dict(estimate=e, variance=v, stats=k.solver_stats())

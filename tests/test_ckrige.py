@@ -138,3 +138,55 @@ def test_drift_and_correction_terms_matter(name):
                variance=o["variance"].reshape(ref["variance"].shape))
     ee, ev = ck_errors(got, ref)
     assert max(ee, ev) > 1e-3, (ee, ev)
+
+
+GRAD = problems.grad_cases()
+
+
+@pytest.mark.parametrize("mode", ["table", "analytic"])
+@pytest.mark.parametrize("spec", GRAD, ids=[c["name"] for c in GRAD])
+def test_grad_oracle_matches_compiled_reference(spec, mode):
+    """increment (gradient-pair) constraints: the numpy oracle against the
+    compiled reference's committed vectors"""
+    v = np.load(problems.GOLDEN + "/grad_vectors.npz")
+    o = ckrige.solve(spec, table_mode=(mode == "table"))
+    ref = dict(estimate=v[f"{spec['name']}/{mode}/estimate"], variance=v[f"{spec['name']}/{mode}/variance"])
+    got = dict(estimate=o["estimate"].reshape(ref["estimate"].shape),
+               variance=o["variance"].reshape(ref["variance"].shape))
+    ee, ev = ck_errors(got, ref)
+    assert ee < 1e-10 and ev < 1e-10, (ee, ev)
+
+
+@pytest.mark.parametrize("change", ["drop", "cross_zero", "sign", "no_var"])
+def test_grad_terms_matter(change):
+    """the vectors tell the increment terms apart: dropping the increments,
+    filling the increment-increment block across variables, flipping the
+    pair order of the covariance, or dropping the increment variance each
+    misses them by far more than the bar"""
+    name = "grad_ck" if change == "cross_zero" else "grad_ok"
+    spec = {c["name"]: c for c in GRAD}[name]
+    v = np.load(problems.GOLDEN + "/grad_vectors.npz")
+    if change == "drop":
+        off = dict(spec)
+        off.pop("grad")
+        o = ckrige.solve(off)
+    elif change == "no_var":
+        off = dict(spec)
+        off["grad"] = [dict(g, variance=None) for g in spec["grad"]]
+        o = ckrige.solve(off)
+    else:
+        src = open(ckrige.__file__).read()
+        if change == "cross_zero":
+            # the block of one variable's increments, filled across variables too
+            src = src.replace("K[np.ix_(no + ga, no + ga)] = blk", "K[np.ix_(no + ga, no + ga)] = blk\n"
+                              "            K[no:n, no:n] = np.where(K[no:n, no:n] == 0.0, 0.05, K[no:n, no:n])")
+        else:
+            src = src.replace("c = (cov(g, i, l1) - cov(g, i, l2))", "c = (cov(g, i, l1) + cov(g, i, l2))")
+        ns = {}
+        exec(compile(src, "oracle/ckrige.py (changed)", "exec"), ns)
+        o = ns["solve"](spec)
+    ref = dict(estimate=v[f"{name}/table/estimate"], variance=v[f"{name}/table/variance"])
+    got = dict(estimate=o["estimate"].reshape(ref["estimate"].shape),
+               variance=o["variance"].reshape(ref["variance"].shape))
+    ee, ev = ck_errors(got, ref)
+    assert max(ee, ev) > 1e-4, (ee, ev)
