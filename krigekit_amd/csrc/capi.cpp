@@ -505,12 +505,17 @@ struct Obs {
   double time_at = 1.0;        // space-time search scaling of the time axis
   std::vector<double> drift;   // (ndrift, n): external drift values (zeros until set_obs_drift)
   HostTree tree;
+  // increment observations z(x1) - z(x2) of this variable (set_grad):
+  // coordinates (ndim, ngrad), values, variances, drift values (ndrift, ngrad)
+  int ngrad = 0;
+  std::vector<double> gx1, gx2, gval, gvar, gdrift;
 };
 
 // device copies of one variable's observations, tree and neighbour lists
 // (multi-variable solves)
 struct DevVar {
   DBuf ocoord, socoord, ovalue, ovar, rdata, ind, ni, nd, qcoord, nnear, inear, sflag, xlist, hdis, hidx, odrift;
+  DBuf gx1, gx2, gval, gvar, gdrift;  // increment observations (set_grad)
   int mmax = 0;
 };
 
@@ -672,8 +677,14 @@ struct Krige {
 // and single-variable kriging with external drift or weight correction
 // (both solved by the bordered solver); stored-weight replays keep the
 // single-variable ones.
+int grad_total(const Krige& K) {
+  int n = 0;
+  for (const Obs& O : K.obs) n += O.ngrad;
+  return n;
+}
+
 bool mv_results(const Krige& K) {
-  return K.nvar > 1 || (!K.use_old_weight && (K.ndrift > 0 || K.weight_correction));
+  return K.nvar > 1 || (!K.use_old_weight && (K.ndrift > 0 || K.weight_correction || grad_total(K) > 0));
 }
 
 std::vector<Krige*> g_reg;
@@ -1856,6 +1867,12 @@ bool validate_for_solve(Krige& K) {
     set_error(W, "this backend solves kriging and cokriging with nvar <= 4, nsim=0, ndrift <= 4");
     return false;
   }
+  if (grad_total(K) > 0 && (K.st || K.store_weight || K.weight_correction || K.cross_validation ||
+                            K.varying_vgm)) {
+    set_error(W, "increment (gradient) constraints are provided without a weight store, weight correction, "
+                 "cross-validation or space-time");
+    return false;
+  }
   if (K.st) {
     if (K.nvar != 1) {
       set_error(W, "space-time kriging with nvar > 1 is not supported by this backend");
@@ -2251,6 +2268,9 @@ bool do_solve_mv(Krige& K, bool fetch) {
           !upload(D.socoord, O.aniso ? O.rcoord : O.coord, W))
         return false;
       if (K.ndrift > 0 && !upload(D.odrift, O.drift, W)) return false;
+      if (O.ngrad > 0 && (!upload(D.gx1, O.gx1, W) || !upload(D.gx2, O.gx2, W) || !upload(D.gval, O.gval, W) ||
+                          !upload(D.gvar, O.gvar, W) || (K.ndrift > 0 && !upload(D.gdrift, O.gdrift, W))))
+        return false;
       if (O.need_search && (!upload(D.rdata, O.tree.rdata, W) || !upload(D.ind, O.tree.ind, W) ||
                             !upload(D.ni, O.tree.ni, W) || !upload(D.nd, O.tree.nd, W)))
         return false;
@@ -2280,6 +2300,8 @@ bool do_solve_mv(Krige& K, bool fetch) {
     rows += K.dv[v].mmax;
   }
   HIPCHK(hipEventRecord(K.ev[1], st), W);
+  // increment rows join every block's system
+  rows += grad_total(K);
   const int cols = rows + kk::mv_extra_cols(nvar, K.std_ck, K.unbias, K.ndrift);
   if (cols > 128) {
     set_error(W, "cokriging systems larger than 128 rows + right-hand sides are not supported by the GPU solver (" +
@@ -2300,7 +2322,8 @@ bool do_solve_mv(Krige& K, bool fetch) {
   a.unbias = K.unbias;
   a.nsub = nsub;
   a.rows = rows;
-  a.simple = K.mv_simple;
+  // increment rows are assembled by the general-shape instances only
+  a.simple = K.mv_simple && grad_total(K) == 0;
   for (int v = 0; v < nvar; ++v) a.sk_mean[v] = K.obs[v].sk_mean;
   a.bound_lo = K.bounds[0];
   a.bound_hi = K.bounds[1];
@@ -2324,6 +2347,16 @@ bool do_solve_mv(Krige& K, bool fetch) {
   if (K.ndrift > 0) {
     for (int v = 0; v < nvar; ++v) a.odrift[v] = K.dv[v].odrift.as<double>();
     a.bdrift = K.d_bdrift.as<double>();
+  }
+  for (int v = 0; v < nvar; ++v) {
+    a.ngrad[v] = K.obs[v].ngrad;
+    if (a.ngrad[v] > 0) {
+      a.gx1[v] = K.dv[v].gx1.as<double>();
+      a.gx2[v] = K.dv[v].gx2.as<double>();
+      a.gval[v] = K.dv[v].gval.as<double>();
+      a.gvar[v] = K.dv[v].gvar.as<double>();
+      a.gdrift[v] = K.ndrift > 0 ? K.dv[v].gdrift.as<double>() : nullptr;
+    }
   }
   // weight correction needs the weights: the weights instance
   a.wcorr = K.weight_correction && nvar == 1;
@@ -3217,7 +3250,17 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
     }
   }
   Obs& O = K->obs[ivar - 1];
-  O = Obs();
+  {
+    // the variable's increment observations are set apart (set_grad)
+    Obs fresh;
+    fresh.ngrad = O.ngrad;
+    fresh.gx1.swap(O.gx1);
+    fresh.gx2.swap(O.gx2);
+    fresh.gval.swap(O.gval);
+    fresh.gvar.swap(O.gvar);
+    fresh.gdrift.swap(O.gdrift);
+    O = std::move(fresh);
+  }
   O.n = nobs;
   K->perm_ok = false;
   O.coord.assign(coord, coord + (size_t)ndim_c * nobs);
@@ -3695,11 +3738,59 @@ int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azi
 }
 
 
-int krige_set_grad(int64_t h, int, int, int, const double*, const double*, const double*, const double*, int,
-                   const double*) {
+int krige_set_grad(int64_t h, int ivar, int ngrad, int ndim_c, const double* coord1, const double* coord2,
+                   const double* grad_val, const double* variance, int ndrift_c, const double* drift_ext) {
   clear_error();
-  if (!checked(h)) return ierr();
-  return unsupported("krige_set_grad", "gradient constraints are not provided by this backend");
+  Krige* K = checked(h);
+  if (!K) return ierr();
+  // kriging_capi.F90:472-506 -> set_grad / reset_grad, kriging_base.F90:2591-2690
+  const char* W = "t_kriging_base%set_grad";
+  if (!K->initialized) {
+    set_error(W, "Call initialize() before set_grad.");
+    return ierr();
+  }
+  if (ivar < 1 || ivar > K->nvar) {
+    set_error(W, "ivar out of range: " + std::to_string(ivar) + " not in 1.." + std::to_string(K->nvar));
+    return ierr();
+  }
+  Obs& O = K->obs[ivar - 1];
+  K->dev_inputs_ok = false;
+  K->solved = false;
+  if (ngrad <= 0) {
+    O.ngrad = 0;
+    O.gx1.clear();
+    O.gx2.clear();
+    O.gval.clear();
+    O.gvar.clear();
+    O.gdrift.clear();
+    return 0;
+  }
+  if (ndim_c != K->ndim) {
+    set_error(W, "size(coord,1) /= nlag or size(coord2,1) /= nlag");
+    return ierr();
+  }
+  if (ndrift_c > 0 && ndrift_c != K->ndrift) {
+    set_error(W, "size(drift_ext,1) /= ndrift or size(drift_ext,2) /= ngrad");
+    return ierr();
+  }
+  if (K->ndrift > 0 && ndrift_c <= 0) {
+    set_error(W, "drift_ext must be provided when ndrift > 0");
+    return ierr();
+  }
+  if (grad_total(*K) - O.ngrad + ngrad > kk::MAXGRAD) {
+    set_error(W, "more than " + std::to_string(kk::MAXGRAD) +
+                     " increment observations over all variables are not supported by the GPU solver");
+    return ierr();
+  }
+  O.ngrad = ngrad;
+  O.gx1.assign(coord1, coord1 + (size_t)ndim_c * ngrad);
+  O.gx2.assign(coord2, coord2 + (size_t)ndim_c * ngrad);
+  O.gval.assign(grad_val, grad_val + ngrad);
+  O.gvar.assign(ngrad, 0.0);
+  if (variance) O.gvar.assign(variance, variance + ngrad);
+  O.gdrift.assign((size_t)std::max(K->ndrift, 1) * ngrad, 0.0);
+  if (K->ndrift > 0) O.gdrift.assign(drift_ext, drift_ext + (size_t)K->ndrift * ngrad);
+  return 0;
 }
 
 int krige_prepare(int64_t h) {

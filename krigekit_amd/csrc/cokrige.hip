@@ -83,6 +83,56 @@ __device__ inline double mv_cov(const Model* models, int a, int b, int nvar, dou
   return cov_general<DIM>(M, dx, dy, dz);
 }
 
+// Increment (gradient-pair) rows, out of line so that the kernels' register
+// allocation does not pay for them (kriging.F90:796-840, 904-920).
+// Element (i, j), j <= i, of increment row i >= no: against a data row
+// C(x1 - x) - C(x2 - x) with vgm(increment variable, data variable);
+// between increments of one variable c11 - c12 - c21 + c22 of the pair (j, i);
+// zero across variables; diagonal 2 C(0) - 2 C(x1 - x2) + variance, without
+// the local nugget.  sx: first points (and data rows), stride ldx; sx2: second
+// points of the increment rows (index i - no), stride MAXGRAD; all relative
+// to the block centre.
+template <int DIM, bool SIMPLE>
+__device__ __noinline__ double mv_inc_cov(const Model* models, int nvar, const double* sx, int ldx, const double* sx2,
+                                          const int* svr, const double* sov, int i, int j, int no, bool scaled,
+                                          double rs) {
+  auto c = [&](int va, int vb, const double* u, int su, int iu, const double* w, int sw_, int iw) {
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < DIM; ++k) {
+      d[k] = u[k * su + iu] - w[k * sw_ + iw];
+      if (scaled) d[k] = d[k] / rs;
+    }
+    return mv_cov<DIM, SIMPLE>(models, va, vb, nvar, d[0], d[1], d[2]);
+  };
+  const int vi = svr[i], vj = svr[j];
+  if (j < no) return c(vi, vj, sx, ldx, i, sx, ldx, j) - c(vi, vj, sx2, MAXGRAD, i - no, sx, ldx, j);
+  if (j == i)
+    return (2.0 * models[vi + nvar * vi].cov0 - 2.0 * c(vi, vi, sx, ldx, i, sx2, MAXGRAD, i - no)) + sov[i];
+  if (vj != vi) return 0.0;
+  const double c11 = c(vi, vi, sx, ldx, j, sx, ldx, i);
+  const double c12 = c(vi, vi, sx, ldx, j, sx2, MAXGRAD, i - no);
+  const double c21 = c(vi, vi, sx2, MAXGRAD, j - no, sx, ldx, i);
+  const double c22 = c(vi, vi, sx2, MAXGRAD, j - no, sx2, MAXGRAD, i - no);
+  return ((c11 - c12) - c21) + c22;
+}
+
+// sum_k w_k C_{vgm(v, q)}(x2 - g_k) over nodes k0..k1 of increment row gi
+template <int DIM, bool SIMPLE>
+__device__ __noinline__ double mv_inc_rhs(const Model* models, int nvar, int v, int q, const double* sx2, int gi,
+                                          const double* sg, const double* sgw, int k0, int k1, bool scaled,
+                                          double rs) {
+  double s = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < DIM; ++m) {
+      d[m] = sx2[m * MAXGRAD + gi] - sg[m * MAXSUB + k];
+      if (scaled) d[m] = d[m] / rs;
+    }
+    s += mv_cov<DIM, SIMPLE>(models, v, q, nvar, d[0], d[1], d[2]) * sgw[k];
+  }
+  return s;
+}
+
 __device__ inline double readlane_d(double v, int lane) {
   const long long u = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), lane);
@@ -139,6 +189,10 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // rows + extra rows fit 16 RB (the instance is chosen by the column count)
   constexpr int ROWS = 16 * RB;
   __shared__ double sx[DIM][ROWS];
+  // increment rows (general-shape instances only; the host runs problems with
+  // increments there): the second point
+  constexpr bool GRAD = !SIMPLE;
+  __shared__ double sx2[GRAD ? DIM : 1][GRAD ? MAXGRAD : 1];
   __shared__ double sz[ROWS];
   __shared__ double sov[ROWS];
   __shared__ int svr[ROWS];
@@ -191,9 +245,18 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
     st[v] = n;
     n += nv[v];
   }
+  // increment rows after the neighbour groups (gst[v]: first of variable v)
+  const int no = n;
+  int gst[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    gst[v] = n;
+    if constexpr (GRAD) n += a.ngrad[v];
+  }
   const double qnan = __builtin_nan("");
-  if (n < 2) {
-    // no neighbour: NaN; one neighbour: the reference skips the solve
+  if (no == 0 || n < 2) {
+    // no neighbour (increments alone do not make a system,
+    // kriging_base.F90:3198-3207): NaN; one row: the reference skips the solve
     // (kriging_base.F90:2127), NaN here
     if (!a.weights_only) {
       if (t < nvar) a.est[b + (size_t)a.nb * t] = qnan;
@@ -224,7 +287,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   for (int k = 0; k < DIM; ++k) x0[k] = a.bcoord[k + DIM * (size_t)b];
 
   // ---- stage rows (coordinates relative to the block centre) and nodes -------
-  if (t < n) {
+  if (t < no) {
     // the row's variable, by selects (a runtime index into the argument
     // arrays would copy them to scratch)
     int vr = 0, s0 = 0, mm = a.mmax[0];
@@ -254,6 +317,31 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
         if (v == vr) odr = a.odrift[v];
       for (int c = 0; c < nd; ++c) sdr[c][t] = odr[c + nd * (size_t)idx];
     }
+  } else if (GRAD && t < n) {
+    // increment row: both points, value, variance, drift values
+    int vr = 0, g0 = gst[0];
+    const double *p1 = a.gx1[0], *p2 = a.gx2[0], *pv = a.gval[0], *pe = a.gvar[0], *pd = a.gdrift[0];
+#pragma unroll
+    for (int v = 1; v < NV; ++v)
+      if (t >= gst[v]) {
+        vr = v;
+        g0 = gst[v];
+        p1 = a.gx1[v];
+        p2 = a.gx2[v];
+        pv = a.gval[v];
+        pe = a.gvar[v];
+        pd = a.gdrift[v];
+      }
+    const int gi = t - g0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      sx[k][t] = p1[k + DIM * (size_t)gi] - x0[k];
+      if constexpr (GRAD) sx2[k][t - no] = p2[k + DIM * (size_t)gi] - x0[k];
+    }
+    sz[t] = pv[gi];
+    sov[t] = pe[gi];
+    svr[t] = vr;
+    for (int c = 0; c < nd; ++c) sdr[c][t] = pd[c + nd * (size_t)gi];
   }
   const int ns = a.nsub;
   if (t < ns) {
@@ -279,7 +367,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       const int q = rq / n, r = rq - q * n;
       const int k0 = h * ns / NPART, k1 = (h + 1) * ns / NPART;
       const int vr = svr[r];
-      double s = 0.0;
+      double s = 0.0, s2 = 0.0;
       for (int k = k0; k < k1; ++k) {
         double dx = sx[0][r] - sg[0][k];
         double dy = DIM > 1 ? sx[1][r] - sg[1][k] : 0.0;
@@ -291,7 +379,12 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
         }
         s += mv_cov<DIM, SIMPLE>(smodels, vr, q, nvar, dx, dy, dz) * sgw[k];
       }
-      sb[h][q][r] = s;
+      // increment row: sum_k w_k C(x1 - g_k) - sum_k w_k C(x2 - g_k)
+      if constexpr (GRAD) {
+        if (r >= no)
+          s2 = mv_inc_rhs<DIM, SIMPLE>(smodels, nvar, vr, q, &sx2[0][0], r - no, &sg[0][0], sgw, k0, k1, scaled, rs);
+      }
+      sb[h][q][r] = r >= no ? s - s2 : s;
     }
   }
   // ---- block base covariances sum_k sum_l w_k w_l C_ij(g_k - g_l) -------------
@@ -358,7 +451,10 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
     for (int cb = 0; cb < RB; ++cb) {
       const int j = tc + 16 * cb;
       double v = 0.0;
-      if (ri && j <= i) {
+      if (GRAD && ri && j <= i && i >= no) {
+        if constexpr (GRAD)
+          v = mv_inc_cov<DIM, SIMPLE>(smodels, nvar, &sx[0][0], ROWS, &sx2[0][0], svr, sov, i, j, no, scaled, rs);
+      } else if (ri && j <= i) {
         if (j == i) {
           v = smodels[vi + nvar * vi].cov0 + sov[i] + ln;
         } else {
@@ -388,12 +484,13 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
           v = w[0];
         }
         else if (e < ez) {
+          // increment rows: their drift values, 0 in the unbiasedness rows
           const int c = e - ef;
-          v = c < nd ? sdr[c][j] : ((a.std_ck ? (vj == c - nd) : true) ? 1.0 : 0.0);
+          v = c < nd ? sdr[c][j] : (j < no && (a.std_ck ? (vj == c - nd) : true) ? 1.0 : 0.0);
         }
         else if (e == ez) v = sz[j];
-        else if (isa) v = (vj == e - ei) ? 1.0 : 0.0;
-        else v = 1.0;  // simple kriging: the ones column for the mean term
+        else if (isa) v = (j < no && vj == e - ei) ? 1.0 : 0.0;
+        else v = j < no ? 1.0 : 0.0;  // simple kriging: the ones column for the mean term (data rows)
       }
       sbuf[cb * MVT + t] = v;
     }
@@ -771,7 +868,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     gm[v] = 0.0;
-    if (isa && nv[v] > 0) gm[v] = wg_sum((t < n && svr[t] == v) ? sz[t] : 0.0, part) / nv[v];
+    if (isa && nv[v] > 0) gm[v] = wg_sum((t < no && svr[t] == v) ? sz[t] : 0.0, part) / nv[v];
   }
   __syncthreads();
   if (!WEIGHTS && t != 0) return;
@@ -970,8 +1067,8 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       a.var[b] = v;
     }
   }
-  if (t < n) {
-    // weight(k, group v, target q) = x(q, istart(v) + k)
+  if (t < no) {
+    // weight(k, group v, target q) = x(q, istart(v) + k) (data rows only)
     const int v = svr[t];
     int s0 = 0;
 #pragma unroll
@@ -1000,10 +1097,14 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
   const int b = list[t];
   const int nvar = a.nvar;
   const int ld = a.rows + MAXP;
-  int nsum = 0;
+  int nsum = 0, ngt = 0;
 #pragma unroll
-  for (int v = 0; v < MAXVAR; ++v) nsum += v < nvar ? a.nnear[v][b] : 0;
-  if (nsum < 2) return;  // nothing to solve (the weights-only pass lists every block)
+  for (int v = 0; v < MAXVAR; ++v) {
+    nsum += v < nvar ? a.nnear[v][b] : 0;
+    ngt += v < nvar ? a.ngrad[v] : 0;
+  }
+  // nothing to solve (the weights-only pass lists every block)
+  if (nsum == 0 || nsum + ngt < 2) return;
   double* K = scratch + (size_t)t * mv_fallback_words(a.rows);
   double* A = K + (size_t)ld * ld;
   double* B = A + (size_t)ld * ld;       // right-hand sides, then solutions
@@ -1014,6 +1115,13 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
     nv[v] = v < nvar ? a.nnear[v][b] : 0;
     st[v] = n;
     n += nv[v];
+  }
+  // increment rows after the data rows, variable by variable
+  const int no = n;
+  int gst[MAXVAR];
+  for (int v = 0; v < MAXVAR; ++v) {
+    gst[v] = n;
+    n += v < nvar ? a.ngrad[v] : 0;
   }
   const int nd = a.ndrift;
   const int p = nd + (a.unbias ? (a.std_ck ? nvar : 1) : 0), m = n + p;
@@ -1031,15 +1139,60 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
     if (f < nd) return a.bdrift[f + nd * ((size_t)q + nvar * (size_t)b)];
     return a.std_ck ? (f - nd == q ? 1.0 : 0.0) : 1.0;
   };
-  // row r: variable, observation index
+  // row r: variable (data and increment rows), observation index
   auto rowvar = [&](int r) {
     int v = 0;
     for (int u = 1; u < nvar; ++u)
-      if (r >= st[u]) v = u;
+      if (r >= (r < no ? st[u] : gst[u])) v = u;
     return v;
   };
+  auto cv = [&](int va, int vb, const double* u, const double* w) {
+    const double dx = (u[0] - w[0]) / rs;
+    const double dy = DIM > 1 ? (u[1] - w[1]) / rs : 0.0;
+    const double dz = DIM > 2 ? (u[2] - w[2]) / rs : 0.0;
+    return mv_cov<DIM, false>(a.models, va, vb, nvar, dx, dy, dz);
+  };
   auto rowx = [&](int r, int v) { return a.ocoord[v] + DIM * (size_t)a.inear[v][(size_t)b * a.mmax[v] + (r - st[v])]; };
-  for (int c = 0; c < n; ++c) {
+  for (int c = no; c < n; ++c) {
+    // increment row c (kriging.F90:796-840, 904-920)
+    const int vc = rowvar(c), gc = c - gst[vc];
+    const double* x1c = a.gx1[vc] + DIM * (size_t)gc;
+    const double* x2c = a.gx2[vc] + DIM * (size_t)gc;
+    K[c + ld * c] = (2.0 * a.models[vc + nvar * vc].cov0 - 2.0 * cv(vc, vc, x1c, x2c)) + a.gvar[vc][gc];
+    for (int r = 0; r < c; ++r) {
+      const int vr = rowvar(r);
+      double v;
+      if (r < no) {
+        const double* xr = rowx(r, vr);
+        v = cv(vc, vr, x1c, xr) - cv(vc, vr, x2c, xr);
+      } else if (vr != vc) {
+        v = 0.0;
+      } else {
+        // the pair (r, c), r first
+        const double* x1r = a.gx1[vr] + DIM * (size_t)(r - gst[vr]);
+        const double* x2r = a.gx2[vr] + DIM * (size_t)(r - gst[vr]);
+        v = ((cv(vc, vc, x1r, x1c) - cv(vc, vc, x1r, x2c)) - cv(vc, vc, x2r, x1c)) + cv(vc, vc, x2r, x2c);
+      }
+      K[r + ld * c] = v;
+      K[c + ld * r] = v;
+    }
+    for (int f = 0; f < p; ++f) {
+      const double e = f < nd ? a.gdrift[vc][f + nd * (size_t)gc] : 0.0;
+      K[(n + f) + ld * c] = e;
+      K[c + ld * (n + f)] = e;
+    }
+    for (int q = 0; q < nvar; ++q) {
+      double tmp = 0.0, tmp2 = 0.0;
+      for (int k = 0; k < ns; ++k) {
+        const double* g = ns > 1 ? a.scoord + DIM * ((size_t)b * ns + k) : x0;
+        const double w = ns > 1 ? a.sweight[(size_t)b * ns + k] : 1.0;
+        tmp = tmp + cv(vc, q, x1c, g) * w;
+        tmp2 = tmp2 + cv(vc, q, x2c, g) * w;
+      }
+      B[c + ld * q] = tmp - tmp2;
+    }
+  }
+  for (int c = 0; c < no; ++c) {
     const int vc = rowvar(c);
     const double* xc = rowx(c, vc);
     const int ic = a.inear[vc][(size_t)b * a.mmax[vc] + (c - st[vc])];
@@ -1156,6 +1309,9 @@ __global__ void k_fallback_mv(MVArgs a, const int* list, int nlist, double* scra
       if (isa && v != q && nv[v] > 0) avg = avg + (tm[q] - tm[v]) * tw;
       twsum = twsum + tw;
     }
+    // increments: their weights times their values (kriging_base.F90:3457-3466)
+    for (int v = 0; v < nvar; ++v)
+      for (int k = 0; k < a.ngrad[v]; ++k) avg = avg + B[(gst[v] + k) + ld * q] * a.gval[v][k];
     if (!a.unbias && a.sk_mean[q] != 0.0) avg = avg + (1.0 - twsum) * a.sk_mean[q];
     avg = avg < a.bound_lo ? a.bound_lo : avg;
     avg = avg > a.bound_hi ? a.bound_hi : avg;

@@ -144,6 +144,7 @@ struct GradArgs {
 constexpr int MAXVAR = 4;
 constexpr int MAXDRIFT = 4;                // external drift functions
 constexpr int MAXP = MAXVAR + MAXDRIFT;    // constraint rows: drift + unbiasedness
+constexpr int MAXGRAD = 32;                // increment observations over all variables
 struct MVArgs {
   const Model* models;        // vgm(a+1, b+1) at a + nvar * b
   int ndim;
@@ -193,6 +194,16 @@ struct MVArgs {
   // clipped to 0 and the rest rescaled to sum 1 before the estimate and
   // variance (needs weights, the weights instance)
   int wcorr;
+  // increment (gradient-pair) observations z(x1) - z(x2) (kriging.F90:796-840,
+  // 904-920): ngrad[v] of variable v join every block's system after the
+  // neighbour groups, in variable order; coordinates (ndim, ngrad[v]),
+  // drift values (ndrift, ngrad[v])
+  int ngrad[MAXVAR];
+  const double* gx1[MAXVAR];
+  const double* gx2[MAXVAR];
+  const double* gval[MAXVAR];
+  const double* gvar[MAXVAR];
+  const double* gdrift[MAXVAR];
 };
 
 // member blocks per work item of the single-variable solve (factor reuse):

@@ -62,6 +62,7 @@ def _load_lib():
     sig("krige_set_grid_block", [_i64, _c_int, _c_int, _c_int, _pd, _c_int, _pi, _pd, _pd, _pd, _pd])
     sig("krige_set_grid_cv", [_i64])
     sig("krige_set_sim", [_i64, _c_int, ctypes.c_void_p, _c_int, _c_int, ctypes.c_void_p])
+    sig("krige_set_grad", [_i64, _c_int, _c_int, _c_int, _pd, _pd, _pd, _pd, _c_int, _pd])
     sig("krige_set_search", [_i64, _c_int] + [_c_double] * 5 + [_c_int])
     sig("krige_solve", [_i64, _c_int, _c_int])
     sig("krige_get_nblocks", [_i64, _pi])
@@ -304,6 +305,32 @@ class Kriging:
     def set_grid_cv(self):
         _check(_load_lib().krige_set_grid_cv(self._handle), "krige_set_grid_cv")
         self._nblock = int(self._nobs[0])
+
+    def set_grad(self, coord1, coord2, grad_value, ivar: int = 1, variance=None, drift_ext=None):
+        """Increment (gradient-pair) observations Z(coord1[i]) - Z(coord2[i]) =
+        grad_value[i] of variable ivar (reference kriging.py set_grad): every
+        pair joins every block's system; variance relaxes the constraint;
+        drift_ext (ngrad, ndrift) holds the drift differences when ndrift > 0.
+        An empty set clears the variable's pairs."""
+        c1, c2 = _fcoord(coord1), _fcoord(coord2)
+        ng = c1.shape[1] if np.asarray(coord1).size else 0
+        gv = np.asfortranarray(np.asarray(grad_value, dtype=np.float64).ravel())
+        if gv.size != ng or c2.shape != c1.shape:
+            raise ValueError("coord1, coord2 and grad_value must describe the same pairs")
+        var = np.zeros(max(ng, 1)) if variance is None else np.asfortranarray(variance, dtype=np.float64).ravel()
+        if variance is not None and var.size != ng:
+            raise ValueError(f"variance length ({var.size}) must match ngrad ({ng})")
+        if drift_ext is not None:
+            de = np.asarray(drift_ext, dtype=np.float64)
+            de = de.reshape(ng, -1) if de.ndim < 2 else de
+            df, nd = np.asfortranarray(de.T), de.shape[1]
+        else:
+            df, nd = np.zeros((1, max(ng, 1)), order="F"), 0
+        if ng == 0:
+            c1 = c2 = np.zeros((self.ndim, 1), order="F")
+            gv = np.zeros(1)
+        _check(_load_lib().krige_set_grad(self._handle, ivar, ng, c1.shape[0], _dp(c1), _dp(c2), _dp(gv), _dp(var),
+                                          nd, _dp(df)), "krige_set_grad")
 
     def set_sim(self, randpath: Optional[np.ndarray] = None, sample: Optional[np.ndarray] = None):
         """Sequential Gaussian simulation path and draws (reference

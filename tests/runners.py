@@ -196,6 +196,9 @@ def make_gpu_ck(spec, **kw):
         k.set_vgm(**v)
     for s in spec["search"]:
         k.set_search(**s)
+    for g in spec.get("grad", []):
+        k.set_grad(g["coord1"], g["coord2"], g["value"], ivar=g["ivar"], variance=g.get("variance"),
+                   drift_ext=g.get("drift"))
     return k
 
 

@@ -88,9 +88,9 @@ def test_bad_handle():
 def test_out_of_scope_entry_points_report(K):
     from krigekit_amd import kriging as kk
     lib = kk._load_lib()
-    lib.krige_set_grad.argtypes = [ctypes.c_int64] + [ctypes.c_int] * 3 + [ctypes.c_void_p] * 4 + \
-        [ctypes.c_int, ctypes.c_void_p]
-    assert lib.krige_set_grad(ctypes.c_int64(K._handle), 1, 0, 2, None, None, None, None, 0, None) == 1
+    lib.krige_set_nscore.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    assert lib.krige_set_nscore(ctypes.c_int64(K._handle), 1, 0.0, 1.0, 1, 1, 0.0, 1.0, 0, None) == 1
     assert "not provided by this backend" in kk._last_error()
 
 
