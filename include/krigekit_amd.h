@@ -90,7 +90,14 @@ int krige_set_sim(int64_t handle, int nblocks, const void *randpath, int nsim, i
 /* src/libkriging/kriging_capi.F90:448-467 (builds the kd-tree) */
 int krige_set_search(int64_t handle, int ivar, double anis1, double anis2, double azimuth,
                      double dip, double plunge, int sector_search);
-/* src/libkriging/kriging_capi.F90 (krige_set_grad): returns 1, not provided */
+/* src/libkriging/kriging_capi.F90:472-506, kriging_base.F90:2591-2660: increment
+ * observations z(coord1) - z(coord2) of variable ivar (1-based), with their
+ * values, variances (may be NULL: 0) and, when the model has external drift,
+ * ndrift drift values per pair ((ngrad, ndrift) column-major).  ngrad <= 0
+ * clears the variable's increments.  Up to 32 increments in all; every
+ * increment joins every block's system, after the neighbour groups.  Not
+ * combined with space-time, stored weights, weight correction, cross-
+ * validation or varying variograms (the solve reports the error). */
 int krige_set_grad(int64_t handle, int ivar, int ngrad, int ndim, const double *coord1,
                    const double *coord2, const double *grad_val, const double *variance,
                    int ndrift, const double *drift_ext);
