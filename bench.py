@@ -224,7 +224,7 @@ def errors_vs_reference(runs, c2_ref, c2_nnear, stride=4):
     return out
 
 
-def loo_gradient(steps=3):
+def loo_gradient(steps=10):
     """leave-one-out MSE gradient on 100,000 observations (90,000 uniform +
     10,000 duplicates jittered by 0.01 m), nmax 16, anisotropic spherical
     (nugget 0.1, sill 1, ranges 1500/600, azimuth 45): adjoint kernel timing
@@ -235,13 +235,22 @@ def loo_gradient(steps=3):
     spec = synth.loo2d()
     names = ["nugget", "sill", "a_major", "a_minor1", "azimuth"]
     k = from_spec(spec, table_mode=False)
-    k.loo_grad()
+    v0 = dict(spec["vgm"][0])
+    vset = {key: v0[key] for key in ("vtype", "nugget", "sill", "a_major", "a_minor1", "azimuth")}
     device_synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        mse, g = k.loo_grad()
+    k.loo_grad()  # first evaluation: the search and a full pass
+    first = time.perf_counter() - t0
+    # an optimiser's loop: a new variogram per evaluation, the neighbour
+    # lists kept (model rebuild, adjoint kernel, device reduction)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        k.set_vgm(1, 1, append=False, **dict(vset, sill=v0["sill"] * (1.0 + 1e-3 * (i % 2 + 1))))
+        k.loo_grad()
     wall = (time.perf_counter() - t0) / steps
     tm = k.timing
+    k.set_vgm(1, 1, append=False, **vset)
+    mse, g = k.loo_grad()
     k.close()
 
     def mse_at(name, val):
@@ -257,15 +266,14 @@ def loo_gradient(steps=3):
         ok = np.isfinite(e)
         return float(np.mean((e[ok] - z[ok]) ** 2))
 
-    v0 = spec["vgm"][0]
     rel = {}
     for n in names:
         h = 1e-4 * abs(v0[n])
         fd = (mse_at(n, v0[n] + h) - mse_at(n, v0[n] - h)) / (2 * h)
         rel[n] = abs(g[n] - fd) / abs(fd)
     return dict(obs=len(spec["obs"][0]["value"]), mse=mse, grad={n: g[n] for n in names},
-                ms_per_eval=wall * 1e3, grad_kernel_ms=tm["loo_grad_kernel_ms"],
-                forward_device_ms=tm["total_ms"], fd_rel_err=rel, fd_max_rel_err=max(rel.values()))
+                ms_per_eval=wall * 1e3, grad_kernel_ms=tm["loo_grad_kernel_ms"], first_eval_ms=first * 1e3,
+                fd_rel_err=rel, fd_max_rel_err=max(rel.values()))
 
 
 def resident_rate(k, nodes, steps=3):

@@ -603,9 +603,14 @@ struct Krige {
   // device residency
   bool dev_inputs_ok = false;
   bool model_ok = false;
+  // d_nnear / d_inear hold the single-variable lists of the current
+  // observations, blocks and search (cleared with dev_inputs_ok): the
+  // leave-one-out gradient re-evaluates a new variogram on them
+  bool lists_ok = false;
+  int list_rows = 0;  // largest neighbour count in those lists
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_tmodel, d_ttab;
+  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_gcount, d_tmodel, d_ttab;
   // factor reuse over identical neighbour sets: -1 auto (on unless
   // leave-one-out, whose sets are all distinct), 0 off, 1 on; members per
   // work item
@@ -3273,6 +3278,7 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
   O.sk_mean = sk_mean;
   if (K->ndrift > 0) O.drift.assign((size_t)K->ndrift * nobs, 0.0);
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->solved = false;
   return 0;
 }
@@ -3311,6 +3317,7 @@ int krige_set_obs_drift(int64_t h, int ivar, int ndrift_c, int nobs, const doubl
   }
   O.drift.assign(drift, drift + (size_t)ndrift_c * nobs);
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->solved = false;
   return 0;
 }
@@ -3325,6 +3332,7 @@ int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
   }
   K->obs[ivar - 1].value.assign(value, value + nobs);
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   // a filled weight store and a finished solve switch the next solve to
   // the stored weights (kriging_base.F90:669-678)
   if (K->ws.alloc && K->ws.stored() && K->solved) {
@@ -3415,6 +3423,7 @@ static int set_grid_common(Krige* K, int nblock, const double* rangescale, const
   if (localnugget && !ln_zero) K->localnugget.assign(localnugget, localnugget + nblock);
   K->cross_validation = false;
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->perm_ok = false;
   K->solved = false;
   K->bdrift.assign((size_t)K->ndrift * K->nvar * nblock, 0.0);
@@ -3599,6 +3608,7 @@ int krige_set_grid_drift(int64_t h, int ivar, int ndrift_c, int nblocks, const d
       if (ivar < 0 || q == ivar - 1)
         for (int k = 0; k < nd; ++k) K->bdrift[k + (size_t)nd * (q + (size_t)nv * b)] = drift[k + (size_t)nd * b];
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->solved = false;
   return 0;
 }
@@ -3700,6 +3710,7 @@ int krige_set_sim(int64_t h, int nblocks, const void* randpath, int nsim_c, int 
   K->sgs_sample = std::move(draws);
   K->sim_set = true;
   K->solved = false;
+  K->lists_ok = false;
   return 0;
 }
 
@@ -3733,6 +3744,7 @@ int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azi
   if (O.aniso) calc_rotmat(azimuth, dip, plunge, anis1, anis2, O.rot);
   if (!build_search(K, O, false)) return ierr();
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->solved = false;
   return 0;
 }
@@ -3755,6 +3767,7 @@ int krige_set_grad(int64_t h, int ivar, int ngrad, int ndim_c, const double* coo
   }
   Obs& O = K->obs[ivar - 1];
   K->dev_inputs_ok = false;
+  K->lists_ok = false;
   K->solved = false;
   if (ngrad <= 0) {
     O.ngrad = 0;
@@ -4580,8 +4593,31 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
     set_error(W, "the gradient does not support per-block range scale or local nugget");
     return ierr();
   }
-  if (!do_solve(*K, false)) return ierr();
+  if (K->nsim > 0 || K->use_old_weight) {
+    set_error(W, "the gradient needs its own kriging pass (no simulation, no stored-weight replay)");
+    return ierr();
+  }
+  // The neighbour lists do not depend on the variogram: after the first
+  // evaluation (one full pass) only the model is rebuilt, and the adjoint
+  // kernel solves every block's system once for both the estimate and the
+  // gradient.
   const int nb = K->nblock, nd = K->ndim;
+  if (K->lists_ok) {
+    if (!validate_for_solve(*K) || !stage_inputs(*K)) return ierr();
+  } else {
+    if (!do_solve(*K, false)) return ierr();
+    // the largest system decides how many blocks share a wave
+    int rows = 0;
+    if (!K->d_gcount.ensure(sizeof(int))) {
+      set_error(W, "device allocation failed");
+      return ierr();
+    }
+    HIPCHK_I(kk::launch_max_count(K->d_nnear.as<int>(), nb, K->d_gcount.as<int>(), K->stream), W);
+    HIPCHK_I(hipMemcpyAsync(&rows, K->d_gcount.p, sizeof(int), hipMemcpyDeviceToHost, K->stream), W);
+    HIPCHK_I(hipStreamSynchronize(K->stream), W);
+    K->list_rows = rows;
+    K->lists_ok = true;
+  }
   if (!K->d_gz.ensure(sizeof(double) * nb) || !K->d_gdz.ensure(sizeof(double) * (size_t)nb * kk::NGRAD) ||
       !K->d_gred.ensure(sizeof(double) * kk::loo_reduce_words())) {
     set_error(W, "device allocation failed");
@@ -4592,6 +4628,7 @@ int kk_loo_grad(int64_t h, double* mse, double* grad, double* zhat, double* dz) 
   ga.ndim = nd;
   ga.nb = nb;
   ga.mmax = K->mmax;
+  ga.rows = K->list_rows;
   ga.bcoord = K->d_bcoord.as<double>();
   ga.ocoord = K->d_ocoord.as<double>();
   ga.ovalue = K->d_ovalue.as<double>();

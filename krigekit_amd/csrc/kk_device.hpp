@@ -102,6 +102,8 @@ __host__ __device__ constexpr size_t fallback_words(int mmax) {
 }
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
+// largest of nnear[0..nb) into the device int *out
+hipError_t launch_max_count(const int* nnear, int nb, int* out, hipStream_t st);
 // device reduction of the LOO squared error and gradient: work holds
 // loo_reduce_words() doubles; the NGRAD + 2 sums (sum r^2, sum r dz_p, count)
 // are left at its end, work + loo_reduce_words() - (NGRAD + 2)

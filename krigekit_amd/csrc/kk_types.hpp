@@ -129,6 +129,7 @@ struct GradArgs {
   int ndim;
   int nb;
   int mmax;
+  int rows;                   // largest neighbour count of any block (<= mmax)
   const double* bcoord;       // (ndim, nb) targets (the observations themselves)
   const double* ocoord;       // (ndim, nobs)
   const double* ovalue;

@@ -31,10 +31,13 @@ def gpu_mse(spec, table):
     return float(np.mean((e[ok] - z[ok]) ** 2))
 
 
+@pytest.mark.parametrize("nmax", [16, 6, 24, 40])
 @pytest.mark.parametrize("table", [False, True], ids=["analytic", "table"])
-def test_gradient_matches_oracle(table):
-    spec = problems.loo_grad_problem()
-    print("loo_grad_problem table", table, flush=True)
+def test_gradient_matches_oracle(table, nmax):
+    """nmax 6, 16 and 24 run 8, 16 and 32 lanes per block (several blocks per
+    wave); nmax 40 one block per workgroup"""
+    spec = problems.loo_grad_problem(nmax=nmax)
+    print("loo_grad_problem table", table, "nmax", nmax, flush=True)
     mse, g, zh, dz = gpu_grad(spec, table, per_block=True)
     omse, og, ozh, odz = lg.loo_grad(spec, table_mode=table)
     assert mse == pytest.approx(omse, rel=1e-12)
@@ -85,3 +88,49 @@ def test_autograd_function():
     np.testing.assert_allclose(theta.grad.numpy(), [og[n] for n in NAMES], rtol=1e-9)
     assert float(loss.detach()) == pytest.approx(lg.loo_mse(spec, False), rel=1e-12)
     k.close()
+
+
+def _same(a, b):
+    assert a[0] == b[0]
+    assert a[1] == b[1]
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[3], b[3])
+
+
+@pytest.mark.parametrize("table", [False, True], ids=["analytic", "table"])
+def test_new_variogram_keeps_lists(table):
+    """after set_vgm the handle keeps its neighbour lists and re-runs only the
+    adjoint kernel: the result equals a fresh handle's at the new variogram,
+    bit for bit"""
+    spec = problems.loo_grad_problem()
+    print("loo keep lists table", table, flush=True)
+    k = make_gpu(spec, table_mode=table)
+    k.loo_grad(per_block=True)
+    v = dict(spec["vgm"][0], sill=1.3, a_major=1700.0, azimuth=30.0)
+    k.set_vgm(append=False, **v)
+    again = k.loo_grad(per_block=True)
+    k.close()
+    s2 = dict(spec, vgm=[v])
+    _same(again, gpu_grad(s2, table, per_block=True))
+
+
+def test_new_inputs_rebuild_lists():
+    """new observation values or a new search (here nmax) are searched again:
+    the result equals a fresh handle's"""
+    spec = problems.loo_grad_problem()
+    print("loo rebuild lists", flush=True)
+    k = make_gpu(spec, table_mode=False)
+    k.loo_grad(per_block=True)
+    z2 = np.asarray(spec["obs"][0]["value"]) * 1.5 + 0.25
+    k.update_obs_value(1, z2)
+    got = k.loo_grad(per_block=True)
+    o2 = dict(spec["obs"][0], value=z2)
+    _same(got, gpu_grad(dict(spec, obs=[o2]), False, per_block=True))
+    o3 = dict(spec["obs"][0], nmax=8)
+    k.set_obs(1, o3["coord"], o3["value"], nmax=8)
+    k.set_grid_cv()
+    for sr in spec["search"]:
+        k.set_search(**sr)
+    got = k.loo_grad(per_block=True)
+    k.close()
+    _same(got, gpu_grad(dict(spec, obs=[o3]), False, per_block=True))
