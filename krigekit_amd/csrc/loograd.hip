@@ -137,10 +137,11 @@ __device__ inline double wave_sum(double v) {
 
 constexpr int GMAX = 64;
 
-// registers for five waves per SIMD (95 VGPRs, a few spills): faster than
-// four at 109 VGPRs
+// One block per 64-lane workgroup, K in LDS: systems of 33-64 rows.  Their
+// K takes 9-33 KB of LDS, so LDS caps the waves per SIMD (at most 4), and
+// no register bound is set (one for five waves only added spills).
 template <int DIM>
-__global__ __launch_bounds__(64, 5) void k_loo_grad(GradArgs a) {
+__global__ __launch_bounds__(64) void k_loo_grad(GradArgs a) {
   extern __shared__ double Ks[];  // mmax x (mmax + 1)
   __shared__ double sx[DIM][GMAX];
   __shared__ double rh[3][GMAX];
