@@ -183,15 +183,20 @@ __device__ inline double wg_sum(double v, double* part) {
 constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 
-template <int DIM, int NV, int RB, bool SIMPLE, bool WEIGHTS, bool MF = false>
+// DRIFT instances hold up to MAXDRIFT external-drift constraint rows; the
+// others none, so that their one-thread Schur step stays NV x NV (sized for
+// the drifts it took the 2-variable solve from 73 to 82 ms).
+template <int DIM, int NV, int RB, bool SIMPLE, bool WEIGHTS, bool MF = false, bool GRAD = false, bool DRIFT = GRAD>
 __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k_solve_mv(MVArgs a) {
-  static_assert(!MF || (RB == 5 && !WEIGHTS), "the FP32 MFMA instance covers up to 80 columns, no weight store");
+  static_assert(!MF || (RB == 5 && !WEIGHTS && !DRIFT),
+                "the FP32 MFMA instance covers up to 80 columns, no weight store, no drift");
+  static_assert(!GRAD || (!SIMPLE && !WEIGHTS && !MF && DRIFT), "increment rows: general-shape FP64 instances only");
+  constexpr int ND = DRIFT ? MAXDRIFT : 0;  // drift constraint rows at most
   // rows + extra rows fit 16 RB (the instance is chosen by the column count)
   constexpr int ROWS = 16 * RB;
   __shared__ double sx[DIM][ROWS];
-  // increment rows (general-shape instances only; the host runs problems with
-  // increments there): the second point
-  constexpr bool GRAD = !SIMPLE;
+  // increment rows (GRAD instances only: their code in every general
+  // instance took those from 2 to 1 wave per SIMD): the second point
   __shared__ double sx2[GRAD ? DIM : 1][GRAD ? MAXGRAD : 1];
   __shared__ double sz[ROWS];
   __shared__ double sov[ROWS];
@@ -214,13 +219,13 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
   // MF: the refined solutions x_f (columns of K^-1 X', FP64), the matrix-vector
   // products M [x_f; -e_f], 1 / L(k, k), K's diagonal
-  constexpr int MXE = 2 * NV + 2 + MAXDRIFT;  // extra columns at most (non-standard cokriging, drift)
+  constexpr int MXE = 2 * NV + 2 + ND;  // extra columns at most (non-standard cokriging, drift)
   __shared__ double sxd[MF ? MXE : 1][MF ? ROWS : 1];
   __shared__ double red[MF ? ROWS : 1][MF ? MXE : 1];
   __shared__ float sli[MF ? ROWS : 1];
   __shared__ float sdg[MF ? ROWS : 1];
   __shared__ double gram[MAXE][MAXE];
-  __shared__ double sdr[MAXDRIFT][ROWS];  // external drift values of the rows
+  __shared__ double sdr[DRIFT ? MAXDRIFT : 1][DRIFT ? ROWS : 1];  // external drift values of the rows
   __shared__ double sbase[NV * NV];
   __shared__ double part[4];
   __shared__ double partb[4][NV * NV];
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // WEIGHTS: A(i, j), j <= i, j < n, at sl[i (i + 1) / 2 + j]; multipliers;
   // the weights finished so far
   __shared__ double sl[WEIGHTS ? ROWS * (ROWS + 1) / 2 : 1];
-  __shared__ double smu[NV][NV + MAXDRIFT];
+  __shared__ double smu[NV][NV + ND];
   __shared__ double sw[NV][WEIGHTS ? ROWS : 1];
   const int b = a.list ? a.list[blockIdx.x] : (int)blockIdx.x, t = threadIdx.x;
   // column class tc = t / 16: the 16 threads of a column class (all row
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   }
   // constraint rows: the external drifts, then unbiasedness (one per variable
   // under std_ck, else one row of ones)
-  const int nd = a.ndrift;
+  const int nd = DRIFT ? a.ndrift : 0;
   const int p = nd + (a.unbias ? (a.std_ck ? nvar : 1) : 0);
   const bool isa = !a.std_ck && a.unbias && nvar > 1;  // Isaaks-Srivastava shift
   // right-hand side of constraint row c for target q
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // neighbours leaves a zero row in S under std_ck) sends the block to the
   // Bunch-Kaufman fallback like the reference's failed SPOTRF of S
   if (t == 0) {
-    constexpr int PM = NV + MAXDRIFT;  // constraint rows at most
+    constexpr int PM = NV + ND;  // constraint rows at most
     double mu[NV][PM];  // mu[q][c]
     bool ok = true;
 #pragma unroll
@@ -1009,7 +1014,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       const int rb_ = n + eb + q;
       double v = sl[rb_ * (rb_ + 1) / 2 + t];
 #pragma unroll
-      for (int f = 0; f < NV + MAXDRIFT; ++f) {
+      for (int f = 0; f < NV + ND; ++f) {
         if (f >= p) break;
         const int rf = n + ef + f;
         v -= smu[q][f] * sl[rf * (rf + 1) / 2 + t];
@@ -1335,10 +1340,13 @@ int mv_extra_cols(int nvar, int std_ck, int unbias, int ndrift) {
   return nvar + p + 1 + (isa ? nvar : 0) + (unbias ? 0 : 1);
 }
 
-// FP32 + MFMA + refinement runs for systems up to 80 columns (weights: FP64)
+// FP32 + MFMA + refinement runs for systems up to 80 columns (weights,
+// external drift and increment rows: FP64)
 bool mv_fp32r_instance(const MVArgs& a) {
   const int cols = a.rows + mv_extra_cols(a.nvar, a.std_ck, a.unbias, a.ndrift);
-  return a.prec == PREC_FP32R && !a.weights && cols <= 80;
+  bool grad = false;
+  for (int v = 0; v < a.nvar; ++v) grad = grad || a.ngrad[v] > 0;
+  return a.prec == PREC_FP32R && !a.weights && !grad && a.ndrift == 0 && cols <= 80;
 }
 
 hipError_t launch_solve_mv(const MVArgs& a, hipStream_t st) {
@@ -1365,12 +1373,34 @@ template <int NV>
 static void launch_nv(const MVArgs& a, int rb, hipStream_t st) {
   constexpr int D = KK_MV_DIM;
   dim3 g(a.list ? a.nlist : a.nb), blk(MVT);
+  const bool drift = a.ndrift > 0;
 #if KK_MV_W
   // the weight store: general covariance evaluation only
-  if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, true>), g, blk, 0, st, a);
-  else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, true>), g, blk, 0, st, a);
+  if (drift) {
+    if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, true, false, false, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, true, false, false, true>), g, blk, 0, st, a);
+  } else {
+    if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, true>), g, blk, 0, st, a);
+  }
 #else
-  if (rb == 5 && a.prec == PREC_FP32R) {
+  bool grad = false;
+  for (int v = 0; v < a.nvar; ++v) grad = grad || a.ngrad[v] > 0;
+  if (grad) {
+    // increment rows: the FP64 general instance that holds them (either
+    // precision; their systems are solved in FP64)
+    if (rb == 5) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false, false, true>), g, blk, 0, st, a);
+    else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, false, false, true>), g, blk, 0, st, a);
+  } else if (drift) {
+    // external drift rows: FP64 instances (either precision)
+    if (rb == 5) {
+      if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, true, false, false, false, true>), g, blk, 0, st, a);
+      else hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false, false, false, true>), g, blk, 0, st, a);
+    } else {
+      if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 8, true, false, false, false, true>), g, blk, 0, st, a);
+      else hipLaunchKernelGGL((k_solve_mv<D, NV, 8, false, false, false, false, true>), g, blk, 0, st, a);
+    }
+  } else if (rb == 5 && a.prec == PREC_FP32R) {
     if (a.simple) hipLaunchKernelGGL((k_solve_mv<D, NV, 5, true, false, true>), g, blk, 0, st, a);
     else hipLaunchKernelGGL((k_solve_mv<D, NV, 5, false, false, true>), g, blk, 0, st, a);
   } else if (rb == 5) {
