@@ -213,9 +213,14 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   constexpr int LD32 = ROWS + 4;
   constexpr int SBW = NPART * NV * ROWS;
   constexpr int UW = MF ? cmax3(SBW + RB * MVT, ROWS * LD32 / 2, ROWS * (ROWS + 1) / 2) : SBW + RB * MVT;
-  __shared__ __align__(16) double uarea[UW];
-  double(*const sb)[NV][ROWS] = reinterpret_cast<double(*)[NV][ROWS]>(uarea);
-  double* const sbuf = uarea + SBW;
+  // (MF only: the FP64 instances keep sb and sbuf as arrays of their own,
+  // whose accesses the compiler can tell apart; carved out of one array they
+  // cost the 2-variable FP64 solve 5%)
+  __shared__ __align__(16) double uarea[MF ? UW : 1];
+  __shared__ double sb_own[MF ? 1 : NPART][NV][ROWS];
+  __shared__ __align__(16) double sbuf_own[MF ? 1 : RB * MVT];
+  double(*const sb)[NV][ROWS] = MF ? reinterpret_cast<double(*)[NV][ROWS]>(uarea) : sb_own;
+  double* const sbuf = MF ? uarea + SBW : sbuf_own;
   double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
   // MF: the refined solutions x_f (columns of K^-1 X', FP64), the matrix-vector
   // products M [x_f; -e_f], 1 / L(k, k), K's diagonal
