@@ -209,7 +209,10 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // factorisation, the panel broadcast L(:, c0..c0+3), double-buffered.
   // MF: after the assembly the same area holds the packed lower triangle
   // (mirror exchange), then the FP32 matrix (column stride LD32 floats).
-  static_assert(RB * MVT >= 2 * ROWS * 4, "panel buffer fits the assembly slots");
+  // panel rows padded to PLD doubles (48 bytes): the 16 rows one wave's
+  // 128-bit reads take in the trailing update then start in distinct banks
+  constexpr int PLD = 6;
+  static_assert(RB * MVT >= 2 * ROWS * PLD, "panel buffer fits the assembly slots");
   constexpr int LD32 = ROWS + 4;
   constexpr int SBW = NPART * NV * ROWS;
   constexpr int UW = MF ? cmax3(SBW + RB * MVT, ROWS * LD32 / 2, ROWS * (ROWS + 1) / 2) : SBW + RB * MVT;
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   __shared__ __align__(16) double sbuf_own[MF ? 1 : RB * MVT];
   double(*const sb)[NV][ROWS] = MF ? reinterpret_cast<double(*)[NV][ROWS]>(uarea) : sb_own;
   double* const sbuf = MF ? uarea + SBW : sbuf_own;
-  double(*const vpan)[ROWS][4] = reinterpret_cast<double(*)[ROWS][4]>(sbuf);
+  double(*const vpan)[ROWS][PLD] = reinterpret_cast<double(*)[ROWS][PLD]>(sbuf);
   // MF: the refined solutions x_f (columns of K^-1 X', FP64), the matrix-vector
   // products M [x_f; -e_f], 1 / L(k, k), K's diagonal
   constexpr int MXE = 2 * NV + 2 + ND;  // extra columns at most (non-standard cokriging, drift)
@@ -754,7 +757,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
   // the panel buffer takes the assembly slots: cleared once every thread has
   // read its own (rows past nx keep zero factors throughout)
   __syncthreads();
-  for (int w = t; w < 2 * ROWS * 4; w += MVT) sbuf[w] = 0.0;
+  for (int w = t; w < 2 * ROWS * PLD; w += MVT) sbuf[w] = 0.0;
   __syncthreads();
 
   // ---- Cholesky of the n pivots, four columns per barrier ------------------------
@@ -777,7 +780,7 @@ __global__ __launch_bounds__(MVT, MF ? 4 : (RB == 5 && !WEIGHTS) ? 5 : 1) void k
       if (c0 >= n) break;
       const int np = n - c0 < 4 ? n - c0 : 4;  // pivots in this panel
       const int last = c0 + np - 1;
-      double(*vp)[4] = vpan[pp & 1];
+      double(*vp)[PLD] = vpan[pp & 1];
       if (wave == pp) {
         const int kl = (t & 63) >> 4;  // this lane's panel column: tc = 4 pp + kl
         for (int k = 0; k < np; ++k) {
