@@ -47,6 +47,7 @@ UNITS = [
     ("sets.hip", [], "sets.o"),
     ("replay.hip", [], "replay.o"),
     ("sgsim.hip", [], "sgsim.o"),
+    ("order.hip", [], "order.o"),
     ("capi.cpp", ["-x", "hip"], "capi.o"),
 ]
 

@@ -610,7 +610,7 @@ struct Krige {
   int list_rows = 0;  // largest neighbour count in those lists
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
-  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_gcount, d_tmodel, d_ttab;
+  DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_gcount, d_tmodel, d_ttab, d_owork;
   // factor reuse over identical neighbour sets: -1 auto (on unless
   // leave-one-out, whose sets are all distinct), 0 off, 1 on; members per
   // work item
@@ -823,37 +823,17 @@ bool stage_inputs(Krige& K) {
 bool morton_perm(Krige& K) {
   if (K.perm_ok) return true;
   const char* W = "t_kriging_base%solve_base";
-  const std::vector<double>& bc = K.cross_validation ? K.obs[0].coord : K.bcoord;
   const int nd = K.ndim, nb = K.nblock;
-  // space-time queries are ordered in the search metric (t * time_at)
+  // the queries (cross-validation: the observations) are resident in
+  // d_bcoord; space-time queries are ordered in the search metric (t * time_at)
   const double tsc = K.st ? K.obs[0].time_at : 1.0;
-  auto coordk = [&](int k, int b) { return bc[k + (size_t)nd * b] * (k == 3 ? tsc : 1.0); };
-  double lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
-  for (int k = 0; k < nd; ++k) {
-    lo[k] = hi[k] = coordk(k, 0);
-    for (int b = 1; b < nb; ++b) {
-      lo[k] = std::min(lo[k], coordk(k, b));
-      hi[k] = std::max(hi[k], coordk(k, b));
-    }
+  const size_t wb = kk::order_work_bytes(nb, nd);
+  if (K.d_bcoord.bytes < sizeof(double) * (size_t)nd * nb || !K.d_perm.ensure(sizeof(int) * (size_t)nb) ||
+      !K.d_owork.ensure(wb)) {
+    set_error(W, "device allocation failed (query order)");
+    return false;
   }
-  double ext = 0.0;
-  for (int k = 0; k < nd; ++k) ext = std::max(ext, hi[k] - lo[k]);
-  if (!(ext > 0.0)) ext = 1.0;
-  const int bits = nd == 1 ? 30 : nd == 2 ? 21 : nd == 3 ? 20 : 15;
-  const double scale = (double)((1u << bits) - 1) / ext;
-  std::vector<std::pair<uint64_t, int>> key(nb);
-  for (int b = 0; b < nb; ++b) {
-    uint64_t code = 0;
-    uint32_t c[4] = {0, 0, 0, 0};
-    for (int k = 0; k < nd; ++k) c[k] = (uint32_t)((coordk(k, b) - lo[k]) * scale);
-    for (int bit = bits - 1; bit >= 0; --bit)
-      for (int k = 0; k < nd; ++k) code = (code << 1) | ((c[k] >> bit) & 1u);
-    key[b] = {code, b};
-  }
-  std::sort(key.begin(), key.end());
-  std::vector<int> perm(nb);
-  for (int b = 0; b < nb; ++b) perm[b] = key[b].second;
-  if (!upload(K.d_perm, perm, W)) return false;
+  HIPCHK(kk::launch_order(K.d_bcoord.as<double>(), nb, nd, tsc, K.d_perm.as<int>(), K.d_owork.p, wb, K.stream), W);
   K.perm_ok = true;
   return true;
 }

@@ -101,6 +101,11 @@ __host__ __device__ constexpr size_t fallback_words(int mmax) {
   return 2 * (size_t)(mmax + 1) * (mmax + 1) + 3 * (size_t)(mmax + 1);
 }
 // leave-one-out estimates and their parameter derivatives, one block per workgroup
+// Morton order of nb query points (ndim nd, the fourth coordinate scaled by
+// tsc) into perm; work holds order_work_bytes(nb, nd) bytes
+size_t order_work_bytes(int nb, int nd);
+hipError_t launch_order(const double* bc, int nb, int nd, double tsc, int* perm, void* work, size_t work_bytes,
+                        hipStream_t st);
 hipError_t launch_loo_grad(const GradArgs& a, hipStream_t st);
 // largest of nnear[0..nb) into the device int *out
 hipError_t launch_max_count(const int* nnear, int nb, int* out, hipStream_t st);
