@@ -608,6 +608,9 @@ struct Krige {
   // leave-one-out gradient re-evaluates a new variogram on them
   bool lists_ok = false;
   int list_rows = 0;  // largest neighbour count in those lists
+  // blocks without neighbours in the last single-variable pass (counted by
+  // its bucketing; -1 unknown)
+  long pass_empty = -1;
   DBuf d_ocoord, d_socoord, d_ovalue, d_ovar, d_rdata, d_ind, d_ni, d_nd, d_qcoord, d_bcoord;
   DBuf d_rs, d_ln, d_soff, d_scoord, d_sweight, d_basecov, d_tab;
   DBuf d_model, d_sflag, d_perm, d_xlist, d_gz, d_gdz, d_gred, d_gcount, d_tmodel, d_ttab, d_owork;
@@ -1496,6 +1499,7 @@ bool run_pipeline(Krige& K, const PassBufs* ext = nullptr) {
     K.stats[1] = nb - nempty - nfb;
     K.stats[2] = 0;
   }
+  K.pass_empty = nempty;
   K.stats[3] = nfb;
   K.stats[4] = 0;
   K.timing[5] = nref;
@@ -1829,8 +1833,9 @@ bool fetch_results(Krige& K) {
     }
   }
   // blocks without neighbours are an error unless neglect_error
-  // (kriging_base.F90:3198-3207)
-  if (!K.neglect_error) {
+  // (kriging_base.F90:3198-3207); the pass's bucketing counted them, so the
+  // counts are read back only to name the first
+  if (!K.neglect_error && K.pass_empty != 0) {
     std::vector<int> nn(nb);
     HIPCHK(hipMemcpy(nn.data(), K.d_nnear.p, sizeof(int) * nb, hipMemcpyDeviceToHost), W);
     for (int b = 0; b < nb; ++b)
@@ -3259,6 +3264,7 @@ int krige_set_obs(int64_t h, int ivar, int nobs, int ndim_c, const double* coord
   if (K->ndrift > 0) O.drift.assign((size_t)K->ndrift * nobs, 0.0);
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->solved = false;
   return 0;
 }
@@ -3298,6 +3304,7 @@ int krige_set_obs_drift(int64_t h, int ivar, int ndrift_c, int nobs, const doubl
   O.drift.assign(drift, drift + (size_t)ndrift_c * nobs);
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->solved = false;
   return 0;
 }
@@ -3313,6 +3320,7 @@ int krige_update_obs_value(int64_t h, int ivar, int nobs, const double* value) {
   K->obs[ivar - 1].value.assign(value, value + nobs);
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   // a filled weight store and a finished solve switch the next solve to
   // the stored weights (kriging_base.F90:669-678)
   if (K->ws.alloc && K->ws.stored() && K->solved) {
@@ -3404,6 +3412,7 @@ static int set_grid_common(Krige* K, int nblock, const double* rangescale, const
   K->cross_validation = false;
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->perm_ok = false;
   K->solved = false;
   K->bdrift.assign((size_t)K->ndrift * K->nvar * nblock, 0.0);
@@ -3589,6 +3598,7 @@ int krige_set_grid_drift(int64_t h, int ivar, int ndrift_c, int nblocks, const d
         for (int k = 0; k < nd; ++k) K->bdrift[k + (size_t)nd * (q + (size_t)nv * b)] = drift[k + (size_t)nd * b];
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->solved = false;
   return 0;
 }
@@ -3691,6 +3701,7 @@ int krige_set_sim(int64_t h, int nblocks, const void* randpath, int nsim_c, int 
   K->sim_set = true;
   K->solved = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   return 0;
 }
 
@@ -3725,6 +3736,7 @@ int krige_set_search(int64_t h, int ivar, double anis1, double anis2, double azi
   if (!build_search(K, O, false)) return ierr();
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->solved = false;
   return 0;
 }
@@ -3748,6 +3760,7 @@ int krige_set_grad(int64_t h, int ivar, int ngrad, int ndim_c, const double* coo
   Obs& O = K->obs[ivar - 1];
   K->dev_inputs_ok = false;
   K->lists_ok = false;
+  K->pass_empty = -1;
   K->solved = false;
   if (ngrad <= 0) {
     O.ngrad = 0;
