@@ -171,9 +171,9 @@ __device__ inline double wg_sum(double v, double* part) {
 }
 
 // The second launch bound is the minimum number of waves per SIMD the
-// registers must allow: five for the 80-column instance (its 25 KB of LDS
-// admits five workgroups per CU; measured faster than four despite a few
-// spills), unconstrained for the 128-column one.
+// registers must allow: five for the 80-column instance (its 31 KB of LDS at
+// two variables admits five workgroups per CU; measured faster than four
+// despite a few spills), unconstrained for the 128-column one.
 //
 // WEIGHTS: the instance also writes the kriging weights (the weight store,
 // kriging_base.F90:2428-2439, 2760-2782).  The factor is kept in LDS
